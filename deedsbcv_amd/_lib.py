@@ -1,0 +1,124 @@
+"""ctypes binding of libdeeds_hip.so (the C ABI declared in include/deeds_hip.h).
+
+The library is built in-tree by ``python -m deedsbcv_amd.build``.  Loading
+fails loudly when it is missing: there is no CPU fallback in this package.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libdeeds_hip.so"
+
+MAX_LEVELS = 10
+c_int, c_float, c_void_p, c_char_p = ctypes.c_int, ctypes.c_float, ctypes.c_void_p, ctypes.c_char_p
+
+
+class Schedule(ctypes.Structure):
+    _fields_ = [("levels", c_int), ("grid", c_int * MAX_LEVELS), ("radius", c_int * MAX_LEVELS),
+                ("quant", c_int * MAX_LEVELS), ("alpha", c_float)]
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [("levels", c_int), ("mind", ctypes.c_double * MAX_LEVELS), ("data", ctypes.c_double * MAX_LEVELS),
+                ("mst_reg", ctypes.c_double * MAX_LEVELS), ("transf", ctypes.c_double * MAX_LEVELS),
+                ("total", ctypes.c_double), ("hamming_evals", ctypes.c_double), ("ssd_before", c_float),
+                ("ssd_after", c_float), ("jac_std", c_float * MAX_LEVELS), ("jac_neg", c_float * MAX_LEVELS)]
+
+    def as_dict(self):
+        L = self.levels
+        return {"levels": L, "mind": list(self.mind[:L]), "data": list(self.data[:L]),
+                "mst_reg": list(self.mst_reg[:L]), "transf": list(self.transf[:L]), "total": self.total,
+                "hamming_evals": self.hamming_evals, "ssd_before": self.ssd_before, "ssd_after": self.ssd_after,
+                "jac_std": list(self.jac_std[:L]), "jac_neg": list(self.jac_neg[:L])}
+
+
+# name -> argtypes (all return int status unless listed in _RESTYPES)
+I, F, V = c_int, c_float, c_void_p
+SIGNATURES = {
+    "deeds_version": [],
+    "deeds_last_error": [],
+    "deeds_set_default_device": [I],
+    "deeds_ctx_create": [I, ctypes.POINTER(c_void_p)],
+    "deeds_ctx_destroy": [V],
+    "deeds_ctx_last_error": [V],
+    "deeds_ctx_set_stream": [V, V],
+    "deeds_ctx_synchronize": [V],
+    "deeds_ctx_trim": [V],
+    "deeds_descriptor": [V, V, I, I, I, I],
+    "deeds_datacost": [V, V, V, I, I, I, I, I, F, F, I],
+    "deeds_prims_graph": [V, V, V, V, I, I, I, I],
+    "deeds_regularisation": [V, V, V, V, V, V, V, I, I, F, V, V, V, I, I, I],
+    "deeds_upsample_deformations": [V, V, V, V, V, V, I, I, I, I, I, I],
+    "deeds_consistent_mapping": [V, V, V, V, V, V, I, I, I, I],
+    "deeds_jacobian": [V, V, V, I, I, I, I, V, V],
+    "deeds_warp_affine": [V, V, V, V, V, V, V, I, I, I, V],
+    "deeds_warp_image": [V, V, V, V, V, V, I, I, I, V],
+    "deeds_warp_affine_s": [V, V, V, V, V, V, I, I, I],
+    "deeds_register": [V, V, V, I, I, I, ctypes.POINTER(Schedule), V, V, ctypes.POINTER(Timing)],
+    "deeds_apply_seg": [V, V, V, V, I, I, I, I],
+    "deeds_apply_float": [V, V, V, V, I, I, I, I],
+    "deeds_ctx_descriptor": [V, V, V, I, I, I, I],
+    "deeds_ctx_datacost": [V, V, V, V, I, I, I, I, I, F, F, I],
+    "deeds_ctx_regularisation": [V, V, V, V, V, V, V, V, I, I, F, V, V, V, I, I, I, V],
+    "deeds_ctx_register": [V, V, V, V, I, I, I, ctypes.POINTER(Schedule), V, V, ctypes.POINTER(Timing)],
+}
+_RESTYPES = {"deeds_version": c_char_p, "deeds_last_error": c_char_p, "deeds_ctx_last_error": c_char_p}
+
+_lib = None
+
+
+class DeedsError(RuntimeError):
+    """A deeds_hip call returned a nonzero status."""
+
+    def __init__(self, fn, code, msg):
+        super().__init__(f"{fn} failed with status {code}: {msg}")
+        self.code = code
+
+
+def load(path: Path | str | None = None):
+    """Load (once) and return the ctypes library with all signatures set."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise ImportError(f"deeds_hip library not found at {p}; build it with `python -m deedsbcv_amd.build`")
+    lib = ctypes.CDLL(str(p))
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = _RESTYPES.get(name, c_int)
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def call(name, *args):
+    """Call `name`, raising DeedsError on a nonzero status."""
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc != 0:
+        msg = lib.deeds_last_error()
+        raise DeedsError(name, rc, msg.decode() if msg else "")
+    return rc
+
+
+def ctx_call(ctx, name, *args):
+    lib = load()
+    rc = getattr(lib, name)(ctx, *args)
+    if rc != 0:
+        msg = lib.deeds_ctx_last_error(ctx)
+        raise DeedsError(name, rc, msg.decode() if msg else "")
+    return rc
+
+
+def make_schedule(grid, radius, quant, alpha=1.6):
+    if not (len(grid) == len(radius) == len(quant)) or not 1 <= len(grid) <= MAX_LEVELS:
+        raise ValueError("grid, radius and quant must have the same length (1..10)")
+    s = Schedule()
+    s.levels = len(grid)
+    for i, (g, r, q) in enumerate(zip(grid, radius, quant)):
+        s.grid[i], s.radius[i], s.quant[i] = int(g), int(r), int(q)
+    s.alpha = float(alpha)
+    return s

@@ -1,0 +1,572 @@
+// C-ABI entry points of deeds_hip: context management and the host-pointer
+// forms, one per reference function (see include/deeds_hip.h for the
+// reference file:line each replaces).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+#include "common.h"
+#include "kernels.h"
+#include "mst_host.h"
+
+using namespace deeds;
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+
+void* deeds_ctx::buf(const std::string& name, size_t bytes) {
+    DevBuf& b = ws[name];
+    if (b.bytes >= bytes && b.ptr) return b.ptr;
+    if (b.ptr) {
+        DEEDS_HIP(hipStreamSynchronize(stream));
+        if (aux_stream) DEEDS_HIP(hipStreamSynchronize(aux_stream));
+        DEEDS_HIP(hipFree(b.ptr));
+        b.ptr = nullptr;
+        b.bytes = 0;
+    }
+    size_t want = std::max<size_t>(bytes, 256);
+    hipError_t e = hipMalloc(&b.ptr, want);
+    if (e != hipSuccess) {
+        b.ptr = nullptr;
+        (void)hipGetLastError();
+        fail(DEEDS_ERR_ALLOC, "hipMalloc of " + std::to_string(want) + " bytes for '" + name + "' failed: " +
+                                  hipGetErrorString(e));
+    }
+    b.bytes = want;
+    return b.ptr;
+}
+
+void deeds_ctx::release_all() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (aux_stream) (void)hipStreamSynchronize(aux_stream);
+    for (auto& kv : ws)
+        if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+    ws.clear();
+}
+
+namespace deeds {
+
+namespace {
+thread_local std::string t_error;
+// The per-thread default context is intentionally never destroyed at thread or
+// process exit (the HIP runtime may already be shut down by then); it is
+// replaced only by deeds_set_default_device.
+thread_local deeds_ctx* t_ctx = nullptr;
+int g_default_device = 0;
+}  // namespace
+
+void set_thread_error(const std::string& msg) { t_error = msg; }
+
+deeds_ctx* default_ctx() {
+    if (!t_ctx) {
+        deeds_ctx* c = nullptr;
+        int rc = deeds_ctx_create(g_default_device, &c);
+        if (rc != DEEDS_OK) fail(rc, "cannot create the default context: " + t_error);
+        t_ctx = c;
+    }
+    return t_ctx;
+}
+
+}  // namespace deeds
+
+extern "C" {
+
+const char* deeds_version(void) { return "deeds_hip 0.1 (gfx950)"; }
+
+const char* deeds_last_error(void) { return t_error.c_str(); }
+
+int deeds_set_default_device(int device) {
+    return guarded(nullptr, [&] {
+        int count = 0;
+        DEEDS_HIP(hipGetDeviceCount(&count));
+        DEEDS_REQUIRE(device >= 0 && device < count, "device index out of range");
+        g_default_device = device;
+        if (t_ctx) {
+            deeds_ctx_destroy(t_ctx);
+            t_ctx = nullptr;
+        }
+    });
+}
+
+int deeds_ctx_create(int device, deeds_ctx** out) {
+    if (!out) {
+        set_thread_error("deeds_ctx_create: out is NULL");
+        return DEEDS_ERR_ARG;
+    }
+    *out = nullptr;
+    auto ctx = new (std::nothrow) deeds_ctx();
+    if (!ctx) return DEEDS_ERR_ALLOC;
+    int rc = guarded(nullptr, [&] {
+        int count = 0;
+        DEEDS_HIP(hipGetDeviceCount(&count));
+        DEEDS_REQUIRE(device >= 0 && device < count, "device index out of range (no GPU visible?)");
+        ctx->device = device;
+        DEEDS_HIP(hipSetDevice(device));
+        DEEDS_HIP(hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking));
+        DEEDS_HIP(hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking));
+        ctx->stream = ctx->own_stream;
+    });
+    if (rc != DEEDS_OK) {
+        if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+        if (ctx->aux_stream) (void)hipStreamDestroy(ctx->aux_stream);
+        delete ctx;
+        return rc;
+    }
+    *out = ctx;
+    return DEEDS_OK;
+}
+
+int deeds_ctx_destroy(deeds_ctx* ctx) {
+    if (!ctx) return DEEDS_OK;
+    (void)hipSetDevice(ctx->device);
+    ctx->release_all();
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+    if (ctx->aux_stream) (void)hipStreamDestroy(ctx->aux_stream);
+    delete ctx;
+    return DEEDS_OK;
+}
+
+const char* deeds_ctx_last_error(const deeds_ctx* ctx) { return ctx ? ctx->err.c_str() : t_error.c_str(); }
+
+int deeds_ctx_set_stream(deeds_ctx* ctx, void* hip_stream) {
+    if (!ctx) return DEEDS_ERR_ARG;
+    return guarded(ctx, [&] {
+        DEEDS_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->stream = hip_stream ? (hipStream_t)hip_stream : ctx->own_stream;
+    });
+}
+
+int deeds_ctx_synchronize(deeds_ctx* ctx) {
+    if (!ctx) return DEEDS_ERR_ARG;
+    return guarded(ctx, [&] {
+        DEEDS_HIP(hipStreamSynchronize(ctx->stream));
+        DEEDS_HIP(hipStreamSynchronize(ctx->aux_stream));
+    });
+}
+
+int deeds_ctx_trim(deeds_ctx* ctx) {
+    if (!ctx) return DEEDS_ERR_ARG;
+    return guarded(ctx, [&] { ctx->release_all(); });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// argument checks shared by both forms
+// ---------------------------------------------------------------------------
+namespace {
+
+void check_dims(int m, int n, int o) {
+    DEEDS_REQUIRE(m > 0 && n > 0 && o > 0, "volume dimensions must be positive");
+    DEEDS_REQUIRE((long long)m * n * o < (1LL << 31), "volume too large (more than 2^31 voxels)");
+}
+
+void check_mind(int m, int n, int o, int qs) {
+    check_dims(m, n, o);
+    DEEDS_REQUIRE(qs >= 1, "MIND patch radius qs must be >= 1");
+    DEEDS_REQUIRE(m >= 2 * qs + 1 && n >= 2 * qs + 1 && o >= 2 * qs + 1,
+                  "each dimension must be >= 2*qs+1 (box filter window)");
+}
+
+void check_grid(int m, int n, int o, int step) {
+    check_dims(m, n, o);
+    DEEDS_REQUIRE(step >= 1, "grid step must be >= 1");
+    DEEDS_REQUIRE(m / step >= 1 && n / step >= 1 && o / step >= 1, "grid step larger than a dimension");
+}
+
+void check_labels(int hw, float quant) {
+    DEEDS_REQUIRE(hw >= 0 && hw <= kMaxBPRadius, "search radius hw must be in [0, 10]");
+    DEEDS_REQUIRE(quant >= 1.0f && quant == std::floor(quant), "quantisation must be an integer >= 1");
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// device-pointer forms
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int deeds_ctx_descriptor(deeds_ctx* ctx, uint64_t* d_mindq, const float* d_im, int m, int n, int o, int qs) {
+    if (!ctx) return DEEDS_ERR_ARG;
+    return guarded(ctx, [&] {
+        check_mind(m, n, o, qs);
+        DEEDS_REQUIRE(d_mindq && d_im, "null pointer");
+        size_t sz6 = (size_t)m * n * o * 6;
+        float* w0 = ctx->get<float>("mind.w0", sz6);
+        float* w1 = ctx->get<float>("mind.w1", sz6);
+        float* w2 = ctx->get<float>("mind.w2", sz6);
+        launch_descriptor(d_mindq, d_im, m, n, o, qs, w0, w1, w2, ctx->stream);
+    });
+}
+
+int deeds_ctx_datacost(deeds_ctx* ctx, const uint64_t* d_F, const uint64_t* d_M, float* d_cost, int m, int n, int o,
+                       int step, int hw, float quant, float alpha, int randnum) {
+    if (!ctx) return DEEDS_ERR_ARG;
+    return guarded(ctx, [&] {
+        check_grid(m, n, o, step);
+        check_labels(hw, quant);
+        DEEDS_REQUIRE(d_F && d_M && d_cost, "null pointer");
+        CostPlan plan = datacost_plan(m, n, o, step, hw, quant, alpha, randnum);
+        uint64_t* Mp = ctx->get<uint64_t>("cost.mpad", (size_t)plan.mp * plan.np * plan.op);
+        launch_pad_codes(Mp, d_M, m, n, o, plan.pad, ctx->stream);
+        launch_datacost(d_cost, d_F, Mp, m, n, o, step, hw, (int)quant, plan, ctx->stream);
+    });
+}
+
+}  // extern "C"
+
+namespace deeds {
+
+size_t tree_ints_bound(int N) { return 7 * (size_t)N + 8; }
+
+// Packs a host tree (ordered/parents + schedule, edgemst) into the pinned
+// staging area (tree_ints_bound(N) ints followed by N floats) and enqueues its
+// copy into the device area d_stage of the same layout.  The staging memory
+// must stay untouched until the copy has executed on `s`.
+BPDevTree stage_tree(const Tree& t, void* pinned, void* d_stage, hipStream_t s) {
+    size_t ints = t.ordered.size() + t.parents.size() + t.depth_start.size() + t.grp_parent.size() +
+                  t.grp_child.size() + t.child_list.size();
+    size_t bound = tree_ints_bound(t.N);
+    if (ints > bound) fail(DEEDS_ERR_INTERNAL, "tree schedule larger than its staging bound");
+    int* hi = static_cast<int*>(pinned);
+    int* di = static_cast<int*>(d_stage);
+    size_t at = 0;
+    BPDevTree out{};
+    auto put = [&](const std::vector<int>& v) {
+        const int* dst = di + at;
+        std::copy(v.begin(), v.end(), hi + at);
+        at += v.size();
+        return dst;
+    };
+    out.ordered = put(t.ordered);
+    out.parents = put(t.parents);
+    out.depth_start = put(t.depth_start);
+    out.grp_parent = put(t.grp_parent);
+    out.grp_child = put(t.grp_child);
+    out.child_list = put(t.child_list);
+    float* hf = reinterpret_cast<float*>(hi + bound);
+    float* df = reinterpret_cast<float*>(di + bound);
+    std::copy(t.edgemst.begin(), t.edgemst.end(), hf);
+    out.edgemst = df;
+    DEEDS_HIP(hipMemcpyAsync(di, hi, at * sizeof(int), hipMemcpyHostToDevice, s));
+    DEEDS_HIP(hipMemcpyAsync(df, hf, t.edgemst.size() * sizeof(float), hipMemcpyHostToDevice, s));
+    return out;
+}
+
+void run_bp(const Tree& t, const BPDevTree& dt, float* d_cost, int16_t* d_inds, int* d_sel, const float* u0,
+            const float* v0, const float* w0, float* u1, float* v1, float* w1, int hw, float quant, hipStream_t s) {
+    int len = 2 * hw + 1;
+    for (int d = t.maxd; d >= 1; d--) {
+        int g0 = t.grp_depth_start[d - 1], g1 = t.grp_depth_start[d];
+        launch_bp_level(d_cost, d_inds, dt, g0, g1 - g0, u0, v0, w0, len, quant, s);
+    }
+    launch_bp_select(d_cost, d_inds, dt, t.N, t.maxd, u0, v0, w0, u1, v1, w1, d_sel, hw, quant, s);
+}
+
+void bp_prepare() {
+    static bool done = false;
+    if (done) return;
+    // allow the BP kernel's LDS footprint above the 64 KB default
+    bp_set_smem_limit();
+    done = true;
+}
+
+}  // namespace deeds
+
+extern "C" {
+
+int deeds_ctx_regularisation(deeds_ctx* ctx, float* d_cost, const float* d_u0, const float* d_v0, const float* d_w0,
+                             float* d_u1, float* d_v1, float* d_w1, int hw, int step, float quant, const int* ordered,
+                             const int* parents, const float* edgemst, int m, int n, int o, int16_t* d_allinds) {
+    if (!ctx) return DEEDS_ERR_ARG;
+    return guarded(ctx, [&] {
+        check_grid(m, n, o, step);
+        check_labels(hw, quant);
+        DEEDS_REQUIRE(d_cost && d_u0 && d_v0 && d_w0 && d_u1 && d_v1 && d_w1 && ordered && parents && edgemst,
+                      "null pointer");
+        bp_prepare();
+        int m1 = m / step, n1 = n / step, o1 = o / step;
+        Tree t;
+        t.m1 = m1; t.n1 = n1; t.o1 = o1; t.N = m1 * n1 * o1;
+        t.ordered.assign(ordered, ordered + t.N);
+        t.parents.assign(parents, parents + t.N);
+        t.edgemst.assign(edgemst, edgemst + t.N);
+        for (int v : t.ordered) DEEDS_REQUIRE(v >= 0 && v < t.N, "ordered holds an out-of-range node");
+        try {
+            build_schedule(t);
+        } catch (const std::runtime_error& e) {
+            fail(DEEDS_ERR_ARG, e.what());
+        }
+        int len = 2 * hw + 1;
+        size_t len3 = (size_t)len * len * len;
+        size_t stage_bytes = tree_ints_bound(t.N) * sizeof(int) + (size_t)t.N * sizeof(float);
+        void* pinned = nullptr;
+        DEEDS_HIP(hipHostMalloc(&pinned, stage_bytes, hipHostMallocDefault));
+        std::unique_ptr<void, hipError_t (*)(void*)> hold(pinned, hipHostFree);
+        void* d_stage = ctx->buf("bp.tree", stage_bytes);
+        BPDevTree dt = stage_tree(t, pinned, d_stage, ctx->stream);
+        int16_t* inds = d_allinds ? d_allinds : ctx->get<int16_t>("bp.inds", (size_t)t.N * len3);
+        int* sel = ctx->get<int>("bp.sel", t.N);
+        run_bp(t, dt, d_cost, inds, sel, d_u0, d_v0, d_w0, d_u1, d_v1, d_w1, hw, quant, ctx->stream);
+        // the staging copy must complete before the pinned block is released
+        DEEDS_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+// ---------------------------------------------------------------------------
+// host-pointer forms
+// ---------------------------------------------------------------------------
+
+int deeds_descriptor(uint64_t* mindq, const float* im, int m, int n, int o, int qs) {
+    return guarded(nullptr, [&] {
+        DEEDS_REQUIRE(mindq && im, "null pointer");
+        check_mind(m, n, o, qs);
+        deeds_ctx* c = default_ctx();
+        DEEDS_HIP(hipSetDevice(c->device));
+        size_t sz = (size_t)m * n * o;
+        float* d_im = c->get<float>("h.im", sz);
+        uint64_t* d_out = c->get<uint64_t>("h.mind", sz);
+        h2d(d_im, im, sz, c->stream);
+        int rc = deeds_ctx_descriptor(c, d_out, d_im, m, n, o, qs);
+        if (rc) fail(rc, c->err);
+        d2h(mindq, d_out, sz, c->stream);
+        DEEDS_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int deeds_datacost(const uint64_t* F, const uint64_t* M, float* cost, int m, int n, int o, int step, int hw,
+                   float quant, float alpha, int randnum) {
+    return guarded(nullptr, [&] {
+        DEEDS_REQUIRE(F && M && cost, "null pointer");
+        check_grid(m, n, o, step);
+        check_labels(hw, quant);
+        deeds_ctx* c = default_ctx();
+        DEEDS_HIP(hipSetDevice(c->device));
+        size_t sz = (size_t)m * n * o;
+        int len = 2 * hw + 1;
+        size_t ncost = (size_t)(m / step) * (n / step) * (o / step) * len * len * len;
+        uint64_t* dF = c->get<uint64_t>("h.F", sz);
+        uint64_t* dM = c->get<uint64_t>("h.M", sz);
+        float* dc = c->get<float>("h.cost", ncost);
+        h2d(dF, F, sz, c->stream);
+        h2d(dM, M, sz, c->stream);
+        int rc = deeds_ctx_datacost(c, dF, dM, dc, m, n, o, step, hw, quant, alpha, randnum);
+        if (rc) fail(rc, c->err);
+        d2h(cost, dc, ncost, c->stream);
+        DEEDS_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int deeds_prims_graph(const float* im, int* ordered, int* parents, float* edgemst, int step, int m, int n, int o) {
+    return guarded(nullptr, [&] {
+        DEEDS_REQUIRE(im && ordered && parents && edgemst, "null pointer");
+        check_grid(m, n, o, step);
+        deeds_ctx* c = default_ctx();
+        DEEDS_HIP(hipSetDevice(c->device));
+        size_t sz = (size_t)m * n * o;
+        int m1 = m / step, n1 = n / step, o1 = o / step;
+        size_t N = (size_t)m1 * n1 * o1;
+        float* d_im = c->get<float>("h.im", sz);
+        float* d_ec = c->get<float>("h.ec", 6 * N);
+        h2d(d_im, im, sz, c->stream);
+        launch_edge_sad(d_ec, d_im, step, m, n, o, c->stream);
+        std::vector<float> ec(6 * N);
+        d2h(ec.data(), d_ec, 6 * N, c->stream);
+        float mean = 0, sd = 0;
+        image_stats(im, (long long)sz, &mean, &sd);
+        DEEDS_HIP(hipStreamSynchronize(c->stream));
+        Tree t;
+        build_tree(ec.data(), sd, step, m1, n1, o1, t);
+        std::copy(t.ordered.begin(), t.ordered.end(), ordered);
+        std::copy(t.parents.begin(), t.parents.end(), parents);
+        std::copy(t.edgemst.begin(), t.edgemst.end(), edgemst);
+    });
+}
+
+int deeds_regularisation(float* cost, const float* u0, const float* v0, const float* w0, float* u1, float* v1,
+                         float* w1, int hw, int step, float quant, const int* ordered, const int* parents,
+                         const float* edgemst, int m, int n, int o) {
+    return guarded(nullptr, [&] {
+        DEEDS_REQUIRE(cost && u0 && v0 && w0 && u1 && v1 && w1 && ordered && parents && edgemst, "null pointer");
+        check_grid(m, n, o, step);
+        check_labels(hw, quant);
+        deeds_ctx* c = default_ctx();
+        DEEDS_HIP(hipSetDevice(c->device));
+        int len = 2 * hw + 1;
+        size_t N = (size_t)(m / step) * (n / step) * (o / step);
+        size_t ncost = N * len * len * len;
+        float* dc = c->get<float>("h.cost", ncost);
+        float* df = c->get<float>("h.fields", 6 * N);
+        h2d(dc, cost, ncost, c->stream);
+        h2d(df, u0, N, c->stream);
+        h2d(df + N, v0, N, c->stream);
+        h2d(df + 2 * N, w0, N, c->stream);
+        int rc = deeds_ctx_regularisation(c, dc, df, df + N, df + 2 * N, df + 3 * N, df + 4 * N, df + 5 * N, hw, step,
+                                          quant, ordered, parents, edgemst, m, n, o, nullptr);
+        if (rc) fail(rc, c->err);
+        d2h(u1, df + 3 * N, N, c->stream);
+        d2h(v1, df + 4 * N, N, c->stream);
+        d2h(w1, df + 5 * N, N, c->stream);
+        DEEDS_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int deeds_upsample_deformations(float* u1, float* v1, float* w1, const float* u0, const float* v0, const float* w0,
+                                int m, int n, int o, int m2, int n2, int o2) {
+    return guarded(nullptr, [&] {
+        DEEDS_REQUIRE(u1 && v1 && w1 && u0 && v0 && w0, "null pointer");
+        check_dims(m, n, o);
+        check_dims(m2, n2, o2);
+        deeds_ctx* c = default_ctx();
+        DEEDS_HIP(hipSetDevice(c->device));
+        size_t sz = (size_t)m * n * o, sz2 = (size_t)m2 * n2 * o2;
+        float* din = c->get<float>("h.up.in", 3 * sz2);
+        float* dout = c->get<float>("h.up.out", 3 * sz);
+        h2d(din, u0, sz2, c->stream);
+        h2d(din + sz2, v0, sz2, c->stream);
+        h2d(din + 2 * sz2, w0, sz2, c->stream);
+        launch_upsample(dout, dout + sz, dout + 2 * sz, din, din + sz2, din + 2 * sz2, m, n, o, m2, n2, o2, c->stream);
+        d2h(u1, dout, sz, c->stream);
+        d2h(v1, dout + sz, sz, c->stream);
+        d2h(w1, dout + 2 * sz, sz, c->stream);
+        DEEDS_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int deeds_consistent_mapping(float* u, float* v, float* w, float* u2, float* v2, float* w2, int m, int n, int o,
+                             int factor) {
+    return guarded(nullptr, [&] {
+        DEEDS_REQUIRE(u && v && w && u2 && v2 && w2, "null pointer");
+        check_dims(m, n, o);
+        DEEDS_REQUIRE(factor != 0, "factor must be nonzero");
+        deeds_ctx* c = default_ctx();
+        DEEDS_HIP(hipSetDevice(c->device));
+        size_t sz = (size_t)m * n * o;
+        float* df = c->get<float>("h.cm", 12 * sz);
+        float* hf[6] = {u, v, w, u2, v2, w2};
+        float* f[6];
+        for (int i = 0; i < 6; i++) {
+            f[i] = df + i * sz;
+            h2d(f[i], hf[i], sz, c->stream);
+        }
+        launch_consistent(f, df + 6 * sz, m, n, o, factor, c->stream);
+        for (int i = 0; i < 6; i++) d2h(hf[i], f[i], sz, c->stream);
+        DEEDS_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int deeds_jacobian(const float* u, const float* v, const float* w, int m, int n, int o, int factor, float* std_out,
+                   float* neg_frac_out) {
+    return guarded(nullptr, [&] {
+        DEEDS_REQUIRE(u && v && w, "null pointer");
+        check_dims(m, n, o);
+        DEEDS_REQUIRE(factor != 0, "factor must be nonzero");
+        deeds_ctx* c = default_ctx();
+        DEEDS_HIP(hipSetDevice(c->device));
+        size_t sz = (size_t)m * n * o;
+        float* df = c->get<float>("h.jac", 3 * sz);
+        double* acc = c->get<double>("h.jac.acc", 4);
+        h2d(df, u, sz, c->stream);
+        h2d(df + sz, v, sz, c->stream);
+        h2d(df + 2 * sz, w, sz, c->stream);
+        launch_jacobian(acc, df, df + sz, df + 2 * sz, m, n, o, factor, c->stream);
+        double h[3];
+        d2h(h, acc, 3, c->stream);
+        DEEDS_HIP(hipStreamSynchronize(c->stream));
+        double N = (double)sz;
+        double mean = h[0] / N;
+        double var = sz > 1 ? (h[1] - N * mean * mean) / (N - 1) : 0.0;
+        if (std_out) *std_out = (float)std::sqrt(std::max(var, 0.0));
+        if (neg_frac_out) *neg_frac_out = (float)(h[2] / N);
+    });
+}
+
+namespace {
+const float kIdentityX[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+}
+
+int deeds_warp_affine(float* out, const float* in, const float* im1b, const float* X, const float* u, const float* v,
+                      const float* w, int m, int n, int o, float* ssd01) {
+    return guarded(nullptr, [&] {
+        DEEDS_REQUIRE(out && in && u && v && w, "null pointer");
+        check_dims(m, n, o);
+        deeds_ctx* c = default_ctx();
+        DEEDS_HIP(hipSetDevice(c->device));
+        size_t sz = (size_t)m * n * o;
+        float* d = c->get<float>("h.warp", 6 * sz);
+        h2d(d, in, sz, c->stream);
+        h2d(d + sz, u, sz, c->stream);
+        h2d(d + 2 * sz, v, sz, c->stream);
+        h2d(d + 3 * sz, w, sz, c->stream);
+        Affine A = affine_from_X(X ? X : kIdentityX);
+        launch_warp_affine(d + 4 * sz, d, A, d + sz, d + 2 * sz, d + 3 * sz, m, n, o, c->stream);
+        if (ssd01 && im1b) {
+            double* acc = c->get<double>("h.ssd", 2);
+            DEEDS_HIP(hipMemsetAsync(acc, 0, 2 * sizeof(double), c->stream));
+            h2d(d + 5 * sz, im1b, sz, c->stream);
+            launch_ssd(acc, d + 5 * sz, d + 4 * sz, d, (long long)sz, c->stream);
+            double h[2];
+            d2h(h, acc, 2, c->stream);
+            DEEDS_HIP(hipStreamSynchronize(c->stream));
+            ssd01[0] = (float)(h[0] / (double)sz);
+            ssd01[1] = (float)(h[1] / (double)sz);
+        }
+        d2h(out, d + 4 * sz, sz, c->stream);
+        DEEDS_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int deeds_warp_image(float* out, const float* im1, const float* im1b, const float* u, const float* v, const float* w,
+                     int m, int n, int o, float* ssd01) {
+    return guarded(nullptr, [&] {
+        DEEDS_REQUIRE(out && im1 && u && v && w, "null pointer");
+        check_dims(m, n, o);
+        deeds_ctx* c = default_ctx();
+        DEEDS_HIP(hipSetDevice(c->device));
+        size_t sz = (size_t)m * n * o;
+        float* d = c->get<float>("h.warp", 6 * sz);
+        h2d(d, im1, sz, c->stream);
+        h2d(d + sz, u, sz, c->stream);
+        h2d(d + 2 * sz, v, sz, c->stream);
+        h2d(d + 3 * sz, w, sz, c->stream);
+        launch_interp3(d + 4 * sz, d, d + sz, d + 2 * sz, d + 3 * sz, m, n, o, m, n, o, 1, c->stream);
+        if (ssd01 && im1b) {
+            double* acc = c->get<double>("h.ssd", 2);
+            DEEDS_HIP(hipMemsetAsync(acc, 0, 2 * sizeof(double), c->stream));
+            h2d(d + 5 * sz, im1b, sz, c->stream);
+            launch_ssd(acc, d + 5 * sz, d + 4 * sz, d, (long long)sz, c->stream);
+            double h[2];
+            d2h(h, acc, 2, c->stream);
+            DEEDS_HIP(hipStreamSynchronize(c->stream));
+            ssd01[0] = (float)(h[0] / (double)sz);
+            ssd01[1] = (float)(h[1] / (double)sz);
+        }
+        d2h(out, d + 4 * sz, sz, c->stream);
+        DEEDS_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int deeds_warp_affine_s(int16_t* out, const int16_t* in, const float* X, const float* u, const float* v,
+                        const float* w, int m, int n, int o) {
+    return guarded(nullptr, [&] {
+        DEEDS_REQUIRE(out && in && u && v && w, "null pointer");
+        check_dims(m, n, o);
+        deeds_ctx* c = default_ctx();
+        DEEDS_HIP(hipSetDevice(c->device));
+        size_t sz = (size_t)m * n * o;
+        float* d = c->get<float>("h.warp", 3 * sz);
+        int16_t* s = c->get<int16_t>("h.seg", 2 * sz);
+        h2d(s, in, sz, c->stream);
+        h2d(d, u, sz, c->stream);
+        h2d(d + sz, v, sz, c->stream);
+        h2d(d + 2 * sz, w, sz, c->stream);
+        Affine A = affine_from_X(X ? X : kIdentityX);
+        launch_warp_affine_s(s + sz, s, A, d, d + sz, d + 2 * sz, m, n, o, c->stream);
+        d2h(out, s + sz, sz, c->stream);
+        DEEDS_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,103 @@
+// Internal shared definitions of the deeds_hip library: error handling, the
+// context (device, streams, cached device workspace) and small helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/deeds_hip.h"
+
+namespace deeds {
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] inline void fail(int code, const std::string& msg) { throw Error{code, msg}; }
+
+#define DEEDS_HIP(expr)                                                                       \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            ::deeds::fail(DEEDS_ERR_HIP, std::string(#expr) + " failed: " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define DEEDS_REQUIRE(cond, msg)                                     \
+    do {                                                             \
+        if (!(cond)) ::deeds::fail(DEEDS_ERR_ARG, std::string(msg)); \
+    } while (0)
+
+#define DEEDS_LAUNCH_CHECK() DEEDS_HIP(hipGetLastError())
+
+// A device allocation that grows on demand and is kept for reuse.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace deeds
+
+struct deeds_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;     // active stream (own or caller's)
+    hipStream_t aux_stream = nullptr;  // second registration direction
+    std::string err;
+    std::map<std::string, deeds::DevBuf> ws;
+
+    // Device buffer `name` of at least `bytes` (contents unspecified).
+    void* buf(const std::string& name, size_t bytes);
+    template <class T>
+    T* get(const std::string& name, size_t count) {
+        return static_cast<T*>(buf(name, count * sizeof(T)));
+    }
+    void release_all();
+};
+
+namespace deeds {
+
+// Thread-local default context used by the host-pointer entry points.
+deeds_ctx* default_ctx();
+void set_thread_error(const std::string& msg);
+
+// Runs f, converting exceptions to a status and recording the message.
+template <class F>
+int guarded(deeds_ctx* ctx, F&& f) {
+    try {
+        if (ctx) DEEDS_HIP(hipSetDevice(ctx->device));
+        f();
+        return DEEDS_OK;
+    } catch (const Error& e) {
+        if (ctx) ctx->err = e.msg;
+        set_thread_error(e.msg);
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        if (ctx) ctx->err = "host allocation failed";
+        set_thread_error("host allocation failed");
+        return DEEDS_ERR_ALLOC;
+    } catch (const std::exception& e) {
+        if (ctx) ctx->err = e.what();
+        set_thread_error(e.what());
+        return DEEDS_ERR_INTERNAL;
+    }
+}
+
+inline unsigned ceil_div(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
+
+// Upload / download helpers on a stream.
+template <class T>
+void h2d(T* dst, const T* src, size_t count, hipStream_t s) {
+    DEEDS_HIP(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+}
+template <class T>
+void d2h(T* dst, const T* src, size_t count, hipStream_t s) {
+    DEEDS_HIP(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToHost, s));
+}
+
+}  // namespace deeds

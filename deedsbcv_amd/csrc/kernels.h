@@ -1,0 +1,82 @@
+// Host-side launchers of the deeds_hip kernels (all take device pointers and
+// enqueue on the given stream; shapes are validated by the callers).
+#pragma once
+
+#include "common.h"
+
+namespace deeds {
+
+// ---- MIND-SSC (mind.hip) --------------------------------------------------
+// The five quantisation thresholds -log((c+1.5)/6) in float.
+void mind_thresholds(float thr[5]);
+// Workspace: three buffers of 6*m*n*o floats.
+void launch_descriptor(uint64_t* out, const float* im, int m, int n, int o, int qs,
+                       float* w0, float* w1, float* w2, hipStream_t s);
+
+// ---- Hamming data cost (datacost.hip) -------------------------------------
+struct CostPlan {
+    int skip_i, skip_j, skip_k;  // sample strides along i, j, k
+    float scale;                 // float factor applied to the integer sum
+    int pad;                     // quant*hw padding of the moving codes
+    int mp, np, op;              // padded dims
+    long long samples;           // block samples per label
+};
+CostPlan datacost_plan(int m, int n, int o, int step, int hw, float quant, float alpha, int randnum);
+void launch_pad_codes(uint64_t* Mp, const uint64_t* M, int m, int n, int o, int pad, hipStream_t s);
+void launch_datacost(float* cost, const uint64_t* F, const uint64_t* Mp, int m, int n, int o,
+                     int step, int hw, int q, const CostPlan& plan, hipStream_t s);
+
+// ---- MST edge costs (mst.hip) ----------------------------------------------
+// ec[node + nb*N] = float SAD of the two step^3 blocks (0 off-grid).
+void launch_edge_sad(float* ec, const float* im, int step, int m, int n, int o, hipStream_t s);
+
+// ---- belief propagation (bp.hip) ------------------------------------------
+struct BPDevTree {
+    const int* ordered;     // N
+    const int* parents;     // N
+    const int* depth_start; // maxd+2
+    const int* grp_parent;  // G
+    const int* grp_child;   // G+1 (CSR offsets into child_list)
+    const int* child_list;  // N-1
+    const float* edgemst;   // N
+};
+size_t bp_level_smem(int len);
+// Largest label radius the LDS-resident BP kernel supports (len^3 floats x3 + int16 x2 in LDS).
+constexpr int kMaxBPRadius = 10;
+// Raise the BP kernel's dynamic LDS limit (call once per process before launches).
+void bp_set_smem_limit();
+void launch_bp_level(float* cost, int16_t* allinds, const BPDevTree& t, int grp_begin, int grp_count,
+                     const float* u0, const float* v0, const float* w0, int len, float quant, hipStream_t s);
+void launch_bp_select(const float* cost, const int16_t* allinds, const BPDevTree& t, int N, int maxd,
+                      const float* u0, const float* v0, const float* w0, float* u1, float* v1, float* w1,
+                      int* sel, int hw, float quant, hipStream_t s);
+
+// ---- transforms (warp.hip) -------------------------------------------------
+struct Affine {
+    float r[12];  // rows of the 3x4 matrix: y = r[0..3], x = r[4..7], z = r[8..11]
+};
+Affine affine_from_X(const float* X);  // X: 16 floats column-wise (deedsBCV0.cpp:161-165)
+void launch_upsample(float* u1, float* v1, float* w1, const float* u0, const float* v0, const float* w0,
+                     int m, int n, int o, int m2, int n2, int o2, hipStream_t s);
+void launch_interp3(float* out, const float* in, const float* x1, const float* y1, const float* z1,
+                    int m, int n, int o, int m2, int n2, int o2, int add_identity, hipStream_t s);
+void launch_warp_affine(float* out, const float* in, const Affine& A, const float* u, const float* v,
+                        const float* w, int m, int n, int o, hipStream_t s);
+// Fused: trilinear upsample of the (m2,n2,o2) field to full resolution on the
+// fly, then the affine warp (identical per-voxel arithmetic to the two-step path).
+void launch_warp_affine_grid(float* out, const float* in, const Affine& A, const float* u0, const float* v0,
+                             const float* w0, int m, int n, int o, int m2, int n2, int o2, hipStream_t s);
+void launch_warp_image_grid(float* out, const float* in, const float* u0, const float* v0, const float* w0,
+                            int m, int n, int o, int m2, int n2, int o2, hipStream_t s);
+void launch_warp_affine_s(int16_t* out, const int16_t* in, const Affine& A, const float* u, const float* v,
+                          const float* w, int m, int n, int o, hipStream_t s);
+// In place on the six fields; tmp holds 6*m*n*o floats.
+void launch_consistent(float* f[6], float* tmp, int m, int n, int o, int factor, hipStream_t s);
+// acc[0] += sum (ref-before)^2, acc[1] += sum (ref-warped)^2 in double (logging only).
+void launch_ssd(double* acc, const float* ref, const float* warped, const float* before, long long sz,
+                hipStream_t s);
+// out[0] = std(det J), out[1] = fraction det J < 0 (logging only); scratch >= 3 doubles.
+void launch_jacobian(double* scratch, const float* u, const float* v, const float* w, int m, int n, int o,
+                     int factor, hipStream_t s);
+
+}  // namespace deeds

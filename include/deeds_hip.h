@@ -1,0 +1,163 @@
+/* deeds_hip -- MI355X (gfx950) implementation of deedsBCV's dense-displacement
+ * registration core, exported as a plain C ABI.
+ *
+ * Conventions (all entry points):
+ *   - Volumes use the reference voxel order idx = i + j*m + k*m*n, i along the
+ *     first NIfTI dimension; m, n, o are the three dimensions.
+ *   - Displacements: u along j (dim 2), v along i (dim 1), w along k (dim 3)
+ *     (deedsBCV0.cpp:193-194).  Labels l = ys + xs*len + zs*len^2, len = 2*hw+1.
+ *   - Every function returns 0 on success and a nonzero deeds_status on error;
+ *     deeds_last_error() (thread-local) / deeds_ctx_last_error() give the text.
+ *     Nothing aborts the process.
+ *   - Host-pointer forms (no ctx argument) take and return host memory, run
+ *     synchronously on a per-thread default context (device 0 unless
+ *     deeds_set_default_device was called) and mirror one reference function
+ *     each, with the reference's globals (image_m/n/o) made explicit arguments.
+ *   - deeds_ctx_* forms take DEVICE pointers, enqueue on the context's stream
+ *     and return before completion (deeds_ctx_synchronize waits), except the
+ *     registration drivers, which return when the result is complete.
+ *   - The caller owns every buffer passed in; a context owns its workspace.
+ *     A context may be used by one host thread at a time; distinct contexts
+ *     are independent.
+ */
+#ifndef DEEDS_HIP_H
+#define DEEDS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum deeds_status {
+    DEEDS_OK = 0,
+    DEEDS_ERR_ARG = 1,      /* invalid argument (sizes, radii, null pointers) */
+    DEEDS_ERR_HIP = 2,      /* a HIP runtime call failed */
+    DEEDS_ERR_ALLOC = 3,    /* device or host allocation failed */
+    DEEDS_ERR_INTERNAL = 4  /* internal consistency check failed */
+};
+
+typedef struct deeds_ctx deeds_ctx;
+
+/* Registration schedule: deedsBCV's -a -G -L -Q flags (parseArguments.h:12-174,
+ * defaults deedsBCV0.cpp:76-79). */
+#define DEEDS_MAX_LEVELS 10
+typedef struct deeds_schedule {
+    int levels;                       /* number of levels (-l / count of -G values) */
+    int grid[DEEDS_MAX_LEVELS];       /* control-point spacing per level (-G) */
+    int radius[DEEDS_MAX_LEVELS];     /* label search radius hw per level (-L) */
+    int quant[DEEDS_MAX_LEVELS];      /* label quantisation per level (-Q) */
+    float alpha;                      /* regularisation weight (-a) */
+} deeds_schedule;
+
+/* Per-level timings in seconds plus work counters, the figures the reference
+ * prints per level (deedsBCV0.cpp:308) and in total (deedsBCV0.cpp:381). */
+typedef struct deeds_timing {
+    int levels;
+    double mind[DEEDS_MAX_LEVELS];
+    double data[DEEDS_MAX_LEVELS];
+    double mst_reg[DEEDS_MAX_LEVELS];
+    double transf[DEEDS_MAX_LEVELS];
+    double total;                     /* the reference's "Total time" span (:217 -> :336) */
+    double hamming_evals;             /* algorithmic 64-bit Hamming evaluations, both directions */
+    float ssd_before, ssd_after;      /* logging-only SSD of the last warp (dataCostD.h:300-312) */
+    float jac_std[DEEDS_MAX_LEVELS];  /* std(J) logged per level (transformations.h:166) */
+    float jac_neg[DEEDS_MAX_LEVELS];  /* fraction of J<0 per level (transformations.h:168) */
+} deeds_timing;
+
+/* ---- library / context ------------------------------------------------- */
+const char* deeds_version(void);
+const char* deeds_last_error(void);
+int deeds_set_default_device(int device);
+int deeds_ctx_create(int device, deeds_ctx** out);
+int deeds_ctx_destroy(deeds_ctx* ctx);
+const char* deeds_ctx_last_error(const deeds_ctx* ctx);
+/* Run subsequent work of ctx on an existing HIP stream (e.g. torch's); NULL restores the own stream. */
+int deeds_ctx_set_stream(deeds_ctx* ctx, void* hip_stream);
+int deeds_ctx_synchronize(deeds_ctx* ctx);
+/* Release the context's cached workspace (it is re-grown on demand). */
+int deeds_ctx_trim(deeds_ctx* ctx);
+
+/* ---- host-pointer forms: one per reference function ---------------------- */
+
+/* MINDSSCbox.h:127 descriptor(): 12-channel MIND-SSC, 60-bit codes. */
+int deeds_descriptor(uint64_t* mindq, const float* im, int m, int n, int o, int qs);
+
+/* dataCostD.h:108 dataCostCL(): cost[node*len3 + l], len3 = (2hw+1)^3, node over
+ * the (m/step)x(n/step)x(o/step) control grid. */
+int deeds_datacost(const uint64_t* fixed_mind, const uint64_t* moving_mind, float* cost,
+                   int m, int n, int o, int step, int hw, float quant, float alpha, int randnum);
+
+/* primsMST.h:52 primsGraph(): MST of the control grid, BFS-ordered. */
+int deeds_prims_graph(const float* im, int* ordered, int* parents, float* edgemst,
+                      int step, int m, int n, int o);
+
+/* regularisation.h:115 regularisationCL(): tree BP; cost is consumed (overwritten). */
+int deeds_regularisation(float* cost, const float* u0, const float* v0, const float* w0,
+                         float* u1, float* v1, float* w1, int hw, int step, float quant,
+                         const int* ordered, const int* parents, const float* edgemst,
+                         int m, int n, int o);
+
+/* transformations.h:245 upsampleDeformationsCL(): (m2,n2,o2) field -> (m,n,o). */
+int deeds_upsample_deformations(float* u1, float* v1, float* w1,
+                                const float* u0, const float* v0, const float* w0,
+                                int m, int n, int o, int m2, int n2, int o2);
+
+/* transformations.h:189 consistentMappingCL(): in place on both fields. */
+int deeds_consistent_mapping(float* u, float* v, float* w, float* u2, float* v2, float* w2,
+                             int m, int n, int o, int factor);
+
+/* transformations.h:84 jacobian(): std of det(J) of x + u/factor, and the J<0 fraction. */
+int deeds_jacobian(const float* u, const float* v, const float* w, int m, int n, int o, int factor,
+                   float* std_out, float* neg_frac_out);
+
+/* dataCostD.h:267 warpAffine(): trilinear warp of `in` by X (16 floats,
+ * column-wise as read at deedsBCV0.cpp:161-165) plus the field; ssd01
+ * (optional, 2 floats) receives the logging SSD before/after vs im1b. */
+int deeds_warp_affine(float* out, const float* in, const float* im1b, const float* X,
+                      const float* u, const float* v, const float* w, int m, int n, int o, float* ssd01);
+
+/* dataCostD.h:213 warpImageCL(): trilinear warp of im1 by x + field. */
+int deeds_warp_image(float* out, const float* im1, const float* im1b,
+                     const float* u, const float* v, const float* w, int m, int n, int o, float* ssd01);
+
+/* dataCostD.h:241 warpAffineS(): nearest-neighbour label warp. */
+int deeds_warp_affine_s(int16_t* out, const int16_t* in, const float* X,
+                        const float* u, const float* v, const float* w, int m, int n, int o);
+
+/* deedsBCV0.cpp:139-356: the full symmetric registration of fixed/moving
+ * (HU values, m*n*o floats each).  flow_out receives the final control-grid
+ * field [u;v;w] (3 * (m/g)(n/g)(o/g) floats for the last grid g) -- the
+ * _displacements.dat payload; warped_out (optional) the deformed moving image
+ * in HU.  X may be NULL for identity.  timing is optional. */
+int deeds_register(const float* fixed, const float* moving, const float* X, int m, int n, int o,
+                   const deeds_schedule* sched, float* flow_out, float* warped_out, deeds_timing* timing);
+
+/* applyBCV.cpp:161-198 / applyBCVfloat.cpp:186-198: apply a _displacements.dat
+ * field (3*sz1 floats on grid `grid_step`) to a label volume (nearest) or an
+ * image (trilinear). */
+int deeds_apply_seg(int16_t* out, const int16_t* in, const float* X, const float* flow, int grid_step,
+                    int m, int n, int o);
+int deeds_apply_float(float* out, const float* in, const float* X, const float* flow, int grid_step,
+                      int m, int n, int o);
+
+/* ---- device-pointer forms (asynchronous on the context stream) ---------- */
+int deeds_ctx_descriptor(deeds_ctx* ctx, uint64_t* d_mindq, const float* d_im, int m, int n, int o, int qs);
+int deeds_ctx_datacost(deeds_ctx* ctx, const uint64_t* d_fixed_mind, const uint64_t* d_moving_mind, float* d_cost,
+                       int m, int n, int o, int step, int hw, float quant, float alpha, int randnum);
+/* BP over a tree given on the host (ordered/parents/edgemst as returned by
+ * deeds_prims_graph); d_cost is consumed.  d_allinds (optional, sz1*len3
+ * int16) receives the per-node argmin tables. */
+int deeds_ctx_regularisation(deeds_ctx* ctx, float* d_cost, const float* d_u0, const float* d_v0, const float* d_w0,
+                             float* d_u1, float* d_v1, float* d_w1, int hw, int step, float quant,
+                             const int* ordered, const int* parents, const float* edgemst,
+                             int m, int n, int o, int16_t* d_allinds);
+int deeds_ctx_register(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, const float* X,
+                       int m, int n, int o, const deeds_schedule* sched,
+                       float* d_flow, float* d_warped, deeds_timing* timing);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

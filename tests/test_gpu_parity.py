@@ -1,0 +1,221 @@
+"""GPU parity: the HIP library (through its C ABI) against the oracle and the
+golden vectors of the unmodified reference.  Bar: bit-exact for codes, costs,
+trees, BP labels and every warp (the kernels reproduce the reference's float /
+double evaluation exactly), so every comparison below is array_equal."""
+import numpy as np
+import pytest
+
+from deedsbcv_amd import api, phantom
+
+pytestmark = pytest.mark.gpu
+
+
+def _eq(a, b, what):
+    a, b = np.asarray(a), np.asarray(b)
+    assert a.shape == b.shape, (what, a.shape, b.shape)
+    if not np.array_equal(a, b):
+        bad = np.flatnonzero(a.ravel() != b.ravel())
+        raise AssertionError(f"{what}: {bad.size} of {a.size} differ; first at {bad[:5]}: "
+                             f"{a.ravel()[bad[:5]]} vs {b.ravel()[bad[:5]]}")
+
+
+# --------------------------------------------------------------- MIND -------
+
+def test_mind_golden(golden):
+    print("case: mind golden 40x36x32", flush=True)
+    for qs in (1, 2):
+        _eq(api.descriptor(golden["im"], qs), golden[f"mind_q{qs}"], f"mind qs={qs}")
+    _eq(api.descriptor(golden["im2"], 2), golden["mind2_q2"], "mind im2")
+
+
+@pytest.mark.parametrize("shape,seed", [((37, 33, 29), 21), ((64, 48, 40), 22), ((23, 70, 31), 23)])
+def test_mind_vs_oracle(oracle, shape, seed):
+    print(f"case: mind vs oracle {shape}", flush=True)
+    f, _ = phantom.make_pair(shape, seed)
+    im = f.astype(np.float32) + 1024
+    for qs in (1, 2, 3):
+        _eq(api.descriptor(im, qs), oracle.descriptor(im, qs), f"mind {shape} qs={qs}")
+
+
+# ---------------------------------------------------------- data cost -------
+
+@pytest.mark.parametrize("case", [(5, 2, 2), (4, 2, 1), (6, 1, 3)])
+def test_cost_golden(golden, case):
+    step, hw, q = case
+    print(f"case: datacost golden {case}", flush=True)
+    _eq(api.dataCostCL(golden["mind_q2"], golden["mind2_q2"], step, hw, q),
+        golden[f"cost_s{step}h{hw}q{q}"], f"cost {case}")
+
+
+@pytest.mark.parametrize("case", [(8, 3, 5, 1), (7, 2, 4, 1), (5, 4, 2, 1), (4, 4, 1, 1), (3, 2, 1, 1),
+                                  (8, 2, 3, 2), (5, 2, 2, 0), (4, 1, 1, 2)])
+def test_cost_vs_oracle(oracle, case):
+    step, hw, q, randnum = case
+    print(f"case: datacost vs oracle {case}", flush=True)
+    f, mv = phantom.make_pair((45, 41, 38), 31)
+    F = oracle.descriptor(f.astype(np.float32) + 1024, 2)
+    M = oracle.descriptor(mv.astype(np.float32) + 1024, 2)
+    _eq(api.dataCostCL(F, M, step, hw, q, 1.6, randnum), oracle.datacost(F, M, step, hw, q, 1.6, randnum),
+        f"cost {case}")
+
+
+# ---------------------------------------------------------------- MST -------
+
+@pytest.mark.parametrize("case", [(5, 2, 2), (4, 2, 1), (6, 1, 3)])
+def test_prims_golden(golden, case):
+    step = case[0]
+    tag = "s%dh%dq%d" % case
+    print(f"case: prims golden {case}", flush=True)
+    got = api.primsGraph(golden["im2"] if step == 4 else golden["im"], step)
+    for a, name in zip(got, ("ordered", "parents", "edgemst")):
+        _eq(a, golden[f"{name}_{tag}"], name)
+
+
+@pytest.mark.parametrize("step", [3, 4, 5, 7, 8])
+def test_prims_vs_oracle(oracle, step):
+    print(f"case: prims vs oracle step={step}", flush=True)
+    f, mv = phantom.make_pair((57, 50, 44), 41)
+    for im in (f.astype(np.float32) + 1024, mv.astype(np.float32) * 0.73 + 1000.25):  # non-integer image too
+        got = api.primsGraph(im, step)
+        want = oracle.prims(im, step)
+        for a, b, name in zip(got, want, ("ordered", "parents", "edgemst")):
+            _eq(a, b, f"{name} step={step}")
+
+
+# ----------------------------------------------------------------- BP -------
+
+@pytest.mark.parametrize("case", [(5, 2, 2), (4, 2, 1), (6, 1, 3)])
+def test_bp_golden(golden, case):
+    step, hw, q = case
+    tag = f"s{step}h{hw}q{q}"
+    print(f"case: bp golden {case}", flush=True)
+    u0 = golden[f"u0_{tag}"]
+    got = api.regularisationCL(golden[f"cost_{tag}"], u0[0], u0[1], u0[2], hw, step, q, golden[f"ordered_{tag}"],
+                               golden[f"parents_{tag}"], golden[f"edgemst_{tag}"], (32, 36, 40))
+    _eq(np.stack(got), golden[f"u1_{tag}"], f"u1 {case}")
+
+
+@pytest.mark.parametrize("case", [(8, 8, 5), (6, 6, 3), (4, 4, 1), (5, 3, 2), (3, 1, 1)])
+def test_bp_vs_oracle(oracle, case):
+    step, hw, q = case
+    print(f"case: bp vs oracle {case}", flush=True)
+    rng = np.random.default_rng(step * 100 + hw)
+    f, mv = phantom.make_pair((48, 44, 40), 51)
+    im = f.astype(np.float32) + 1024
+    F = oracle.descriptor(im, 2)
+    M = oracle.descriptor(mv.astype(np.float32) + 1024, 2)
+    cost = oracle.datacost(F, M, step, hw, q)
+    tree = oracle.prims(im, step)
+    o1, n1, m1 = 40 // step, 44 // step, 48 // step
+    u0 = (rng.normal(size=(3, o1, n1, m1)) * 2).astype(np.float32)
+    got = api.regularisationCL(cost, u0[0], u0[1], u0[2], hw, step, q, *tree, (40, 44, 48))
+    want = oracle.regularisation(cost, u0[0], u0[1], u0[2], hw, step, q, *tree, (48, 44, 40))
+    _eq(np.stack(got), np.stack(want), f"bp {case}")
+
+
+# ---------------------------------------------------------- transforms ------
+
+def test_transforms_golden(golden):
+    print("case: transforms golden", flush=True)
+    g = golden["grid6"]
+    _eq(np.stack(api.upsampleDeformationsCL(g[0], g[1], g[2], (32, 36, 40))), golden["up"], "upsample")
+    _eq(np.stack(api.consistentMappingCL(*g, 5)), golden["cons"], "consistent")
+    ux, vx, wx = golden["up"]
+    w, ssd = api.warpAffine(golden["im2"], golden["im"], golden["X"], ux, vx, wx)
+    _eq(w, golden["warp_affine"], "warpAffine")
+    np.testing.assert_allclose(ssd, golden["warp_affine_ssd"], rtol=1e-4)  # logging-only figure
+    w, ssd = api.warpImageCL(golden["im2"], golden["im"], ux, vx, wx)
+    _eq(w, golden["warp_image"], "warpImageCL")
+    np.testing.assert_allclose(ssd, golden["warp_image_ssd"], rtol=1e-4)
+    _eq(api.warpAffineS(golden["seg"], golden["X"], ux, vx, wx), golden["warp_seg"], "warpAffineS")
+
+
+def test_transforms_vs_oracle(oracle):
+    print("case: transforms vs oracle", flush=True)
+    rng = np.random.default_rng(8)
+    for (m2, n2, o2), (m, n, o) in (((7, 6, 5), (50, 45, 41)), ((12, 9, 10), (12, 9, 10)), ((3, 4, 5), (31, 29, 23))):
+        g = (rng.normal(size=(6, o2, n2, m2)) * 3).astype(np.float32)
+        _eq(np.stack(api.upsampleDeformationsCL(g[0], g[1], g[2], (o, n, m))),
+            np.stack(oracle.upsample(g[0], g[1], g[2], (m, n, o))), "upsample")
+        for factor in (4, 5, 8):
+            _eq(np.stack(api.consistentMappingCL(*g, factor)), np.stack(oracle.consistent(*g, factor)), "consistent")
+
+
+def test_jacobian_logging(oracle):
+    rng = np.random.default_rng(9)
+    g = (rng.normal(size=(3, 9, 10, 11)) * 2).astype(np.float32)
+    s, neg = api.jacobian(g[0], g[1], g[2], 4)
+    s0, neg0 = oracle.jacobian(g[0], g[1], g[2], 4)
+    assert abs(s - float(s0)) <= 1e-4 * max(1.0, float(s0))
+    assert abs(neg - float(neg0)) < 1e-6
+
+
+def test_apply_matches_reference_applyBCV(golden):
+    print("case: applyBCV golden", flush=True)
+    s = golden["e2e_schedule"]
+    seg = golden["e2e_seg"]
+    _eq(api.apply_seg(seg, golden["e2e_flow"], int(s[0][-1])), golden["e2e_apply_seg"], "applyBCV")
+
+
+# --------------------------------------------------------- end to end -------
+
+def test_register_golden(golden):
+    print("case: register golden 48x40x36", flush=True)
+    s = golden["e2e_schedule"]
+    flow, warped, t = api.register(golden["e2e_fixed"], golden["e2e_moving"], s[0], s[1], s[2])
+    _eq(flow, golden["e2e_flow"], "flow")
+    _eq(warped, golden["e2e_warped"], "warped")
+    seg = api.apply_seg(golden["e2e_seg"], flow, int(s[0][-1]))
+    _eq(seg, golden["e2e_warped_seg"], "warped seg")
+
+
+def test_register_small_workload_vs_oracle(oracle):
+    """The 96x96x80 three-level workload: GPU == oracle == reference binary."""
+    print("case: register small workload vs oracle", flush=True)
+    f, mv = phantom.workload_small(0)
+    sch = phantom.SCHEDULE_SMALL
+    flow, warped, t = api.register(f, mv, sch["grid"], sch["radius"], sch["quant"], sch["alpha"])
+    oflow, owarped = oracle.register(f, mv, sch["grid"], sch["radius"], sch["quant"], sch["alpha"])
+    _eq(flow, oflow, "flow")
+    _eq(warped, owarped, "warped")
+    assert t["hamming_evals"] > 0 and t["total"] > 0
+
+
+def test_register_deterministic_and_affine():
+    print("case: register determinism with affine", flush=True)
+    f, mv = phantom.make_pair((64, 56, 48), 61)
+    X = np.eye(4, dtype=np.float32)
+    X[0, 3], X[1, 1], X[2, 0] = 1.25, 1.02, 0.01
+    Xc = X.T.ravel().copy()
+    a = api.register(f, mv, (6, 5, 4), (4, 3, 2), (3, 2, 1), 1.6, X=Xc)
+    b = api.register(f, mv, (6, 5, 4), (4, 3, 2), (3, 2, 1), 1.6, X=Xc)
+    _eq(a[0], b[0], "flow determinism")
+    _eq(a[1], b[1], "warped determinism")
+
+
+def test_register_with_affine_vs_oracle(oracle):
+    print("case: register with affine vs oracle", flush=True)
+    f, mv = phantom.make_pair((52, 48, 44), 71)
+    X = np.eye(4, dtype=np.float32)
+    X[0, 3], X[1, 1], X[2, 0], X[1, 2] = -1.5, 0.97, 0.015, -0.02
+    Xc = X.T.ravel().copy()
+    flow, warped, _ = api.register(f, mv, (6, 4), (3, 2), (2, 1), 1.6, X=Xc)
+    oflow, owarped = oracle.register(f, mv, (6, 4), (3, 2), (2, 1), 1.6, X=Xc)
+    _eq(flow, oflow, "flow")
+    _eq(warped, owarped, "warped")
+
+
+# ------------------------------------------------------------ errors --------
+
+def test_errors_are_statuses_not_faults():
+    im = np.zeros((10, 10, 10), np.float32)
+    with pytest.raises(api.DeedsError):
+        api.descriptor(im, 6)  # box window larger than the volume
+    with pytest.raises(api.DeedsError):
+        api.dataCostCL(np.zeros((10, 10, 10), np.uint64), np.zeros((10, 10, 10), np.uint64), 20, 1, 1)
+    with pytest.raises(api.DeedsError):
+        api.regularisationCL(np.zeros((8, 27), np.float32), *np.zeros((3, 2, 2, 2), np.float32), 1, 5, 1,
+                             np.array([0, 0, 1, 2, 3, 4, 5, 6], np.int32), np.zeros(8, np.int32),
+                             np.ones(8, np.float32), (10, 10, 10))
+    # the library still works afterwards
+    assert api.descriptor(im, 1).shape == im.shape

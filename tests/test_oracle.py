@@ -1,0 +1,296 @@
+"""CPU tests of the oracle (the C restatement in oracle/) -- no GPU.
+
+The oracle is pinned three ways:
+  1. bit-for-bit against the golden vectors written by the unmodified
+     reference (tests/golden/make_golden.py);
+  2. bit-for-bit against the reference compiled from its own sources
+     (oracle/_ref), on fresh seeded inputs, where that build exists;
+  3. against what the mathematics fixes (brute-force window sums, popcount of
+     thermometer codes, Kruskal's MST weight, brute-force min-convolution and
+     brute-force MAP labelings on tiny trees), so that a dropped term, a wrong
+     sign or a transposed axis fails even where 1 and 2 cannot see it.
+"""
+import numpy as np
+import pytest
+from scipy.sparse import coo_matrix
+from scipy.sparse.csgraph import minimum_spanning_tree
+
+from deedsbcv_amd import phantom
+
+# ---------------------------------------------------------------- golden ----
+
+
+def test_golden_mind(oracle, golden):
+    for qs in (1, 2):
+        assert np.array_equal(oracle.descriptor(golden["im"], qs), golden[f"mind_q{qs}"])
+    assert np.array_equal(oracle.descriptor(golden["im2"], 2), golden["mind2_q2"])
+
+
+@pytest.mark.parametrize("case", [(5, 2, 2), (4, 2, 1), (6, 1, 3)])
+def test_golden_cost_tree_bp(oracle, golden, case):
+    step, hw, q = case
+    tag = f"s{step}h{hw}q{q}"
+    cost = oracle.datacost(golden["mind_q2"], golden["mind2_q2"], step, hw, q)
+    assert np.array_equal(cost, golden[f"cost_{tag}"])
+    tree = oracle.prims(golden["im2"] if step == 4 else golden["im"], step)
+    for a, name in zip(tree, ("ordered", "parents", "edgemst")):
+        assert np.array_equal(a, golden[f"{name}_{tag}"]), name
+    u0 = golden[f"u0_{tag}"]
+    u1 = oracle.regularisation(cost, u0[0], u0[1], u0[2], hw, step, q, *tree, (40, 36, 32))
+    assert np.array_equal(np.stack(u1), golden[f"u1_{tag}"])
+
+
+@pytest.mark.parametrize("ln", [3, 5, 9])
+def test_golden_message_dt(oracle, golden, ln):
+    off = golden[f"dt_off_{ln}"]
+    msg, inds = oracle.message_dt(golden[f"dt_in_{ln}"], off[0], off[1], off[2])
+    assert np.array_equal(msg, golden[f"dt_msg_{ln}"])
+    assert np.array_equal(inds, golden[f"dt_inds_{ln}"])
+
+
+def test_golden_transforms(oracle, golden):
+    g = golden["grid6"]
+    assert np.array_equal(np.stack(oracle.upsample(g[0], g[1], g[2], (40, 36, 32))), golden["up"])
+    assert np.array_equal(np.stack(oracle.consistent(*g, 5)), golden["cons"])
+    ux, vx, wx = golden["up"]
+    w, ssd = oracle.warp_affine(golden["im2"], golden["im"], golden["X"], ux, vx, wx)
+    assert np.array_equal(w, golden["warp_affine"])
+    assert np.array_equal(ssd, golden["warp_affine_ssd"])
+    w, ssd = oracle.warp_image(golden["im2"], golden["im"], ux, vx, wx)
+    assert np.array_equal(w, golden["warp_image"])
+    assert np.array_equal(ssd, golden["warp_image_ssd"])
+    assert np.array_equal(oracle.warp_affine_s(golden["seg"], golden["X"], ux, vx, wx), golden["warp_seg"])
+
+
+def test_golden_end_to_end(oracle, golden):
+    s = golden["e2e_schedule"]
+    flow, warped = oracle.register(golden["e2e_fixed"], golden["e2e_moving"], s[0], s[1], s[2])
+    assert np.array_equal(flow, golden["e2e_flow"])
+    assert np.array_equal(warped, golden["e2e_warped"])
+
+
+# ------------------------------------------------- against the reference ----
+
+
+@pytest.mark.parametrize("seed", [21, 22])
+def test_ref_mind_cost_tree(oracle, ref, seed):
+    f, mv = phantom.make_pair((37, 33, 29), seed)  # odd sizes: partial blocks at the border
+    im, im2 = f.astype(np.float32) + 1024, mv.astype(np.float32) + 1024
+    for qs in (1, 2, 3):
+        assert np.array_equal(oracle.descriptor(im, qs), ref.descriptor(im, qs))
+    F, M = oracle.descriptor(im, 2), oracle.descriptor(im2, 2)
+    for step, hw, q in ((5, 2, 2), (4, 3, 1), (7, 1, 4), (8, 1, 5)):
+        assert np.array_equal(oracle.datacost(F, M, step, hw, q), ref.datacost(F, M, step, hw, q))
+    for step in (3, 4, 5, 6):
+        for a, b in zip(oracle.prims(im, step), ref.prims(im, step)):
+            assert np.array_equal(a, b)
+
+
+def test_ref_randnum_variants(oracle, ref):
+    f, mv = phantom.make_pair((32, 32, 32), 3)
+    F = oracle.descriptor(f.astype(np.float32) + 1024, 1)
+    M = oracle.descriptor(mv.astype(np.float32) + 1024, 1)
+    for randnum in (0, 2):
+        for step in (4, 5, 8):
+            assert np.array_equal(oracle.datacost(F, M, step, 1, 2, 1.6, randnum),
+                                  ref.datacost(F, M, step, 1, 2, 1.6, randnum))
+
+
+def test_ref_bp_random_trees(oracle, ref):
+    rng = np.random.default_rng(7)
+    f, _ = phantom.make_pair((30, 30, 25), 9)
+    im = f.astype(np.float32) + 1024
+    for step, hw, q in ((5, 2, 2), (3, 1, 1), (6, 3, 1)):
+        tree = oracle.prims(im, step)
+        m1, n1, o1 = 30 // step, 30 // step, 25 // step
+        cost = (rng.random(size=(m1 * n1 * o1, (2 * hw + 1) ** 3)) * 20).astype(np.float32)
+        u0 = (rng.normal(size=(3, o1, n1, m1)) * 2).astype(np.float32)
+        a = oracle.regularisation(cost, u0[0], u0[1], u0[2], hw, step, q, *tree, (30, 30, 25))
+        b = ref.regularisation(cost, u0[0], u0[1], u0[2], hw, step, q, *tree, (30, 30, 25))
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+
+
+def test_ref_jacobian(oracle, ref):
+    rng = np.random.default_rng(3)
+    g = (rng.normal(size=(3, 9, 10, 11)) * 2).astype(np.float32)
+    assert oracle.jacobian(g[0], g[1], g[2], 4)[0] == ref.jacobian(g[0], g[1], g[2], 4)
+
+
+# ------------------------------------------------ mathematical pins ---------
+
+
+def _window_sum(x, hw):
+    """Brute-force clamped box sum along all three axes (float64)."""
+    out = x.astype(np.float64)
+    for ax in range(3):
+        L = out.shape[ax]
+        acc = np.zeros_like(out)
+        for t in range(L):
+            lo, hi = max(t - hw, 0), min(t + hw, L - 1)
+            sl = [slice(None)] * 3
+            sl[ax] = slice(lo, hi + 1)
+            tgt = [slice(None)] * 3
+            tgt[ax] = t
+            acc[tuple(tgt)] = out[tuple(sl)].sum(axis=ax)
+        out = acc
+    return out
+
+
+def test_boxfilter_is_window_sum(oracle):
+    rng = np.random.default_rng(0)
+    x = rng.integers(0, 50, size=(9, 11, 13)).astype(np.float32)  # small integers: float sums exact
+    for hw in (1, 2, 3):
+        assert np.array_equal(oracle.boxfilter(x, hw), _window_sum(x, hw).astype(np.float32))
+
+
+def test_mind_code_structure(oracle):
+    f, _ = phantom.make_pair((24, 22, 20), 4)
+    codes = oracle.descriptor(f.astype(np.float32) + 1024, 2)
+    assert not np.any(codes >> np.uint64(60))
+    for l in range(12):
+        field = (codes >> np.uint64(5 * l)) & np.uint64(31)
+        assert set(np.unique(field).tolist()) <= {0, 1, 3, 7, 15, 31}
+    # the smallest of the 12 normalised values is 0 -> its field is full (31)
+    full = np.zeros(codes.shape, bool)
+    for l in range(12):
+        full |= ((codes >> np.uint64(5 * l)) & np.uint64(31)) == 31
+    assert full.all()
+
+
+def test_mind_thresholds(oracle):
+    """-log((c+1.5f)/6): float argument, log correctly rounded to float."""
+    import math
+    t = oracle.mind_thresholds()
+    args = [np.float32(np.float32(c + 1.5) / np.float32(6)) for c in range(5)]
+    want = np.array([np.float32(-math.log(float(a))) for a in args], np.float32)
+    assert np.array_equal(t, want)
+
+
+def test_datacost_brute_force(oracle):
+    """popcount of two thermometer codes == sum over fields of |level difference|."""
+    rng = np.random.default_rng(1)
+    m, n, o = 12, 11, 10
+    levels_f = rng.integers(0, 6, size=(o, n, m, 12))
+    levels_m = rng.integers(0, 6, size=(o, n, m, 12))
+    pack = lambda L: sum(((np.uint64(1) << L[..., l].astype(np.uint64)) - np.uint64(1)) << np.uint64(5 * l)
+                         for l in range(12)).astype(np.uint64)
+    F, M = pack(levels_f), pack(levels_m)
+    step, hw, q = 3, 1, 2
+    cost = oracle.datacost(F, M, step, hw, q, 1.6, 1)
+    scale, skip = oracle.datacost_scale(step, q, 1.6, 1)
+    len_ = 2 * hw + 1
+    for node in range(cost.shape[0]):
+        y, x, z = node % (m // step), (node // (m // step)) % (n // step), node // ((m // step) * (n // step))
+        for l in range(len_ ** 3):
+            ls, lx, lz = l % len_, (l // len_) % len_, l // len_ ** 2
+            s = 0
+            for k in range(0, step, skip[2]):
+                for j in range(0, step, skip[1]):
+                    for i in range(0, step, skip[0]):
+                        fi, fj, fk = y * step + i, x * step + j, z * step + k
+                        mi = min(max(fi + (ls - hw) * q, 0), m - 1)
+                        mj = min(max(fj + (lx - hw) * q, 0), n - 1)
+                        mk = min(max(fk + (lz - hw) * q, 0), o - 1)
+                        s += int(np.abs(levels_f[fk, fj, fi] - levels_m[mk, mj, mi]).sum())
+            assert cost[node, l] == np.float32(s) * scale
+
+
+def test_prims_is_a_minimum_spanning_tree(oracle):
+    f, _ = phantom.make_pair((40, 36, 32), 5)
+    im = f.astype(np.float32) + 1024
+    step = 4
+    m1, n1, o1 = 10, 9, 8
+    N = m1 * n1 * o1
+    ordered, parents, edgemst = oracle.prims(im, step)
+    ec, nb = oracle.edge_sad(im, step)
+    _, sd = oracle.image_stats(im)
+    w = -np.exp(-(ec / np.float32(step ** 3)).astype(np.float64) / (2.0 * float(sd)))
+    # spanning tree: root first, every other node's parent precedes it
+    root = m1 // 2 + (n1 // 2) * m1 + (o1 // 2) * m1 * n1
+    assert ordered[0] == root and parents[root] == -1
+    pos = np.empty(N, int)
+    pos[ordered] = np.arange(N)
+    assert sorted(ordered.tolist()) == list(range(N))
+    assert all(pos[parents[c]] < pos[c] for c in range(N) if c != root)
+    # BFS depth order, ties by vertex id
+    depth = np.zeros(N, int)
+    for c in ordered[1:]:
+        depth[c] = depth[parents[c]] + 1
+    keys = list(zip(depth[ordered], ordered))
+    assert keys == sorted(keys)
+    # total weight equals Kruskal's (shift weights positive for scipy)
+    rows, cols, vals = [], [], []
+    for e in range(6 * N):
+        if nb[e] >= 0:
+            rows.append(e % N), cols.append(nb[e]), vals.append(w[e] + 2.0)
+    G = coo_matrix((vals, (rows, cols)), shape=(N, N)).tocsr()
+    kruskal = minimum_spanning_tree(G).sum() - 2.0 * (N - 1)
+    tree_w = 0.0
+    for c in range(N):
+        if c == root:
+            continue
+        p = parents[c]
+        e = [k for k in range(6) if nb[c + k * N] == p]
+        assert len(e) == 1
+        tree_w += w[c + e[0] * N]
+        # float expf vs double exp: equal to within one float ulp
+        assert abs(float(edgemst[c]) + w[c + e[0] * N]) <= 1.2e-7 * abs(w[c + e[0] * N])
+    assert abs(tree_w - kruskal) < 1e-6 * N
+
+
+def test_message_dt_brute_force(oracle):
+    rng = np.random.default_rng(2)
+    for ln in (3, 5):
+        cube = (rng.normal(size=ln ** 3) * 3).astype(np.float32)
+        ox, oy, oz = rng.uniform(-1.5, 1.5, size=3).astype(np.float32)
+        msg, inds = oracle.message_dt(cube, ox, oy, oz)
+        v = cube.reshape(ln, ln, ln)  # [k, j, i]
+        t = np.arange(ln)
+        zy = ((t[:, None] - t[None, :]).astype(np.float32) + oy) ** 2  # [i_out, i_in]
+        zx = ((t[:, None] - t[None, :]).astype(np.float32) + ox) ** 2
+        zz = ((t[:, None] - t[None, :]).astype(np.float32) + oz) ** 2
+        for l in range(ln ** 3):
+            io, jo, ko = l % ln, (l // ln) % ln, l // ln ** 2
+            # same association as the separable passes: ((v + zy) + zx) + zz
+            a = (v + zy[io][None, None, :]).astype(np.float32)
+            a = (a + zx[jo][None, :, None]).astype(np.float32)
+            a = (a + zz[ko][:, None, None]).astype(np.float32)
+            assert msg[l] == a.min()
+            src = inds[l]
+            assert a.reshape(-1)[src] == a.min()
+
+
+def test_regularisation_is_map_on_chains(oracle):
+    """On a 3-node star the BP labeling minimises the tree energy
+    D_root(l_r) + sum_c [ w_c D_c(l_c) + |l_r - l_c + (u0_r - u0_c)/q|^2 ]
+    (the child's cost is scaled by its edge weight before the distance
+    transform, the pairwise term is not)."""
+    rng = np.random.default_rng(4)
+    hw, step, q = 1, 1, 1.0
+    ln = 2 * hw + 1
+    L = ln ** 3
+    # a 3x1x1 grid; root = node 1 (m/2), children 0 and 2
+    m, n, o = 3, 1, 1
+    ordered = np.array([1, 0, 2], np.int32)
+    parents = np.array([1, -1, 1], np.int32)
+    for trial in range(5):
+        cost = (rng.random(size=(3, L)) * 6).astype(np.float32)
+        edgemst = np.array([rng.uniform(0.3, 1), 0, rng.uniform(0.3, 1)], np.float32)
+        u0 = (rng.normal(size=(3, 1, 1, 3)) * 0.7).astype(np.float32)
+        u1, v1, w1, inds, sel = oracle.regularisation(cost, u0[0], u0[1], u0[2], hw, step, q, ordered, parents,
+                                                      edgemst, (m, n, o), want_inds=True)
+        lab = np.arange(L)
+        li, lj, lk = lab % ln, (lab // ln) % ln, lab // ln ** 2
+
+        def pair(p, c):  # d(parent label, child label) in label units
+            offx = (u0[0].ravel()[p] - u0[0].ravel()[c]) / q
+            offy = (u0[1].ravel()[p] - u0[1].ravel()[c]) / q
+            offz = (u0[2].ravel()[p] - u0[2].ravel()[c]) / q
+            return ((li[:, None] - li[None, :] + offy) ** 2 + (lj[:, None] - lj[None, :] + offx) ** 2 +
+                    (lk[:, None] - lk[None, :] + offz) ** 2)
+
+        E = (cost[1][:, None, None] + (edgemst[0] * cost[0][None, :, None] + pair(1, 0)[:, :, None]) +
+             (edgemst[2] * cost[2][None, None, :] + pair(1, 2)[:, None, :]))
+        s = sel
+        assert abs(E[s[1], s[0], s[2]] - E.min()) < 1e-4
