@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""Benchmark: symmetric 5-level deeds registrations per second on one MI355X.
+
+A step is one full symmetric registration (deedsBCV0.cpp:183-356: five levels,
+grid 8..4, radius 8..4, quantisation 5..1, alpha 1.6, forward + backward with
+consistent mapping, final warp of the moving image) of the seeded 256x256x128
+CT-like pair, with both volumes already resident in HBM when the timed region
+starts.  Prints one JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--workload full|small|crossmodal]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+VALU_INT_PEAK_TOPS = 39.3  # 256 CU x 64 lanes x 2.4 GHz, 32-bit integer VALU ops
+OPS_PER_HAMMING = 4        # 2 x v_xor_b32 + 2 x v_bcnt_u32_b32 per 64-bit code pair
+
+
+def workload(name):
+    from deedsbcv_amd import phantom as ph
+    if name == "full":
+        f, m = ph.workload_full(1)
+        sched = ph.SCHEDULE_FULL
+    elif name == "small":
+        f, m = ph.workload_small(0)
+        sched = ph.SCHEDULE_SMALL
+    elif name == "crossmodal":
+        f, m = ph.workload_crossmodal(2)
+        sched = ph.SCHEDULE_FULL
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return f, m, sched
+
+
+def cpu_baseline(f, m, sched, threads, budget_s):
+    """The reference's own CPU path on the host cores, on the same pair."""
+    ref = ROOT / "oracle" / "_ref" / "deedsBCV"
+    if ref.exists():
+        from deedsbcv_amd import nifti
+        with tempfile.TemporaryDirectory() as d:
+            nifti.write_nifti(f"{d}/f.nii.gz", f)
+            nifti.write_nifti(f"{d}/m.nii.gz", m)
+            env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+            cmd = [str(ref), "-F", f"{d}/f.nii.gz", "-M", f"{d}/m.nii.gz", "-O", f"{d}/o",
+                   "-G", "x".join(map(str, sched["grid"])), "-L", "x".join(map(str, sched["radius"])),
+                   "-Q", "x".join(map(str, sched["quant"])), "-a", str(sched["alpha"])]
+            print(f"[bench] cpu baseline: reference deedsBCV with {threads} OpenMP threads", file=sys.stderr,
+                  flush=True)
+            r = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=budget_s)
+            mt = re.search(r"Total time: ([0-9.eE+-]+) sec", r.stdout)
+            if r.returncode != 0 or not mt:
+                return None
+            total = float(mt.group(1))
+            data = sum(float(x) for x in re.findall(r"data=([0-9.eE+-]+)", r.stdout))
+            return {"value": 1.0 / total, "unit": "registrations/s", "cores": threads, "kind": "reference",
+                    "sample": "one full symmetric registration of the same pair by the unmodified reference "
+                              "(its 'Total time' span, deedsBCV0.cpp:217-336)",
+                    "total_s": total, "datacost_s": data}
+    # fallback: the single-threaded C restatement on the small three-level pair
+    sys.path.insert(0, str(ROOT / "tests"))
+    from cbind import load_oracle
+    from deedsbcv_amd import phantom as ph
+    fs, ms = ph.workload_small(0)
+    s = ph.SCHEDULE_SMALL
+    t0 = time.perf_counter()
+    load_oracle().register(fs, ms, s["grid"], s["radius"], s["quant"], s["alpha"])
+    dt = time.perf_counter() - t0
+    return {"value": 1.0 / dt, "unit": "registrations/s (96x96x80, 3 levels)", "cores": 1, "kind": "port",
+            "sample": "one 3-level registration of the 96x96x80 pair by the C restatement"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="full", choices=["full", "small", "crossmodal"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
+    ap.add_argument("--cpu-budget", type=float, default=240.0)
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the registration path runs on a single device "
+              "(multi-GPU sharding is a design note, not part of this build).", file=sys.stderr)
+        raise SystemExit(2)
+
+    import torch
+
+    from deedsbcv_amd import api
+
+    print(f"[bench] generating workload {a.workload}", file=sys.stderr, flush=True)
+    f, m, sched = workload(a.workload)
+    o, n, mm = f.shape
+    dev = torch.device("cuda:0")
+    fixed = torch.from_numpy(f.astype(np.float32)).to(dev)
+    moving = torch.from_numpy(m.astype(np.float32)).to(dev)
+    g = sched["grid"][-1]
+    sz1 = (mm // g) * (n // g) * (o // g)
+    flow = torch.empty(3 * sz1, dtype=torch.float32, device=dev)
+    warped = torch.empty_like(fixed)
+    ctx = api.Context(0)
+    kw = dict(grid=sched["grid"], radius=sched["radius"], quant=sched["quant"], alpha=sched["alpha"])
+
+    for i in range(a.warmup):
+        ctx.register(fixed, moving, flow, warped, **kw)
+        print(f"[bench] warmup {i + 1}/{a.warmup} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    timings = []
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        timings.append(ctx.register(fixed, moving, flow, warped, **kw))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    print(f"[bench] {a.steps} timed steps: {wall:.3f} s", file=sys.stderr, flush=True)
+    fl = flow.cpu().numpy()
+    if not np.all(np.isfinite(fl)):
+        raise SystemExit("non-finite displacement field")
+
+    evals = timings[0]["hamming_evals"]
+    dc_s = sum(t["datacost_kernel_s"] for t in timings)
+    dc_n = sum(t["datacost_launches"] for t in timings)
+    achieved = OPS_PER_HAMMING * evals * a.steps / dc_s / 1e12
+    stage = {k: float(np.mean([sum(t[k]) for t in timings])) for k in ("mind", "data", "mst_reg", "transf")}
+    result = {
+        "metric": "symmetric 5-level registrations/s @256x256x128; dataCost G Hamming evals/s",
+        "value": a.steps / wall,
+        "unit": "registrations/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * wall / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64 Hamming / f32",
+        "data": "synthetic (seeded CT-like phantom pair, B-spline warp)",
+        "config": {"workload": f"{a.workload}: {mm}x{n}x{o} int16 pair, grid {sched['grid']}, "
+                               f"radius {sched['radius']}, quant {sched['quant']}, alpha {sched['alpha']}, symmetric",
+                   "dims": [mm, n, o], "levels": len(sched["grid"]), "parallelism": "single GPU"},
+        "hamming_gevals_per_s": evals * a.steps / wall / 1e9,
+        "datacost_kernel_gevals_per_s": evals * a.steps / dc_s / 1e9,
+        "hamming_evals_per_registration": evals,
+        "stage_ms_fwd_stream": {k: 1e3 * v for k, v in stage.items()},
+        "roofline": {
+            "kernel": "k_datacost (Hamming cost volume)",
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": VALU_INT_PEAK_TOPS,
+            "unit": "TOP/s (int32 VALU)",
+            "frac": achieved / VALU_INT_PEAK_TOPS,
+            "traffic": None,
+            "launches_timed": dc_n,
+            "avg_launch_ms": 1e3 * dc_s / dc_n,
+        },
+    }
+    if not a.no_cpu_baseline:
+        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        cb = cpu_baseline(f, m, sched, threads, a.cpu_budget) if a.workload != "small" or True else None
+        if cb is not None and cb.get("datacost_s"):
+            cb["hamming_gevals_per_s"] = evals / cb["datacost_s"] / 1e9
+        result["cpu_baseline"] = cb
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

@@ -23,14 +23,20 @@ class Timing(ctypes.Structure):
     _fields_ = [("levels", c_int), ("mind", ctypes.c_double * MAX_LEVELS), ("data", ctypes.c_double * MAX_LEVELS),
                 ("mst_reg", ctypes.c_double * MAX_LEVELS), ("transf", ctypes.c_double * MAX_LEVELS),
                 ("total", ctypes.c_double), ("hamming_evals", ctypes.c_double), ("ssd_before", c_float),
-                ("ssd_after", c_float), ("jac_std", c_float * MAX_LEVELS), ("jac_neg", c_float * MAX_LEVELS)]
+                ("ssd_after", c_float), ("jac_std", c_float * MAX_LEVELS), ("jac_neg", c_float * MAX_LEVELS),
+                ("datacost_kernel_s", ctypes.c_double), ("datacost_launches", c_int),
+                ("datacost_level_s", ctypes.c_double * MAX_LEVELS),
+                ("datacost_level_evals", ctypes.c_double * MAX_LEVELS)]
 
     def as_dict(self):
         L = self.levels
         return {"levels": L, "mind": list(self.mind[:L]), "data": list(self.data[:L]),
                 "mst_reg": list(self.mst_reg[:L]), "transf": list(self.transf[:L]), "total": self.total,
                 "hamming_evals": self.hamming_evals, "ssd_before": self.ssd_before, "ssd_after": self.ssd_after,
-                "jac_std": list(self.jac_std[:L]), "jac_neg": list(self.jac_neg[:L])}
+                "jac_std": list(self.jac_std[:L]), "jac_neg": list(self.jac_neg[:L]),
+                "datacost_kernel_s": self.datacost_kernel_s, "datacost_launches": self.datacost_launches,
+                "datacost_level_s": list(self.datacost_level_s[:L]),
+                "datacost_level_evals": list(self.datacost_level_evals[:L])}
 
 
 # name -> argtypes (all return int status unless listed in _RESTYPES)

@@ -144,6 +144,10 @@ void register_pair(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, 
     std::vector<std::unique_ptr<Event>> ev_lvl;
     for (int i = 0; i < 6 * L; i++) ev_lvl.emplace_back(new Event());
     auto EV = [&](int l, int k) { return ev_lvl[6 * l + k]->e; };
+    // events bracketing every data-cost kernel launch, on its own stream
+    std::vector<std::unique_ptr<Event>> ev_dc;
+    for (int i = 0; i < 4 * L; i++) ev_dc.emplace_back(new Event());
+    auto DC = [&](int l, int dir, int end) { return ev_dc[4 * l + 2 * dir + end]->e; };
 
     DEEDS_HIP(hipEventRecord(ev_start.e, s0));
     auto t0 = clock::now();
@@ -252,8 +256,10 @@ void register_pair(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, 
             launch_descriptor(mind_w[dir], warped[dir], m, n, o, c.mind_qs, mw[dir][0], mw[dir][1], mw[dir][2], s);
             if (dir == 0) DEEDS_HIP(hipEventRecord(EV(l, 2), s));
             launch_pad_codes(mpad[dir], mind_w[dir], m, n, o, c.plan.pad, s);
+            DEEDS_HIP(hipEventRecord(DC(l, dir, 0), s));
             launch_datacost(cost[dir], dir ? mind_mov : mind_fix, mpad[dir], m, n, o, c.step, c.hw, c.quant, c.plan,
                             s);
+            DEEDS_HIP(hipEventRecord(DC(l, dir, 1), s));
             if (dir == 0) DEEDS_HIP(hipEventRecord(EV(l, 3), s));
             hamming += (double)c.sz1 * c.len3 * (double)c.plan.samples;
         }
@@ -321,6 +327,18 @@ void register_pair(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, 
         DEEDS_HIP(hipEventElapsedTime(&tot, ev_start.e, ev_stop.e));
         timing->total = tot * 1e-3;
         timing->hamming_evals = hamming;
+        for (int l = 0; l < L; l++) {
+            double lvl = 0.0;
+            for (int dir = 0; dir < 2; dir++) {
+                float ms = 0;
+                DEEDS_HIP(hipEventElapsedTime(&ms, DC(l, dir, 0), DC(l, dir, 1)));
+                lvl += ms * 1e-3;
+            }
+            timing->datacost_level_s[l] = lvl;
+            timing->datacost_level_evals[l] = 2.0 * lv[l].sz1 * lv[l].len3 * (double)lv[l].plan.samples;
+            timing->datacost_kernel_s += lvl;
+            timing->datacost_launches += 2;
+        }
         std::vector<double> hj(4 * (size_t)L);
         DEEDS_HIP(hipMemcpy(hj.data(), jac, hj.size() * sizeof(double), hipMemcpyDeviceToHost));
         for (int l = 0; l < L; l++) {

@@ -64,6 +64,12 @@ typedef struct deeds_timing {
     float ssd_before, ssd_after;      /* logging-only SSD of the last warp (dataCostD.h:300-312) */
     float jac_std[DEEDS_MAX_LEVELS];  /* std(J) logged per level (transformations.h:166) */
     float jac_neg[DEEDS_MAX_LEVELS];  /* fraction of J<0 per level (transformations.h:168) */
+    /* Hamming data-cost kernel: summed launch durations (HIP events on the
+     * launching stream), number of launches and evaluations per level. */
+    double datacost_kernel_s;
+    int datacost_launches;
+    double datacost_level_s[DEEDS_MAX_LEVELS];
+    double datacost_level_evals[DEEDS_MAX_LEVELS];
 } deeds_timing;
 
 /* ---- library / context ------------------------------------------------- */

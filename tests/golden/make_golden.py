@@ -92,6 +92,10 @@ def main():
         out["e2e_flow"] = nifti.read_displacements(f"{d}/o_displacements.dat")
         out["e2e_warped"] = nifti.read_nifti(f"{d}/o_deformed.nii.gz")[0]
         out["e2e_warped_seg"] = nifti.read_nifti(f"{d}/o_deformed_seg.nii.gz")[0]
+        import gzip
+        for name in ("deformed", "deformed_seg"):
+            with gzip.open(f"{d}/o_{name}.nii.gz", "rb") as fh:
+                out[f"e2e_file_{name}"] = np.frombuffer(fh.read(), np.uint8).copy()
         subprocess.run([str(ROOT / "oracle/_ref/applyBCV"), "-M", f"{d}/s.nii.gz", "-O", f"{d}/o", "-D",
                         f"{d}/a.nii.gz"], check=True, capture_output=True)
         out["e2e_apply_seg"] = nifti.read_nifti(f"{d}/a.nii.gz")[0]
