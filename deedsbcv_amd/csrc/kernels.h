@@ -53,9 +53,9 @@ void launch_bp_select(const float* cost, const int16_t* allinds, const BPDevTree
 
 // ---- transforms (warp.hip) -------------------------------------------------
 struct Affine {
-    float r[12];  // rows of the 3x4 matrix: y = r[0..3], x = r[4..7], z = r[8..11]
+    float r[12];  // coefficients of y = r[0..3], x = r[4..7], z = r[8..11] (dataCostD.h:285-287)
 };
-Affine affine_from_X(const float* X);  // X: 16 floats column-wise (deedsBCV0.cpp:161-165)
+Affine affine_from_X(const float* X);  // X: the 16 floats as stored by deedsBCV0.cpp:161-165
 void launch_upsample(float* u1, float* v1, float* w1, const float* u0, const float* v0, const float* w0,
                      int m, int n, int o, int m2, int n2, int o2, hipStream_t s);
 void launch_interp3(float* out, const float* in, const float* x1, const float* y1, const float* z1,

@@ -302,9 +302,10 @@ constexpr int TB = 256;
 }  // namespace
 
 Affine affine_from_X(const float* X) {
+    // the warps index X flat: y uses X[0..3], x X[4..7], z X[8..11]
+    // (dataCostD.h:285-287); X holds the text file's rows column-wise
     Affine a{};
-    for (int r = 0; r < 3; r++)
-        for (int c = 0; c < 4; c++) a.r[r * 4 + c] = X[r + 4 * c];
+    for (int c = 0; c < 12; c++) a.r[c] = X[c];
     return a;
 }
 
