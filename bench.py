@@ -156,6 +156,9 @@ def main():
         "datacost_kernel_gevals_per_s": evals * a.steps / dc_s / 1e9,
         "hamming_evals_per_registration": evals,
         "stage_ms_fwd_stream": {k: 1e3 * v for k, v in stage.items()},
+        "host_tree_build_ms": 1e3 * float(np.mean([t["tree_build_s"] for t in timings])),
+        "host_tree_wait_ms": 1e3 * float(np.mean([t["tree_wait_s"] for t in timings])),
+        "datacost_level_ms": [1e3 * x for x in np.mean([t["datacost_level_s"] for t in timings], axis=0)],
         "roofline": {
             "kernel": "k_datacost (Hamming cost volume)",
             "bound": "valu",

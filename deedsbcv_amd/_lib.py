@@ -26,7 +26,8 @@ class Timing(ctypes.Structure):
                 ("ssd_after", c_float), ("jac_std", c_float * MAX_LEVELS), ("jac_neg", c_float * MAX_LEVELS),
                 ("datacost_kernel_s", ctypes.c_double), ("datacost_launches", c_int),
                 ("datacost_level_s", ctypes.c_double * MAX_LEVELS),
-                ("datacost_level_evals", ctypes.c_double * MAX_LEVELS)]
+                ("datacost_level_evals", ctypes.c_double * MAX_LEVELS),
+                ("tree_build_s", ctypes.c_double), ("tree_wait_s", ctypes.c_double)]
 
     def as_dict(self):
         L = self.levels
@@ -36,7 +37,8 @@ class Timing(ctypes.Structure):
                 "jac_std": list(self.jac_std[:L]), "jac_neg": list(self.jac_neg[:L]),
                 "datacost_kernel_s": self.datacost_kernel_s, "datacost_launches": self.datacost_launches,
                 "datacost_level_s": list(self.datacost_level_s[:L]),
-                "datacost_level_evals": list(self.datacost_level_evals[:L])}
+                "datacost_level_evals": list(self.datacost_level_evals[:L]),
+                "tree_build_s": self.tree_build_s, "tree_wait_s": self.tree_wait_s}
 
 
 # name -> argtypes (all return int status unless listed in _RESTYPES)

@@ -226,8 +226,8 @@ size_t tree_ints_bound(int N) { return 7 * (size_t)N + 8; }
 // copy into the device area d_stage of the same layout.  The staging memory
 // must stay untouched until the copy has executed on `s`.
 BPDevTree stage_tree(const Tree& t, void* pinned, void* d_stage, hipStream_t s) {
-    size_t ints = t.ordered.size() + t.parents.size() + t.depth_start.size() + t.grp_parent.size() +
-                  t.grp_child.size() + t.child_list.size();
+    size_t ints = t.ordered.size() + t.parents.size() + t.depth_start.size() + t.child_start.size() +
+                  t.child_list.size() + t.heads.size();
     size_t bound = tree_ints_bound(t.N);
     if (ints > bound) fail(DEEDS_ERR_INTERNAL, "tree schedule larger than its staging bound");
     int* hi = static_cast<int*>(pinned);
@@ -243,9 +243,10 @@ BPDevTree stage_tree(const Tree& t, void* pinned, void* d_stage, hipStream_t s) 
     out.ordered = put(t.ordered);
     out.parents = put(t.parents);
     out.depth_start = put(t.depth_start);
-    out.grp_parent = put(t.grp_parent);
-    out.grp_child = put(t.grp_child);
+    out.child_start = put(t.child_start);
     out.child_list = put(t.child_list);
+    out.heads = put(t.heads);
+    out.root = t.N > 0 ? t.ordered[0] : 0;
     float* hf = reinterpret_cast<float*>(hi + bound);
     float* df = reinterpret_cast<float*>(di + bound);
     std::copy(t.edgemst.begin(), t.edgemst.end(), hf);
@@ -255,22 +256,11 @@ BPDevTree stage_tree(const Tree& t, void* pinned, void* d_stage, hipStream_t s) 
     return out;
 }
 
+// Bottom-up rounds, then the top-down selection.
 void run_bp(const Tree& t, const BPDevTree& dt, float* d_cost, int16_t* d_inds, int* d_sel, const float* u0,
             const float* v0, const float* w0, float* u1, float* v1, float* w1, int hw, float quant, hipStream_t s) {
-    int len = 2 * hw + 1;
-    for (int d = t.maxd; d >= 1; d--) {
-        int g0 = t.grp_depth_start[d - 1], g1 = t.grp_depth_start[d];
-        launch_bp_level(d_cost, d_inds, dt, g0, g1 - g0, u0, v0, w0, len, quant, s);
-    }
+    launch_bp_rounds(d_cost, d_inds, dt, t.round_start.data(), t.rounds, u0, v0, w0, hw, quant, s);
     launch_bp_select(d_cost, d_inds, dt, t.N, t.maxd, u0, v0, w0, u1, v1, w1, d_sel, hw, quant, s);
-}
-
-void bp_prepare() {
-    static bool done = false;
-    if (done) return;
-    // allow the BP kernel's LDS footprint above the 64 KB default
-    bp_set_smem_limit();
-    done = true;
 }
 
 }  // namespace deeds
@@ -286,7 +276,6 @@ int deeds_ctx_regularisation(deeds_ctx* ctx, float* d_cost, const float* d_u0, c
         check_labels(hw, quant);
         DEEDS_REQUIRE(d_cost && d_u0 && d_v0 && d_w0 && d_u1 && d_v1 && d_w1 && ordered && parents && edgemst,
                       "null pointer");
-        bp_prepare();
         int m1 = m / step, n1 = n / step, o1 = o / step;
         Tree t;
         t.m1 = m1; t.n1 = n1; t.o1 = o1; t.N = m1 * n1 * o1;

@@ -1,35 +1,109 @@
 // Tree belief propagation on gfx950 (reference: regularisation.h:6-303).
 //
-// Bottom-up, one launch per BFS depth d (deepest first): one workgroup per
-// parent p at depth d-1 that has children at depth d.  The workgroup walks its
-// children in `ordered` order (the reference's serial fold order,
-// regularisation.h:221-232), and for each child c:
-//   - stages cost[c] * edgemst[c] in LDS                              (:197-202)
-//   - runs the three separable min-convolution passes y, x, z with the
-//     quadratic (t - len + offset)^2 in label units and the reference's
-//     strict-< lowest-index tie rule, tracking the composite argmin   (:6-113)
-//   - folds  acc[p] += msg - min(msg)  into an LDS accumulator        (:221-232)
-// and finally writes the accumulator back to cost[p].  Each 1-D pass is the
-// exact brute-force loop of the reference (same float operations in the same
-// order), so messages and argmin tables are bit-identical; with len <= 17 the
-// O(len) inner loop is cheap next to the per-depth latency.
-// Top-down: a single workgroup selects labels depth by depth (:259-293).
+// Bottom-up pass: one launch per "round" of the tree schedule, one workgroup
+// per chain head.  A chain head is an internal node with several children or
+// with a single leaf child; its children are leaves or chain ends finished in
+// earlier rounds.  The workgroup
+//   - accumulates the head's cost in LDS, and for each child c in `ordered`
+//     order stages cost[c] * edgemst[c] (:197-202), runs the three separable
+//     min-convolution passes y, x, z with the quadratic ((t - len) + offset)^2
+//     in label units and the reference's strict-< lowest-index tie rule,
+//     writing the composite argmin table of c (:6-113), and folds
+//     acc += msg - min(msg) (:221-232);
+//   - then, while the parent of the finished node has that node as its only
+//     child, messages it straight from LDS into the parent (the reference's
+//     fold with a single child), so a chain of single-child links costs one
+//     round instead of one launch per BFS depth;
+//   - writes the last finished node's cost back to HBM.
+// Rounds replace the reference's loop over BFS depths (:190-235) with fewer
+// global synchronisation points (the kernel boundary makes the chain ends of
+// one round visible to the next on every XCD).  The arithmetic per node is the
+// reference's, in its order, so messages and argmin tables are bit-identical.
+//
+// Each 1-D pass keeps the reference's brute-force O(len) loop per output, but
+// register-blocked: a thread loads a whole line of the label cube and a window
+// of the quadratic table into VGPRs and produces its outputs from registers
+// (LEN is a template parameter).
+//
+// Top-down: one workgroup walks the BFS depths selecting labels (:259-293).
 #include "kernels.h"
 
 namespace deeds {
 
 namespace {
 
-constexpr int BP_THREADS = 256;
+template <int LEN>
+struct BPShape {
+    static constexpr int L2 = LEN * LEN;
+    static constexpr int L3 = L2 * LEN;
+    static constexpr int L3E = (L3 + 1) & ~1;
+    // S threads share one line (each produces OUT consecutive outputs)
+    static constexpr int S = (L2 * 8 <= 1024) ? 8 : (L2 * 4 <= 1024) ? 4 : (L2 * 2 <= 1024) ? 2 : 1;
+    static constexpr int OUT = (LEN + S - 1) / S;
+    static constexpr int BLOCK = ((L2 * S + 63) / 64) * 64;
+    static constexpr int ZT = 2 * LEN + 1 + 2 * S;  // quadratic table, padded for the last window
+    static constexpr size_t SMEM = 2 * (size_t)L3 * 4 + 2 * (size_t)L3E * 2 + 3 * (size_t)ZT * 4;
+};
+
+// One pass over all lines: line `ln` has elements base(ln) + j*STRIDE,
+// j = 0..LEN-1; output i of the line goes to base(ln) + i*STRIDE.
+// MODE 0: y pass (I1 = j + line*LEN), 1: x pass (I2 = I1[src of j]),
+// 2: z pass (allinds = I2[src of j]).  Returns the thread's minimum output.
+template <int LEN, int MODE>
+__device__ __forceinline__ float dt_pass(const float* __restrict__ src, float* __restrict__ dst,
+                                         const float* __restrict__ zt, const int16_t* __restrict__ iin,
+                                         int16_t* __restrict__ iout, int16_t* __restrict__ gout) {
+    using B = BPShape<LEN>;
+    constexpr int STRIDE = MODE == 0 ? 1 : MODE == 1 ? LEN : B::L2;
+    const int t = threadIdx.x;
+    const int line = t / B::S;
+    const int sub = t - line * B::S;
+    float mn = 3.402823466e38f;
+    if (line >= B::L2) return mn;
+    int base;
+    if (MODE == 0) base = line * LEN;
+    else if (MODE == 1) base = (line % LEN) + (line / LEN) * B::L2;
+    else base = line;
+    float v[LEN];
+#pragma unroll
+    for (int j = 0; j < LEN; j++) v[j] = src[base + j * STRIDE];
+    const int i0 = sub * B::OUT;
+    // window of the table: zw[q] = zt[i0 + q], so zt[i - j + LEN] = zw[r - j + LEN] for i = i0 + r
+    float zw[B::OUT + LEN];
+#pragma unroll
+    for (int q = 0; q < B::OUT + LEN; q++) zw[q] = zt[i0 + q];
+#pragma unroll
+    for (int r = 0; r < B::OUT; r++) {
+        const int i = i0 + r;
+        if (i < LEN) {
+            float best = v[0] + zw[r + LEN];
+            int bj = 0;
+#pragma unroll
+            for (int j = 0; j < LEN; j++) {
+                float cnd = v[j] + zw[r - j + LEN];
+                if (cnd < best) {
+                    best = cnd;
+                    bj = j;
+                }
+            }
+            const int o = base + i * STRIDE;
+            dst[o] = best;
+            mn = best < mn ? best : mn;
+            if (MODE == 0) iout[o] = (int16_t)(bj + line * LEN);
+            else if (MODE == 1) iout[o] = iin[base + bj * STRIDE];
+            else gout[o] = iin[base + bj * STRIDE];
+        }
+    }
+    return mn;
+}
 
 __device__ __forceinline__ float block_min(float v, float* red) {
-    // exact: min is order independent
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         float o = __shfl_xor(v, off, 64);
         v = o < v ? o : v;
     }
-    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) red[wid] = v;
     __syncthreads();
     float r = red[0];
@@ -38,99 +112,107 @@ __device__ __forceinline__ float block_min(float v, float* red) {
     return r;
 }
 
-__global__ __launch_bounds__(BP_THREADS) void k_bp_level(float* __restrict__ cost, int16_t* __restrict__ allinds,
-                                                         const int* __restrict__ grp_parent,
-                                                         const int* __restrict__ grp_child,
-                                                         const int* __restrict__ child_list,
-                                                         const float* __restrict__ edgemst,
-                                                         const float* __restrict__ u0, const float* __restrict__ v0,
-                                                         const float* __restrict__ w0, int grp_begin, int len,
-                                                         float quant) {
-    extern __shared__ float smem[];
-    __shared__ float red[BP_THREADS / 64];
-    const int len2 = len * len, len3 = len2 * len;
-    float* A = smem;
-    float* B = A + len3;
-    float* C = B + len3;
-    const int len3e = (len3 + 1) & ~1;  // int16 regions padded to keep the float tables 4-byte aligned
-    int16_t* I1 = (int16_t*)(C + len3);
-    int16_t* I2 = I1 + len3e;
-    float* zt = (float*)(I2 + len3e);  // three tables of 2*len+1
-    float* zy = zt;
-    float* zx = zt + (2 * len + 1);
-    float* zz = zx + (2 * len + 1);
+struct BPArgs {
+    float* cost;
+    int16_t* allinds;
+    const int* heads;
+    const int* parents;
+    const int* child_start;
+    const int* child_list;
+    const float* edgemst;
+    const float* u0;
+    const float* v0;
+    const float* w0;
+    float quant;
+    int root;
+};
 
-    const int g = grp_begin + blockIdx.x;
-    const int p = grp_parent[g];
-    const long long pb = (long long)p * len3;
-    for (int l = threadIdx.x; l < len3; l += blockDim.x) C[l] = cost[pb + l];
-
-    for (int ci = grp_child[g]; ci < grp_child[g + 1]; ci++) {
-        const int c = child_list[ci];
-        const long long cb = (long long)c * len3;
-        const float wgt = edgemst[c];
-        for (int l = threadIdx.x; l < len3; l += blockDim.x) A[l] = cost[cb + l] * wgt;
-        // offsets in label units (regularisation.h:208-210)
-        const float ox = (u0[p] - u0[c]) / quant;
-        const float oy = (v0[p] - v0[c]) / quant;
-        const float oz = (w0[p] - w0[c]) / quant;
-        for (int t = threadIdx.x; t < 2 * len + 1; t += blockDim.x) {
-            float dy = (float)(t - len) + oy;
-            float dx = (float)(t - len) + ox;
-            float dz = (float)(t - len) + oz;
-            zy[t] = dy * dy;
-            zx[t] = dx * dx;
-            zz[t] = dz * dz;
-        }
-        __syncthreads();
-        // pass y (contiguous axis): A -> B, I1 = composite index of the source
-        for (int out = threadIdx.x; out < len3; out += blockDim.x) {
-            const int line = out / len, i = out - line * len;
-            const float* src = A + line * len;
-            float best = src[0] + zy[i + len];
-            int bj = 0;
-            for (int j = 0; j < len; j++) {
-                float cnd = src[j] + zy[i - j + len];
-                if (cnd < best) { best = cnd; bj = j; }
-            }
-            B[out] = best;
-            I1[out] = (int16_t)(bj + line * len);
-        }
-        __syncthreads();
-        // pass x (stride len): B -> A, I2
-        for (int out = threadIdx.x; out < len3; out += blockDim.x) {
-            const int k1 = out / len2, rem = out - k1 * len2, i = rem / len, i1 = rem - i * len;
-            const int base = i1 + k1 * len2;
-            float best = B[base] + zx[i + len];
-            int bj = 0;
-            for (int j = 0; j < len; j++) {
-                float cnd = B[base + j * len] + zx[i - j + len];
-                if (cnd < best) { best = cnd; bj = j; }
-            }
-            A[out] = best;
-            I2[out] = I1[base + bj * len];
-        }
-        __syncthreads();
-        // pass z (stride len^2): A -> B, argmin table to global
-        for (int out = threadIdx.x; out < len3; out += blockDim.x) {
-            const int i = out / len2, c2 = out - i * len2;
-            float best = A[c2] + zz[i + len];
-            int bj = 0;
-            for (int j = 0; j < len; j++) {
-                float cnd = A[c2 + j * len2] + zz[i - j + len];
-                if (cnd < best) { best = cnd; bj = j; }
-            }
-            B[out] = best;
-            allinds[cb + out] = I2[c2 + bj * len2];
-        }
-        __syncthreads();
-        float mn = 3.402823466e38f;
-        for (int l = threadIdx.x; l < len3; l += blockDim.x) mn = B[l] < mn ? B[l] : mn;
-        mn = block_min(mn, red);
-        for (int l = threadIdx.x; l < len3; l += blockDim.x) C[l] += (B[l] - mn);
-        __syncthreads();
+// Message of node c (cost in `src`, already scaled by its edge weight in A)
+// towards parent p, folded into the accumulator C.  Leaves the argmin table
+// of c in allinds.
+template <int LEN>
+__device__ __forceinline__ void message_into(const BPArgs& a, int c, int p, float* A, float* Bf, float* C,
+                                             int16_t* I1, int16_t* I2, float* zy, float* zx, float* zz,
+                                             float* red) {
+    using B = BPShape<LEN>;
+    const int t = threadIdx.x;
+    const float ox = (a.u0[p] - a.u0[c]) / a.quant;
+    const float oy = (a.v0[p] - a.v0[c]) / a.quant;
+    const float oz = (a.w0[p] - a.w0[c]) / a.quant;
+    for (int q = t; q < B::ZT; q += B::BLOCK) {
+        float dy = (float)(q - LEN) + oy;
+        float dx = (float)(q - LEN) + ox;
+        float dz = (float)(q - LEN) + oz;
+        zy[q] = dy * dy;
+        zx[q] = dx * dx;
+        zz[q] = dz * dz;
     }
-    for (int l = threadIdx.x; l < len3; l += blockDim.x) cost[pb + l] = C[l];
+    __syncthreads();
+    dt_pass<LEN, 0>(A, Bf, zy, nullptr, I1, nullptr);
+    __syncthreads();
+    dt_pass<LEN, 1>(Bf, A, zx, I1, I2, nullptr);
+    __syncthreads();
+    float mn = dt_pass<LEN, 2>(A, Bf, zz, I2, nullptr, a.allinds + (long long)c * B::L3);
+    mn = block_min(mn, red);  // its barriers also complete Bf
+    for (int l = t; l < B::L3; l += B::BLOCK) C[l] += (Bf[l] - mn);
+    __syncthreads();
+}
+
+template <int LEN>
+__global__ __launch_bounds__(BPShape<LEN>::BLOCK) void k_bp_chain(BPArgs a, int head_begin) {
+    using B = BPShape<LEN>;
+    extern __shared__ float smem[];
+    __shared__ float red[16];
+    float* A = smem;
+    float* Bf = A + B::L3;
+    float* C = Bf + B::L3;
+    int16_t* I1 = (int16_t*)(C + B::L3);
+    int16_t* I2 = I1 + B::L3E;
+    float* zy = (float*)(I2 + B::L3E);
+    float* zx = zy + B::ZT;
+    float* zz = zx + B::ZT;
+    const int t = threadIdx.x;
+
+    int x = a.heads[head_begin + blockIdx.x];
+    long long xb = (long long)x * B::L3;
+    for (int l = t; l < B::L3; l += B::BLOCK) C[l] = a.cost[xb + l];
+    // fold every child of the head, in `ordered` order
+    for (int ci = a.child_start[x]; ci < a.child_start[x + 1]; ci++) {
+        const int c = a.child_list[ci];
+        const float w = a.edgemst[c];
+        const long long cb = (long long)c * B::L3;
+        for (int l = t; l < B::L3; l += B::BLOCK) A[l] = a.cost[cb + l] * w;
+        message_into<LEN>(a, c, x, A, Bf, C, I1, I2, zy, zx, zz, red);
+    }
+    // climb while the finished node is its parent's only child
+    while (x != a.root) {
+        const int q = a.parents[x];
+        if (a.child_start[q + 1] - a.child_start[q] != 1) break;
+        const float w = a.edgemst[x];
+        const long long qb = (long long)q * B::L3;
+        for (int l = t; l < B::L3; l += B::BLOCK) {
+            A[l] = C[l] * w;
+            C[l] = a.cost[qb + l];
+        }
+        message_into<LEN>(a, x, q, A, Bf, C, I1, I2, zy, zx, zz, red);
+        x = q;
+        xb = qb;
+    }
+    for (int l = t; l < B::L3; l += B::BLOCK) a.cost[xb + l] = C[l];
+}
+
+template <int LEN>
+void launch_round_len(const BPArgs& a, int head_begin, int count, hipStream_t s) {
+    using B = BPShape<LEN>;
+    constexpr size_t smem = B::SMEM + (size_t)B::L3 * 4;  // + the accumulator C
+    static bool attr_set = false;
+    if (!attr_set) {
+        DEEDS_HIP(hipFuncSetAttribute((const void*)k_bp_chain<LEN>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)smem));
+        attr_set = true;
+    }
+    k_bp_chain<LEN><<<count, B::BLOCK, smem, s>>>(a, head_begin);
+    DEEDS_LAUNCH_CHECK();
 }
 
 // Root argmin (first strict minimum), then label selection depth by depth
@@ -189,25 +271,27 @@ __global__ __launch_bounds__(1024) void k_bp_select(const float* __restrict__ co
 
 }  // namespace
 
-size_t bp_level_smem(int len) {
-    size_t len3 = (size_t)len * len * len;
-    return 3 * len3 * sizeof(float) + 2 * ((len3 + 1) & ~(size_t)1) * sizeof(int16_t) +
-           3 * (2 * len + 1) * sizeof(float);
-}
-
-void bp_set_smem_limit() {
-    int len = 2 * kMaxBPRadius + 1;
-    DEEDS_HIP(hipFuncSetAttribute((const void*)k_bp_level, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)bp_level_smem(len)));
-}
-
-void launch_bp_level(float* cost, int16_t* allinds, const BPDevTree& t, int grp_begin, int grp_count,
-                     const float* u0, const float* v0, const float* w0, int len, float quant, hipStream_t s) {
-    if (grp_count <= 0) return;
-    size_t smem = bp_level_smem(len);
-    k_bp_level<<<grp_count, BP_THREADS, smem, s>>>(cost, allinds, t.grp_parent, t.grp_child, t.child_list,
-                                                   t.edgemst, u0, v0, w0, grp_begin, len, quant);
-    DEEDS_LAUNCH_CHECK();
+void launch_bp_rounds(float* cost, int16_t* allinds, const BPDevTree& t, const int* round_start, int rounds,
+                      const float* u0, const float* v0, const float* w0, int hw, float quant, hipStream_t s) {
+    BPArgs a{cost, allinds, t.heads, t.parents, t.child_start, t.child_list, t.edgemst, u0, v0, w0, quant, t.root};
+    for (int r = 0; r < rounds; r++) {
+        const int b = round_start[r], n = round_start[r + 1] - b;
+        if (n <= 0) continue;
+        switch (2 * hw + 1) {
+            case 1: launch_round_len<1>(a, b, n, s); break;
+            case 3: launch_round_len<3>(a, b, n, s); break;
+            case 5: launch_round_len<5>(a, b, n, s); break;
+            case 7: launch_round_len<7>(a, b, n, s); break;
+            case 9: launch_round_len<9>(a, b, n, s); break;
+            case 11: launch_round_len<11>(a, b, n, s); break;
+            case 13: launch_round_len<13>(a, b, n, s); break;
+            case 15: launch_round_len<15>(a, b, n, s); break;
+            case 17: launch_round_len<17>(a, b, n, s); break;
+            case 19: launch_round_len<19>(a, b, n, s); break;
+            case 21: launch_round_len<21>(a, b, n, s); break;
+            default: fail(DEEDS_ERR_ARG, "search radius outside [0, 10]");
+        }
+    }
 }
 
 void launch_bp_select(const float* cost, const int16_t* allinds, const BPDevTree& t, int N, int maxd,

@@ -32,21 +32,20 @@ void launch_edge_sad(float* ec, const float* im, int step, int m, int n, int o, 
 
 // ---- belief propagation (bp.hip) ------------------------------------------
 struct BPDevTree {
-    const int* ordered;     // N
-    const int* parents;     // N
-    const int* depth_start; // maxd+2
-    const int* grp_parent;  // G
-    const int* grp_child;   // G+1 (CSR offsets into child_list)
-    const int* child_list;  // N-1
-    const float* edgemst;   // N
+    const int* ordered;      // N
+    const int* parents;      // N
+    const int* depth_start;  // maxd+2
+    const int* child_start;  // N+1, CSR offsets into child_list
+    const int* child_list;   // N-1, children of each node in `ordered` order
+    const int* heads;        // chain heads grouped by round (see mst_host.h)
+    const float* edgemst;    // N
+    int root;
 };
-size_t bp_level_smem(int len);
-// Largest label radius the LDS-resident BP kernel supports (len^3 floats x3 + int16 x2 in LDS).
+// Largest label radius the LDS-resident BP kernel supports (2 len^3 floats + 2 len^3 int16 in LDS).
 constexpr int kMaxBPRadius = 10;
-// Raise the BP kernel's dynamic LDS limit (call once per process before launches).
-void bp_set_smem_limit();
-void launch_bp_level(float* cost, int16_t* allinds, const BPDevTree& t, int grp_begin, int grp_count,
-                     const float* u0, const float* v0, const float* w0, int len, float quant, hipStream_t s);
+// Bottom-up pass: one launch per round; round r covers heads[round_start[r] .. round_start[r+1]).
+void launch_bp_rounds(float* cost, int16_t* allinds, const BPDevTree& t, const int* round_start, int rounds,
+                      const float* u0, const float* v0, const float* w0, int hw, float quant, hipStream_t s);
 void launch_bp_select(const float* cost, const int16_t* allinds, const BPDevTree& t, int N, int maxd,
                       const float* u0, const float* v0, const float* w0, float* u1, float* v1, float* w1,
                       int* sel, int hw, float quant, hipStream_t s);

@@ -12,20 +12,80 @@
 #include <cmath>
 #include <cstdint>
 #include <stdexcept>
-#include <unordered_map>
 
 namespace deeds {
 
 namespace {
 
-struct HeapEdge {
-    double weight;  // the reference stores the float weight in a double
-    int from;
-    int to;
-};
-// Edge::operator< of primsMST.h:24-26
-struct LighterFirst {
-    bool operator()(const HeapEdge& a, const HeapEdge& b) const { return a.weight > b.weight; }
+// Lazy Prim heap with exactly the sift rules of the C++ standard library's
+// std::push_heap / std::pop_heap (libstdc++ __push_heap / __adjust_heap), the
+// containers the reference's Prim's uses, ordered by the reference's
+// Edge::operator< (primsMST.h:24-26: a < b iff a.weight > b.weight, so the
+// top is the lightest edge).  Ties among equal weights resolve exactly as
+// in the reference only because the sequence of sifts is the same.  Storage is
+// split into a float key array (the reference's double weight holds a float,
+// so comparisons are identical) and an edge array, which keeps the sifts in
+// cache for grids of 10^5 nodes.
+class EdgeHeap {
+   public:
+    explicit EdgeHeap(size_t reserve) {
+        key_.reserve(reserve);
+        edge_.reserve(reserve);
+    }
+    bool empty() const { return key_.empty(); }
+    void push(float w, int from, int to) {
+        key_.push_back(w);
+        edge_.push_back(pack(from, to));
+        sift_up((long)key_.size() - 1, 0, w, edge_.back());
+    }
+    // removes the top (lightest) edge and returns it
+    void pop(int& from, int& to) {
+        unpack(edge_[0], from, to);
+        long n = (long)key_.size() - 1;
+        float vk = key_[n];
+        uint64_t ve = edge_[n];
+        key_[n] = key_[0];
+        edge_[n] = edge_[0];
+        // __adjust_heap(first, 0, n, value)
+        long hole = 0, child = 0;
+        while (child < (n - 1) / 2) {
+            child = 2 * (child + 1);
+            if (key_[child] > key_[child - 1]) child--;  // comp(child, child-1): child.w > (child-1).w
+            key_[hole] = key_[child];
+            edge_[hole] = edge_[child];
+            hole = child;
+        }
+        if ((n & 1) == 0 && child == (n - 2) / 2) {
+            child = 2 * (child + 1);
+            key_[hole] = key_[child - 1];
+            edge_[hole] = edge_[child - 1];
+            hole = child - 1;
+        }
+        sift_up(hole, 0, vk, ve);
+        key_.pop_back();
+        edge_.pop_back();
+    }
+
+   private:
+    static uint64_t pack(int a, int b) { return ((uint64_t)(uint32_t)a << 32) | (uint32_t)b; }
+    static void unpack(uint64_t e, int& a, int& b) {
+        a = (int)(e >> 32);
+        b = (int)(uint32_t)e;
+    }
+    // __push_heap(first, hole, top, value): move parents down while comp(parent, value)
+    void sift_up(long hole, long top, float vk, uint64_t ve) {
+        long parent = (hole - 1) / 2;
+        while (hole > top && key_[parent] > vk) {
+            key_[hole] = key_[parent];
+            edge_[hole] = edge_[parent];
+            hole = parent;
+            parent = (hole - 1) / 2;
+        }
+        key_[hole] = vk;
+        edge_[hole] = ve;
+    }
+    std::vector<float> key_;
+    std::vector<uint64_t> edge_;
 };
 
 const int kNbI[6] = {0, 0, -1, 1, 0, 0};
@@ -72,28 +132,22 @@ void build_tree(const float* ec, float sd, int step, int m1, int n1, int o1, Tre
     const int root = m1 / 2 + (n1 / 2) * m1 + (o1 / 2) * m1 * n1;
     std::vector<uint8_t> in_tree(N, 0);
     std::vector<int> depth(N, 0);
-    std::vector<HeapEdge> heap;
-    heap.reserve((size_t)4 * N + 16);
+    EdgeHeap heap((size_t)4 * N + 16);
     in_tree[root] = 1;
     int last = root;
-    LighterFirst cmp;
     for (int it = 0; it < N - 1; it++) {
         for (int nb = 0; nb < 6; nb++) {
             int v = neighbour(last, nb);
-            if (v >= 0) {
-                heap.push_back(HeapEdge{(double)w[last + (size_t)nb * N], last, v});
-                std::push_heap(heap.begin(), heap.end(), cmp);
-            }
+            if (v >= 0) heap.push(w[last + (size_t)nb * N], last, v);
         }
         int from = -1, to = -1;
         while (to < 0) {
             if (heap.empty()) throw std::runtime_error("prims: heap exhausted");
-            HeapEdge e = heap.front();
-            std::pop_heap(heap.begin(), heap.end(), cmp);
-            heap.pop_back();
-            bool a = in_tree[e.from], b = in_tree[e.to];
+            int ea, eb;
+            heap.pop(ea, eb);
+            bool a = in_tree[ea], b = in_tree[eb];
             if (a != b) {
-                if (a) { from = e.from; to = e.to; } else { from = e.to; to = e.from; }
+                if (a) { from = ea; to = eb; } else { from = eb; to = ea; }
             }
         }
         in_tree[to] = 1;
@@ -150,46 +204,46 @@ void build_schedule(Tree& t) {
     for (int p = 0; p < N; p++)
         if (depth[t.ordered[p]] < (p == 0 ? 0 : depth[t.ordered[p - 1]]))
             throw std::runtime_error("tree: `ordered` is not sorted by depth");
-    // groups of children per parent, per depth, in order of first appearance
-    t.grp_depth_start.assign(maxd + 1, 0);
-    t.grp_parent.clear();
-    t.grp_child.assign(1, 0);
-    t.child_list.clear();
-    t.child_list.reserve(N > 0 ? N - 1 : 0);
-    std::unordered_map<int, int> slot;
-    std::vector<int> counts;
-    for (int d = 1; d <= maxd; d++) {
-        int g0 = (int)t.grp_parent.size();
-        t.grp_depth_start[d - 1] = g0;
-        slot.clear();
-        counts.clear();
-        int s = t.depth_start[d], e = t.depth_start[d + 1];
-        for (int p = s; p < e; p++) {
-            int par = t.parents[t.ordered[p]];
-            auto it = slot.find(par);
-            if (it == slot.end()) {
-                slot.emplace(par, (int)counts.size());
-                t.grp_parent.push_back(par);
-                counts.push_back(1);
-            } else {
-                counts[it->second]++;
-            }
-        }
-        int base = (int)t.child_list.size();
-        std::vector<int> pos(counts.size());
-        for (size_t g = 0; g < counts.size(); g++) {
-            pos[g] = base;
-            base += counts[g];
-            t.grp_child.push_back(base);
-        }
-        t.child_list.resize(base);
-        for (int p = s; p < e; p++) {
-            int c = t.ordered[p];
-            int g = slot[t.parents[c]];
-            t.child_list[pos[g]++] = c;
+    // children of every node in `ordered` order, and the leaves, deepest first
+    t.child_start.assign(N + 1, 0);
+    for (int p = 1; p < N; p++) t.child_start[t.parents[t.ordered[p]] + 1]++;
+    for (int v = 0; v < N; v++) t.child_start[v + 1] += t.child_start[v];
+    t.child_list.assign(N > 0 ? N - 1 : 0, 0);
+    std::vector<int> fill(t.child_start.begin(), t.child_start.end() - 1);
+    for (int p = 1; p < N; p++) {
+        int c = t.ordered[p];
+        t.child_list[fill[t.parents[c]]++] = c;
+    }
+    // rounds of chain heads (a node's round is final once its children's are)
+    auto nch = [&](int v) { return t.child_start[v + 1] - t.child_start[v]; };
+    std::vector<int> round(N, 0);
+    std::vector<char> is_head(N, 0);
+    for (int p = N - 1; p >= 0; p--) {
+        int v = t.ordered[p];
+        int k = nch(v);
+        if (k == 0) continue;
+        int c0 = t.child_list[t.child_start[v]];
+        if (k == 1 && nch(c0) > 0) {
+            round[v] = round[c0];  // continues the chain of its only child
+        } else {
+            int r = 0;
+            for (int ci = t.child_start[v]; ci < t.child_start[v + 1]; ci++) r = std::max(r, round[t.child_list[ci]]);
+            round[v] = r + 1;
+            is_head[v] = 1;
         }
     }
-    if (maxd >= 1) t.grp_depth_start[maxd] = (int)t.grp_parent.size();
+    t.rounds = N > 0 ? round[t.ordered[0]] : 0;
+    t.round_start.assign(t.rounds + 1, 0);
+    for (int v = 0; v < N; v++)
+        if (is_head[v]) t.round_start[round[v]]++;  // count of round r stored at index r (r >= 1)
+    // exclusive prefix: heads of round r (1-based) occupy [round_start[r-1], round_start[r])
+    for (int r = 1; r <= t.rounds; r++) t.round_start[r] += t.round_start[r - 1];
+    t.heads.assign(t.round_start[t.rounds], 0);
+    std::vector<int> fill_r(t.round_start.begin(), t.round_start.end());
+    for (int p = 0; p < N; p++) {
+        int v = t.ordered[p];
+        if (is_head[v]) t.heads[fill_r[round[v] - 1]++] = v;
+    }
 }
 
 }  // namespace deeds

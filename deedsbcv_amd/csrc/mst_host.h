@@ -15,12 +15,17 @@ struct Tree {
     // schedule: depth_start[d] = first position of depth d in `ordered` (d = 0..maxd+1)
     int maxd = 0;
     std::vector<int> depth_start;
-    // bottom-up groups: for every depth d >= 1, the parents (depth d-1) with children
-    // at depth d; grp_depth_start[d-1 .. d] spans the groups of depth d.
-    std::vector<int> grp_depth_start;  // maxd + 1 entries (index 0 unused = 0)
-    std::vector<int> grp_parent;
-    std::vector<int> grp_child;        // CSR offsets into child_list (size G+1)
-    std::vector<int> child_list;       // children of each group, in `ordered` order
+    // children of every node in `ordered` order (CSR by node id)
+    std::vector<int> child_start;  // N + 1
+    std::vector<int> child_list;   // N - 1
+    // Bottom-up rounds.  A chain head is an internal node with several
+    // children or with one leaf child; from a head the chain climbs while the
+    // parent has a single child.  round(head) = 1 + max round of the chains
+    // ending at its children (leaves: 0), so a round only needs chain ends of
+    // earlier rounds.  heads are grouped by round (ordered order within one).
+    std::vector<int> heads;
+    std::vector<int> round_start;  // rounds + 1 offsets into heads
+    int rounds = 0;
 };
 
 // primsMST.h:101-110: float mean and std of the image, accumulated exactly as

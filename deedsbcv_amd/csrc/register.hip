@@ -26,7 +26,6 @@ size_t tree_ints_bound(int N);
 BPDevTree stage_tree(const Tree& t, void* pinned, void* d_stage, hipStream_t s);
 void run_bp(const Tree& t, const BPDevTree& dt, float* d_cost, int16_t* d_inds, int* d_sel, const float* u0,
             const float* v0, const float* w0, float* u1, float* v1, float* w1, int hw, float quant, hipStream_t s);
-void bp_prepare();
 
 namespace {
 
@@ -88,7 +87,6 @@ void register_pair(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, 
     const long long sz = (long long)m * n * o;
     const int L = S.levels;
     hipStream_t s0 = ctx->stream, s1 = ctx->aux_stream;
-    bp_prepare();
 
     // ---- schedule (deedsBCV0.cpp:183-187, :222-242) ----
     std::vector<LevelCfg> lv(L);
@@ -191,16 +189,21 @@ void register_pair(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, 
     };
     std::shared_future<float> sd_fut[2] = {std::async(std::launch::async, stats_task, 0).share(),
                                            std::async(std::launch::async, stats_task, 1).share()};
+    std::vector<double> build_time(2 * L, 0.0);
+    double wait_time = 0.0;
     std::vector<std::future<std::unique_ptr<Tree>>> trees[2];
     for (int l = 0; l < L; l++)
         for (int dir = 0; dir < 2; dir++) {
             const float* ec = h_ec.as<float>() + dir * ec_total + ec_off[l];
             const LevelCfg c = lv[l];
             std::shared_future<float> sdf = sd_fut[dir];
-            trees[dir].push_back(std::async(std::launch::async, [ec, c, sdf]() {
+            double* spent = &build_time[2 * l + dir];
+            trees[dir].push_back(std::async(std::launch::async, [ec, c, sdf, spent]() {
                 float sd = sdf.get();
+                auto tb = std::chrono::steady_clock::now();
                 std::unique_ptr<Tree> t(new Tree());
                 build_tree(ec, sd, c.step, c.m1, c.n1, c.o1, *t);
+                *spent = std::chrono::duration<double>(std::chrono::steady_clock::now() - tb).count();
                 return t;
             }));
         }
@@ -266,7 +269,9 @@ void register_pair(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, 
         // BP of both directions (regularisationCL, :276 / :303)
         for (int dir = 0; dir < 2; dir++) {
             hipStream_t s = dir ? s1 : s0;
+            auto tw = clock::now();
             std::unique_ptr<Tree> t = trees[dir][l].get();
+            wait_time += std::chrono::duration<double>(clock::now() - tw).count();
             tree_maxd[2 * l + dir] = t->maxd;
             size_t half = (stage_off[l + 1 < L ? l + 1 : l] - stage_off[l]) / 2;
             if (l + 1 >= L) half = (stage_total - stage_off[l]) / 2;
@@ -327,6 +332,7 @@ void register_pair(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, 
         DEEDS_HIP(hipEventElapsedTime(&tot, ev_start.e, ev_stop.e));
         timing->total = tot * 1e-3;
         timing->hamming_evals = hamming;
+        timing->tree_wait_s = wait_time;
         for (int l = 0; l < L; l++) {
             double lvl = 0.0;
             for (int dir = 0; dir < 2; dir++) {
@@ -337,6 +343,7 @@ void register_pair(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, 
             timing->datacost_level_s[l] = lvl;
             timing->datacost_level_evals[l] = 2.0 * lv[l].sz1 * lv[l].len3 * (double)lv[l].plan.samples;
             timing->datacost_kernel_s += lvl;
+            timing->tree_build_s += build_time[2 * l] + build_time[2 * l + 1];
             timing->datacost_launches += 2;
         }
         std::vector<double> hj(4 * (size_t)L);

@@ -70,6 +70,10 @@ typedef struct deeds_timing {
     int datacost_launches;
     double datacost_level_s[DEEDS_MAX_LEVELS];
     double datacost_level_evals[DEEDS_MAX_LEVELS];
+    /* host MST work: summed build time of all trees, and the time the
+     * launching thread spent waiting for trees that were not ready yet */
+    double tree_build_s;
+    double tree_wait_s;
 } deeds_timing;
 
 /* ---- library / context ------------------------------------------------- */
