@@ -158,6 +158,10 @@ def main():
         "stage_ms_fwd_stream": {k: 1e3 * v for k, v in stage.items()},
         "host_tree_build_ms": 1e3 * float(np.mean([t["tree_build_s"] for t in timings])),
         "host_tree_wait_ms": 1e3 * float(np.mean([t["tree_wait_s"] for t in timings])),
+        "host_tree_wait_level_ms": [1e3 * x for x in np.mean([t["tree_wait_level_s"] for t in timings], axis=0)],
+        "host_tree_build_level_ms": [1e3 * x for x in np.mean([t["tree_build_level_s"] for t in timings], axis=0)],
+        "host_stats_ms": 1e3 * float(np.mean([t["stats_s"] for t in timings])),
+        "host_cpus": len(os.sched_getaffinity(0)),
         "datacost_level_ms": [1e3 * x for x in np.mean([t["datacost_level_s"] for t in timings], axis=0)],
         "roofline": {
             "kernel": "k_datacost (Hamming cost volume)",

@@ -27,7 +27,9 @@ class Timing(ctypes.Structure):
                 ("datacost_kernel_s", ctypes.c_double), ("datacost_launches", c_int),
                 ("datacost_level_s", ctypes.c_double * MAX_LEVELS),
                 ("datacost_level_evals", ctypes.c_double * MAX_LEVELS),
-                ("tree_build_s", ctypes.c_double), ("tree_wait_s", ctypes.c_double)]
+                ("tree_build_s", ctypes.c_double), ("tree_wait_s", ctypes.c_double),
+                ("tree_build_level_s", ctypes.c_double * MAX_LEVELS),
+                ("tree_wait_level_s", ctypes.c_double * MAX_LEVELS), ("stats_s", ctypes.c_double)]
 
     def as_dict(self):
         L = self.levels
@@ -38,7 +40,9 @@ class Timing(ctypes.Structure):
                 "datacost_kernel_s": self.datacost_kernel_s, "datacost_launches": self.datacost_launches,
                 "datacost_level_s": list(self.datacost_level_s[:L]),
                 "datacost_level_evals": list(self.datacost_level_evals[:L]),
-                "tree_build_s": self.tree_build_s, "tree_wait_s": self.tree_wait_s}
+                "tree_build_s": self.tree_build_s, "tree_wait_s": self.tree_wait_s,
+                "tree_build_level_s": list(self.tree_build_level_s[:L]),
+                "tree_wait_level_s": list(self.tree_wait_level_s[:L]), "stats_s": self.stats_s}
 
 
 # name -> argtypes (all return int status unless listed in _RESTYPES)

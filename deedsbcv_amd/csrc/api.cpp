@@ -168,6 +168,8 @@ void check_mind(int m, int n, int o, int qs) {
     DEEDS_REQUIRE(qs >= 1, "MIND patch radius qs must be >= 1");
     DEEDS_REQUIRE(m >= 2 * qs + 1 && n >= 2 * qs + 1 && o >= 2 * qs + 1,
                   "each dimension must be >= 2*qs+1 (box filter window)");
+    DEEDS_REQUIRE(m <= mind_max_dim() && n <= mind_max_dim() && o <= mind_max_dim(),
+                  "a dimension exceeds the box filter's LDS tile limit");
 }
 
 void check_grid(int m, int n, int o, int step) {
@@ -196,8 +198,7 @@ int deeds_ctx_descriptor(deeds_ctx* ctx, uint64_t* d_mindq, const float* d_im, i
         size_t sz6 = (size_t)m * n * o * 6;
         float* w0 = ctx->get<float>("mind.w0", sz6);
         float* w1 = ctx->get<float>("mind.w1", sz6);
-        float* w2 = ctx->get<float>("mind.w2", sz6);
-        launch_descriptor(d_mindq, d_im, m, n, o, qs, w0, w1, w2, ctx->stream);
+        launch_descriptor(d_mindq, d_im, m, n, o, qs, w0, w1, ctx->stream);
     });
 }
 

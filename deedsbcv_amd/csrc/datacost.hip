@@ -28,6 +28,48 @@ __global__ void k_pad_codes(uint64_t* __restrict__ Mp, const uint64_t* __restric
     Mp[p] = M[i + (long long)j * m + (long long)k * m * n];
 }
 
+// Blocks are dispatched round-robin over the 8 XCDs; hand each XCD a
+// contiguous run of control points so neighbouring points, whose moving-code
+// neighbourhoods overlap almost entirely, share that XCD's L2.
+__device__ __forceinline__ int xcd_node(int b, int nodes) {
+    const int per = (nodes + 7) / 8;
+    return (b & 7) * per + (b >> 3);
+}
+
+// Sample loops unrolled at compile time (STEP and the skips are template
+// arguments), so every wave keeps the moving-code loads of a whole block in
+// flight instead of one at a time.
+template <int STEP, int SI, int SJ, int SK>
+__global__ __launch_bounds__(256) void k_datacost_t(float* __restrict__ cost, const uint64_t* __restrict__ F,
+                                                    const uint64_t* __restrict__ Mp, int m, int n, int m1, int n1,
+                                                    int nodes, int len, int q, float scale, int mp, int np) {
+    const int node = xcd_node(blockIdx.x, nodes);
+    if (node >= nodes) return;
+    const int y = node % m1;
+    const int x = (node / m1) % n1;
+    const int z = node / (m1 * n1);
+    const int len3 = len * len * len;
+    const long long mn = (long long)m * n, mpnp = (long long)mp * np;
+    const uint64_t* fblk = F + (long long)y * STEP + (long long)x * STEP * m + (long long)z * STEP * mn;
+    const uint64_t* mblk = Mp + (long long)y * STEP + (long long)x * STEP * mp + (long long)z * STEP * mpnp;
+    for (int l = threadIdx.x; l < len3; l += blockDim.x) {
+        const int ls = l % len, lx = (l / len) % len, lz = l / (len * len);
+        const uint64_t* mb = mblk + ls * q + (long long)(lx * q) * mp + (long long)(lz * q) * mpnp;
+        unsigned sum = 0;
+#pragma unroll
+        for (int k = 0; k < STEP; k += SK) {
+#pragma unroll
+            for (int j = 0; j < STEP; j += SJ) {
+                const uint64_t* fr = fblk + k * mn + (long long)j * m;
+                const uint64_t* mr = mb + k * mpnp + (long long)j * mp;
+#pragma unroll
+                for (int i = 0; i < STEP; i += SI) sum += __builtin_popcountll(fr[i] ^ mr[i]);
+            }
+        }
+        cost[(long long)node * len3 + l] = (float)sum * scale;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_datacost(float* __restrict__ cost, const uint64_t* __restrict__ F,
                                                   const uint64_t* __restrict__ Mp, int m, int n, int m1, int n1,
                                                   int step, int len, int q, int si, int sj, int sk, float scale,
@@ -95,6 +137,19 @@ void launch_datacost(float* cost, const uint64_t* F, const uint64_t* Mp, int m, 
     long long nodes = (long long)m1 * n1 * o1;
     if (nodes == 0) return;
     int len = 2 * hw + 1;
+    const unsigned grid = (unsigned)(8 * ((nodes + 7) / 8));
+    const int N = (int)nodes;
+#define DEEDS_DC(ST, SI, SJ, SK)                                                                              \
+    if (step == ST && plan.skip_i == SI && plan.skip_j == SJ && plan.skip_k == SK) {                        \
+        k_datacost_t<ST, SI, SJ, SK><<<grid, 256, 0, s>>>(cost, F, Mp, m, n, m1, n1, N, len, q, plan.scale, \
+                                                          plan.mp, plan.np);                                \
+        DEEDS_LAUNCH_CHECK();                                                                               \
+        return;                                                                                             \
+    }
+    // the schedules of deedsBCV (RAND_SAMPLES = 1: skip 2 in j and k above step 4)
+    DEEDS_DC(8, 1, 2, 2) DEEDS_DC(7, 1, 2, 2) DEEDS_DC(6, 1, 2, 2) DEEDS_DC(5, 1, 2, 2)
+    DEEDS_DC(4, 1, 1, 1) DEEDS_DC(3, 1, 1, 1) DEEDS_DC(2, 1, 1, 1)
+#undef DEEDS_DC
     k_datacost<<<(unsigned)nodes, 256, 0, s>>>(cost, F, Mp, m, n, m1, n1, step, len, q, plan.skip_i, plan.skip_j,
                                                plan.skip_k, plan.scale, plan.mp, plan.np);
     DEEDS_LAUNCH_CHECK();

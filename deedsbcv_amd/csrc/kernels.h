@@ -6,12 +6,17 @@
 
 namespace deeds {
 
+// LDS available to one workgroup on gfx950 (160 KiB per CU).
+constexpr int kMaxLds = 160 * 1024;
+
 // ---- MIND-SSC (mind.hip) --------------------------------------------------
 // The five quantisation thresholds -log((c+1.5)/6) in float.
 void mind_thresholds(float thr[5]);
-// Workspace: three buffers of 6*m*n*o floats.
+// Workspace: two buffers of 6*m*n*o floats.
 void launch_descriptor(uint64_t* out, const float* im, int m, int n, int o, int qs,
-                       float* w0, float* w1, float* w2, hipStream_t s);
+                       float* w0, float* w1, hipStream_t s);
+// Largest dimension the LDS-tiled box filter accepts.
+int mind_max_dim();
 
 // ---- Hamming data cost (datacost.hip) -------------------------------------
 struct CostPlan {

@@ -1,15 +1,18 @@
 // MIND-SSC descriptor on gfx950 (reference: MINDSSCbox.h:2-232).
 //
-// Pipeline per call, all in HBM-resident float planes (6 channels):
-//   k_ssd6      squared difference to the six shifted copies      (:79-90, :113-117)
-//   k_box_line  per-line sequential prefix sum + clamped window    (:2-76), once per axis
-//   k_mind_code 12 SSC values -> min/noise normalise -> 5-level    (:172-222)
-//               thermometer fields packed into a 64-bit code
-// The box filter must reproduce the reference's float rounding: each line is
-// one sequential prefix sum (values exceed 2^24, so a tree scan would change
-// bits), hence one thread per line.  Lines along j and k are coalesced across
-// the wavefront (consecutive lanes = consecutive i); lines along i are
-// per-lane sequential and served from L1/L2.
+// Pipeline per call (6 float channels in HBM):
+//   k_ssd_box_i   squared difference to the shifted image (:79-90, :113-117)
+//                 fused with the box pass along i (:2-29)
+//   k_box_cols    box pass along j, then along k (:31-73)
+//   k_mind_code   12 SSC values -> min/noise normalisation -> 6-level
+//                 thermometer, packed into a 64-bit code (:172-222)
+// The box filter must reproduce the reference's float rounding: every line is
+// one sequential float prefix sum (the sums exceed 2^24, so a tree scan would
+// change bits) followed by the clamped window difference.  Each workgroup
+// stages TILE lines in LDS with coalesced loads, one thread per line runs the
+// sequential prefix in LDS (the loads are independent of the add chain, so
+// they stay in flight), and all threads write the window differences back
+// coalesced.
 #include "kernels.h"
 
 #include <cmath>
@@ -18,6 +21,9 @@ namespace deeds {
 
 namespace {
 
+constexpr int TILE = 32;      // lines per workgroup
+constexpr int BOX_THREADS = 256;
+
 // shift of the six SSD channels in units of qs, as (di, dj, dk)
 __constant__ int c_ssd_shift[6][3] = {{+1, +1, 0}, {-1, +1, 0}, {0, -1, +1}, {-1, 0, +1}, {0, +1, +1}, {+1, 0, +1}};
 // twelve SSC pairs: neighbour offset (di, dj, dk) in units of qs and channel
@@ -25,60 +31,86 @@ __constant__ int c_ssc[12][4] = {{0, -1, 0, 0}, {-1, 0, 0, 0}, {0, -1, 0, 1}, {+
                                  {0, 0, -1, 2}, {0, +1, 0, 2}, {0, 0, -1, 3}, {+1, 0, 0, 3},
                                  {0, 0, -1, 4}, {0, -1, 0, 4}, {0, 0, -1, 5}, {-1, 0, 0, 5}};
 
-__global__ void k_ssd6(float* __restrict__ out, const float* __restrict__ im, int m, int n, int o, int qs) {
-    long long sz = (long long)m * n * o;
-    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= sz) return;
-    int i = (int)(p % m);
-    int j = (int)((p / m) % n);
-    int k = (int)(p / ((long long)m * n));
-    float c = im[p];
-#pragma unroll
-    for (int l = 0; l < 6; l++) {
-        int ii = i + c_ssd_shift[l][0] * qs, jj = j + c_ssd_shift[l][1] * qs, kk = k + c_ssd_shift[l][2] * qs;
-        bool in = ii >= 0 && ii < m && jj >= 0 && jj < n && kk >= 0 && kk < o;
-        float s = in ? im[ii + (long long)jj * m + (long long)kk * m * n] : c;
-        float d = s - c;
-        out[l * sz + p] = d * d;
+// Sequential prefix sum of `L` values at p[0], p[stride], ... in place.
+__device__ __forceinline__ void prefix_in_place(float* p, int L, int stride) {
+    float acc = p[0];
+    int t = 1;
+    for (; t + 4 <= L; t += 4) {
+        float x0 = p[t * stride], x1 = p[(t + 1) * stride], x2 = p[(t + 2) * stride], x3 = p[(t + 3) * stride];
+        acc = acc + x0;
+        p[t * stride] = acc;
+        acc = acc + x1;
+        p[(t + 1) * stride] = acc;
+        acc = acc + x2;
+        p[(t + 2) * stride] = acc;
+        acc = acc + x3;
+        p[(t + 3) * stride] = acc;
+    }
+    for (; t < L; t++) {
+        acc = acc + p[t * stride];
+        p[t * stride] = acc;
     }
 }
 
-// One thread per (line, channel).  axis 0: lines along i (stride 1),
-// axis 1: along j (stride m), axis 2: along k (stride m*n).  The prefix sums
-// go to `pre`, the windowed differences to `dst`.
-__global__ void k_box_line(float* __restrict__ dst, float* __restrict__ pre, const float* __restrict__ src,
-                           int m, int n, int o, int axis, int hw) {
-    long long sz = (long long)m * n * o;
-    long long lines = axis == 0 ? (long long)n * o : axis == 1 ? (long long)m * o : (long long)m * n;
-    long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid >= lines * 6) return;
-    int ch = (int)(tid / lines);
-    long long line = tid - ch * lines;
-    long long base, stride;
-    int L;
-    if (axis == 0) {
-        base = line * m; stride = 1; L = m;
-    } else if (axis == 1) {
-        long long i = line % m, k = line / m;
-        base = i + k * (long long)m * n; stride = m; L = n;
-    } else {
-        base = line; stride = (long long)m * n; L = o;
+// Clamped window of the prefix sums P (MINDSSCbox.h:17-27): P[min(t+hw,L-1)] - P[t-hw-1], second term absent for t <= hw.
+__device__ __forceinline__ float window(const float* P, int t, int L, int hw, int stride) {
+    int hi = t + hw < L - 1 ? t + hw : L - 1;
+    float v = P[hi * stride];
+    if (t > hw) v = v - P[(t - hw - 1) * stride];
+    return v;
+}
+
+// i pass: tile = TILE rows (j0 .. j0+TILE-1 at fixed k) of channel ch; LDS row stride m+1.
+__global__ __launch_bounds__(BOX_THREADS) void k_ssd_box_i(float* __restrict__ out, const float* __restrict__ im,
+                                                            int m, int n, int o, int qs) {
+    extern __shared__ float tile[];
+    const int j0 = blockIdx.x * TILE, k = blockIdx.y, ch = blockIdx.z;
+    const int rows = min(TILE, n - j0);
+    const int ld = m + 1;
+    const long long mn = (long long)m * n;
+    const int di = c_ssd_shift[ch][0] * qs, dj = c_ssd_shift[ch][1] * qs, dk = c_ssd_shift[ch][2] * qs;
+    for (int idx = threadIdx.x; idx < rows * m; idx += blockDim.x) {
+        const int r = idx / m, i = idx - r * m, j = j0 + r;
+        const long long p = i + (long long)j * m + k * mn;
+        const float c = im[p];
+        const int ii = i + di, jj = j + dj, kk = k + dk;
+        const bool in = ii >= 0 && ii < m && jj >= 0 && jj < n && kk >= 0 && kk < o;
+        const float s = in ? im[ii + (long long)jj * m + kk * mn] : c;
+        const float d = s - c;
+        tile[r * ld + i] = d * d;
     }
-    base += ch * sz;
-    const float* x = src + base;
-    float* P = pre + base;
-    float* y = dst + base;
-    float acc = x[0];
-    P[0] = acc;
-    for (int t = 1; t < L; t++) {
-        acc = acc + x[t * stride];
-        P[t * stride] = acc;
+    __syncthreads();
+    if ((int)threadIdx.x < rows) prefix_in_place(tile + threadIdx.x * ld, m, 1);
+    __syncthreads();
+    float* dst = out + (long long)ch * mn * o;
+    for (int idx = threadIdx.x; idx < rows * m; idx += blockDim.x) {
+        const int r = idx / m, i = idx - r * m;
+        dst[i + (long long)(j0 + r) * m + k * mn] = window(tile + r * ld, i, m, qs, 1);
     }
-    for (int t = 0; t < L; t++) {
-        int hi = t + hw < L - 1 ? t + hw : L - 1;
-        float v = P[hi * stride];
-        if (t > hw) v = v - P[(t - hw - 1) * stride];
-        y[t * stride] = v;
+}
+
+// j pass (axis 1) or k pass (axis 2): tile = TILE consecutive i at fixed
+// (k) resp. (j), lines along the axis; LDS layout [t][r] with stride TILE+1.
+__global__ __launch_bounds__(BOX_THREADS) void k_box_cols(float* __restrict__ dst, const float* __restrict__ src,
+                                                           int m, int n, int o, int axis, int hw) {
+    extern __shared__ float tile[];
+    const int i0 = blockIdx.x * TILE, other = blockIdx.y, ch = blockIdx.z;
+    const int cols = min(TILE, m - i0);
+    const long long mn = (long long)m * n;
+    const int L = axis == 1 ? n : o;
+    const long long stride = axis == 1 ? (long long)m : mn;
+    const long long base = (long long)ch * mn * o + i0 + (axis == 1 ? other * mn : (long long)other * m);
+    constexpr int ld = TILE + 1;
+    for (int idx = threadIdx.x; idx < L * TILE; idx += blockDim.x) {
+        const int t = idx / TILE, r = idx - t * TILE;
+        if (r < cols) tile[t * ld + r] = src[base + t * stride + r];
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < cols) prefix_in_place(tile + threadIdx.x, L, ld);
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < L * TILE; idx += blockDim.x) {
+        const int t = idx / TILE, r = idx - t * TILE;
+        if (r < cols) dst[base + t * stride + r] = window(tile + r, t, L, hw, ld);
     }
 }
 
@@ -125,7 +157,12 @@ __global__ void k_mind_code(uint64_t* __restrict__ out, const float* __restrict_
     out[p] = code;
 }
 
+size_t rows_smem(int m) { return (size_t)TILE * (m + 1) * sizeof(float); }
+size_t cols_smem(int L) { return (size_t)L * (TILE + 1) * sizeof(float); }
+
 }  // namespace
+
+int mind_max_dim() { return (int)(kMaxLds / ((TILE + 1) * sizeof(float))) - 1; }
 
 void mind_thresholds(float thr[5]) {
     // -log((c+1.5)/6) rounded once to float (the reference evaluates the float
@@ -135,22 +172,24 @@ void mind_thresholds(float thr[5]) {
 }
 
 void launch_descriptor(uint64_t* out, const float* im, int m, int n, int o, int qs, float* w0, float* w1,
-                       float* w2, hipStream_t s) {
+                       hipStream_t s) {
+    static bool attr = false;
+    if (!attr) {
+        DEEDS_HIP(hipFuncSetAttribute((const void*)k_ssd_box_i, hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds));
+        DEEDS_HIP(hipFuncSetAttribute((const void*)k_box_cols, hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds));
+        attr = true;
+    }
     long long sz = (long long)m * n * o;
-    const int B = 256;
-    k_ssd6<<<ceil_div(sz, B), B, 0, s>>>(w0, im, m, n, o, qs);
+    // i pass (with the SSD): im -> w0; j pass: w0 -> w1; k pass: w1 -> w0
+    k_ssd_box_i<<<dim3(ceil_div(n, TILE), o, 6), BOX_THREADS, rows_smem(m), s>>>(w0, im, m, n, o, qs);
     DEEDS_LAUNCH_CHECK();
-    // i pass: w0 -> w1, j pass: w1 -> w0, k pass: w0 -> w1 (w2 holds prefix sums)
-    long long li = (long long)n * o * 6, lj = (long long)m * o * 6, lk = (long long)m * n * 6;
-    k_box_line<<<ceil_div(li, 64), 64, 0, s>>>(w1, w2, w0, m, n, o, 0, qs);
+    k_box_cols<<<dim3(ceil_div(m, TILE), o, 6), BOX_THREADS, cols_smem(n), s>>>(w1, w0, m, n, o, 1, qs);
     DEEDS_LAUNCH_CHECK();
-    k_box_line<<<ceil_div(lj, B), B, 0, s>>>(w0, w2, w1, m, n, o, 1, qs);
-    DEEDS_LAUNCH_CHECK();
-    k_box_line<<<ceil_div(lk, B), B, 0, s>>>(w1, w2, w0, m, n, o, 2, qs);
+    k_box_cols<<<dim3(ceil_div(m, TILE), n, 6), BOX_THREADS, cols_smem(o), s>>>(w0, w1, m, n, o, 2, qs);
     DEEDS_LAUNCH_CHECK();
     Thr thr;
     mind_thresholds(thr.t);
-    k_mind_code<<<ceil_div(sz, B), B, 0, s>>>(out, w1, m, n, o, qs, thr);
+    k_mind_code<<<ceil_div(sz, 256), 256, 0, s>>>(out, w0, m, n, o, qs, thr);
     DEEDS_LAUNCH_CHECK();
 }
 

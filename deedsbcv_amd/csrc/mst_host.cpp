@@ -26,66 +26,59 @@ namespace {
 // split into a float key array (the reference's double weight holds a float,
 // so comparisons are identical) and an edge array, which keeps the sifts in
 // cache for grids of 10^5 nodes.
+// An entry is the float weight and the edge coded as from*8 + neighbour slot.
 class EdgeHeap {
    public:
-    explicit EdgeHeap(size_t reserve) {
-        key_.reserve(reserve);
-        edge_.reserve(reserve);
+    struct Entry {
+        float key;
+        uint32_t code;
+    };
+    explicit EdgeHeap(size_t reserve) { h_.reserve(reserve); }
+    bool empty() const { return h_.empty(); }
+    void push(float w, uint32_t code) {
+        h_.push_back(Entry{w, code});
+        sift_up((long)h_.size() - 1, 0, h_.back());
     }
-    bool empty() const { return key_.empty(); }
-    void push(float w, int from, int to) {
-        key_.push_back(w);
-        edge_.push_back(pack(from, to));
-        sift_up((long)key_.size() - 1, 0, w, edge_.back());
-    }
-    // removes the top (lightest) edge and returns it
-    void pop(int& from, int& to) {
-        unpack(edge_[0], from, to);
-        long n = (long)key_.size() - 1;
-        float vk = key_[n];
-        uint64_t ve = edge_[n];
-        key_[n] = key_[0];
-        edge_[n] = edge_[0];
-        // __adjust_heap(first, 0, n, value)
+    // removes the top (lightest) edge and returns its code
+    uint32_t pop() {
+        const uint32_t top = h_[0].code;
+        const long n = (long)h_.size() - 1;
+        const Entry v = h_[n];
+        h_[n] = h_[0];
+        // __adjust_heap(first, 0, n, value): the hole walks down along the
+        // child that is not "less" (comp(child, child-1) <=> child.w > (child-1).w)
+        Entry* h = h_.data();
         long hole = 0, child = 0;
         while (child < (n - 1) / 2) {
             child = 2 * (child + 1);
-            if (key_[child] > key_[child - 1]) child--;  // comp(child, child-1): child.w > (child-1).w
-            key_[hole] = key_[child];
-            edge_[hole] = edge_[child];
+            __builtin_prefetch(h + 2 * (child + 1) - 1);
+            if (h[child].key > h[child - 1].key) child--;
+            h[hole] = h[child];
             hole = child;
         }
         if ((n & 1) == 0 && child == (n - 2) / 2) {
             child = 2 * (child + 1);
-            key_[hole] = key_[child - 1];
-            edge_[hole] = edge_[child - 1];
+            h[hole] = h[child - 1];
             hole = child - 1;
         }
-        sift_up(hole, 0, vk, ve);
-        key_.pop_back();
-        edge_.pop_back();
+        sift_up(hole, 0, v);
+        h_.pop_back();
+        return top;
     }
 
    private:
-    static uint64_t pack(int a, int b) { return ((uint64_t)(uint32_t)a << 32) | (uint32_t)b; }
-    static void unpack(uint64_t e, int& a, int& b) {
-        a = (int)(e >> 32);
-        b = (int)(uint32_t)e;
-    }
     // __push_heap(first, hole, top, value): move parents down while comp(parent, value)
-    void sift_up(long hole, long top, float vk, uint64_t ve) {
+    void sift_up(long hole, long top, Entry v) {
+        Entry* h = h_.data();
         long parent = (hole - 1) / 2;
-        while (hole > top && key_[parent] > vk) {
-            key_[hole] = key_[parent];
-            edge_[hole] = edge_[parent];
+        while (hole > top && h[parent].key > v.key) {
+            h[hole] = h[parent];
             hole = parent;
             parent = (hole - 1) / 2;
         }
-        key_[hole] = vk;
-        edge_[hole] = ve;
+        h[hole] = v;
     }
-    std::vector<float> key_;
-    std::vector<uint64_t> edge_;
+    std::vector<Entry> h_;
 };
 
 const int kNbI[6] = {0, 0, -1, 1, 0, 0};
@@ -135,16 +128,19 @@ void build_tree(const float* ec, float sd, int step, int m1, int n1, int o1, Tre
     EdgeHeap heap((size_t)4 * N + 16);
     in_tree[root] = 1;
     int last = root;
+    // linear offset of each neighbour slot (valid wherever the neighbour exists)
+    const int delta[6] = {-m1, m1, -1, 1, -m1 * n1, m1 * n1};
+    if ((long long)N * 8 >= (1LL << 32)) throw std::runtime_error("prims: grid too large");
     for (int it = 0; it < N - 1; it++) {
         for (int nb = 0; nb < 6; nb++) {
             int v = neighbour(last, nb);
-            if (v >= 0) heap.push(w[last + (size_t)nb * N], last, v);
+            if (v >= 0) heap.push(w[last + (size_t)nb * N], (uint32_t)last * 8u + (uint32_t)nb);
         }
         int from = -1, to = -1;
         while (to < 0) {
             if (heap.empty()) throw std::runtime_error("prims: heap exhausted");
-            int ea, eb;
-            heap.pop(ea, eb);
+            const uint32_t code = heap.pop();
+            const int ea = (int)(code >> 3), eb = ea + delta[code & 7u];
             bool a = in_tree[ea], b = in_tree[eb];
             if (a != b) {
                 if (a) { from = ea; to = eb; } else { from = eb; to = ea; }

@@ -111,6 +111,8 @@ void register_pair(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, 
     }
     DEEDS_REQUIRE(m >= 2 * max_qs + 1 && n >= 2 * max_qs + 1 && o >= 2 * max_qs + 1,
                   "volume too small for the MIND patch radius of the schedule");
+    DEEDS_REQUIRE(m <= mind_max_dim() && n <= mind_max_dim() && o <= mind_max_dim(),
+                  "a dimension exceeds the box filter's LDS tile limit");
 
     // ---- device workspace ----
     float* im1 = ctx->get<float>("r.im1", sz);
@@ -120,9 +122,9 @@ void register_pair(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, 
     uint64_t* mind_fix = ctx->get<uint64_t>("r.mind.fix", sz);
     float* warped[2] = {ctx->get<float>("r.warped.0", sz), ctx->get<float>("r.warped.1", sz)};
     uint64_t* mind_w[2] = {ctx->get<uint64_t>("r.mindw.0", sz), ctx->get<uint64_t>("r.mindw.1", sz)};
-    float* mw[2][3];
+    float* mw[2][2];
     for (int d = 0; d < 2; d++)
-        for (int b = 0; b < 3; b++)
+        for (int b = 0; b < 2; b++)
             mw[d][b] = ctx->get<float>("r.mindtmp." + std::to_string(d) + std::to_string(b), 6 * sz);
     uint64_t* mpad[2] = {ctx->get<uint64_t>("r.mpad.0", max_pad), ctx->get<uint64_t>("r.mpad.1", max_pad)};
     float* cost[2] = {ctx->get<float>("r.cost.0", max_cost), ctx->get<float>("r.cost.1", max_cost)};
@@ -181,16 +183,20 @@ void register_pair(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, 
     DEEDS_HIP(hipEventRecord(ev_sad.e, s0));
 
     hipEvent_t sad_event = ev_sad.e;
+    double stats_time[2] = {0.0, 0.0};
     auto stats_task = [&, sad_event](int dir) {
         DEEDS_HIP(hipEventSynchronize(sad_event));
+        auto ts = std::chrono::steady_clock::now();
         float mean = 0, sd = 0;
         image_stats(h_img.as<float>() + dir * sz, sz, &mean, &sd);
+        stats_time[dir] = std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count();
         return sd;
     };
     std::shared_future<float> sd_fut[2] = {std::async(std::launch::async, stats_task, 0).share(),
                                            std::async(std::launch::async, stats_task, 1).share()};
     std::vector<double> build_time(2 * L, 0.0);
     double wait_time = 0.0;
+    std::vector<double> wait_level(L, 0.0);
     std::vector<std::future<std::unique_ptr<Tree>>> trees[2];
     for (int l = 0; l < L; l++)
         for (int dir = 0; dir < 2; dir++) {
@@ -256,7 +262,7 @@ void register_pair(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, 
             else
                 launch_warp_image_grid(warped[1], im1b, u0, u0 + n1, u0 + 2 * n1, m, n, o, c.m1, c.n1, c.o1, s);
             if (dir == 0) DEEDS_HIP(hipEventRecord(EV(l, 1), s));
-            launch_descriptor(mind_w[dir], warped[dir], m, n, o, c.mind_qs, mw[dir][0], mw[dir][1], mw[dir][2], s);
+            launch_descriptor(mind_w[dir], warped[dir], m, n, o, c.mind_qs, mw[dir][0], mw[dir][1], s);
             if (dir == 0) DEEDS_HIP(hipEventRecord(EV(l, 2), s));
             launch_pad_codes(mpad[dir], mind_w[dir], m, n, o, c.plan.pad, s);
             DEEDS_HIP(hipEventRecord(DC(l, dir, 0), s));
@@ -271,7 +277,9 @@ void register_pair(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, 
             hipStream_t s = dir ? s1 : s0;
             auto tw = clock::now();
             std::unique_ptr<Tree> t = trees[dir][l].get();
-            wait_time += std::chrono::duration<double>(clock::now() - tw).count();
+            double waited = std::chrono::duration<double>(clock::now() - tw).count();
+            wait_time += waited;
+            wait_level[l] += waited;
             tree_maxd[2 * l + dir] = t->maxd;
             size_t half = (stage_off[l + 1 < L ? l + 1 : l] - stage_off[l]) / 2;
             if (l + 1 >= L) half = (stage_total - stage_off[l]) / 2;
@@ -333,6 +341,7 @@ void register_pair(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, 
         timing->total = tot * 1e-3;
         timing->hamming_evals = hamming;
         timing->tree_wait_s = wait_time;
+        timing->stats_s = std::max(stats_time[0], stats_time[1]);
         for (int l = 0; l < L; l++) {
             double lvl = 0.0;
             for (int dir = 0; dir < 2; dir++) {
@@ -344,6 +353,8 @@ void register_pair(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, 
             timing->datacost_level_evals[l] = 2.0 * lv[l].sz1 * lv[l].len3 * (double)lv[l].plan.samples;
             timing->datacost_kernel_s += lvl;
             timing->tree_build_s += build_time[2 * l] + build_time[2 * l + 1];
+            timing->tree_build_level_s[l] = std::max(build_time[2 * l], build_time[2 * l + 1]);
+            timing->tree_wait_level_s[l] = wait_level[l];
             timing->datacost_launches += 2;
         }
         std::vector<double> hj(4 * (size_t)L);

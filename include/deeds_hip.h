@@ -74,6 +74,9 @@ typedef struct deeds_timing {
      * launching thread spent waiting for trees that were not ready yet */
     double tree_build_s;
     double tree_wait_s;
+    double tree_build_level_s[DEEDS_MAX_LEVELS];  /* per level: max of the two directions' build times */
+    double tree_wait_level_s[DEEDS_MAX_LEVELS];   /* per level: host wait for that level's trees */
+    double stats_s;                               /* host image mean/std (both images, concurrent) */
 } deeds_timing;
 
 /* ---- library / context ------------------------------------------------- */
