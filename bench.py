@@ -92,6 +92,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
     ap.add_argument("--cpu-budget", type=float, default=240.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the timed steps, copy a 1 GiB device buffer once (a known-size kernel that "
+                         "calibrates HBM byte counters in a profiler run)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the registration path runs on a single device "
@@ -130,6 +133,12 @@ def main():
     fl = flow.cpu().numpy()
     if not np.all(np.isfinite(fl)):
         raise SystemExit("non-finite displacement field")
+    if a.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)  # 1 GiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     evals = timings[0]["hamming_evals"]
     dc_s = sum(t["datacost_kernel_s"] for t in timings)

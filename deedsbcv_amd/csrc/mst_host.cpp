@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstring>
 #include <stdexcept>
 
 namespace deeds {
@@ -22,11 +23,10 @@ namespace {
 // containers the reference's Prim's uses, ordered by the reference's
 // Edge::operator< (primsMST.h:24-26: a < b iff a.weight > b.weight, so the
 // top is the lightest edge).  Ties among equal weights resolve exactly as
-// in the reference only because the sequence of sifts is the same.  Storage is
-// split into a float key array (the reference's double weight holds a float,
-// so comparisons are identical) and an edge array, which keeps the sifts in
-// cache for grids of 10^5 nodes.
-// An entry is the float weight and the edge coded as from*8 + neighbour slot.
+// in the reference only because the sequence of sifts is the same.  An entry
+// is 8 bytes, the float key (the reference's double weight holds a float, so
+// comparisons are identical) and the edge coded as from*8 + neighbour slot,
+// which keeps the sifts in cache for grids of 10^5 nodes.
 class EdgeHeap {
    public:
     struct Entry {
@@ -100,21 +100,102 @@ void image_stats(const float* im, long long count, float* mean, float* sd) {
     *sd = std::sqrt(var / (float)count);
 }
 
+namespace {
+
+// edge cost per voxel pair, c = SAD / step^3 (primsMST.h:112-114)
+std::vector<float> edge_costs(const float* ec, int step, size_t count) {
+    const float vol = (float)((double)step * step * step);
+    std::vector<float> c(count);
+    for (size_t e = 0; e < count; e++) c[e] = ec[e] / vol;
+    return c;
+}
+
+// the reference's weight of an edge of cost c (primsMST.h:48-50, :115-117)
+inline float edge_weight(float c, float two_sd) { return -std::exp(-c / two_sd); }
+
+// Prim's with the given keys (smaller key = lighter edge), then the BFS
+// order; records the neighbour slot of every node's edge to its parent with
+// the reference's last-match scan (primsMST.h:198-211).
+void prim_tree(const float* key, int m1, int n1, int o1, Tree& t);
+
+}  // namespace
+
+bool weights_distinguish_costs(const float* ec, int step, int m1, int n1, int o1, float sd) {
+    // Prim's only compares weights.  w(c) = -exp(-c/(2 sd)) never decreases
+    // with c, so ordering by c equals ordering by w (ties included) exactly
+    // when distinct costs of the pushed (valid) edges get distinct weights.
+    const int N = m1 * n1 * o1;
+    std::vector<float> c = edge_costs(ec, step, (size_t)6 * N);
+    std::vector<uint32_t> bits;
+    bits.reserve((size_t)6 * N);
+    for (int v = 0; v < N; v++) {
+        int k = v / (m1 * n1), j = (v / m1) % n1, i = v % m1;
+        for (int nb = 0; nb < 6; nb++) {
+            int ii = i + kNbI[nb], jj = j + kNbJ[nb], kk = k + kNbK[nb];
+            if (ii < 0 || ii >= m1 || jj < 0 || jj >= n1 || kk < 0 || kk >= o1) continue;
+            uint32_t b;
+            std::memcpy(&b, &c[v + (size_t)nb * N], 4);
+            bits.push_back(b);  // costs are >= 0: the bit patterns sort like the values
+        }
+    }
+    // LSD radix sort on 11-bit digits
+    std::vector<uint32_t> tmp(bits.size());
+    for (int shift = 0; shift < 33; shift += 11) {
+        size_t cnt[2049] = {0};
+        for (uint32_t b : bits) cnt[((b >> shift) & 2047u) + 1]++;
+        for (int d = 0; d < 2048; d++) cnt[d + 1] += cnt[d];
+        for (uint32_t b : bits) tmp[cnt[(b >> shift) & 2047u]++] = b;
+        bits.swap(tmp);
+    }
+    const float two_sd = 2.0f * sd;
+    float prev_w = 0.0f;
+    for (size_t e = 0; e < bits.size(); e++) {
+        if (e > 0 && bits[e] == bits[e - 1]) continue;
+        float cv;
+        std::memcpy(&cv, &bits[e], 4);
+        float wv = edge_weight(cv, two_sd);
+        if (e > 0 && !(wv > prev_w)) return false;  // two costs, one weight
+        prev_w = wv;
+    }
+    return true;
+}
+
+void build_tree_shape(const float* ec, int step, int m1, int n1, int o1, Tree& t) {
+    std::vector<float> c = edge_costs(ec, step, (size_t)6 * m1 * n1 * o1);
+    prim_tree(c.data(), m1, n1, o1, t);
+}
+
+void finish_tree(const float* ec, float sd, int step, Tree& t) {
+    const int N = t.N;
+    const float vol = (float)((double)step * step * step);
+    const float two_sd = 2.0f * sd;
+    t.edgemst.assign(N, 0.0f);
+    for (int v = 0; v < N; v++) {
+        const int nb = t.parent_slot[v];
+        if (nb >= 0) t.edgemst[v] = -edge_weight(ec[v + (size_t)nb * N] / vol, two_sd);
+    }
+    build_schedule(t);
+}
+
 void build_tree(const float* ec, float sd, int step, int m1, int n1, int o1, Tree& t) {
+    // weights -exp(-(c / step^3) / (2 sd)) as the keys (primsMST.h:112-117)
+    const int N = m1 * n1 * o1;
+    std::vector<float> w = edge_costs(ec, step, (size_t)6 * N);
+    const float two_sd = 2.0f * sd;
+    for (float& x : w) x = edge_weight(x, two_sd);
+    prim_tree(w.data(), m1, n1, o1, t);
+    finish_tree(ec, sd, step, t);
+}
+
+namespace {
+
+void prim_tree(const float* w, int m1, int n1, int o1, Tree& t) {
     const int N = m1 * n1 * o1;
     t.m1 = m1; t.n1 = n1; t.o1 = o1; t.N = N;
     t.ordered.assign(N, 0);
     t.parents.assign(N, -1);
-    t.edgemst.assign(N, 0.0f);
+    t.parent_slot.assign(N, -1);
     if (N == 0) return;
-    // weights -exp(-(c / step^3) / (2 sd)) (primsMST.h:112-117)
-    const float vol = (float)((double)step * step * step);
-    const float two_sd = 2.0f * sd;
-    std::vector<float> w((size_t)6 * N);
-    for (size_t e = 0; e < w.size(); e++) {
-        float c = ec[e] / vol;
-        w[e] = -std::exp(-c / two_sd);
-    }
     auto neighbour = [&](int v, int nb) -> int {
         int k = v / (m1 * n1), j = (v / m1) % n1, i = v % m1;
         int ii = i + kNbI[nb], jj = j + kNbJ[nb], kk = k + kNbK[nb];
@@ -159,18 +240,19 @@ void build_tree(const float* ec, float sd, int step, int m1, int n1, int o1, Tre
     for (int d = 1; d <= maxd + 1; d++) start[d] += start[d - 1];
     std::vector<int> fill(start.begin(), start.end());
     for (int v = 0; v < N; v++) t.ordered[fill[depth[v]]++] = v;
-    // weight of the edge to the parent; the reference scans all six slots and
+    // slot of the edge to the parent; the reference scans all six slots and
     // keeps the last whose linear index equals the parent (primsMST.h:198-211)
     for (int p = 1; p < N; p++) {
         int c = t.ordered[p], par = t.parents[c];
         int z = c / (m1 * n1), x = (c - z * m1 * n1) / m1, y = c - z * m1 * n1 - x * m1;
         for (int nb = 0; nb < 6; nb++) {
             int idx = y + kNbI[nb] + (x + kNbJ[nb]) * m1 + (z + kNbK[nb]) * m1 * n1;
-            if (idx == par) t.edgemst[c] = -w[c + (size_t)nb * N];
+            if (idx == par) t.parent_slot[c] = nb;
         }
     }
-    build_schedule(t);
 }
+
+}  // namespace
 
 void build_schedule(Tree& t) {
     const int N = t.N;

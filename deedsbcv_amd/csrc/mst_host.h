@@ -12,6 +12,7 @@ struct Tree {
     std::vector<int> ordered;      // nodes sorted by (depth, id); ordered[0] is the root
     std::vector<int> parents;      // parent of each node, -1 for the root
     std::vector<float> edgemst;    // weight of the edge to the parent (0 for the root)
+    std::vector<int> parent_slot;  // neighbour slot (0..5) of that edge, -1 for the root
     // schedule: depth_start[d] = first position of depth d in `ordered` (d = 0..maxd+1)
     int maxd = 0;
     std::vector<int> depth_start;
@@ -35,6 +36,15 @@ void image_stats(const float* im, long long count, float* mean, float* sd);
 // primsMST.h:112-223 given the SAD edge costs ec[node + nb*N] of the (m1,n1,o1)
 // control grid, the image std `sd` and the grid step.
 void build_tree(const float* ec, float sd, int step, int m1, int n1, int o1, Tree& t);
+
+// The same tree in two steps, so the image statistics need not be known when
+// Prim's runs: build_tree_shape orders edges by their cost c = SAD/step^3,
+// which is the reference's order by weight whenever
+// weights_distinguish_costs(...) holds for the final sd; finish_tree then
+// computes edgemst and the schedule.  When the check fails, use build_tree.
+void build_tree_shape(const float* ec, int step, int m1, int n1, int o1, Tree& t);
+bool weights_distinguish_costs(const float* ec, int step, int m1, int n1, int o1, float sd);
+void finish_tree(const float* ec, float sd, int step, Tree& t);
 
 // Fills depth_start and the bottom-up groups from ordered/parents (any tree
 // in BFS order, e.g. one handed in by a caller).

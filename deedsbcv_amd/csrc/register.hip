@@ -204,12 +204,21 @@ void register_pair(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, 
             const LevelCfg c = lv[l];
             std::shared_future<float> sdf = sd_fut[dir];
             double* spent = &build_time[2 * l + dir];
-            trees[dir].push_back(std::async(std::launch::async, [ec, c, sdf, spent]() {
-                float sd = sdf.get();
+            trees[dir].push_back(std::async(std::launch::async, [ec, c, sdf, spent, sad_event]() {
+                // Prim's on the edge costs runs before the image statistics are
+                // known; the exact weights then confirm the order (else rebuild)
+                DEEDS_HIP(hipEventSynchronize(sad_event));
                 auto tb = std::chrono::steady_clock::now();
                 std::unique_ptr<Tree> t(new Tree());
-                build_tree(ec, sd, c.step, c.m1, c.n1, c.o1, *t);
-                *spent = std::chrono::duration<double>(std::chrono::steady_clock::now() - tb).count();
+                build_tree_shape(ec, c.step, c.m1, c.n1, c.o1, *t);
+                double shape_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - tb).count();
+                float sd = sdf.get();
+                auto tf = std::chrono::steady_clock::now();
+                if (weights_distinguish_costs(ec, c.step, c.m1, c.n1, c.o1, sd))
+                    finish_tree(ec, sd, c.step, *t);
+                else
+                    build_tree(ec, sd, c.step, c.m1, c.n1, c.o1, *t);
+                *spent = shape_s + std::chrono::duration<double>(std::chrono::steady_clock::now() - tf).count();
                 return t;
             }));
         }
