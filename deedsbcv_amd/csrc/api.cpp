@@ -220,15 +220,18 @@ int deeds_ctx_datacost(deeds_ctx* ctx, const uint64_t* d_F, const uint64_t* d_M,
 
 namespace deeds {
 
-size_t tree_ints_bound(int N) { return 7 * (size_t)N + 8; }
+size_t tree_ints_bound(int N) { return tree_device_ints(N); }
 
-// Packs a host tree (ordered/parents + schedule, edgemst) into the pinned
-// staging area (tree_ints_bound(N) ints followed by N floats) and enqueues its
-// copy into the device area d_stage of the same layout.  The staging memory
+// Bytes of a tree's staging area: the schedule ints, then 3*N floats of
+// device scratch for the BP parent offsets (not copied).
+size_t tree_stage_bytes(int N) { return tree_ints_bound(N) * sizeof(int) + 3 * (size_t)N * sizeof(float); }
+
+// Packs a host tree's BP schedule (ordered, parents, depth_start, prog_start,
+// prog) into the pinned staging area and enqueues its copy into the device
+// area d_stage of the same layout (tree_stage_bytes(N)).  The pinned memory
 // must stay untouched until the copy has executed on `s`.
 BPDevTree stage_tree(const Tree& t, void* pinned, void* d_stage, hipStream_t s) {
-    size_t ints = t.ordered.size() + t.parents.size() + t.depth_start.size() + t.child_start.size() +
-                  t.child_list.size() + t.heads.size();
+    size_t ints = t.ordered.size() + t.parents.size() + t.depth_start.size() + t.prog_start.size() + t.prog.size();
     size_t bound = tree_ints_bound(t.N);
     if (ints > bound) fail(DEEDS_ERR_INTERNAL, "tree schedule larger than its staging bound");
     int* hi = static_cast<int*>(pinned);
@@ -244,23 +247,18 @@ BPDevTree stage_tree(const Tree& t, void* pinned, void* d_stage, hipStream_t s) 
     out.ordered = put(t.ordered);
     out.parents = put(t.parents);
     out.depth_start = put(t.depth_start);
-    out.child_start = put(t.child_start);
-    out.child_list = put(t.child_list);
-    out.heads = put(t.heads);
+    out.prog_start = put(t.prog_start);
+    out.prog = put(t.prog);
     out.root = t.N > 0 ? t.ordered[0] : 0;
-    float* hf = reinterpret_cast<float*>(hi + bound);
-    float* df = reinterpret_cast<float*>(di + bound);
-    std::copy(t.edgemst.begin(), t.edgemst.end(), hf);
-    out.edgemst = df;
+    out.off3 = reinterpret_cast<float*>(di + bound);
     DEEDS_HIP(hipMemcpyAsync(di, hi, at * sizeof(int), hipMemcpyHostToDevice, s));
-    DEEDS_HIP(hipMemcpyAsync(df, hf, t.edgemst.size() * sizeof(float), hipMemcpyHostToDevice, s));
     return out;
 }
 
 // Bottom-up rounds, then the top-down selection.
 void run_bp(const Tree& t, const BPDevTree& dt, float* d_cost, int16_t* d_inds, int* d_sel, const float* u0,
             const float* v0, const float* w0, float* u1, float* v1, float* w1, int hw, float quant, hipStream_t s) {
-    launch_bp_rounds(d_cost, d_inds, dt, t.round_start.data(), t.rounds, u0, v0, w0, hw, quant, s);
+    launch_bp_rounds(d_cost, d_inds, dt.off3, dt, t.N, t.round_start.data(), t.rounds, u0, v0, w0, hw, quant, s);
     launch_bp_select(d_cost, d_inds, dt, t.N, t.maxd, u0, v0, w0, u1, v1, w1, d_sel, hw, quant, s);
 }
 
@@ -291,7 +289,7 @@ int deeds_ctx_regularisation(deeds_ctx* ctx, float* d_cost, const float* d_u0, c
         }
         int len = 2 * hw + 1;
         size_t len3 = (size_t)len * len * len;
-        size_t stage_bytes = tree_ints_bound(t.N) * sizeof(int) + (size_t)t.N * sizeof(float);
+        size_t stage_bytes = tree_stage_bytes(t.N);
         void* pinned = nullptr;
         DEEDS_HIP(hipHostMalloc(&pinned, stage_bytes, hipHostMallocDefault));
         std::unique_ptr<void, hipError_t (*)(void*)> hold(pinned, hipHostFree);

@@ -115,30 +115,34 @@ __device__ __forceinline__ float block_min(float v, float* red) {
 struct BPArgs {
     float* cost;
     int16_t* allinds;
-    const int* heads;
-    const int* parents;
-    const int* child_start;
-    const int* child_list;
-    const float* edgemst;
-    const float* u0;
-    const float* v0;
-    const float* w0;
-    float quant;
+    const int* prog_start;
+    const int* prog;
+    const float* off3;  // per node: label-unit offset (x, y, z) of its parent's field
     int root;
 };
 
-// Message of node c (cost in `src`, already scaled by its edge weight in A)
-// towards parent p, folded into the accumulator C.  Leaves the argmin table
-// of c in allinds.
+// Offsets of every node's parent field in label units (regularisation.h:208-210):
+// off3[c] = ((u0[p]-u0[c])/q, (v0[p]-v0[c])/q, (w0[p]-w0[c])/q), p = parents[c].
+__global__ void k_parent_offsets(float* __restrict__ off3, const int* __restrict__ parents,
+                                 const float* __restrict__ u0, const float* __restrict__ v0,
+                                 const float* __restrict__ w0, float quant, int N) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    const int p = parents[c];
+    if (p < 0) return;
+    off3[3 * c + 0] = (u0[p] - u0[c]) / quant;
+    off3[3 * c + 1] = (v0[p] - v0[c]) / quant;
+    off3[3 * c + 2] = (w0[p] - w0[c]) / quant;
+}
+
+// One message: A holds the sender's cost scaled by its edge weight; the
+// result is folded into the accumulator C.  Leaves the argmin table in allinds.
 template <int LEN>
-__device__ __forceinline__ void message_into(const BPArgs& a, int c, int p, float* A, float* Bf, float* C,
-                                             int16_t* I1, int16_t* I2, float* zy, float* zx, float* zz,
+__device__ __forceinline__ void message_into(int16_t* allinds_c, float ox, float oy, float oz, float* A, float* Bf,
+                                             float* C, int16_t* I1, int16_t* I2, float* zy, float* zx, float* zz,
                                              float* red) {
     using B = BPShape<LEN>;
     const int t = threadIdx.x;
-    const float ox = (a.u0[p] - a.u0[c]) / a.quant;
-    const float oy = (a.v0[p] - a.v0[c]) / a.quant;
-    const float oz = (a.w0[p] - a.w0[c]) / a.quant;
     for (int q = t; q < B::ZT; q += B::BLOCK) {
         float dy = (float)(q - LEN) + oy;
         float dx = (float)(q - LEN) + ox;
@@ -152,15 +156,20 @@ __device__ __forceinline__ void message_into(const BPArgs& a, int c, int p, floa
     __syncthreads();
     dt_pass<LEN, 1>(Bf, A, zx, I1, I2, nullptr);
     __syncthreads();
-    float mn = dt_pass<LEN, 2>(A, Bf, zz, I2, nullptr, a.allinds + (long long)c * B::L3);
+    float mn = dt_pass<LEN, 2>(A, Bf, zz, I2, nullptr, allinds_c);
     mn = block_min(mn, red);  // its barriers also complete Bf
     for (int l = t; l < B::L3; l += B::BLOCK) C[l] += (Bf[l] - mn);
     __syncthreads();
 }
 
+// One workgroup runs one head's program: fold the head's children into its
+// accumulator, then climb while the finished node is its parent's only child.
+// The next message's record, offsets and cost row are loaded while the
+// current message computes.
 template <int LEN>
-__global__ __launch_bounds__(BPShape<LEN>::BLOCK) void k_bp_chain(BPArgs a, int head_begin) {
+__global__ __launch_bounds__(BPShape<LEN>::BLOCK) void k_bp_prog(BPArgs a, int head_begin) {
     using B = BPShape<LEN>;
+    constexpr int ELEMS = (B::L3 + B::BLOCK - 1) / B::BLOCK;
     extern __shared__ float smem[];
     __shared__ float red[16];
     float* A = smem;
@@ -173,32 +182,72 @@ __global__ __launch_bounds__(BPShape<LEN>::BLOCK) void k_bp_chain(BPArgs a, int 
     float* zz = zx + B::ZT;
     const int t = threadIdx.x;
 
-    int x = a.heads[head_begin + blockIdx.x];
-    long long xb = (long long)x * B::L3;
-    for (int l = t; l < B::L3; l += B::BLOCK) C[l] = a.cost[xb + l];
-    // fold every child of the head, in `ordered` order
-    for (int ci = a.child_start[x]; ci < a.child_start[x + 1]; ci++) {
-        const int c = a.child_list[ci];
-        const float w = a.edgemst[c];
-        const long long cb = (long long)c * B::L3;
-        for (int l = t; l < B::L3; l += B::BLOCK) A[l] = a.cost[cb + l] * w;
-        message_into<LEN>(a, c, x, A, Bf, C, I1, I2, zy, zx, zz, red);
-    }
-    // climb while the finished node is its parent's only child
-    while (x != a.root) {
-        const int q = a.parents[x];
-        if (a.child_start[q + 1] - a.child_start[q] != 1) break;
-        const float w = a.edgemst[x];
-        const long long qb = (long long)q * B::L3;
-        for (int l = t; l < B::L3; l += B::BLOCK) {
-            A[l] = C[l] * w;
-            C[l] = a.cost[qb + l];
+    const int* pg = a.prog + a.prog_start[head_begin + blockIdx.x];
+    const int x = pg[0], k = pg[1], M = k + pg[2];
+    const int* rec = pg + 3;
+    for (int l = t; l < B::L3; l += B::BLOCK) C[l] = a.cost[(long long)x * B::L3 + l];
+    int last = x;
+    if (M > 0) {
+        // the row a message needs from HBM: the sender's cost for a child, the
+        // receiver's data cost (the new accumulator) for a climb
+        auto row_of = [&](int j, int src, int dst) { return j < k ? src : dst; };
+        int src = rec[0], dst = rec[1], wb = rec[2];
+        int nsrc = 0, ndst = 0, nwb = 0;
+        if (M > 1) { nsrc = rec[3]; ndst = rec[4]; nwb = rec[5]; }
+        float ox = a.off3[3 * src], oy = a.off3[3 * src + 1], oz = a.off3[3 * src + 2];
+        float row[ELEMS];
+        {
+            const long long rb = (long long)row_of(0, src, dst) * B::L3;
+#pragma unroll
+            for (int e = 0; e < ELEMS; e++) {
+                const int l = t + e * B::BLOCK;
+                row[e] = l < B::L3 ? a.cost[rb + l] : 0.0f;
+            }
         }
-        message_into<LEN>(a, x, q, A, Bf, C, I1, I2, zy, zx, zz, red);
-        x = q;
-        xb = qb;
+        for (int j = 0; j < M; j++) {
+            // prefetch message j+1 (record j+2, offsets and row of j+1)
+            int n2src = 0, n2dst = 0, n2wb = 0;
+            if (j + 2 < M) { n2src = rec[3 * (j + 2)]; n2dst = rec[3 * (j + 2) + 1]; n2wb = rec[3 * (j + 2) + 2]; }
+            float nox = 0.f, noy = 0.f, noz = 0.f;
+            float nrow[ELEMS];
+            if (j + 1 < M) {
+                nox = a.off3[3 * nsrc];
+                noy = a.off3[3 * nsrc + 1];
+                noz = a.off3[3 * nsrc + 2];
+                const long long rb = (long long)row_of(j + 1, nsrc, ndst) * B::L3;
+#pragma unroll
+                for (int e = 0; e < ELEMS; e++) {
+                    const int l = t + e * B::BLOCK;
+                    nrow[e] = l < B::L3 ? a.cost[rb + l] : 0.0f;
+                }
+            }
+            const float w = __int_as_float(wb);
+            if (j < k) {
+#pragma unroll
+                for (int e = 0; e < ELEMS; e++) {
+                    const int l = t + e * B::BLOCK;
+                    if (l < B::L3) A[l] = row[e] * w;
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < ELEMS; e++) {
+                    const int l = t + e * B::BLOCK;
+                    if (l < B::L3) {
+                        A[l] = C[l] * w;
+                        C[l] = row[e];
+                    }
+                }
+            }
+            message_into<LEN>(a.allinds + (long long)src * B::L3, ox, oy, oz, A, Bf, C, I1, I2, zy, zx, zz, red);
+            if (j >= k) last = dst;
+            src = nsrc; dst = ndst; wb = nwb;
+            nsrc = n2src; ndst = n2dst; nwb = n2wb;
+            ox = nox; oy = noy; oz = noz;
+#pragma unroll
+            for (int e = 0; e < ELEMS; e++) row[e] = nrow[e];
+        }
     }
-    for (int l = t; l < B::L3; l += B::BLOCK) a.cost[xb + l] = C[l];
+    for (int l = t; l < B::L3; l += B::BLOCK) a.cost[(long long)last * B::L3 + l] = C[l];
 }
 
 template <int LEN>
@@ -207,11 +256,11 @@ void launch_round_len(const BPArgs& a, int head_begin, int count, hipStream_t s)
     constexpr size_t smem = B::SMEM + (size_t)B::L3 * 4;  // + the accumulator C
     static bool attr_set = false;
     if (!attr_set) {
-        DEEDS_HIP(hipFuncSetAttribute((const void*)k_bp_chain<LEN>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        DEEDS_HIP(hipFuncSetAttribute((const void*)k_bp_prog<LEN>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)smem));
         attr_set = true;
     }
-    k_bp_chain<LEN><<<count, B::BLOCK, smem, s>>>(a, head_begin);
+    k_bp_prog<LEN><<<count, B::BLOCK, smem, s>>>(a, head_begin);
     DEEDS_LAUNCH_CHECK();
 }
 
@@ -271,9 +320,14 @@ __global__ __launch_bounds__(1024) void k_bp_select(const float* __restrict__ co
 
 }  // namespace
 
-void launch_bp_rounds(float* cost, int16_t* allinds, const BPDevTree& t, const int* round_start, int rounds,
-                      const float* u0, const float* v0, const float* w0, int hw, float quant, hipStream_t s) {
-    BPArgs a{cost, allinds, t.heads, t.parents, t.child_start, t.child_list, t.edgemst, u0, v0, w0, quant, t.root};
+void launch_bp_rounds(float* cost, int16_t* allinds, float* off3, const BPDevTree& t, int N,
+                      const int* round_start, int rounds, const float* u0, const float* v0, const float* w0, int hw,
+                      float quant, hipStream_t s) {
+    if (N > 0) {
+        k_parent_offsets<<<ceil_div(N, 256), 256, 0, s>>>(off3, t.parents, u0, v0, w0, quant, N);
+        DEEDS_LAUNCH_CHECK();
+    }
+    BPArgs a{cost, allinds, t.prog_start, t.prog, off3, t.root};
     for (int r = 0; r < rounds; r++) {
         const int b = round_start[r], n = round_start[r + 1] - b;
         if (n <= 0) continue;

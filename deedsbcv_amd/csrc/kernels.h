@@ -40,17 +40,18 @@ struct BPDevTree {
     const int* ordered;      // N
     const int* parents;      // N
     const int* depth_start;  // maxd+2
-    const int* child_start;  // N+1, CSR offsets into child_list
-    const int* child_list;   // N-1, children of each node in `ordered` order
-    const int* heads;        // chain heads grouped by round (see mst_host.h)
-    const float* edgemst;    // N
+    const int* prog_start;   // heads+1 (heads in round order)
+    const int* prog;         // per head: [x, k, L] + (k+L) records (src, dst, edgemst bits)
+    float* off3;             // 3*N floats of scratch (parent offsets)
     int root;
 };
-// Largest label radius the LDS-resident BP kernel supports (2 len^3 floats + 2 len^3 int16 in LDS).
+// Largest label radius the LDS-resident BP kernel supports (3 len^3 floats + 2 len^3 int16 in LDS).
 constexpr int kMaxBPRadius = 10;
-// Bottom-up pass: one launch per round; round r covers heads[round_start[r] .. round_start[r+1]).
-void launch_bp_rounds(float* cost, int16_t* allinds, const BPDevTree& t, const int* round_start, int rounds,
-                      const float* u0, const float* v0, const float* w0, int hw, float quant, hipStream_t s);
+// Bottom-up pass: one launch per round; round r covers heads round_start[r] .. round_start[r+1]
+// (host array).  off3: 3*N floats of scratch for the parent offsets.
+void launch_bp_rounds(float* cost, int16_t* allinds, float* off3, const BPDevTree& t, int N,
+                      const int* round_start, int rounds, const float* u0, const float* v0, const float* w0, int hw,
+                      float quant, hipStream_t s);
 void launch_bp_select(const float* cost, const int16_t* allinds, const BPDevTree& t, int N, int maxd,
                       const float* u0, const float* v0, const float* w0, float* u1, float* v1, float* w1,
                       int* sel, int hw, float quant, hipStream_t s);

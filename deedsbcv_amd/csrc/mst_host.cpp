@@ -322,6 +322,39 @@ void build_schedule(Tree& t) {
         int v = t.ordered[p];
         if (is_head[v]) t.heads[fill_r[round[v] - 1]++] = v;
     }
+    // the program of every head: its children, then the single-child climb
+    const int root = N > 0 ? t.ordered[0] : -1;
+    t.prog_start.assign(t.heads.size() + 1, 0);
+    t.prog.clear();
+    t.prog.reserve(3 * t.heads.size() + 3 * (size_t)N);
+    auto record = [&](int src, int dst) {
+        int wb;
+        std::memcpy(&wb, &t.edgemst[src], 4);
+        t.prog.push_back(src);
+        t.prog.push_back(dst);
+        t.prog.push_back(wb);
+    };
+    for (size_t h = 0; h < t.heads.size(); h++) {
+        const int x = t.heads[h];
+        t.prog_start[h] = (int)t.prog.size();
+        const size_t hdr = t.prog.size();
+        t.prog.push_back(x);
+        t.prog.push_back(nch(x));
+        t.prog.push_back(0);
+        for (int ci = t.child_start[x]; ci < t.child_start[x + 1]; ci++) record(t.child_list[ci], x);
+        int L = 0, cur = x;
+        while (cur != root) {
+            const int q = t.parents[cur];
+            if (nch(q) != 1) break;
+            record(cur, q);
+            cur = q;
+            L++;
+        }
+        t.prog[hdr + 2] = L;
+    }
+    t.prog_start[t.heads.size()] = (int)t.prog.size();
 }
+
+size_t tree_device_ints(int N) { return 12 * (size_t)N + 16; }
 
 }  // namespace deeds

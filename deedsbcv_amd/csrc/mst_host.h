@@ -3,6 +3,7 @@
 // device belief propagation walks.
 #pragma once
 
+#include <cstddef>
 #include <vector>
 
 namespace deeds {
@@ -27,7 +28,16 @@ struct Tree {
     std::vector<int> heads;
     std::vector<int> round_start;  // rounds + 1 offsets into heads
     int rounds = 0;
+    // Per head (in round order) the program one workgroup executes:
+    // [x, k, L] then k + L message records (src, dst, edgemst[src] as bits):
+    // first the k children of x, then the L single-child climbs above x.
+    std::vector<int> prog_start;  // heads + 1 offsets into prog
+    std::vector<int> prog;
 };
+
+// Upper bound on the ints a device copy of a tree's BP schedule needs
+// (ordered, parents, depth_start, prog_start, prog).
+size_t tree_device_ints(int N);
 
 // primsMST.h:101-110: float mean and std of the image, accumulated exactly as
 // the reference does (sequential float sum; float += double square).

@@ -22,7 +22,7 @@
 
 namespace deeds {
 
-size_t tree_ints_bound(int N);
+size_t tree_stage_bytes(int N);
 BPDevTree stage_tree(const Tree& t, void* pinned, void* d_stage, hipStream_t s);
 void run_bp(const Tree& t, const BPDevTree& dt, float* d_cost, int16_t* d_inds, int* d_sel, const float* u0,
             const float* v0, const float* w0, float* u1, float* v1, float* w1, int hw, float quant, hipStream_t s);
@@ -234,7 +234,7 @@ void register_pair(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, 
     std::vector<size_t> stage_off(L);
     size_t stage_total = 0, stage_max = 0;
     for (int l = 0; l < L; l++) {
-        size_t b = tree_ints_bound(lv[l].sz1) * sizeof(int) + (size_t)lv[l].sz1 * sizeof(float);
+        size_t b = tree_stage_bytes(lv[l].sz1);
         b = (b + 255) & ~(size_t)255;
         stage_off[l] = stage_total;
         stage_total += 2 * b;
