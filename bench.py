@@ -46,6 +46,64 @@ def workload(name):
     return f, m, sched
 
 
+def _atlas_pair(args):
+    from deedsbcv_amd import phantom as ph
+    seed, dims = args
+    return ph.workload_crossmodal(seed, dims)
+
+
+def atlas_bench(a):
+    """Multi-atlas: 16 pairs of drawn sizes, batch throughput vs one-at-a-time."""
+    import concurrent.futures as cf
+
+    import torch
+
+    from deedsbcv_amd import api
+    from deedsbcv_amd import phantom as ph
+    sched = ph.SCHEDULE_FULL
+    dims = ph.atlas_dims(100, 16)
+    print(f"[bench] generating 16 atlas pairs", file=sys.stderr, flush=True)
+    with cf.ProcessPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        vols = list(ex.map(_atlas_pair, [(100 + i, d) for i, d in enumerate(dims)]))
+    g = sched["grid"][-1]
+    pairs = []
+    for (f, m), (mm, n, o) in zip(vols, dims):
+        pairs.append(dict(fixed=torch.from_numpy(f.astype(np.float32)).cuda(),
+                          moving=torch.from_numpy(m.astype(np.float32)).cuda(),
+                          flow=torch.empty(3 * (mm // g) * (n // g) * (o // g), device="cuda"),
+                          warped=torch.empty(f.shape, device="cuda")))
+    kw = dict(grid=sched["grid"], radius=sched["radius"], quant=sched["quant"], alpha=sched["alpha"])
+    for _ in range(max(1, a.warmup)):
+        api.register_batch_device(pairs, concurrency=a.concurrency, **kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        api.register_batch_device(pairs, concurrency=a.concurrency, **kw)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    batch_flows = [p["flow"].cpu().numpy() for p in pairs]
+    # the same 16 pairs one at a time on one context
+    ctx = api.Context(0)
+    for p in pairs[:2]:
+        ctx.register(p["fixed"], p["moving"], p["flow"], p["warped"], **kw)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for p in pairs:
+        ctx.register(p["fixed"], p["moving"], p["flow"], p["warped"], **kw)
+    torch.cuda.synchronize()
+    seq = time.perf_counter() - t1
+    identical = all(np.array_equal(p["flow"].cpu().numpy(), b) for p, b in zip(pairs, batch_flows))
+    n = len(pairs)
+    print(json.dumps({
+        "metric": "multi-atlas registrations/s (16 pairs, dims U[128,224]xU[128,192]xU[96,176], 5 levels)",
+        "value": n * a.steps / wall, "unit": "registrations/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u64 Hamming / f32", "data": "synthetic (seeded cross-modal phantom pairs)",
+        "config": {"workload": "atlas: 16 pairs, concurrency %d" % a.concurrency, "dims": dims},
+        "sequential_registrations_per_s": n / seq, "batch_speedup_vs_sequential": (n * a.steps / wall) / (n / seq),
+        "identical_to_sequential": identical}))
+
+
 def cpu_baseline(f, m, sched, threads, budget_s):
     """The reference's own CPU path on the host cores, on the same pair."""
     ref = ROOT / "oracle" / "_ref" / "deedsBCV"
@@ -88,7 +146,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="full", choices=["full", "small", "crossmodal"])
+    ap.add_argument("--workload", default="full", choices=["full", "small", "crossmodal", "atlas"])
+    ap.add_argument("--concurrency", type=int, default=4, help="pairs in flight for --workload atlas")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
     ap.add_argument("--cpu-budget", type=float, default=240.0)
@@ -100,6 +159,9 @@ def main():
         print("bench.py measures one GPU only: the registration path runs on a single device "
               "(multi-GPU sharding is a design note, not part of this build).", file=sys.stderr)
         raise SystemExit(2)
+
+    if a.workload == "atlas":
+        return atlas_bench(a)
 
     import torch
 

@@ -45,6 +45,11 @@ class Timing(ctypes.Structure):
                 "tree_wait_level_s": list(self.tree_wait_level_s[:L]), "stats_s": self.stats_s}
 
 
+class Pair(ctypes.Structure):
+    _fields_ = [("fixed", c_void_p), ("moving", c_void_p), ("X", c_void_p), ("m", c_int), ("n", c_int),
+                ("o", c_int), ("flow", c_void_p), ("warped", c_void_p)]
+
+
 # name -> argtypes (all return int status unless listed in _RESTYPES)
 I, F, V = c_int, c_float, c_void_p
 SIGNATURES = {
@@ -68,6 +73,9 @@ SIGNATURES = {
     "deeds_warp_image": [V, V, V, V, V, V, I, I, I, V],
     "deeds_warp_affine_s": [V, V, V, V, V, V, I, I, I],
     "deeds_register": [V, V, V, I, I, I, ctypes.POINTER(Schedule), V, V, ctypes.POINTER(Timing)],
+    "deeds_register_batch": [ctypes.POINTER(Pair), I, ctypes.POINTER(Schedule), I, ctypes.POINTER(Timing)],
+    "deeds_register_batch_device": [I, ctypes.POINTER(Pair), I, ctypes.POINTER(Schedule), I,
+                                    ctypes.POINTER(Timing)],
     "deeds_apply_seg": [V, V, V, V, I, I, I, I],
     "deeds_apply_float": [V, V, V, V, I, I, I, I],
     "deeds_ctx_descriptor": [V, V, V, I, I, I, I],
@@ -88,6 +96,22 @@ class DeedsError(RuntimeError):
         self.code = code
 
 
+def _init_torch_first():
+    """PyTorch-ROCm ships its own HIP runtime next to the system one this
+    library links; both sit on the same HSA runtime (so device pointers are
+    shared), but torch's fails to find the GPU if the system runtime was
+    initialised first.  When torch is importable, bring its runtime up first."""
+    try:
+        import torch
+    except Exception:
+        return
+    try:
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    except Exception:
+        pass
+
+
 def load(path: Path | str | None = None):
     """Load (once) and return the ctypes library with all signatures set."""
     global _lib
@@ -96,6 +120,7 @@ def load(path: Path | str | None = None):
     p = Path(path) if path else LIB_PATH
     if not p.exists():
         raise ImportError(f"deeds_hip library not found at {p}; build it with `python -m deedsbcv_amd.build`")
+    _init_torch_first()
     lib = ctypes.CDLL(str(p))
     for name, args in SIGNATURES.items():
         fn = getattr(lib, name)

@@ -24,7 +24,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import DeedsError, Timing, make_schedule  # noqa: F401
+from ._lib import DeedsError, Pair, Timing, make_schedule  # noqa: F401
 
 IDENTITY_X = np.eye(4, dtype=np.float32).T.ravel().copy()  # column-wise, deedsBCV0.cpp:161-172
 
@@ -170,6 +170,52 @@ def register(fixed, moving, grid=(8, 7, 6, 5, 4), radius=(8, 7, 6, 5, 4), quant=
     _lib.call("deeds_register", _p(fixed), _p(moving), _p(_f32(X) if X is not None else None), m, n, o,
               ctypes.byref(sched), _p(flow), _p(warped), ctypes.byref(t))
     return flow, warped, t.as_dict()
+
+
+def register_batch(pairs, grid=(8, 7, 6, 5, 4), radius=(8, 7, 6, 5, 4), quant=(5, 4, 3, 2, 1), alpha=1.6,
+                   concurrency=4, want_warped=True):
+    """Multi-atlas batch: `pairs` is a list of (fixed, moving) or (fixed, moving, X) volumes,
+    each with its own shape.  Returns a list of (flow, warped, timing); every result equals
+    register() of that pair alone."""
+    keep, arr = [], (Pair * max(1, len(pairs)))()
+    outs = []
+    for i, p in enumerate(pairs):
+        fixed, moving = _f32(p[0]), _f32(p[1])
+        if fixed.shape != moving.shape:
+            raise ValueError("Inconsistent image sizes (must have same dimensions)")
+        X = _f32(p[2]) if len(p) > 2 and p[2] is not None else None
+        m, n, o = _dims(fixed)
+        g = grid[-1]
+        flow = np.empty(3 * (m // g) * (n // g) * (o // g), np.float32)
+        warped = np.empty(fixed.shape, np.float32) if want_warped else None
+        keep += [fixed, moving, X]
+        arr[i] = Pair(fixed.ctypes.data, moving.ctypes.data, X.ctypes.data if X is not None else None, m, n, o,
+                      flow.ctypes.data, warped.ctypes.data if warped is not None else None)
+        outs.append((flow, warped))
+    sched = make_schedule(grid, radius, quant, alpha)
+    times = (Timing * max(1, len(pairs)))()
+    _lib.call("deeds_register_batch", arr, len(pairs), ctypes.byref(sched), int(concurrency), times)
+    return [(f, w, times[i].as_dict()) for i, (f, w) in enumerate(outs)]
+
+
+def register_batch_device(pairs, device=0, grid=(8, 7, 6, 5, 4), radius=(8, 7, 6, 5, 4), quant=(5, 4, 3, 2, 1),
+                          alpha=1.6, concurrency=4):
+    """Batch on device data: `pairs` is a list of dicts with torch CUDA tensors
+    fixed, moving (o, n, m), flow (3*sz1) and optional warped, X (numpy, host)."""
+    arr = (Pair * max(1, len(pairs)))()
+    keep = []
+    for i, p in enumerate(pairs):
+        o, n, m = p["fixed"].shape
+        X = _f32(p["X"]) if p.get("X") is not None else None
+        keep.append(X)
+        w = p.get("warped")
+        arr[i] = Pair(p["fixed"].data_ptr(), p["moving"].data_ptr(), X.ctypes.data if X is not None else None,
+                      m, n, o, p["flow"].data_ptr(), w.data_ptr() if w is not None else None)
+    sched = make_schedule(grid, radius, quant, alpha)
+    times = (Timing * max(1, len(pairs)))()
+    _lib.call("deeds_register_batch_device", int(device), arr, len(pairs), ctypes.byref(sched), int(concurrency),
+              times)
+    return [times[i].as_dict() for i in range(len(pairs))]
 
 
 def apply_seg(seg, flow, grid_step, X=None):

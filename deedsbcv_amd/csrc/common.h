@@ -90,6 +90,12 @@ int guarded(deeds_ctx* ctx, F&& f) {
 
 inline unsigned ceil_div(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
 
+// Validates a registration schedule for an m x n x o pair (throws Error).
+void check_schedule(const deeds_schedule* S, int m, int n, int o);
+// One symmetric registration on ctx, device pointers (register.hip).
+void register_pair(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, const float* X, int m, int n, int o,
+                   const deeds_schedule& S, float* d_flow, float* d_warped, deeds_timing* timing);
+
 // Upload / download helpers on a stream.
 template <class T>
 void h2d(T* dst, const T* src, size_t count, hipStream_t s) {

@@ -391,9 +391,7 @@ void register_pair(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, 
 
 using namespace deeds;
 
-namespace {
-
-void check_schedule(const deeds_schedule* S, int m, int n, int o) {
+void deeds::check_schedule(const deeds_schedule* S, int m, int n, int o) {
     DEEDS_REQUIRE(S, "schedule is NULL");
     DEEDS_REQUIRE(S->levels >= 1 && S->levels <= DEEDS_MAX_LEVELS, "levels must be in [1, 10]");
     DEEDS_REQUIRE(m > 0 && n > 0 && o > 0 && (long long)m * n * o < (1LL << 31), "bad volume dimensions");
@@ -406,7 +404,6 @@ void check_schedule(const deeds_schedule* S, int m, int n, int o) {
     DEEDS_REQUIRE(std::isfinite(S->alpha) && S->alpha > 0.0f, "alpha must be positive");
 }
 
-}  // namespace
 
 extern "C" {
 

@@ -147,6 +147,28 @@ int deeds_warp_affine_s(int16_t* out, const int16_t* in, const float* X,
 int deeds_register(const float* fixed, const float* moving, const float* X, int m, int n, int o,
                    const deeds_schedule* sched, float* flow_out, float* warped_out, deeds_timing* timing);
 
+/* Multi-atlas batch: register `count` independent pairs (each with its own
+ * dimensions and affine) with the same schedule.  Up to `concurrency` pairs
+ * run at once on a pool of contexts (each with its own streams and host tree
+ * threads), so one pair's host MST work and latency-bound BP overlap another
+ * pair's kernels.  Each result is bit-identical to a deeds_register call on
+ * that pair alone (no reference counterpart: the reference registers one
+ * pair per process, deedsBCV0.cpp:55).  Pointers are host memory for
+ * deeds_register_batch and device memory of `device` for
+ * deeds_register_batch_device; timings (optional) has `count` entries. */
+typedef struct deeds_pair {
+    const float* fixed;   /* m*n*o HU values */
+    const float* moving;  /* m*n*o HU values */
+    const float* X;       /* 16 floats or NULL (identity); always host memory */
+    int m, n, o;
+    float* flow;          /* 3*sz1 floats (final control grid) */
+    float* warped;        /* m*n*o floats or NULL */
+} deeds_pair;
+int deeds_register_batch(const deeds_pair* pairs, int count, const deeds_schedule* sched, int concurrency,
+                         deeds_timing* timings);
+int deeds_register_batch_device(int device, const deeds_pair* pairs, int count, const deeds_schedule* sched,
+                                int concurrency, deeds_timing* timings);
+
 /* applyBCV.cpp:161-198 / applyBCVfloat.cpp:186-198: apply a _displacements.dat
  * field (3*sz1 floats on grid `grid_step`) to a label volume (nearest) or an
  * image (trilinear). */
