@@ -82,8 +82,16 @@ SIGNATURES = {
     "deeds_ctx_datacost": [V, V, V, V, I, I, I, I, I, F, F, I],
     "deeds_ctx_regularisation": [V, V, V, V, V, V, V, V, I, I, F, V, V, V, I, I, I, V],
     "deeds_ctx_register": [V, V, V, V, I, I, I, ctypes.POINTER(Schedule), V, V, ctypes.POINTER(Timing)],
+    "deeds_soft_saved_count": [I, I],
+    "deeds_ctx_soft_features": [V, I, V, V, I, I, I, I],
+    "deeds_ctx_soft_features_backward": [V, I, V, V, V, I, I, I, I],
+    "deeds_ctx_soft_cost": [V, I, V, V, V, I, I, I, I, I, F, F],
+    "deeds_ctx_soft_cost_backward": [V, I, V, V, V, V, V, I, I, I, I, I, F, F],
+    "deeds_ctx_soft_bp": [V, I, V, V, V, V, V, V, V, I, I, I, I, I, F, F],
+    "deeds_ctx_soft_bp_backward": [V, I, V, V, V, V, V, V, V, V, I, I, I, I, I, F, F],
 }
-_RESTYPES = {"deeds_version": c_char_p, "deeds_last_error": c_char_p, "deeds_ctx_last_error": c_char_p}
+_RESTYPES = {"deeds_version": c_char_p, "deeds_last_error": c_char_p, "deeds_ctx_last_error": c_char_p,
+             "deeds_soft_saved_count": ctypes.c_size_t}
 
 _lib = None
 

@@ -1,0 +1,242 @@
+// C ABI of the differentiable soft variant (declared in include/deeds_hip.h).
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <stdexcept>
+
+#include "kernels.h"
+#include "mst_host.h"
+#include "soft.h"
+
+namespace deeds {
+namespace {
+
+void check_real(int real) {
+    DEEDS_REQUIRE(real == DEEDS_REAL_F32 || real == DEEDS_REAL_F64, "real must be DEEDS_REAL_F32 or DEEDS_REAL_F64");
+}
+
+size_t rsize(int real) { return real == DEEDS_REAL_F64 ? sizeof(double) : sizeof(float); }
+
+void check_soft_dims(int m, int n, int o, int qs) {
+    DEEDS_REQUIRE(m > 0 && n > 0 && o > 0, "dimensions must be positive");
+    DEEDS_REQUIRE((long long)m * n * o <= (1LL << 31) / 12, "volume too large for the soft features");
+    DEEDS_REQUIRE(qs >= 1, "qs must be >= 1");
+}
+
+void check_soft_grid(int m, int n, int o, int step, int hw, float quant) {
+    DEEDS_REQUIRE(m > 0 && n > 0 && o > 0, "dimensions must be positive");
+    DEEDS_REQUIRE(step >= 1 && m / step >= 1 && n / step >= 1 && o / step >= 1,
+                  "grid step must leave at least one control point per axis");
+    DEEDS_REQUIRE(hw >= 0, "label radius must be >= 0");
+    DEEDS_REQUIRE(quant >= 1.0f && (float)(int)quant == quant, "quant must be an integer >= 1");
+    DEEDS_REQUIRE((long long)m * n * o <= (1LL << 31) / 12, "volume too large for the soft cost");
+}
+
+// A host tree's depth schedule and children, copied to the device (ctx buffer
+// "soft.tree") through a pinned block that lives until the copy has run.
+struct StagedTree {
+    SoftTree t;
+    void* pinned = nullptr;
+    hipStream_t stream = nullptr;
+    ~StagedTree() {
+        if (pinned) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipHostFree(pinned);
+        }
+    }
+};
+
+void stage_soft_tree(deeds_ctx* ctx, const int* ordered, const int* parents, const float* edgemst, int N,
+                     StagedTree& st) {
+    DEEDS_REQUIRE(ordered && parents && edgemst, "null tree pointer");
+    Tree t;
+    t.N = N;
+    t.ordered.assign(ordered, ordered + N);
+    t.parents.assign(parents, parents + N);
+    t.edgemst.assign(edgemst, edgemst + N);
+    for (int v : t.ordered) DEEDS_REQUIRE(v >= 0 && v < N, "ordered holds an out-of-range node");
+    try {
+        build_schedule(t);
+    } catch (const std::runtime_error& e) {
+        fail(DEEDS_ERR_ARG, e.what());
+    }
+    const size_t ints = (size_t)N + N + (N + 1) + (N > 0 ? N - 1 : 0);
+    const size_t bytes = ints * sizeof(int) + (size_t)N * sizeof(float);
+    st.stream = ctx->stream;
+    DEEDS_HIP(hipHostMalloc(&st.pinned, bytes, hipHostMallocDefault));
+    int* h = static_cast<int*>(st.pinned);
+    int* d = static_cast<int*>(ctx->buf("soft.tree", bytes));
+    size_t at = 0;
+    auto put = [&](const std::vector<int>& v) {
+        const int* p = d + at;
+        std::copy(v.begin(), v.end(), h + at);
+        at += v.size();
+        return p;
+    };
+    st.t.N = N;
+    st.t.maxd = t.maxd;
+    st.t.depth_start = t.depth_start;
+    st.t.d_ordered = put(t.ordered);
+    st.t.d_parents = put(t.parents);
+    st.t.d_child_start = put(t.child_start);
+    st.t.d_child_list = put(t.child_list);
+    DEEDS_REQUIRE(at == ints, "tree arrays have unexpected sizes");
+    std::copy(t.edgemst.begin(), t.edgemst.end(), reinterpret_cast<float*>(h + at));
+    st.t.d_edgemst = reinterpret_cast<const float*>(d + at);
+    DEEDS_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, ctx->stream));
+}
+
+void check_bp_args(int m, int n, int o, int step, int hw, float quant, float T, int real) {
+    check_real(real);
+    check_soft_grid(m, n, o, step, hw, quant);
+    DEEDS_REQUIRE(T > 0.0f, "temperature must be positive");
+    DEEDS_REQUIRE(soft_bp_smem(2 * hw + 1, rsize(real)) <= (size_t)kMaxLds,
+                  "label radius too large for the soft BP (its workgroup keeps six label cubes in LDS)");
+}
+
+}  // namespace
+}  // namespace deeds
+
+using namespace deeds;
+
+extern "C" {
+
+size_t deeds_soft_saved_count(int N, int hw) {
+    const size_t L = 2 * (size_t)(hw < 0 ? 0 : hw) + 1;
+    return 3 * (size_t)(N < 0 ? 0 : N) * L * L * L;
+}
+
+int deeds_ctx_soft_features(deeds_ctx* ctx, int real, void* d_phi, const void* d_im, int m, int n, int o, int qs) {
+    if (!ctx) return DEEDS_ERR_ARG;
+    return guarded(ctx, [&] {
+        check_real(real);
+        check_soft_dims(m, n, o, qs);
+        DEEDS_REQUIRE(d_phi && d_im, "null pointer");
+        const size_t sz = (size_t)m * n * o;
+        if (real == DEEDS_REAL_F32) {
+            float* w0 = ctx->get<float>("soft.f.w0", 6 * sz);
+            float* w1 = ctx->get<float>("soft.f.w1", 6 * sz);
+            soft_features<float>(static_cast<float*>(d_phi), static_cast<const float*>(d_im), m, n, o, qs, w0, w1,
+                                 ctx->stream);
+        } else {
+            double* w0 = ctx->get<double>("soft.d.w0", 6 * sz);
+            double* w1 = ctx->get<double>("soft.d.w1", 6 * sz);
+            soft_features<double>(static_cast<double*>(d_phi), static_cast<const double*>(d_im), m, n, o, qs, w0, w1,
+                                  ctx->stream);
+        }
+    });
+}
+
+int deeds_ctx_soft_features_backward(deeds_ctx* ctx, int real, void* d_gim, const void* d_gphi, const void* d_im,
+                                     int m, int n, int o, int qs) {
+    if (!ctx) return DEEDS_ERR_ARG;
+    return guarded(ctx, [&] {
+        check_real(real);
+        check_soft_dims(m, n, o, qs);
+        DEEDS_REQUIRE(d_gim && d_gphi && d_im, "null pointer");
+        const size_t sz = (size_t)m * n * o;
+        if (real == DEEDS_REAL_F32) {
+            float* w0 = ctx->get<float>("soft.f.w0", 6 * sz);
+            float* w1 = ctx->get<float>("soft.f.w1", 6 * sz);
+            float* w2 = ctx->get<float>("soft.f.w2", 12 * sz);
+            soft_features_backward<float>(static_cast<float*>(d_gim), static_cast<const float*>(d_gphi),
+                                          static_cast<const float*>(d_im), m, n, o, qs, w0, w1, w2, ctx->stream);
+        } else {
+            double* w0 = ctx->get<double>("soft.d.w0", 6 * sz);
+            double* w1 = ctx->get<double>("soft.d.w1", 6 * sz);
+            double* w2 = ctx->get<double>("soft.d.w2", 12 * sz);
+            soft_features_backward<double>(static_cast<double*>(d_gim), static_cast<const double*>(d_gphi),
+                                           static_cast<const double*>(d_im), m, n, o, qs, w0, w1, w2, ctx->stream);
+        }
+    });
+}
+
+int deeds_ctx_soft_cost(deeds_ctx* ctx, int real, void* d_cost, const void* d_fixed_phi, const void* d_moving_phi,
+                        int m, int n, int o, int step, int hw, float quant, float alpha) {
+    if (!ctx) return DEEDS_ERR_ARG;
+    return guarded(ctx, [&] {
+        check_real(real);
+        check_soft_grid(m, n, o, step, hw, quant);
+        DEEDS_REQUIRE(d_cost && d_fixed_phi && d_moving_phi, "null pointer");
+        if (real == DEEDS_REAL_F32)
+            soft_cost<float>(static_cast<float*>(d_cost), static_cast<const float*>(d_fixed_phi),
+                             static_cast<const float*>(d_moving_phi), m, n, o, step, hw, quant, alpha, ctx->stream);
+        else
+            soft_cost<double>(static_cast<double*>(d_cost), static_cast<const double*>(d_fixed_phi),
+                              static_cast<const double*>(d_moving_phi), m, n, o, step, hw, quant, alpha, ctx->stream);
+    });
+}
+
+int deeds_ctx_soft_cost_backward(deeds_ctx* ctx, int real, void* d_gfixed, void* d_gmoving, const void* d_gcost,
+                                 const void* d_fixed_phi, const void* d_moving_phi, int m, int n, int o, int step,
+                                 int hw, float quant, float alpha) {
+    if (!ctx) return DEEDS_ERR_ARG;
+    return guarded(ctx, [&] {
+        check_real(real);
+        check_soft_grid(m, n, o, step, hw, quant);
+        DEEDS_REQUIRE(d_gfixed && d_gmoving && d_gcost && d_fixed_phi && d_moving_phi, "null pointer");
+        const size_t L = 2 * (size_t)hw + 1;
+        DEEDS_REQUIRE(L * L * L * rsize(real) <= (size_t)kMaxLds, "label radius too large for the soft cost backward");
+        if (real == DEEDS_REAL_F32)
+            soft_cost_backward<float>(static_cast<float*>(d_gfixed), static_cast<float*>(d_gmoving),
+                                      static_cast<const float*>(d_gcost), static_cast<const float*>(d_fixed_phi),
+                                      static_cast<const float*>(d_moving_phi), m, n, o, step, hw, quant, alpha,
+                                      ctx->stream);
+        else
+            soft_cost_backward<double>(static_cast<double*>(d_gfixed), static_cast<double*>(d_gmoving),
+                                       static_cast<const double*>(d_gcost), static_cast<const double*>(d_fixed_phi),
+                                       static_cast<const double*>(d_moving_phi), m, n, o, step, hw, quant, alpha,
+                                       ctx->stream);
+    });
+}
+
+int deeds_ctx_soft_bp(deeds_ctx* ctx, int real, void* d_u1, void* d_saved, const void* d_cost, const void* d_u0,
+                      const int* ordered, const int* parents, const float* edgemst, int m, int n, int o, int step,
+                      int hw, float quant, float T) {
+    if (!ctx) return DEEDS_ERR_ARG;
+    return guarded(ctx, [&] {
+        check_bp_args(m, n, o, step, hw, quant, T, real);
+        DEEDS_REQUIRE(d_u1 && d_saved && d_cost && d_u0, "null pointer");
+        const int N = (m / step) * (n / step) * (o / step);
+        StagedTree st;
+        stage_soft_tree(ctx, ordered, parents, edgemst, N, st);
+        if (real == DEEDS_REAL_F32)
+            soft_bp_forward<float>(st.t, static_cast<const float*>(d_cost), static_cast<const float*>(d_u0),
+                                   static_cast<float*>(d_u1), static_cast<float*>(d_saved), hw, quant, T,
+                                   ctx->stream);
+        else
+            soft_bp_forward<double>(st.t, static_cast<const double*>(d_cost), static_cast<const double*>(d_u0),
+                                    static_cast<double*>(d_u1), static_cast<double*>(d_saved), hw, quant, T,
+                                    ctx->stream);
+        // the tree copy must complete before the pinned staging block is released
+        DEEDS_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int deeds_ctx_soft_bp_backward(deeds_ctx* ctx, int real, void* d_gcost, const void* d_gu1, const void* d_saved,
+                               const void* d_cost, const void* d_u0, const int* ordered, const int* parents,
+                               const float* edgemst, int m, int n, int o, int step, int hw, float quant, float T) {
+    if (!ctx) return DEEDS_ERR_ARG;
+    return guarded(ctx, [&] {
+        check_bp_args(m, n, o, step, hw, quant, T, real);
+        DEEDS_REQUIRE(d_gcost && d_gu1 && d_saved && d_cost && d_u0, "null pointer");
+        const int N = (m / step) * (n / step) * (o / step);
+        const size_t L = 2 * (size_t)hw + 1;
+        StagedTree st;
+        stage_soft_tree(ctx, ordered, parents, edgemst, N, st);
+        const size_t scratch = 2 * (size_t)N * L * L * L;
+        if (real == DEEDS_REAL_F32)
+            soft_bp_backward<float>(st.t, static_cast<const float*>(d_cost), static_cast<const float*>(d_u0),
+                                    static_cast<const float*>(d_saved), static_cast<const float*>(d_gu1),
+                                    static_cast<float*>(d_gcost), ctx->get<float>("soft.bp.scratch", scratch), hw,
+                                    quant, T, ctx->stream);
+        else
+            soft_bp_backward<double>(st.t, static_cast<const double*>(d_cost), static_cast<const double*>(d_u0),
+                                     static_cast<const double*>(d_saved), static_cast<const double*>(d_gu1),
+                                     static_cast<double*>(d_gcost), ctx->get<double>("soft.bp.scratch", scratch), hw,
+                                     quant, T, ctx->stream);
+        DEEDS_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+}  // extern "C"

@@ -164,5 +164,11 @@ def workload_atlas(first_seed=100, count=16):
     return [workload_crossmodal(first_seed + i, d) for i, d in enumerate(dims)]
 
 
-def workload_soft(seed=4):
-    return workload_crossmodal(seed, (160, 160, 160), dtype="float32")
+def workload_soft(seed=4, crop=None):
+    """The 160^3 float32 pair of the soft variant; `crop` = edge of a centred cube cut from it."""
+    f, m = workload_crossmodal(seed, (160, 160, 160), dtype="float32")
+    if crop:
+        a = (160 - crop) // 2
+        sl = (slice(a, a + crop),) * 3
+        f, m = np.ascontiguousarray(f[sl]), np.ascontiguousarray(m[sl])
+    return f, m

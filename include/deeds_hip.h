@@ -192,6 +192,42 @@ int deeds_ctx_register(deeds_ctx* ctx, const float* d_fixed, const float* d_movi
                        int m, int n, int o, const deeds_schedule* sched,
                        float* d_flow, float* d_warped, deeds_timing* timing);
 
+/* ---- differentiable soft variant (no reference counterpart) --------------
+ * A smooth relaxation of one registration level beside the plain path, for
+ * callers that need gradients (learning pipelines, torch.autograd):
+ *   features  phi[p][l] (l = 0..11): the un-binarised self-similarity values
+ *             that descriptor() thresholds (MINDSSCbox.h:184-201), i.e. the
+ *             twelve box-filtered SSDs minus their minimum over max(mean, 1e-6)
+ *   cost      C[n][l] = a1 * sum over the block samples of sum_ch
+ *             (phi_F[s][ch] - phi_M[clamp(s + d_l)][ch])^2 with the samples,
+ *             clamping and a1 of dataCostCL (dataCostD.h:141-196)
+ *   BP        sum-product belief propagation at temperature T on a tree in
+ *             BFS order (ordered/parents/edgemst as from deeds_prims_graph) for
+ *             E = sum_n C_n + sum_c |l_p - l_c + (u0_p - u0_c)/quant|^2 / edgemst[c];
+ *             d_u1 = d_u0 + the expected label displacement of each marginal
+ * Each forward has an exact backward (vector-Jacobian product).  `real`
+ * selects float (DEEDS_REAL_F32) or double (DEEDS_REAL_F64) for every void*
+ * array of the call.  Layouts: images m*n*o, features [voxel][12], cost
+ * [node][label], fields [3][N] (u, v, w), d_saved deeds_soft_saved_count(N, hw)
+ * reals written by deeds_ctx_soft_bp and read by its backward.  The BP
+ * calls return when their work is complete; the others are asynchronous. */
+enum deeds_real { DEEDS_REAL_F32 = 0, DEEDS_REAL_F64 = 1 };
+size_t deeds_soft_saved_count(int N, int hw);
+int deeds_ctx_soft_features(deeds_ctx* ctx, int real, void* d_phi, const void* d_im, int m, int n, int o, int qs);
+int deeds_ctx_soft_features_backward(deeds_ctx* ctx, int real, void* d_gim, const void* d_gphi, const void* d_im,
+                                     int m, int n, int o, int qs);
+int deeds_ctx_soft_cost(deeds_ctx* ctx, int real, void* d_cost, const void* d_fixed_phi, const void* d_moving_phi,
+                        int m, int n, int o, int step, int hw, float quant, float alpha);
+int deeds_ctx_soft_cost_backward(deeds_ctx* ctx, int real, void* d_gfixed, void* d_gmoving, const void* d_gcost,
+                                 const void* d_fixed_phi, const void* d_moving_phi, int m, int n, int o, int step,
+                                 int hw, float quant, float alpha);
+int deeds_ctx_soft_bp(deeds_ctx* ctx, int real, void* d_u1, void* d_saved, const void* d_cost, const void* d_u0,
+                      const int* ordered, const int* parents, const float* edgemst, int m, int n, int o, int step,
+                      int hw, float quant, float T);
+int deeds_ctx_soft_bp_backward(deeds_ctx* ctx, int real, void* d_gcost, const void* d_gu1, const void* d_saved,
+                               const void* d_cost, const void* d_u0, const int* ordered, const int* parents,
+                               const float* edgemst, int m, int n, int o, int step, int hw, float quant, float T);
+
 #ifdef __cplusplus
 }
 #endif
