@@ -104,6 +104,122 @@ def atlas_bench(a):
         "identical_to_sequential": identical}))
 
 
+def _timed(fn, steps, warmup):
+    """Mean wall seconds of fn() over `steps` calls after `warmup` calls, GPU drained on both sides."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def soft_bench(a):
+    """Differentiable soft level (160^3 float32 pair, grid 6, radius 6, quantisation 2,
+    T = 0.5): forward + backward of the expected displacements with respect to both
+    images, against the plain GPU forward of the same level (MIND of both images,
+    Hamming cost, MST, min-sum BP), both including the MST construction."""
+    import ctypes
+
+    import torch
+
+    from deedsbcv_amd import _lib, api, soft
+    from deedsbcv_amd import phantom as ph
+    print("[bench] generating the 160^3 soft pair", file=sys.stderr, flush=True)
+    f, m = ph.workload_soft(4)
+    o, n, mm = f.shape
+    grid, hw, quant, alpha, T = 6, 6, 2, 1.6, 0.5
+    qs = soft.mind_step(quant)
+    N = (mm // grid) * (n // grid) * (o // grid)
+    L3 = (2 * hw + 1) ** 3
+    dev = torch.device("cuda:0")
+    x = torch.from_numpy(f).to(dev)
+    y = torch.from_numpy(m).to(dev)
+    wgt = torch.randn((3, o // grid, n // grid, mm // grid), generator=torch.Generator().manual_seed(0)).to(dev)
+    V = ctypes.c_void_p
+
+    # plain forward of the same level on one context (device pointers)
+    ctx = api.Context(0)
+    mF = torch.empty(f.shape, dtype=torch.int64, device=dev)
+    mM = torch.empty(f.shape, dtype=torch.int64, device=dev)
+    cost = torch.empty((N, L3), dtype=torch.float32, device=dev)
+    u0 = torch.zeros((3, N), dtype=torch.float32, device=dev)
+    u1 = torch.empty((3, N), dtype=torch.float32, device=dev)
+
+    def plain():
+        ordered, parents, edgemst = api.primsGraph(f, grid)
+        _lib.ctx_call(ctx.h, "deeds_ctx_descriptor", V(mF.data_ptr()), V(x.data_ptr()), mm, n, o, qs)
+        _lib.ctx_call(ctx.h, "deeds_ctx_descriptor", V(mM.data_ptr()), V(y.data_ptr()), mm, n, o, qs)
+        _lib.ctx_call(ctx.h, "deeds_ctx_datacost", V(mF.data_ptr()), V(mM.data_ptr()), V(cost.data_ptr()), mm, n, o,
+                      grid, hw, float(quant), alpha, 1)
+        _lib.ctx_call(ctx.h, "deeds_ctx_regularisation", V(cost.data_ptr()), V(u0[0].data_ptr()),
+                      V(u0[1].data_ptr()), V(u0[2].data_ptr()), V(u1[0].data_ptr()), V(u1[1].data_ptr()),
+                      V(u1[2].data_ptr()), hw, grid, float(quant), _p32(ordered), _p32(parents), _p32(edgemst),
+                      mm, n, o, None)
+        _lib.ctx_call(ctx.h, "deeds_ctx_synchronize")
+
+    def _p32(arr):
+        return arr.ctypes.data_as(V)
+
+    xs = x.clone().requires_grad_(True)
+    ys = y.clone().requires_grad_(True)
+
+    def soft_fb():
+        tree = api.primsGraph(f, grid)
+        xs.grad = ys.grad = None
+        loss = (soft.soft_register(xs, ys, grid, hw, quant, alpha, T, tree=tree) * wgt).sum()
+        loss.backward()
+
+    def soft_f():
+        tree = api.primsGraph(f, grid)
+        with torch.no_grad():
+            soft.soft_register(x, y, grid, hw, quant, alpha, T, tree=tree)
+
+    t_plain = _timed(plain, a.steps, max(1, a.warmup))
+    print(f"[bench] plain forward {1e3 * t_plain:.2f} ms", file=sys.stderr, flush=True)
+    t_sf = _timed(soft_f, a.steps, max(1, a.warmup))
+    t_sfb = _timed(soft_fb, a.steps, max(1, a.warmup))
+    print(f"[bench] soft forward {1e3 * t_sf:.2f} ms, forward+backward {1e3 * t_sfb:.2f} ms", file=sys.stderr,
+          flush=True)
+
+    # stage split of the soft step (each stage timed alone)
+    tree = api.primsGraph(f, grid)
+    F = soft.soft_features(xs, qs)
+    M = soft.soft_features(ys, qs)
+    C = soft.soft_cost(F, M, grid, hw, quant, alpha)
+    U = soft.soft_bp(C, torch.zeros((3, N), device=dev), tree, f.shape, grid, hw, quant, T)
+    gU = wgt.reshape(3, N).contiguous()
+    gC = torch.autograd.grad(U, C, gU, retain_graph=True)[0]
+    gF = torch.autograd.grad(C, (F, M), gC, retain_graph=True)
+    st = {
+        "mst_host_ms": _timed(lambda: api.primsGraph(f, grid), a.steps, 1),
+        "features_fwd_ms": _timed(lambda: soft.soft_features(x, qs), a.steps, 1) * 2,
+        "cost_fwd_ms": _timed(lambda: soft.soft_cost(F.detach(), M.detach(), grid, hw, quant, alpha), a.steps, 1),
+        "bp_fwd_ms": _timed(lambda: soft.soft_bp(C.detach(), torch.zeros((3, N), device=dev), tree, f.shape, grid,
+                                                 hw, quant, T), a.steps, 1),
+        "bp_bwd_ms": _timed(lambda: torch.autograd.grad(U, C, gU, retain_graph=True), a.steps, 1),
+        "cost_bwd_ms": _timed(lambda: torch.autograd.grad(C, (F, M), gC, retain_graph=True), a.steps, 1),
+        "features_bwd_ms": _timed(lambda: torch.autograd.grad(F, xs, gF[0], retain_graph=True), a.steps, 1) * 2,
+    }
+    st = {k: 1e3 * v for k, v in st.items()}
+    terms = N * L3 * 54 * 12  # SSD terms of the cost volume (54 samples per label at grid 6)
+    print(json.dumps({
+        "metric": "soft level forward+backward/s @160^3 (grid 6, radius 6, quant 2, T 0.5)",
+        "value": 1.0 / t_sfb, "unit": "forward+backward/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": 1e3 * t_sfb, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded cross-modal phantom pair, 160^3 float32)",
+        "config": {"workload": "soft: 160x160x160 float32 pair, grid 6, radius 6 (2197 labels), quant 2, "
+                               "alpha 1.6, T 0.5, gradient w.r.t. both images", "nodes": N, "labels": L3},
+        "soft_forward_ms": 1e3 * t_sf, "plain_forward_ms": 1e3 * t_plain,
+        "soft_fwd_bwd_over_plain_forward": t_sfb / t_plain,
+        "soft_stage_ms": st,
+        "cost_fwd_gterms_per_s": terms / (st["cost_fwd_ms"] * 1e-3) / 1e9,
+    }))
+
+
 def cpu_baseline(f, m, sched, threads, budget_s):
     """The reference's own CPU path on the host cores, on the same pair."""
     ref = ROOT / "oracle" / "_ref" / "deedsBCV"
@@ -146,7 +262,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="full", choices=["full", "small", "crossmodal", "atlas"])
+    ap.add_argument("--workload", default="full", choices=["full", "small", "crossmodal", "atlas", "soft"])
     ap.add_argument("--concurrency", type=int, default=4, help="pairs in flight for --workload atlas")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
@@ -162,6 +278,8 @@ def main():
 
     if a.workload == "atlas":
         return atlas_bench(a)
+    if a.workload == "soft":
+        return soft_bench(a)
 
     import torch
 
@@ -248,7 +366,7 @@ def main():
     }
     if not a.no_cpu_baseline:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
-        cb = cpu_baseline(f, m, sched, threads, a.cpu_budget) if a.workload != "small" or True else None
+        cb = cpu_baseline(f, m, sched, threads, a.cpu_budget)
         if cb is not None and cb.get("datacost_s"):
             cb["hamming_gevals_per_s"] = evals / cb["datacost_s"] / 1e9
         result["cpu_baseline"] = cb

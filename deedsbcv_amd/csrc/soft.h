@@ -1,6 +1,28 @@
-// Launchers of the differentiable soft variant (soft.hip).  All pointers are
-// device pointers; R is float or double.  Features are laid out [voxel][12],
-// cost volumes node-major [node][label], displacement fields [3][N] (u, v, w).
+// Differentiable soft variant of one deeds registration level (no counterpart
+// in the reference; the model below is this library's own).
+//
+//   features  phi[p][l], l = 0..11: the un-binarised self-similarity context
+//             of MINDSSCbox.h:127-201 -- six box-filtered squared
+//             differences D_c (:104-124, clamped window sums of radius qs),
+//             read at the twelve SSC offsets (:184-191), minus their minimum,
+//             divided by max(mean, 1e-6) (:192-201) -- before thresholding
+//   cost      C[n,l] = a1 * sum_{samples s of n} sum_ch (F[s,ch] - M[clamp(s+d(l)),ch])^2
+//             with the samples, clamping and a1 of dataCostD.h:141-196
+//   energy    E(L) = sum_n C_n(l_n) + sum_{c != root} |l_p - l_c + off_c|^2 / w_c
+//             on a tree in BFS order (the MST of primsMST.h), w_c = edgemst[c],
+//             off_c = (u0[p] - u0[c]) / quant per axis (regularisation.h:208-210)
+//   inference sum-product BP at temperature T: log-sum-exp distance
+//             transforms separable over the three label axes, upward then
+//             downward messages, marginals P_c = softmax(-g_c / T), output
+//             u1_c = u0_c + E_{P_c}[label displacement]
+//   backward  hand-written adjoints of all three stages; BP keeps the upward
+//             beliefs, upward messages and full beliefs of every node and
+//             recomputes each distance transform in the backward sweeps
+//
+// Launchers of the three stages (soft_features.hip, soft_cost.hip,
+// soft_bp.hip).  All pointers are device pointers; R is float or double.
+// Features are laid out [voxel][12], cost volumes node-major [node][label],
+// displacement fields [3][N] (u, v, w).
 #pragma once
 
 #include <cstddef>
@@ -24,8 +46,23 @@ struct SoftTree {
     const float* d_edgemst = nullptr;
 };
 
-// LDS bytes of one soft BP workgroup for L labels per axis.
-size_t soft_bp_smem(int L, size_t rsize);
+// Workgroup geometry of the soft BP for L labels per axis: each line of the
+// label cube is shared by S threads that produce OUT outputs each, OUT the
+// smallest count that keeps L^2 * S threads within 1024.
+constexpr int soft_bp_out(int L) {
+    int o = 1;
+    while (L * L * ((L + o - 1) / o) > 1024) o++;
+    return o;
+}
+constexpr int soft_bp_s(int L) { return (L + soft_bp_out(L) - 1) / soft_bp_out(L); }
+// LDS bytes of one soft BP workgroup: six label cubes, three padded quadratic
+// tables and a reduction area.
+constexpr size_t soft_bp_smem(int L, size_t rsize) {
+    return (6 * (size_t)L * L * L + 3 * (2 * (size_t)L + 1 + 2 * (size_t)soft_bp_s(L) * soft_bp_out(L)) + 32) *
+           rsize;
+}
+// Largest label radius with a compiled soft BP kernel.
+constexpr int kSoftMaxRadius = 8;
 
 // phi from image im; w0, w1: 6*m*n*o reals of workspace each.
 template <typename R>

@@ -90,6 +90,7 @@ void check_bp_args(int m, int n, int o, int step, int hw, float quant, float T, 
     check_real(real);
     check_soft_grid(m, n, o, step, hw, quant);
     DEEDS_REQUIRE(T > 0.0f, "temperature must be positive");
+    DEEDS_REQUIRE(hw <= kSoftMaxRadius, "soft BP label radius must be <= " + std::to_string(kSoftMaxRadius));
     DEEDS_REQUIRE(soft_bp_smem(2 * hw + 1, rsize(real)) <= (size_t)kMaxLds,
                   "label radius too large for the soft BP (its workgroup keeps six label cubes in LDS)");
 }

@@ -1,0 +1,464 @@
+// Soft variant, stage 3: sum-product belief propagation on the tree and its
+// adjoint (the model is stated in soft.h).
+//
+// Forward sweeps, one launch per BFS depth, one workgroup per node:
+//   upward   (deepest depth first)  h_c = C_c + sum of the children's messages,
+//            m_c = DTup_c(h_c) - min
+//   downward (root side first)      g_c = h_c + DTdown_c(g_p - m_c) - min
+//   marginals P_c = softmax(-g_c / T), u1_c = u0_c + E_{P_c}[displacement]
+// DT is three log-sum-exp passes, one per label axis, against the quadratic
+// pairwise table of the edge (the soft counterpart of messageDT,
+// regularisation.h:6-113).  The label count per axis is a template
+// parameter: the S threads of a line keep the line's values and a window of
+// the table in registers and produce OUT outputs each, so a pass costs no
+// index arithmetic or LDS traffic beyond one read of the line.
+//
+// Backward sweeps: deepest first, A_c = d/dg_c (the marginal's own term plus
+// the adjoints Dt_k of the children's downward transforms) and
+// Dt_c = DTdown_c^T(A_c); then root side first, dH_c = A_c + DTup_c^T(dH_p - Dt_c),
+// which is d/dC_c.  Each transform is recomputed from the saved beliefs in LDS
+// (all four pass buffers kept) and differentiated pass by pass in reverse.
+// The minimum subtracted from each message only shifts beliefs by constants
+// the marginals ignore, so it has no adjoint term.
+#include "soft_common.h"
+
+namespace deeds {
+
+namespace {
+
+using namespace soft_dev;
+
+template <typename R, int LEN>
+struct SS {
+    static constexpr int HW = (LEN - 1) / 2;
+    static constexpr int L2 = LEN * LEN, L3 = L2 * LEN;
+    static constexpr int OUT = soft_bp_out(LEN), S = soft_bp_s(LEN);
+    static constexpr int BLOCK = ((L2 * S + 63) / 64) * 64;
+    static constexpr int PAD = S * OUT;  // table entries beyond both ends read by the last windows
+    static constexpr int ZT = 2 * LEN + 1 + 2 * PAD;
+    static constexpr size_t SMEM = soft_bp_smem(LEN, sizeof(R));
+};
+
+template <typename R, int LEN>
+struct Cubes {
+    R *X0, *X1, *X2, *X3, *Ga, *Gb, *zy, *zx, *zz, *red;
+    __device__ explicit Cubes(R* b) {
+        using H = SS<R, LEN>;
+        X0 = b;
+        X1 = X0 + H::L3;
+        X2 = X1 + H::L3;
+        X3 = X2 + H::L3;
+        Ga = X3 + H::L3;
+        Gb = Ga + H::L3;
+        zy = Gb + H::L3 + H::PAD;  // zy[q] for q = -PAD .. 2*LEN+PAD: offset q - LEN
+        zx = zy + H::ZT;
+        zz = zx + H::ZT;
+        red = zz - H::PAD + H::ZT;
+    }
+};
+
+// Lines of the cube along axis AX (0: i / stride 1, 1: j / stride LEN, 2: k / stride LEN^2).
+template <int LEN, int AX>
+struct Line {
+    static constexpr int STRIDE = AX == 0 ? 1 : AX == 1 ? LEN : LEN * LEN;
+    static __device__ __forceinline__ int base(int line) {
+        return AX == 0 ? line * LEN : AX == 1 ? (line % LEN) + (line / LEN) * LEN * LEN : line;
+    }
+};
+
+// out(a) = mn_a - T ln sum_b exp(-(in(b) + z[a - b]) / T) along axis AX, mn_a = min_b(in(b) + z[a - b]);
+// returns the smallest output this thread wrote.
+template <typename R, int LEN, int AX>
+__device__ __forceinline__ R lse_pass(const R* __restrict__ in, R* __restrict__ out, const R* __restrict__ zt, R iT2,
+                                      R Tln2) {
+    using H = SS<R, LEN>;
+    using G = Line<LEN, AX>;
+    const int line = threadIdx.x / H::S, sub = threadIdx.x - line * H::S;
+    R lo = big<R>();
+    if (line >= H::L2) return lo;
+    const int base = G::base(line);
+    R v[LEN];
+#pragma unroll
+    for (int j = 0; j < LEN; j++) v[j] = in[base + j * G::STRIDE];
+    const int i0 = sub * H::OUT;
+    R zw[H::OUT + LEN];  // zw[r - j + LEN] = zt[(i0 + r) - j + LEN]
+#pragma unroll
+    for (int q = 0; q < H::OUT + LEN; q++) zw[q] = zt[i0 + q];
+#pragma unroll
+    for (int r = 0; r < H::OUT; r++) {
+        if (i0 + r < LEN) {
+            R c[LEN];
+            R mn = big<R>();
+#pragma unroll
+            for (int j = 0; j < LEN; j++) {
+                c[j] = v[j] + zw[r - j + LEN];
+                mn = c[j] < mn ? c[j] : mn;
+            }
+            R s = 0;
+#pragma unroll
+            for (int j = 0; j < LEN; j++) s += FastMath<R>::exp2((mn - c[j]) * iT2);
+            const R o = mn - Tln2 * FastMath<R>::log2(s);
+            out[base + (i0 + r) * G::STRIDE] = o;
+            lo = o < lo ? o : lo;
+        }
+    }
+    return lo;
+}
+
+// Adjoint of lse_pass: gin(b) = sum_a gout(a) exp(-(in(b) + z[a - b] - out(a)) / T).
+template <typename R, int LEN, int AX>
+__device__ __forceinline__ void lse_adj(const R* __restrict__ in, const R* __restrict__ out,
+                                        const R* __restrict__ gout, R* __restrict__ gin, const R* __restrict__ zt,
+                                        R iT2) {
+    using H = SS<R, LEN>;
+    using G = Line<LEN, AX>;
+    const int line = threadIdx.x / H::S, sub = threadIdx.x - line * H::S;
+    if (line >= H::L2) return;
+    const int base = G::base(line);
+    R go[LEN], ou[LEN];
+#pragma unroll
+    for (int a = 0; a < LEN; a++) {
+        go[a] = gout[base + a * G::STRIDE];
+        ou[a] = out[base + a * G::STRIDE];
+    }
+    const int i0 = sub * H::OUT;
+    R zw[H::OUT + LEN - 1];  // zw[a - r + OUT - 1] = zt[a - (i0 + r) + LEN]
+#pragma unroll
+    for (int q = 0; q < H::OUT + LEN - 1; q++) zw[q] = zt[LEN - i0 - (H::OUT - 1) + q];
+#pragma unroll
+    for (int r = 0; r < H::OUT; r++) {
+        const int b = i0 + r;
+        if (b < LEN) {
+            const R x = in[base + b * G::STRIDE];
+            R acc = 0;
+#pragma unroll
+            for (int a = 0; a < LEN; a++) acc += go[a] * FastMath<R>::exp2((ou[a] - x - zw[a - r + H::OUT - 1]) * iT2);
+            gin[base + b * G::STRIDE] = acc;
+        }
+    }
+}
+
+struct SoftTreeDev {
+    const int* ordered;
+    const int* parents;
+    const int* child_start;  // N + 1
+    const int* child_list;   // N - 1
+    const float* edgemst;
+};
+
+template <typename R>
+struct BPState {
+    const R* cost;  // N x L3
+    R* H;           // upward beliefs h_c
+    R* Msg;         // upward messages, shifted to minimum 0
+    R* G;           // full beliefs g_c
+    const R* u0;    // 3 x N (u, v, w)
+    R* u1;          // 3 x N
+    SoftTreeDev t;
+    int N;
+    R quant, T;
+};
+
+template <typename R>
+struct BPGrad {
+    const R* gu1;  // 3 x N
+    R* A;          // total d/dg_c
+    R* Dt;         // d/d(input of the downward transform of c)
+    R* dH;         // d/dh_c == d/dC_c
+};
+
+// Quadratic tables of the edge c -> p: z[a - b] = (a - b + off)^2 / w per axis
+// (label axis 0 is i / v, 1 is j / u, 2 is k / w); sign +1 for the upward
+// transform (a = parent label, b = child label), -1 for the downward one.
+template <typename R, int LEN>
+__device__ void edge_tables(const BPState<R>& a, int c, int p, R sign, const Cubes<R, LEN>& s) {
+    using H = SS<R, LEN>;
+    const float wf = a.t.edgemst[c];
+    const R w = (R)(wf > kSoftMinEdgeWeight ? wf : kSoftMinEdgeWeight);
+    const R ou = sign * (a.u0[p] - a.u0[c]) / a.quant;
+    const R ov = sign * (a.u0[a.N + p] - a.u0[a.N + c]) / a.quant;
+    const R ow = sign * (a.u0[2 * a.N + p] - a.u0[2 * a.N + c]) / a.quant;
+    for (int q = threadIdx.x; q < H::ZT; q += blockDim.x) {
+        const R t = (R)(q - H::PAD - LEN);
+        s.zy[q - H::PAD] = (t + ov) * (t + ov) / w;
+        s.zx[q - H::PAD] = (t + ou) * (t + ou) / w;
+        s.zz[q - H::PAD] = (t + ow) * (t + ow) / w;
+    }
+}
+
+// X0 -> X1 -> X2 -> X3 with the current tables; returns the minimum of X3.
+template <typename R, int LEN>
+__device__ R transform(const Cubes<R, LEN>& s, R T) {
+    const R iT2 = (R)kLog2e / T, Tln2 = T * (R)kLn2;
+    __syncthreads();
+    lse_pass<R, LEN, 0>(s.X0, s.X1, s.zy, iT2, Tln2);
+    __syncthreads();
+    lse_pass<R, LEN, 1>(s.X1, s.X2, s.zx, iT2, Tln2);
+    __syncthreads();
+    const R lo = lse_pass<R, LEN, 2>(s.X2, s.X3, s.zz, iT2, Tln2);
+    return block_reduce<R>(lo, s.red, true);
+}
+
+// d/dX3 in Ga -> d/dX0 in Gb (uses Ga as scratch).
+template <typename R, int LEN>
+__device__ void transform_adj(const Cubes<R, LEN>& s, R T) {
+    const R iT2 = (R)kLog2e / T;
+    __syncthreads();
+    lse_adj<R, LEN, 2>(s.X2, s.X3, s.Ga, s.Gb, s.zz, iT2);
+    __syncthreads();
+    lse_adj<R, LEN, 1>(s.X1, s.X2, s.Gb, s.Ga, s.zx, iT2);
+    __syncthreads();
+    lse_adj<R, LEN, 0>(s.X0, s.X1, s.Ga, s.Gb, s.zy, iT2);
+    __syncthreads();
+}
+
+template <typename R, int LEN>
+__global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_up(BPState<R> a, int pos0) {
+    using H = SS<R, LEN>;
+    extern __shared__ __align__(16) unsigned char sp_smem[];
+    const Cubes<R, LEN> s(reinterpret_cast<R*>(sp_smem));
+    const int c = a.t.ordered[pos0 + blockIdx.x];
+    const long long cb = (long long)c * H::L3;
+    const int c0 = a.t.child_start[c], c1 = a.t.child_start[c + 1];
+    for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) {
+        R h = a.cost[cb + l];
+        for (int ci = c0; ci < c1; ci++) h += a.Msg[(long long)a.t.child_list[ci] * H::L3 + l];
+        a.H[cb + l] = h;
+        s.X0[l] = h;
+    }
+    const int p = a.t.parents[c];
+    if (p < 0) {  // the root's full belief is its upward belief
+        for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) a.G[cb + l] = s.X0[l];
+        return;
+    }
+    edge_tables<R, LEN>(a, c, p, (R)1, s);
+    const R mn = transform<R, LEN>(s, a.T);
+    for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) a.Msg[cb + l] = s.X3[l] - mn;
+}
+
+template <typename R, int LEN>
+__global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_down(BPState<R> a, int pos0) {
+    using H = SS<R, LEN>;
+    extern __shared__ __align__(16) unsigned char sp_smem[];
+    const Cubes<R, LEN> s(reinterpret_cast<R*>(sp_smem));
+    const int c = a.t.ordered[pos0 + blockIdx.x];
+    const int p = a.t.parents[c];
+    const long long cb = (long long)c * H::L3, pb = (long long)p * H::L3;
+    for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) s.X0[l] = a.G[pb + l] - a.Msg[cb + l];
+    edge_tables<R, LEN>(a, c, p, (R)-1, s);
+    const R mn = transform<R, LEN>(s, a.T);
+    for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) a.G[cb + l] = a.H[cb + l] + (s.X3[l] - mn);
+}
+
+// Marginal of node c into P (softmax of -g_c / T); e = expected displacement (u, v, w).
+template <typename R, int LEN>
+__device__ void marginal(const BPState<R>& a, int c, R* P, R* red, R e[3]) {
+    using H = SS<R, LEN>;
+    const long long cb = (long long)c * H::L3;
+    R lo = big<R>();
+    for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) {
+        const R g = a.G[cb + l];
+        P[l] = g;
+        lo = g < lo ? g : lo;
+    }
+    const R mn = block_reduce<R>(lo, red, true);
+    const R iT2 = (R)kLog2e / a.T;
+    R z = 0, eu = 0, ev = 0, ew = 0;
+    for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) {
+        const R pw = FastMath<R>::exp2((mn - P[l]) * iT2);
+        P[l] = pw;
+        const int li = l % LEN, lj = (l / LEN) % LEN, lk = l / H::L2;
+        z += pw;
+        eu += pw * (R)(lj - H::HW);
+        ev += pw * (R)(li - H::HW);
+        ew += pw * (R)(lk - H::HW);
+    }
+    z = block_reduce<R>(z, red, false);
+    eu = block_reduce<R>(eu, red, false);
+    ev = block_reduce<R>(ev, red, false);
+    ew = block_reduce<R>(ew, red, false);
+    const R iz = (R)1 / z;
+    for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) P[l] *= iz;
+    __syncthreads();
+    e[0] = eu * iz * a.quant;
+    e[1] = ev * iz * a.quant;
+    e[2] = ew * iz * a.quant;
+}
+
+template <typename R, int LEN>
+__global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_marginals(BPState<R> a) {
+    extern __shared__ __align__(16) unsigned char sp_smem[];
+    const Cubes<R, LEN> s(reinterpret_cast<R*>(sp_smem));
+    const int c = blockIdx.x;
+    R e[3];
+    marginal<R, LEN>(a, c, s.X0, s.red, e);
+    if (threadIdx.x == 0) {
+        a.u1[c] = a.u0[c] + e[0];
+        a.u1[a.N + c] = a.u0[a.N + c] + e[1];
+        a.u1[2 * a.N + c] = a.u0[2 * a.N + c] + e[2];
+    }
+}
+
+template <typename R, int LEN>
+__global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_bwd_down(BPState<R> a, BPGrad<R> b, int pos0) {
+    using H = SS<R, LEN>;
+    extern __shared__ __align__(16) unsigned char sp_smem[];
+    const Cubes<R, LEN> s(reinterpret_cast<R*>(sp_smem));
+    const int c = a.t.ordered[pos0 + blockIdx.x];
+    const long long cb = (long long)c * H::L3;
+    R e[3];
+    marginal<R, LEN>(a, c, s.X3, s.red, e);
+    const R gu = b.gu1[c], gv = b.gu1[a.N + c], gw = b.gu1[2 * a.N + c];
+    const R iT = (R)1 / a.T;
+    const int c0 = a.t.child_start[c], c1 = a.t.child_start[c + 1];
+    for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) {
+        const int li = l % LEN, lj = (l / LEN) % LEN, lk = l / H::L2;
+        const R du = (R)(lj - H::HW) * a.quant - e[0];
+        const R dv = (R)(li - H::HW) * a.quant - e[1];
+        const R dw = (R)(lk - H::HW) * a.quant - e[2];
+        R g = -(s.X3[l] * iT) * (gu * du + gv * dv + gw * dw);
+        for (int ci = c0; ci < c1; ci++) g += b.Dt[(long long)a.t.child_list[ci] * H::L3 + l];
+        b.A[cb + l] = g;
+        s.Ga[l] = g;
+    }
+    const int p = a.t.parents[c];
+    if (p < 0) return;
+    const long long pb = (long long)p * H::L3;
+    for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) s.X0[l] = a.G[pb + l] - a.Msg[cb + l];
+    edge_tables<R, LEN>(a, c, p, (R)-1, s);
+    transform<R, LEN>(s, a.T);
+    transform_adj<R, LEN>(s, a.T);
+    for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) b.Dt[cb + l] = s.Gb[l];
+}
+
+template <typename R, int LEN>
+__global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_bwd_up(BPState<R> a, BPGrad<R> b, int pos0) {
+    using H = SS<R, LEN>;
+    extern __shared__ __align__(16) unsigned char sp_smem[];
+    const Cubes<R, LEN> s(reinterpret_cast<R*>(sp_smem));
+    const int c = a.t.ordered[pos0 + blockIdx.x];
+    const long long cb = (long long)c * H::L3;
+    const int p = a.t.parents[c];
+    if (p < 0) {
+        for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) b.dH[cb + l] = b.A[cb + l];
+        return;
+    }
+    const long long pb = (long long)p * H::L3;
+    for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) {
+        s.X0[l] = a.H[cb + l];
+        s.Ga[l] = b.dH[pb + l] - b.Dt[cb + l];
+    }
+    edge_tables<R, LEN>(a, c, p, (R)1, s);
+    transform<R, LEN>(s, a.T);
+    transform_adj<R, LEN>(s, a.T);
+    for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) b.dH[cb + l] = b.A[cb + l] + s.Gb[l];
+}
+
+template <typename R>
+BPState<R> bp_state(const SoftTree& t, const R* cost, const R* u0, R* u1, R* saved, int hw, float quant, float T) {
+    const size_t L = 2 * (size_t)hw + 1, L3 = L * L * L;
+    BPState<R> a{};
+    a.cost = cost;
+    a.H = saved;
+    a.Msg = saved + (size_t)t.N * L3;
+    a.G = saved + 2 * (size_t)t.N * L3;
+    a.u0 = u0;
+    a.u1 = u1;
+    a.t = SoftTreeDev{t.d_ordered, t.d_parents, t.d_child_start, t.d_child_list, t.d_edgemst};
+    a.N = t.N;
+    a.quant = (R)quant;
+    a.T = (R)T;
+    return a;
+}
+
+template <typename R, int LEN>
+struct Sweeps {
+    using H = SS<R, LEN>;
+    static void forward(const SoftTree& t, const BPState<R>& a, hipStream_t s) {
+        set_smem((const void*)k_sp_up<R, LEN>, H::SMEM);
+        set_smem((const void*)k_sp_down<R, LEN>, H::SMEM);
+        set_smem((const void*)k_sp_marginals<R, LEN>, H::SMEM);
+        for (int d = t.maxd; d >= 0; d--) {
+            const int b = t.depth_start[d], n = t.depth_start[d + 1] - b;
+            k_sp_up<R, LEN><<<n, H::BLOCK, H::SMEM, s>>>(a, b);
+            DEEDS_LAUNCH_CHECK();
+        }
+        for (int d = 1; d <= t.maxd; d++) {
+            const int b = t.depth_start[d], n = t.depth_start[d + 1] - b;
+            k_sp_down<R, LEN><<<n, H::BLOCK, H::SMEM, s>>>(a, b);
+            DEEDS_LAUNCH_CHECK();
+        }
+        k_sp_marginals<R, LEN><<<t.N, H::BLOCK, H::SMEM, s>>>(a);
+        DEEDS_LAUNCH_CHECK();
+    }
+    static void backward(const SoftTree& t, const BPState<R>& a, const BPGrad<R>& g, hipStream_t s) {
+        set_smem((const void*)k_sp_bwd_down<R, LEN>, H::SMEM);
+        set_smem((const void*)k_sp_bwd_up<R, LEN>, H::SMEM);
+        for (int d = t.maxd; d >= 0; d--) {
+            const int b = t.depth_start[d], n = t.depth_start[d + 1] - b;
+            k_sp_bwd_down<R, LEN><<<n, H::BLOCK, H::SMEM, s>>>(a, g, b);
+            DEEDS_LAUNCH_CHECK();
+        }
+        for (int d = 0; d <= t.maxd; d++) {
+            const int b = t.depth_start[d], n = t.depth_start[d + 1] - b;
+            k_sp_bwd_up<R, LEN><<<n, H::BLOCK, H::SMEM, s>>>(a, g, b);
+            DEEDS_LAUNCH_CHECK();
+        }
+    }
+};
+
+template <typename R, template <typename, int> class Op, typename... Args>
+void dispatch_len(int L, Args&&... args) {
+    switch (L) {
+        case 1: Op<R, 1>::run(args...); break;
+        case 3: Op<R, 3>::run(args...); break;
+        case 5: Op<R, 5>::run(args...); break;
+        case 7: Op<R, 7>::run(args...); break;
+        case 9: Op<R, 9>::run(args...); break;
+        case 11: Op<R, 11>::run(args...); break;
+        case 13: Op<R, 13>::run(args...); break;
+        case 15: Op<R, 15>::run(args...); break;
+        case 17: Op<R, 17>::run(args...); break;
+        default: fail(DEEDS_ERR_ARG, "soft BP label radius must be 0.." + std::to_string(kSoftMaxRadius));
+    }
+}
+
+template <typename R, int LEN>
+struct RunForward {
+    static void run(const SoftTree& t, const BPState<R>& a, hipStream_t s) { Sweeps<R, LEN>::forward(t, a, s); }
+};
+template <typename R, int LEN>
+struct RunBackward {
+    static void run(const SoftTree& t, const BPState<R>& a, const BPGrad<R>& g, hipStream_t s) {
+        Sweeps<R, LEN>::backward(t, a, g, s);
+    }
+};
+
+}  // namespace
+
+template <typename R>
+void soft_bp_forward(const SoftTree& t, const R* cost, const R* u0, R* u1, R* saved, int hw, float quant, float T,
+                     hipStream_t s) {
+    const BPState<R> a = bp_state<R>(t, cost, u0, u1, saved, hw, quant, T);
+    dispatch_len<R, RunForward>(2 * hw + 1, t, a, s);
+}
+
+template <typename R>
+void soft_bp_backward(const SoftTree& t, const R* cost, const R* u0, const R* saved, const R* gu1, R* gcost,
+                      R* scratch, int hw, float quant, float T, hipStream_t s) {
+    const BPState<R> a = bp_state<R>(t, cost, u0, nullptr, const_cast<R*>(saved), hw, quant, T);
+    const size_t L = 2 * (size_t)hw + 1;
+    const BPGrad<R> g{gu1, scratch, scratch + (size_t)t.N * L * L * L, gcost};
+    dispatch_len<R, RunBackward>(2 * hw + 1, t, a, g, s);
+}
+
+template void soft_bp_forward<float>(const SoftTree&, const float*, const float*, float*, float*, int, float, float,
+                                     hipStream_t);
+template void soft_bp_forward<double>(const SoftTree&, const double*, const double*, double*, double*, int, float,
+                                      float, hipStream_t);
+template void soft_bp_backward<float>(const SoftTree&, const float*, const float*, const float*, const float*, float*,
+                                      float*, int, float, float, hipStream_t);
+template void soft_bp_backward<double>(const SoftTree&, const double*, const double*, const double*, const double*,
+                                       double*, double*, int, float, float, hipStream_t);
+
+}  // namespace deeds
