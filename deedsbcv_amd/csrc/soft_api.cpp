@@ -17,6 +17,11 @@ void check_real(int real) {
 
 size_t rsize(int real) { return real == DEEDS_REAL_F64 ? sizeof(double) : sizeof(float); }
 
+// Feature arrays are read twelve values at a time with 16-byte vector loads.
+void check_aligned(const void* p) {
+    DEEDS_REQUIRE(((uintptr_t)p & 15) == 0, "feature arrays must be 16-byte aligned");
+}
+
 void check_soft_dims(int m, int n, int o, int qs) {
     DEEDS_REQUIRE(m > 0 && n > 0 && o > 0, "dimensions must be positive");
     DEEDS_REQUIRE((long long)m * n * o <= (1LL << 31) / 12, "volume too large for the soft features");
@@ -27,6 +32,9 @@ void check_soft_grid(int m, int n, int o, int step, int hw, float quant) {
     DEEDS_REQUIRE(m > 0 && n > 0 && o > 0, "dimensions must be positive");
     DEEDS_REQUIRE(step >= 1 && m / step >= 1 && n / step >= 1 && o / step >= 1,
                   "grid step must leave at least one control point per axis");
+    // the cost kernels keep one thread per block sample (at most 256: steps up to 11)
+    const CostPlan plan = datacost_plan(m, n, o, step, hw, quant, 1.0f, 1);
+    DEEDS_REQUIRE(plan.samples <= 256, "grid step too large for the soft cost (at most 256 samples per block)");
     DEEDS_REQUIRE(hw >= 0, "label radius must be >= 0");
     DEEDS_REQUIRE(quant >= 1.0f && (float)(int)quant == quant, "quant must be an integer >= 1");
     DEEDS_REQUIRE((long long)m * n * o <= (1LL << 31) / 12, "volume too large for the soft cost");
@@ -159,6 +167,8 @@ int deeds_ctx_soft_cost(deeds_ctx* ctx, int real, void* d_cost, const void* d_fi
         check_real(real);
         check_soft_grid(m, n, o, step, hw, quant);
         DEEDS_REQUIRE(d_cost && d_fixed_phi && d_moving_phi, "null pointer");
+        check_aligned(d_fixed_phi);
+        check_aligned(d_moving_phi);
         if (real == DEEDS_REAL_F32)
             soft_cost<float>(static_cast<float*>(d_cost), static_cast<const float*>(d_fixed_phi),
                              static_cast<const float*>(d_moving_phi), m, n, o, step, hw, quant, alpha, ctx->stream);
@@ -176,6 +186,9 @@ int deeds_ctx_soft_cost_backward(deeds_ctx* ctx, int real, void* d_gfixed, void*
         check_real(real);
         check_soft_grid(m, n, o, step, hw, quant);
         DEEDS_REQUIRE(d_gfixed && d_gmoving && d_gcost && d_fixed_phi && d_moving_phi, "null pointer");
+        check_aligned(d_fixed_phi);
+        check_aligned(d_moving_phi);
+        check_aligned(d_gmoving);
         const size_t L = 2 * (size_t)hw + 1;
         DEEDS_REQUIRE(L * L * L * rsize(real) <= (size_t)kMaxLds, "label radius too large for the soft cost backward");
         if (real == DEEDS_REAL_F32)

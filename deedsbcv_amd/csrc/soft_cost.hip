@@ -9,14 +9,21 @@
 // voxel, every (sample, label) pair whose clamped displaced position it is.
 #include "soft_common.h"
 
+// The soft cost reproduces no reference rounding: let a*b+c contract to FMA here.
+#pragma clang fp contract(fast)
+
 namespace deeds {
 
 namespace {
 
 using namespace soft_dev;
 
+constexpr int CB = 256;  // threads per cost workgroup
+
 struct CostGeom {
     int m, n, o, m1, n1, o1, step, len, q, hw, si, sj, sk;
+    int ni, nj, nk, ns;       // samples per axis and per node
+    unsigned lati, latj, latk;  // bit r set when block offset r is sampled along the axis
     double scale;
 };
 
@@ -27,74 +34,140 @@ CostGeom cost_geom(int m, int n, int o, int step, int hw, float quant, float alp
     g.m1 = m / step; g.n1 = n / step; g.o1 = o / step;
     g.step = step; g.len = 2 * hw + 1; g.q = (int)quant; g.hw = hw;
     g.si = plan.skip_i; g.sj = plan.skip_j; g.sk = plan.skip_k;
+    g.ni = (step + g.si - 1) / g.si;
+    g.nj = (step + g.sj - 1) / g.sj;
+    g.nk = (step + g.sk - 1) / g.sk;
+    g.ns = g.ni * g.nj * g.nk;
+    for (int r = 0; r < step && r < 32; r++) {
+        if (r % g.si == 0) g.lati |= 1u << r;
+        if (r % g.sj == 0) g.latj |= 1u << r;
+        if (r % g.sk == 0) g.latk |= 1u << r;
+    }
     g.scale = (double)plan.scale;
     return g;
 }
 
-// one thread per (node, label); features are [voxel][12]
+__device__ __forceinline__ void load12(float (&d)[12], const float* __restrict__ p) {
+    const float4* q = reinterpret_cast<const float4*>(p);
+    const float4 a = q[0], b = q[1], c = q[2];
+    d[0] = a.x; d[1] = a.y; d[2] = a.z; d[3] = a.w;
+    d[4] = b.x; d[5] = b.y; d[6] = b.z; d[7] = b.w;
+    d[8] = c.x; d[9] = c.y; d[10] = c.z; d[11] = c.w;
+}
+__device__ __forceinline__ void load12(double (&d)[12], const double* __restrict__ p) {
+    const double2* q = reinterpret_cast<const double2*>(p);
+#pragma unroll
+    for (int e = 0; e < 6; e++) {
+        const double2 a = q[e];
+        d[2 * e] = a.x;
+        d[2 * e + 1] = a.y;
+    }
+}
+
+// Forward: one workgroup per (node, chunk of CB labels), one thread per label.
+// The node's fixed-image samples are staged in LDS in sample order (k, j, i).
 template <typename R>
-__global__ void k_sc_cost(R* __restrict__ cost, const R* __restrict__ F, const R* __restrict__ M, CostGeom g) {
+__global__ __launch_bounds__(CB) void k_sc_cost(R* __restrict__ cost, const R* __restrict__ F,
+                                                const R* __restrict__ M, CostGeom g) {
+    extern __shared__ __align__(16) unsigned char sc_smem[];
+    R* Fs = reinterpret_cast<R*>(sc_smem);
     const int L3 = g.len * g.len * g.len;
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long nodes = (long long)g.m1 * g.n1 * g.o1;
-    if (idx >= nodes * L3) return;
-    const int node = (int)(idx / L3), l = (int)(idx - (long long)node * L3);
+    const int chunks = (L3 + CB - 1) / CB;
+    const int node = blockIdx.x / chunks;
+    const int l = (blockIdx.x - node * chunks) * CB + threadIdx.x;
     const int y = node % g.m1, x = (node / g.m1) % g.n1, z = node / (g.m1 * g.n1);
+    const long long mn = (long long)g.m * g.n;
+    for (int w = threadIdx.x; w < g.ns * 12; w += CB) {
+        const int ch = w % 12, s = w / 12;
+        const int i = (s % g.ni) * g.si, j = ((s / g.ni) % g.nj) * g.sj, k = (s / (g.ni * g.nj)) * g.sk;
+        Fs[w] = F[12 * (y * g.step + i + (long long)(x * g.step + j) * g.m + (z * g.step + k) * mn) + ch];
+    }
+    __syncthreads();
+    if (l >= L3) return;
     const int ly = l % g.len, lx = (l / g.len) % g.len, lz = l / (g.len * g.len);
     const int dy = (ly - g.hw) * g.q, dx = (lx - g.hw) * g.q, dz = (lz - g.hw) * g.q;
-    const long long mn = (long long)g.m * g.n;
     R acc = 0;
+    const R* fs = Fs;
     for (int k = 0; k < g.step; k += g.sk) {
-        const int fk = z * g.step + k, mk = clampi(fk + dz, 0, g.o - 1);
+        const long long zo = clampi(z * g.step + k + dz, 0, g.o - 1) * mn;
         for (int j = 0; j < g.step; j += g.sj) {
-            const int fj = x * g.step + j, mj = clampi(fj + dx, 0, g.n - 1);
-            for (int i = 0; i < g.step; i += g.si) {
-                const int fi = y * g.step + i, mi = clampi(fi + dy, 0, g.m - 1);
-                const R* fp = F + 12 * (fi + (long long)fj * g.m + fk * mn);
-                const R* mp = M + 12 * (mi + (long long)mj * g.m + mk * mn);
+            const long long xo = zo + (long long)clampi(x * g.step + j + dx, 0, g.n - 1) * g.m;
+            for (int i = 0; i < g.step; i += g.si, fs += 12) {
+                R mv[12];
+                load12(mv, M + 12 * (xo + clampi(y * g.step + i + dy, 0, g.m - 1)));
 #pragma unroll
                 for (int ch = 0; ch < 12; ch++) {
-                    const R t = fp[ch] - mp[ch];
-                    acc += t * t;
+                    const R d = fs[ch] - mv[ch];
+                    acc += d * d;
                 }
             }
         }
     }
-    cost[idx] = acc * (R)g.scale;
+    cost[(long long)node * L3 + l] = acc * (R)g.scale;
 }
 
-// dF at the samples of one node (one workgroup per node, one thread per (sample, channel))
+// dF at the samples of one node: one workgroup per node; thread (s, c) sums
+// g[l] * M[clamp(s + d_l)] over label chunk c for all twelve channels, then
+// the chunks are reduced in LDS: dF = 2 a1 (F * sum_l g - sum_l g M).
 template <typename R>
-__global__ void k_sc_dF(R* __restrict__ dF, const R* __restrict__ gc, const R* __restrict__ F,
-                        const R* __restrict__ M, CostGeom g) {
-    extern __shared__ unsigned char sc_smem[];
-    R* gl = reinterpret_cast<R*>(sc_smem);
-    const int node = blockIdx.x;
+__global__ __launch_bounds__(CB) void k_sc_dF(R* __restrict__ dF, const R* __restrict__ gc, const R* __restrict__ F,
+                                              const R* __restrict__ M, CostGeom g) {
+    extern __shared__ __align__(16) unsigned char sc_smem[];
     const int L = g.len, L3 = L * L * L;
-    for (int l = threadIdx.x; l < L3; l += blockDim.x) gl[l] = gc[(long long)node * L3 + l];
+    R* gl = reinterpret_cast<R*>(sc_smem);
+    R* part = gl + ((L3 + 1) & ~1);  // [chunk][sample][13]
+    const int node = blockIdx.x;
+    for (int l = threadIdx.x; l < L3; l += CB) gl[l] = gc[(long long)node * L3 + l];
     __syncthreads();
     const int y = node % g.m1, x = (node / g.m1) % g.n1, z = node / (g.m1 * g.n1);
-    const int ni = (g.step + g.si - 1) / g.si, nj = (g.step + g.sj - 1) / g.sj, nk = (g.step + g.sk - 1) / g.sk;
     const long long mn = (long long)g.m * g.n;
-    for (int w = threadIdx.x; w < ni * nj * nk * 12; w += blockDim.x) {
-        const int ch = w % 12, s = w / 12;
-        const int fi = y * g.step + (s % ni) * g.si, fj = x * g.step + ((s / ni) % nj) * g.sj,
-                  fk = z * g.step + (s / (ni * nj)) * g.sk;
-        const long long fidx = 12 * (fi + (long long)fj * g.m + fk * mn) + ch;
-        const R f = F[fidx];
-        R acc = 0;
-        int l = 0;
-        for (int lz = 0; lz < L; lz++) {
-            const long long zo = clampi(fk + (lz - g.hw) * g.q, 0, g.o - 1) * mn;
-            for (int lx = 0; lx < L; lx++) {
-                const long long xo = zo + (long long)clampi(fj + (lx - g.hw) * g.q, 0, g.n - 1) * g.m;
-                for (int ly = 0; ly < L; ly++, l++) {
-                    const int mi = clampi(fi + (ly - g.hw) * g.q, 0, g.m - 1);
-                    acc += gl[l] * (f - M[12 * (xo + mi) + ch]);
+    const int CH = CB / g.ns;  // label chunks (ns <= CB)
+    const int t = threadIdx.x;
+    if (t < g.ns * CH) {
+        const int s = t % g.ns, c = t / g.ns;
+        const int fi = y * g.step + (s % g.ni) * g.si, fj = x * g.step + ((s / g.ni) % g.nj) * g.sj,
+                  fk = z * g.step + (s / (g.ni * g.nj)) * g.sk;
+        const int l0 = (int)((long long)c * L3 / CH), l1 = (int)((long long)(c + 1) * L3 / CH);
+        R acc[12], G = 0;
+#pragma unroll
+        for (int ch = 0; ch < 12; ch++) acc[ch] = 0;
+        int ly = l0 % L, lx = (l0 / L) % L, lz = l0 / (L * L);
+        long long xo = clampi(fk + (lz - g.hw) * g.q, 0, g.o - 1) * mn +
+                       (long long)clampi(fj + (lx - g.hw) * g.q, 0, g.n - 1) * g.m;
+        for (int l = l0; l < l1; l++) {
+            const R w = gl[l];
+            R mv[12];
+            load12(mv, M + 12 * (xo + clampi(fi + (ly - g.hw) * g.q, 0, g.m - 1)));
+            G += w;
+#pragma unroll
+            for (int ch = 0; ch < 12; ch++) acc[ch] += w * mv[ch];
+            if (++ly == L) {
+                ly = 0;
+                if (++lx == L) {
+                    lx = 0;
+                    lz++;
                 }
+                xo = clampi(fk + (lz - g.hw) * g.q, 0, g.o - 1) * mn +
+                     (long long)clampi(fj + (lx - g.hw) * g.q, 0, g.n - 1) * g.m;
             }
         }
-        dF[fidx] = (R)(2.0 * g.scale) * acc;
+        R* pp = part + 13 * (c * g.ns + s);
+#pragma unroll
+        for (int ch = 0; ch < 12; ch++) pp[ch] = acc[ch];
+        pp[12] = G;
+    }
+    __syncthreads();
+    for (int w = t; w < g.ns * 12; w += CB) {
+        const int s = w / 12, ch = w % 12;
+        R a = 0, G = 0;
+        for (int c = 0; c < CH; c++) {
+            a += part[13 * (c * g.ns + s) + ch];
+            G += part[13 * (c * g.ns + s) + 12];
+        }
+        const int fi = y * g.step + (s % g.ni) * g.si, fj = x * g.step + ((s / g.ni) % g.nj) * g.sj,
+                  fk = z * g.step + (s / (g.ni * g.nj)) * g.sk;
+        const long long fidx = 12 * (fi + (long long)fj * g.m + fk * mn) + ch;
+        dF[fidx] = (R)(2.0 * g.scale) * (F[fidx] * G - a);
     }
 }
 
@@ -109,14 +182,31 @@ __device__ __forceinline__ void axis_range(int t, int dd, int len, int extent, i
     hi = hi > extent - 1 ? extent - 1 : hi;
 }
 
+// Position s = div * step + mod of a sample coordinate that decreases by q per label.
+struct Walk {
+    int s, div, mod;
+    __device__ Walk(int s0, int step) : s(s0), div(s0 / step), mod(s0 - (s0 / step) * step) {}
+    __device__ __forceinline__ void next(int q, int step) {
+        s -= q;
+        mod -= q;
+        while (mod < 0) {
+            mod += step;
+            div--;
+        }
+    }
+    __device__ __forceinline__ bool ok(int cells, unsigned lat) const { return div < cells && ((lat >> mod) & 1u); }
+};
+
 // dM at one moving voxel t (one thread per voxel, all twelve channels): every
 // (sample s, label l) whose clamped displaced position is t contributes
-// 2 a1 g[n(s), l] (M_t - F_s)
+// 2 a1 g[n(s), l] (M_t - F_s).  Away from the volume faces each label has at
+// most one such sample per axis, s = t - d_l, walked without divisions; on a
+// face the clamping gathers a range of samples.
 template <typename R>
-__global__ void k_sc_dM(R* __restrict__ dM, const R* __restrict__ gc, const R* __restrict__ F,
-                        const R* __restrict__ M, CostGeom g) {
+__global__ __launch_bounds__(CB) void k_sc_dM(R* __restrict__ dM, const R* __restrict__ gc,
+                                              const R* __restrict__ F, const R* __restrict__ M, CostGeom g) {
     const long long sz = (long long)g.m * g.n * g.o;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long t = (long long)blockIdx.x * CB + threadIdx.x;
     if (t >= sz) return;
     const int ti = (int)(t % g.m), tj = (int)((t / g.m) % g.n), tk = (int)(t / ((long long)g.m * g.n));
     const int L = g.len, L3 = L * L * L;
@@ -124,37 +214,63 @@ __global__ void k_sc_dM(R* __restrict__ dM, const R* __restrict__ gc, const R* _
     R accg = 0, accf[12];
 #pragma unroll
     for (int ch = 0; ch < 12; ch++) accf[ch] = 0;
-    for (int lz = 0; lz < L; lz++) {
-        int zlo, zhi;
-        axis_range(tk, (lz - g.hw) * g.q, g.o, g.o1 * g.step, zlo, zhi);
-        for (int sk = zlo; sk <= zhi; sk++) {
-            if ((sk % g.step) % g.sk) continue;
-            const int zn = (sk / g.step) * g.m1 * g.n1;
-            for (int lx = 0; lx < L; lx++) {
-                int xlo, xhi;
-                axis_range(tj, (lx - g.hw) * g.q, g.n, g.n1 * g.step, xlo, xhi);
-                for (int sj = xlo; sj <= xhi; sj++) {
-                    if ((sj % g.step) % g.sj) continue;
-                    const int xn = zn + (sj / g.step) * g.m1;
-                    for (int ly = 0; ly < L; ly++) {
-                        int ylo, yhi;
-                        axis_range(ti, (ly - g.hw) * g.q, g.m, g.m1 * g.step, ylo, yhi);
-                        for (int si = ylo; si <= yhi; si++) {
-                            if ((si % g.step) % g.si) continue;
-                            const int node = xn + si / g.step;
-                            const R w = gc[(long long)node * L3 + ly + lx * L + lz * L * L];
-                            const R* fp = F + 12 * (si + (long long)sj * g.m + sk * mn);
-                            accg += w;
+    auto add = [&](int node, int l, long long fvox) {
+        const R w = gc[(long long)node * L3 + l];
+        R fv[12];
+        load12(fv, F + 12 * fvox);
+        accg += w;
 #pragma unroll
-                            for (int ch = 0; ch < 12; ch++) accf[ch] += w * fp[ch];
+        for (int ch = 0; ch < 12; ch++) accf[ch] += w * fv[ch];
+    };
+    const bool face = ti == 0 || ti == g.m - 1 || tj == 0 || tj == g.n - 1 || tk == 0 || tk == g.o - 1;
+    if (!face) {
+        const int qh = g.q * g.hw;
+        const Walk wi0(ti + qh, g.step);
+        Walk wk(tk + qh, g.step);
+        for (int lz = 0; lz < L && wk.s >= 0; lz++, wk.next(g.q, g.step)) {
+            if (!wk.ok(g.o1, g.latk)) continue;
+            Walk wj(tj + qh, g.step);
+            for (int lx = 0; lx < L && wj.s >= 0; lx++, wj.next(g.q, g.step)) {
+                if (!wj.ok(g.n1, g.latj)) continue;
+                const int nzx = (wk.div * g.n1 + wj.div) * g.m1;
+                const long long fzx = (long long)wk.s * mn + (long long)wj.s * g.m;
+                const int lzx = (lz * L + lx) * L;
+                Walk wi = wi0;
+                for (int ly = 0; ly < L && wi.s >= 0; ly++, wi.next(g.q, g.step)) {
+                    if (wi.ok(g.m1, g.lati)) add(nzx + wi.div, lzx + ly, fzx + wi.s);
+                }
+            }
+        }
+    } else {
+        for (int lz = 0; lz < L; lz++) {
+            int zlo, zhi;
+            axis_range(tk, (lz - g.hw) * g.q, g.o, g.o1 * g.step, zlo, zhi);
+            for (int sk = zlo; sk <= zhi; sk++) {
+                if ((sk % g.step) % g.sk) continue;
+                const int zn = (sk / g.step) * g.m1 * g.n1;
+                for (int lx = 0; lx < L; lx++) {
+                    int xlo, xhi;
+                    axis_range(tj, (lx - g.hw) * g.q, g.n, g.n1 * g.step, xlo, xhi);
+                    for (int sj = xlo; sj <= xhi; sj++) {
+                        if ((sj % g.step) % g.sj) continue;
+                        const int xn = zn + (sj / g.step) * g.m1;
+                        for (int ly = 0; ly < L; ly++) {
+                            int ylo, yhi;
+                            axis_range(ti, (ly - g.hw) * g.q, g.m, g.m1 * g.step, ylo, yhi);
+                            for (int si = ylo; si <= yhi; si++) {
+                                if ((si % g.step) % g.si) continue;
+                                add(xn + si / g.step, ly + lx * L + lz * L * L, si + (long long)sj * g.m + sk * mn);
+                            }
                         }
                     }
                 }
             }
         }
     }
+    R mv[12];
+    load12(mv, M + 12 * t);
 #pragma unroll
-    for (int ch = 0; ch < 12; ch++) dM[12 * t + ch] = (R)(2.0 * g.scale) * (M[12 * t + ch] * accg - accf[ch]);
+    for (int ch = 0; ch < 12; ch++) dM[12 * t + ch] = (R)(2.0 * g.scale) * (mv[ch] * accg - accf[ch]);
 }
 
 template <typename R>
@@ -171,9 +287,14 @@ template <typename R>
 void soft_cost(R* cost, const R* F, const R* M, int m, int n, int o, int step, int hw, float quant, float alpha,
                hipStream_t s) {
     const CostGeom g = cost_geom(m, n, o, step, hw, quant, alpha);
-    const long long total = (long long)g.m1 * g.n1 * g.o1 * g.len * g.len * g.len;
-    if (total == 0) return;
-    k_sc_cost<R><<<ceil_div(total, 256), 256, 0, s>>>(cost, F, M, g);
+    const long long nodes = (long long)g.m1 * g.n1 * g.o1;
+    const int L3 = g.len * g.len * g.len;
+    if (nodes == 0) return;
+    const long long blocks = nodes * ((L3 + CB - 1) / CB);
+    if (blocks > 0x7fffffffLL) fail(DEEDS_ERR_ARG, "soft cost volume too large");
+    const size_t smem = (size_t)g.ns * 12 * sizeof(R);
+    set_smem((const void*)k_sc_cost<R>, smem);
+    k_sc_cost<R><<<(unsigned)blocks, CB, smem, s>>>(cost, F, M, g);
     DEEDS_LAUNCH_CHECK();
 }
 
@@ -187,12 +308,13 @@ void soft_cost_backward(R* dF, R* dM, const R* gcost, const R* F, const R* M, in
     DEEDS_LAUNCH_CHECK();
     const int nodes = g.m1 * g.n1 * g.o1;
     if (nodes > 0) {
-        const size_t smem = (size_t)g.len * g.len * g.len * sizeof(R);
+        const size_t L3 = (size_t)g.len * g.len * g.len;
+        const size_t smem = (((L3 + 1) & ~(size_t)1) + (size_t)(CB / g.ns) * g.ns * 13) * sizeof(R);
         set_smem((const void*)k_sc_dF<R>, smem);
-        k_sc_dF<R><<<nodes, 256, smem, s>>>(dF, gcost, F, M, g);
+        k_sc_dF<R><<<nodes, CB, smem, s>>>(dF, gcost, F, M, g);
         DEEDS_LAUNCH_CHECK();
     }
-    k_sc_dM<R><<<ceil_div(sz, 256), 256, 0, s>>>(dM, gcost, F, M, g);
+    k_sc_dM<R><<<ceil_div(sz, CB), CB, 0, s>>>(dM, gcost, F, M, g);
     DEEDS_LAUNCH_CHECK();
 }
 

@@ -215,18 +215,32 @@ def tree_of(fixed, step):
     return api.primsGraph(fixed.detach().to("cpu", _torch().float32).numpy(), step)
 
 
-def soft_register(fixed, moving, grid=6, radius=6, quant=2, alpha=1.6, T=0.5, tree=None, u0=None):
+_tree_pool = None
+
+
+def soft_register(fixed, moving, grid=6, radius=6, quant=2, alpha=1.6, T=0.5, tree=None, u0=None, tree_image=None):
     """One soft registration level: expected displacements (3, o1, n1, m1) of
-    the control grid, differentiable with respect to both images."""
+    the control grid, differentiable with respect to both images.  Without
+    `tree`, the MST of the fixed image (or of the host array `tree_image`) is
+    built on a host thread while the features and the cost run on the GPU."""
+    global _tree_pool
     torch = _torch()
     o, n, m = fixed.shape
     qs = mind_step(quant)
+    pending = None
     if tree is None:
-        tree = tree_of(fixed, grid)
+        import concurrent.futures as cf
+        if _tree_pool is None:
+            _tree_pool = cf.ThreadPoolExecutor(max_workers=1)
+        from . import api
+        img = tree_image if tree_image is not None else fixed.detach().to("cpu", torch.float32).numpy()
+        pending = _tree_pool.submit(api.primsGraph, img, grid)
     F = soft_features(fixed, qs)
     M = soft_features(moving, qs)
     C = soft_cost(F, M, grid, radius, quant, alpha)
     N = C.shape[0]
+    if pending is not None:
+        tree = pending.result()
     if u0 is None:
         u0 = torch.zeros((3, N), dtype=fixed.dtype, device=fixed.device)
     u1 = soft_bp(C, u0.reshape(3, N), tree, (o, n, m), grid, radius, quant, T)
