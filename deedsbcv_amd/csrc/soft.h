@@ -75,9 +75,14 @@ void soft_features_backward(R* gim, const R* gphi, const R* im, int m, int n, in
 template <typename R>
 void soft_cost(R* cost, const R* F, const R* M, int m, int n, int o, int step, int hw, float quant, float alpha,
                hipStream_t s);
+// part: soft_cost_part_count(...) reals of workspace (sums of the padded positions).
 template <typename R>
 void soft_cost_backward(R* dF, R* dM, const R* gcost, const R* F, const R* M, int m, int n, int o, int step, int hw,
-                        float quant, float alpha, hipStream_t s);
+                        float quant, float alpha, R* part, hipStream_t s);
+inline size_t soft_cost_part_count(int m, int n, int o, int hw, float quant) {
+    const size_t P = (size_t)quant * hw;
+    return 13 * (m + 2 * P) * (n + 2 * P) * (o + 2 * P);
+}
 
 // saved: 3*N*L^3 reals (upward beliefs, upward messages, full beliefs).
 template <typename R>

@@ -191,16 +191,17 @@ int deeds_ctx_soft_cost_backward(deeds_ctx* ctx, int real, void* d_gfixed, void*
         check_aligned(d_gmoving);
         const size_t L = 2 * (size_t)hw + 1;
         DEEDS_REQUIRE(L * L * L * rsize(real) <= (size_t)kMaxLds, "label radius too large for the soft cost backward");
+        const size_t part = soft_cost_part_count(m, n, o, hw, quant);
         if (real == DEEDS_REAL_F32)
             soft_cost_backward<float>(static_cast<float*>(d_gfixed), static_cast<float*>(d_gmoving),
                                       static_cast<const float*>(d_gcost), static_cast<const float*>(d_fixed_phi),
                                       static_cast<const float*>(d_moving_phi), m, n, o, step, hw, quant, alpha,
-                                      ctx->stream);
+                                      ctx->get<float>("soft.cost.part", part), ctx->stream);
         else
             soft_cost_backward<double>(static_cast<double*>(d_gfixed), static_cast<double*>(d_gmoving),
                                        static_cast<const double*>(d_gcost), static_cast<const double*>(d_fixed_phi),
                                        static_cast<const double*>(d_moving_phi), m, n, o, step, hw, quant, alpha,
-                                       ctx->stream);
+                                       ctx->get<double>("soft.cost.part", part), ctx->stream);
     });
 }
 

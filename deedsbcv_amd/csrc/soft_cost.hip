@@ -5,8 +5,9 @@
 // with the block samples, clamping and a1 of the Hamming cost (dataCostD.h:141-196).
 // Features are [voxel][12]; the cost volume is node-major [node][label].
 // Backward: dF at the samples of each node (a correlation of the node's cost
-// gradient with the moving features), dM by gathering, for each moving
-// voxel, every (sample, label) pair whose clamped displaced position it is.
+// gradient with the moving features); dM by gathering, for every position of
+// the zero-padded box, the (sample, label) pairs displaced onto it, then
+// folding the pad region onto the face voxels it clamps to.
 #include "soft_common.h"
 
 // The soft cost reproduces no reference rounding: let a*b+c contract to FMA here.
@@ -171,17 +172,6 @@ __global__ __launch_bounds__(CB) void k_sc_dF(R* __restrict__ dF, const R* __res
     }
 }
 
-// Sample coordinates s along one axis with clamp(s + dd, 0, len-1) == t, restricted
-// to the control-grid region [0, extent).
-__device__ __forceinline__ void axis_range(int t, int dd, int len, int extent, int& lo, int& hi) {
-    lo = t - dd;
-    hi = t - dd;
-    if (t == 0) lo = -(1 << 20);
-    if (t == len - 1) hi = 1 << 20;
-    lo = lo < 0 ? 0 : lo;
-    hi = hi > extent - 1 ? extent - 1 : hi;
-}
-
 // Position s = div * step + mod of a sample coordinate that decreases by q per label.
 struct Walk {
     int s, div, mod;
@@ -197,80 +187,130 @@ struct Walk {
     __device__ __forceinline__ bool ok(int cells, unsigned lat) const { return div < cells && ((lat >> mod) & 1u); }
 };
 
-// dM at one moving voxel t (one thread per voxel, all twelve channels): every
+// dM at a moving voxel t (one thread per voxel, all twelve channels): every
 // (sample s, label l) whose clamped displaced position is t contributes
-// 2 a1 g[n(s), l] (M_t - F_s).  Away from the volume faces each label has at
-// most one such sample per axis, s = t - d_l, walked without divisions; on a
-// face the clamping gathers a range of samples.
+// 2 a1 g[n(s), l] (M_t - F_s).
 template <typename R>
-__global__ __launch_bounds__(CB) void k_sc_dM(R* __restrict__ dM, const R* __restrict__ gc,
-                                              const R* __restrict__ F, const R* __restrict__ M, CostGeom g) {
-    const long long sz = (long long)g.m * g.n * g.o;
-    const long long t = (long long)blockIdx.x * CB + threadIdx.x;
-    if (t >= sz) return;
-    const int ti = (int)(t % g.m), tj = (int)((t / g.m) % g.n), tk = (int)(t / ((long long)g.m * g.n));
-    const int L = g.len, L3 = L * L * L;
-    const long long mn = (long long)g.m * g.n;
-    R accg = 0, accf[12];
+struct DMAcc {
+    R g = 0, f[12];
+    __device__ DMAcc() {
 #pragma unroll
-    for (int ch = 0; ch < 12; ch++) accf[ch] = 0;
-    auto add = [&](int node, int l, long long fvox) {
+        for (int ch = 0; ch < 12; ch++) f[ch] = 0;
+    }
+    __device__ __forceinline__ void add(const R* __restrict__ gc, const R* __restrict__ F, int L3, int node, int l,
+                                        long long fvox) {
         const R w = gc[(long long)node * L3 + l];
         R fv[12];
         load12(fv, F + 12 * fvox);
-        accg += w;
+        g += w;
 #pragma unroll
-        for (int ch = 0; ch < 12; ch++) accf[ch] += w * fv[ch];
-    };
-    const bool face = ti == 0 || ti == g.m - 1 || tj == 0 || tj == g.n - 1 || tk == 0 || tk == g.o - 1;
-    if (!face) {
-        const int qh = g.q * g.hw;
-        const Walk wi0(ti + qh, g.step);
-        Walk wk(tk + qh, g.step);
-        for (int lz = 0; lz < L && wk.s >= 0; lz++, wk.next(g.q, g.step)) {
-            if (!wk.ok(g.o1, g.latk)) continue;
-            Walk wj(tj + qh, g.step);
-            for (int lx = 0; lx < L && wj.s >= 0; lx++, wj.next(g.q, g.step)) {
-                if (!wj.ok(g.n1, g.latj)) continue;
-                const int nzx = (wk.div * g.n1 + wj.div) * g.m1;
-                const long long fzx = (long long)wk.s * mn + (long long)wj.s * g.m;
-                const int lzx = (lz * L + lx) * L;
-                Walk wi = wi0;
-                for (int ly = 0; ly < L && wi.s >= 0; ly++, wi.next(g.q, g.step)) {
-                    if (wi.ok(g.m1, g.lati)) add(nzx + wi.div, lzx + ly, fzx + wi.s);
-                }
-            }
-        }
-    } else {
-        for (int lz = 0; lz < L; lz++) {
-            int zlo, zhi;
-            axis_range(tk, (lz - g.hw) * g.q, g.o, g.o1 * g.step, zlo, zhi);
-            for (int sk = zlo; sk <= zhi; sk++) {
-                if ((sk % g.step) % g.sk) continue;
-                const int zn = (sk / g.step) * g.m1 * g.n1;
-                for (int lx = 0; lx < L; lx++) {
-                    int xlo, xhi;
-                    axis_range(tj, (lx - g.hw) * g.q, g.n, g.n1 * g.step, xlo, xhi);
-                    for (int sj = xlo; sj <= xhi; sj++) {
-                        if ((sj % g.step) % g.sj) continue;
-                        const int xn = zn + (sj / g.step) * g.m1;
-                        for (int ly = 0; ly < L; ly++) {
-                            int ylo, yhi;
-                            axis_range(ti, (ly - g.hw) * g.q, g.m, g.m1 * g.step, ylo, yhi);
-                            for (int si = ylo; si <= yhi; si++) {
-                                if ((si % g.step) % g.si) continue;
-                                add(xn + si / g.step, ly + lx * L + lz * L * L, si + (long long)sj * g.m + sk * mn);
-                            }
-                        }
-                    }
-                }
+        for (int ch = 0; ch < 12; ch++) f[ch] += w * fv[ch];
+    }
+    __device__ __forceinline__ void store(R* __restrict__ dM, const R* __restrict__ M, long long t, double scale) {
+        R mv[12];
+        load12(mv, M + 12 * t);
+#pragma unroll
+        for (int ch = 0; ch < 12; ch++) dM[12 * t + ch] = (R)(2.0 * scale) * (mv[ch] * g - f[ch]);
+    }
+};
+
+// Pull over the padded box: every position P = s + d_l of a (sample, label)
+// pair lies in [-pad, len-1+pad] per axis (pad = q*hw), and for a padded
+// position P each label has at most one sample per axis, s = P - d_l, walked
+// without divisions.  Voxels strictly inside the volume receive exactly the
+// pairs at their own position and write dM; positions on or beyond a face
+// write their sums (g, g*F) to `part` for k_sc_dM_fold, since the clamping
+// sends all of them to the face voxel they clamp to.
+template <typename R>
+__global__ __launch_bounds__(CB) void k_sc_dM_pull(R* __restrict__ dM, R* __restrict__ part, const R* __restrict__ gc,
+                                                   const R* __restrict__ F, const R* __restrict__ M, CostGeom g) {
+    const int P = g.q * g.hw;
+    const int mp = g.m + 2 * P, np = g.n + 2 * P, op = g.o + 2 * P;
+    const long long e = (long long)blockIdx.x * CB + threadIdx.x;
+    if (e >= (long long)mp * np * op) return;
+    const int ip = (int)(e % mp), jp = (int)((e / mp) % np), kp = (int)(e / ((long long)mp * np));
+    const int L = g.len, L3 = L * L * L;
+    const long long mn = (long long)g.m * g.n;
+    DMAcc<R> acc;
+    const Walk wi0(ip, g.step);  // s = (ip - P) - d_l = ip - q*l
+    Walk wk(kp, g.step);
+    for (int lz = 0; lz < L && wk.s >= 0; lz++, wk.next(g.q, g.step)) {
+        if (!wk.ok(g.o1, g.latk)) continue;
+        Walk wj(jp, g.step);
+        for (int lx = 0; lx < L && wj.s >= 0; lx++, wj.next(g.q, g.step)) {
+            if (!wj.ok(g.n1, g.latj)) continue;
+            const int nzx = (wk.div * g.n1 + wj.div) * g.m1;
+            const long long fzx = (long long)wk.s * mn + (long long)wj.s * g.m;
+            const int lzx = (lz * L + lx) * L;
+            Walk wi = wi0;
+            for (int ly = 0; ly < L && wi.s >= 0; ly++, wi.next(g.q, g.step)) {
+                if (wi.ok(g.m1, g.lati)) acc.add(gc, F, L3, nzx + wi.div, lzx + ly, fzx + wi.s);
             }
         }
     }
-    R mv[12];
-    load12(mv, M + 12 * t);
+    const int ti = ip - P, tj = jp - P, tk = kp - P;
+    if (ti >= 1 && ti <= g.m - 2 && tj >= 1 && tj <= g.n - 2 && tk >= 1 && tk <= g.o - 2) {
+        acc.store(dM, M, ti + (long long)tj * g.m + tk * mn, g.scale);
+    } else {
+        R* pp = part + 13 * e;
 #pragma unroll
-    for (int ch = 0; ch < 12; ch++) dM[12 * t + ch] = (R)(2.0 * g.scale) * (mv[ch] * accg - accf[ch]);
+        for (int ch = 0; ch < 12; ch++) pp[ch] = acc.f[ch];
+        pp[12] = acc.g;
+    }
+}
+
+// dM of the face voxels: the sums of every padded position that clamps to
+// the voxel (a line, square or cube of the pad region beyond the faces).
+// Faces are enumerated without overlap: both i-faces whole, then the
+// j-faces and k-faces without the voxels already covered.
+template <typename R>
+__global__ __launch_bounds__(CB) void k_sc_dM_fold(R* __restrict__ dM, const R* __restrict__ part,
+                                                   const R* __restrict__ M, CostGeom g) {
+    const long long e0 = (long long)blockIdx.x * CB + threadIdx.x;
+    const long long mi = g.m > 2 ? g.m - 2 : 0, ni = g.n > 2 ? g.n - 2 : 0;
+    const long long ci = 2LL * g.n * g.o, cj = 2 * mi * g.o, ck = 2 * mi * ni;
+    long long e = e0;
+    int ti, tj, tk;
+    if (e < ci) {
+        const long long half = (long long)g.n * g.o;
+        const int side = (int)(e / half);
+        e -= side * half;
+        ti = side ? g.m - 1 : 0;
+        tj = (int)(e % g.n);
+        tk = (int)(e / g.n);
+    } else if ((e -= ci) < cj) {
+        const long long half = mi * g.o;
+        const int side = (int)(e / half);
+        e -= side * half;
+        tj = side ? g.n - 1 : 0;
+        ti = 1 + (int)(e % mi);
+        tk = (int)(e / mi);
+    } else if ((e -= cj) < ck) {
+        const long long half = mi * ni;
+        const int side = (int)(e / half);
+        e -= side * half;
+        tk = side ? g.o - 1 : 0;
+        ti = 1 + (int)(e % mi);
+        tj = 1 + (int)(e / mi);
+    } else {
+        return;
+    }
+    const int P = g.q * g.hw;
+    const int mp = g.m + 2 * P, np = g.n + 2 * P;
+    // padded index ranges that clamp to (ti, tj, tk)
+    const int i0 = ti == 0 ? 0 : ti + P, i1 = ti == g.m - 1 ? g.m - 1 + 2 * P : ti + P;
+    const int j0 = tj == 0 ? 0 : tj + P, j1 = tj == g.n - 1 ? g.n - 1 + 2 * P : tj + P;
+    const int k0 = tk == 0 ? 0 : tk + P, k1 = tk == g.o - 1 ? g.o - 1 + 2 * P : tk + P;
+    DMAcc<R> acc;
+    for (int kp = k0; kp <= k1; kp++)
+        for (int jp = j0; jp <= j1; jp++)
+            for (int ip = i0; ip <= i1; ip++) {
+                const R* pp = part + 13 * (ip + (long long)jp * mp + (long long)kp * mp * np);
+#pragma unroll
+                for (int ch = 0; ch < 12; ch++) acc.f[ch] += pp[ch];
+                acc.g += pp[12];
+            }
+    acc.store(dM, M, ti + (long long)tj * g.m + (long long)tk * g.m * g.n, g.scale);
 }
 
 template <typename R>
@@ -300,7 +340,7 @@ void soft_cost(R* cost, const R* F, const R* M, int m, int n, int o, int step, i
 
 template <typename R>
 void soft_cost_backward(R* dF, R* dM, const R* gcost, const R* F, const R* M, int m, int n, int o, int step, int hw,
-                        float quant, float alpha, hipStream_t s) {
+                        float quant, float alpha, R* part, hipStream_t s) {
     const CostGeom g = cost_geom(m, n, o, step, hw, quant, alpha);
     const long long sz = (long long)m * n * o;
     // voxels outside every sampled block get no gradient from the cost
@@ -314,14 +354,21 @@ void soft_cost_backward(R* dF, R* dM, const R* gcost, const R* F, const R* M, in
         k_sc_dF<R><<<nodes, CB, smem, s>>>(dF, gcost, F, M, g);
         DEEDS_LAUNCH_CHECK();
     }
-    k_sc_dM<R><<<ceil_div(sz, CB), CB, 0, s>>>(dM, gcost, F, M, g);
+    // pull over the padded box (interior voxels finish there), then fold the pad region onto the faces
+    const long long P = (long long)g.q * g.hw;
+    const long long padded = (m + 2 * P) * (n + 2 * P) * (o + 2 * P);
+    k_sc_dM_pull<R><<<ceil_div(padded, CB), CB, 0, s>>>(dM, part, gcost, F, M, g);
+    DEEDS_LAUNCH_CHECK();
+    const long long mi = m > 2 ? m - 2 : 0, ni = n > 2 ? n - 2 : 0;
+    const long long faces = 2LL * n * o + 2 * mi * o + 2 * mi * ni;
+    k_sc_dM_fold<R><<<ceil_div(faces, CB), CB, 0, s>>>(dM, part, M, g);
     DEEDS_LAUNCH_CHECK();
 }
 
 #define DEEDS_SOFT_COST(R)                                                                                     \
     template void soft_cost<R>(R*, const R*, const R*, int, int, int, int, int, float, float, hipStream_t);         \
     template void soft_cost_backward<R>(R*, R*, const R*, const R*, const R*, int, int, int, int, int, float, float, \
-                                        hipStream_t);
+                                        R*, hipStream_t);
 
 DEEDS_SOFT_COST(float)
 DEEDS_SOFT_COST(double)
