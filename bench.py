@@ -354,6 +354,8 @@ def main():
         "host_tree_wait_level_ms": [1e3 * x for x in np.mean([t["tree_wait_level_s"] for t in timings], axis=0)],
         "host_tree_build_level_ms": [1e3 * x for x in np.mean([t["tree_build_level_s"] for t in timings], axis=0)],
         "host_stats_ms": 1e3 * float(np.mean([t["stats_s"] for t in timings])),
+        "gpu_idle_before_bp_level_ms": [1e3 * x for x in np.mean([t["bp_idle_level_s"] for t in timings], axis=0)],
+        "bp_level_ms_fwd_stream": [1e3 * x for x in np.mean([t["bp_level_s"] for t in timings], axis=0)],
         "host_cpus": len(os.sched_getaffinity(0)),
         "datacost_level_ms": [1e3 * x for x in np.mean([t["datacost_level_s"] for t in timings], axis=0)],
         "roofline": {

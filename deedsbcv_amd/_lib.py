@@ -29,7 +29,8 @@ class Timing(ctypes.Structure):
                 ("datacost_level_evals", ctypes.c_double * MAX_LEVELS),
                 ("tree_build_s", ctypes.c_double), ("tree_wait_s", ctypes.c_double),
                 ("tree_build_level_s", ctypes.c_double * MAX_LEVELS),
-                ("tree_wait_level_s", ctypes.c_double * MAX_LEVELS), ("stats_s", ctypes.c_double)]
+                ("tree_wait_level_s", ctypes.c_double * MAX_LEVELS), ("stats_s", ctypes.c_double),
+                ("bp_idle_level_s", ctypes.c_double * MAX_LEVELS), ("bp_level_s", ctypes.c_double * MAX_LEVELS)]
 
     def as_dict(self):
         L = self.levels
@@ -42,7 +43,8 @@ class Timing(ctypes.Structure):
                 "datacost_level_evals": list(self.datacost_level_evals[:L]),
                 "tree_build_s": self.tree_build_s, "tree_wait_s": self.tree_wait_s,
                 "tree_build_level_s": list(self.tree_build_level_s[:L]),
-                "tree_wait_level_s": list(self.tree_wait_level_s[:L]), "stats_s": self.stats_s}
+                "tree_wait_level_s": list(self.tree_wait_level_s[:L]), "stats_s": self.stats_s,
+                "bp_idle_level_s": list(self.bp_idle_level_s[:L]), "bp_level_s": list(self.bp_level_s[:L])}
 
 
 class Pair(ctypes.Structure):
@@ -65,6 +67,7 @@ SIGNATURES = {
     "deeds_descriptor": [V, V, I, I, I, I],
     "deeds_datacost": [V, V, V, I, I, I, I, I, F, F, I],
     "deeds_prims_graph": [V, V, V, V, I, I, I, I],
+    "deeds_image_stats": [V, ctypes.c_longlong, V, V],
     "deeds_regularisation": [V, V, V, V, V, V, V, I, I, F, V, V, V, I, I, I],
     "deeds_upsample_deformations": [V, V, V, V, V, V, I, I, I, I, I, I],
     "deeds_consistent_mapping": [V, V, V, V, V, V, I, I, I, I],
@@ -79,6 +82,7 @@ SIGNATURES = {
     "deeds_apply_seg": [V, V, V, V, I, I, I, I],
     "deeds_apply_float": [V, V, V, V, I, I, I, I],
     "deeds_ctx_descriptor": [V, V, V, I, I, I, I],
+    "deeds_ctx_image_stats": [V, V, ctypes.c_longlong, V, V],
     "deeds_ctx_datacost": [V, V, V, V, I, I, I, I, I, F, F, I],
     "deeds_ctx_regularisation": [V, V, V, V, V, V, V, V, I, I, F, V, V, V, I, I, I, V],
     "deeds_ctx_register": [V, V, V, V, I, I, I, ctypes.POINTER(Schedule), V, V, ctypes.POINTER(Timing)],

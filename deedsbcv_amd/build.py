@@ -34,7 +34,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-I", str(ROOT / "i
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 LIB_SOURCES = ["mind.hip", "datacost.hip", "mst.hip", "bp.hip", "warp.hip", "register.hip", "api.cpp",
-               "mst_host.cpp", "batch.cpp", "soft_features.hip", "soft_cost.hip",
+               "mst_host.cpp", "stats.hip", "stats_host.cpp", "batch.cpp", "soft_features.hip", "soft_cost.hip",
                "soft_bp.hip", "soft_api.cpp"]
 TOOLS = {"deedsBCV": "tool_deeds.cpp", "applyBCV": "tool_apply.cpp", "applyBCVfloat": "tool_apply.cpp"}
 

@@ -9,6 +9,7 @@
 #include "common.h"
 #include "kernels.h"
 #include "mst_host.h"
+#include "stats.h"
 
 using namespace deeds;
 
@@ -22,6 +23,8 @@ void* deeds_ctx::buf(const std::string& name, size_t bytes) {
     if (b.ptr) {
         DEEDS_HIP(hipStreamSynchronize(stream));
         if (aux_stream) DEEDS_HIP(hipStreamSynchronize(aux_stream));
+        for (hipStream_t ss : side_stream)
+            if (ss) DEEDS_HIP(hipStreamSynchronize(ss));
         DEEDS_HIP(hipFree(b.ptr));
         b.ptr = nullptr;
         b.bytes = 0;
@@ -41,6 +44,8 @@ void* deeds_ctx::buf(const std::string& name, size_t bytes) {
 void deeds_ctx::release_all() {
     if (stream) (void)hipStreamSynchronize(stream);
     if (aux_stream) (void)hipStreamSynchronize(aux_stream);
+    for (hipStream_t ss : side_stream)
+        if (ss) (void)hipStreamSynchronize(ss);
     for (auto& kv : ws)
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);
     ws.clear();
@@ -106,11 +111,14 @@ int deeds_ctx_create(int device, deeds_ctx** out) {
         DEEDS_HIP(hipSetDevice(device));
         DEEDS_HIP(hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking));
         DEEDS_HIP(hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking));
+        for (hipStream_t& ss : ctx->side_stream) DEEDS_HIP(hipStreamCreateWithFlags(&ss, hipStreamNonBlocking));
         ctx->stream = ctx->own_stream;
     });
     if (rc != DEEDS_OK) {
         if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
         if (ctx->aux_stream) (void)hipStreamDestroy(ctx->aux_stream);
+        for (hipStream_t ss : ctx->side_stream)
+            if (ss) (void)hipStreamDestroy(ss);
         delete ctx;
         return rc;
     }
@@ -124,6 +132,8 @@ int deeds_ctx_destroy(deeds_ctx* ctx) {
     ctx->release_all();
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     if (ctx->aux_stream) (void)hipStreamDestroy(ctx->aux_stream);
+    for (hipStream_t ss : ctx->side_stream)
+        if (ss) (void)hipStreamDestroy(ss);
     delete ctx;
     return DEEDS_OK;
 }
@@ -370,6 +380,28 @@ int deeds_prims_graph(const float* im, int* ordered, int* parents, float* edgems
         std::copy(t.ordered.begin(), t.ordered.end(), ordered);
         std::copy(t.parents.begin(), t.parents.end(), parents);
         std::copy(t.edgemst.begin(), t.edgemst.end(), edgemst);
+    });
+}
+
+int deeds_ctx_image_stats(deeds_ctx* ctx, const float* d_im, long long count, float* mean, float* sd) {
+    if (!ctx) return DEEDS_ERR_ARG;
+    return guarded(ctx, [&] {
+        DEEDS_REQUIRE(d_im && mean && sd, "null pointer");
+        DEEDS_REQUIRE(count > 0, "count must be positive");
+        StatsScratch sc;
+        const size_t bytes = StatsScratch::bytes(count);
+        sc.dev = ctx->buf("stats.dev", bytes);
+        DEEDS_HIP(hipHostMalloc(&sc.host, bytes, hipHostMallocDefault));
+        std::unique_ptr<void, hipError_t (*)(void*)> hold(sc.host, hipHostFree);
+        image_stats_device(d_im, count, sc, ctx->stream, mean, sd);
+    });
+}
+
+int deeds_image_stats(const float* im, long long count, float* mean, float* sd) {
+    return guarded(nullptr, [&] {
+        DEEDS_REQUIRE(im && mean && sd, "null pointer");
+        DEEDS_REQUIRE(count > 0, "count must be positive");
+        image_stats(im, count, mean, sd);
     });
 }
 

@@ -48,6 +48,7 @@ struct deeds_ctx {
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;     // active stream (own or caller's)
     hipStream_t aux_stream = nullptr;  // second registration direction
+    hipStream_t side_stream[2] = {nullptr, nullptr};  // small work issued from host helper threads (image stats)
     std::string err;
     std::map<std::string, deeds::DevBuf> ws;
 

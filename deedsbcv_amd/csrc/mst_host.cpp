@@ -87,19 +87,6 @@ const int kNbK[6] = {0, 0, 0, 0, -1, 1};
 
 }  // namespace
 
-void image_stats(const float* im, long long count, float* mean, float* sd) {
-    float mu = 0.0f;
-    for (long long t = 0; t < count; t++) mu = mu + im[t];
-    mu = mu / (float)count;
-    float var = 0.0f;
-    for (long long t = 0; t < count; t++) {
-        double d = (double)(im[t] - mu);
-        var = (float)((double)var + d * d);
-    }
-    *mean = mu;
-    *sd = std::sqrt(var / (float)count);
-}
-
 namespace {
 
 // edge cost per voxel pair, c = SAD / step^3 (primsMST.h:112-114)

@@ -39,8 +39,9 @@ struct Tree {
 // (ordered, parents, depth_start, prog_start, prog).
 size_t tree_device_ints(int N);
 
-// primsMST.h:101-110: float mean and std of the image, accumulated exactly as
-// the reference does (sequential float sum; float += double square).
+// primsMST.h:101-110: float mean and std of the image, bit-identical to the
+// reference's sequential accumulation (float sum; float += double square),
+// computed in parallel chunks (stats_host.cpp).
 void image_stats(const float* im, long long count, float* mean, float* sd);
 
 // primsMST.h:112-223 given the SAD edge costs ec[node + nb*N] of the (m1,n1,o1)

@@ -19,6 +19,7 @@
 #include "common.h"
 #include "kernels.h"
 #include "mst_host.h"
+#include "stats.h"
 
 namespace deeds {
 
@@ -147,6 +148,9 @@ void register_pair(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, 
     // events bracketing every data-cost kernel launch, on its own stream
     std::vector<std::unique_ptr<Event>> ev_dc;
     for (int i = 0; i < 4 * L; i++) ev_dc.emplace_back(new Event());
+    // marker on each direction's stream right before its BP is enqueued
+    std::vector<std::unique_ptr<Event>> ev_pre;
+    for (int i = 0; i < 2 * L; i++) ev_pre.emplace_back(new Event());
     auto DC = [&](int l, int dir, int end) { return ev_dc[4 * l + 2 * dir + end]->e; };
 
     DEEDS_HIP(hipEventRecord(ev_start.e, s0));
@@ -164,7 +168,31 @@ void register_pair(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, 
         launch_warp_affine_grid(warped0, im1, A, zero3, zero3 + 1, zero3 + 2, m, n, o, 1, 1, 1, s0);
     }
 
-    // ---- MST inputs: edge SADs of every level on both images, image stats ----
+    // ---- image statistics of both images (exact, chunks decomposed on the side streams) ----
+    Event ev_w0;
+    DEEDS_HIP(hipEventRecord(ev_w0.e, s0));
+    const size_t stats_bytes = (StatsScratch::bytes((long long)sz) + 255) & ~(size_t)255;
+    char* d_stats = static_cast<char*>(ctx->buf("r.stats", 2 * stats_bytes));
+    PinnedBuf h_stats(2 * stats_bytes);
+    double stats_time[2] = {0.0, 0.0};
+    hipEvent_t w0_event = ev_w0.e;
+    auto stats_task = [&, w0_event](int dir) {
+        auto ts = std::chrono::steady_clock::now();
+        hipStream_t ss = ctx->side_stream[dir];
+        DEEDS_HIP(hipSetDevice(ctx->device));
+        DEEDS_HIP(hipStreamWaitEvent(ss, w0_event, 0));
+        StatsScratch sc;
+        sc.dev = d_stats + dir * stats_bytes;
+        sc.host = h_stats.as<char>() + dir * stats_bytes;
+        float mean = 0, sd = 0;
+        image_stats_device(dir ? warped0 : im1b, (long long)sz, sc, ss, &mean, &sd);
+        stats_time[dir] = std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count();
+        return sd;
+    };
+    std::shared_future<float> sd_fut[2] = {std::async(std::launch::async, stats_task, 0).share(),
+                                           std::async(std::launch::async, stats_task, 1).share()};
+
+    // ---- MST inputs: edge SADs of every level on both images ----
     size_t ec_total = 0;
     std::vector<size_t> ec_off(L);
     for (int l = 0; l < L; l++) {
@@ -176,24 +204,9 @@ void register_pair(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, 
         for (int l = 0; l < L; l++)
             launch_edge_sad(d_ec + dir * ec_total + ec_off[l], dir ? warped0 : im1b, lv[l].step, m, n, o, s0);
     PinnedBuf h_ec(2 * ec_total * sizeof(float));
-    PinnedBuf h_img(2 * sz * sizeof(float));
     d2h(h_ec.as<float>(), d_ec, 2 * ec_total, s0);
-    d2h(h_img.as<float>(), im1b, sz, s0);
-    d2h(h_img.as<float>() + sz, warped0, sz, s0);
     DEEDS_HIP(hipEventRecord(ev_sad.e, s0));
-
     hipEvent_t sad_event = ev_sad.e;
-    double stats_time[2] = {0.0, 0.0};
-    auto stats_task = [&, sad_event](int dir) {
-        DEEDS_HIP(hipEventSynchronize(sad_event));
-        auto ts = std::chrono::steady_clock::now();
-        float mean = 0, sd = 0;
-        image_stats(h_img.as<float>() + dir * sz, sz, &mean, &sd);
-        stats_time[dir] = std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count();
-        return sd;
-    };
-    std::shared_future<float> sd_fut[2] = {std::async(std::launch::async, stats_task, 0).share(),
-                                           std::async(std::launch::async, stats_task, 1).share()};
     std::vector<double> build_time(2 * L, 0.0);
     double wait_time = 0.0;
     std::vector<double> wait_level(L, 0.0);
@@ -290,6 +303,7 @@ void register_pair(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, 
             wait_time += waited;
             wait_level[l] += waited;
             tree_maxd[2 * l + dir] = t->maxd;
+            DEEDS_HIP(hipEventRecord(ev_pre[2 * l + dir]->e, s));
             size_t half = (stage_off[l + 1 < L ? l + 1 : l] - stage_off[l]) / 2;
             if (l + 1 >= L) half = (stage_total - stage_off[l]) / 2;
             char* slot = h_stage.as<char>() + stage_off[l] + dir * half;
@@ -365,6 +379,14 @@ void register_pair(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, 
             timing->tree_build_level_s[l] = std::max(build_time[2 * l], build_time[2 * l + 1]);
             timing->tree_wait_level_s[l] = wait_level[l];
             timing->datacost_launches += 2;
+            for (int dir = 0; dir < 2; dir++) {
+                float idle = 0;
+                DEEDS_HIP(hipEventElapsedTime(&idle, DC(l, dir, 1), ev_pre[2 * l + dir]->e));
+                timing->bp_idle_level_s[l] += std::max(0.0f, idle) * 1e-3;
+            }
+            float bp = 0;
+            DEEDS_HIP(hipEventElapsedTime(&bp, ev_pre[2 * l]->e, EV(l, 4)));
+            timing->bp_level_s[l] = bp * 1e-3;
         }
         std::vector<double> hj(4 * (size_t)L);
         DEEDS_HIP(hipMemcpy(hj.data(), jac, hj.size() * sizeof(double), hipMemcpyDeviceToHost));

@@ -77,6 +77,9 @@ typedef struct deeds_timing {
     double tree_build_level_s[DEEDS_MAX_LEVELS];  /* per level: max of the two directions' build times */
     double tree_wait_level_s[DEEDS_MAX_LEVELS];   /* per level: host wait for that level's trees */
     double stats_s;                               /* host image mean/std (both images, concurrent) */
+    double bp_idle_level_s[DEEDS_MAX_LEVELS];     /* per level: GPU idle between each direction's data cost
+                                                     and its BP (waiting for the host tree), summed */
+    double bp_level_s[DEEDS_MAX_LEVELS];          /* per level: forward-direction BP time on its stream */
 } deeds_timing;
 
 /* ---- library / context ------------------------------------------------- */
@@ -105,6 +108,11 @@ int deeds_datacost(const uint64_t* fixed_mind, const uint64_t* moving_mind, floa
 /* primsMST.h:52 primsGraph(): MST of the control grid, BFS-ordered. */
 int deeds_prims_graph(const float* im, int* ordered, int* parents, float* edgemst,
                       int step, int m, int n, int o);
+
+/* primsMST.h:101-110 (inside primsGraph): the image's float mean and standard
+ * deviation, bit-identical to the reference's sequential accumulation.  Host
+ * only (no GPU work). */
+int deeds_image_stats(const float* im, long long count, float* mean, float* sd);
 
 /* regularisation.h:115 regularisationCL(): tree BP; cost is consumed (overwritten). */
 int deeds_regularisation(float* cost, const float* u0, const float* v0, const float* w0,
@@ -179,6 +187,10 @@ int deeds_apply_float(float* out, const float* in, const float* X, const float* 
 
 /* ---- device-pointer forms (asynchronous on the context stream) ---------- */
 int deeds_ctx_descriptor(deeds_ctx* ctx, uint64_t* d_mindq, const float* d_im, int m, int n, int o, int qs);
+/* primsMST.h:101-110 on a device image: the same bit-identical mean and
+ * standard deviation as deeds_image_stats, the per-chunk work on the GPU.
+ * Returns when the result is complete. */
+int deeds_ctx_image_stats(deeds_ctx* ctx, const float* d_im, long long count, float* mean, float* sd);
 int deeds_ctx_datacost(deeds_ctx* ctx, const uint64_t* d_fixed_mind, const uint64_t* d_moving_mind, float* d_cost,
                        int m, int n, int o, int step, int hw, float quant, float alpha, int randnum);
 /* BP over a tree given on the host (ordered/parents/edgemst as returned by
