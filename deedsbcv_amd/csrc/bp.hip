@@ -32,14 +32,21 @@ namespace deeds {
 
 namespace {
 
+constexpr int out_per_thread(int len) {
+    int o = 1;
+    while (len * len * ((len + o - 1) / o) > 1024) o++;
+    return o;
+}
+
 template <int LEN>
 struct BPShape {
     static constexpr int L2 = LEN * LEN;
     static constexpr int L3 = L2 * LEN;
     static constexpr int L3E = (L3 + 1) & ~1;
-    // S threads share one line (each produces OUT consecutive outputs)
-    static constexpr int S = (L2 * 8 <= 1024) ? 8 : (L2 * 4 <= 1024) ? 4 : (L2 * 2 <= 1024) ? 2 : 1;
-    static constexpr int OUT = (LEN + S - 1) / S;
+    // S threads share one line (each produces OUT consecutive outputs): the
+    // smallest OUT that keeps L2 * S threads within 1024, S = ceil(LEN / OUT)
+    static constexpr int OUT = out_per_thread(LEN);
+    static constexpr int S = (LEN + OUT - 1) / OUT;
     static constexpr int BLOCK = ((L2 * S + 63) / 64) * 64;
     static constexpr int ZT = 2 * LEN + 1 + 2 * S;  // quadratic table, padded for the last window
     static constexpr size_t SMEM = 2 * (size_t)L3 * 4 + 2 * (size_t)L3E * 2 + 3 * (size_t)ZT * 4;
@@ -117,6 +124,8 @@ struct BPArgs {
     int16_t* allinds;
     const int* prog_start;
     const int* prog;
+    const int* leaves;
+    const int* leaf_w;
     const float* off3;  // per node: label-unit offset (x, y, z) of its parent's field
     int root;
 };
@@ -135,12 +144,12 @@ __global__ void k_parent_offsets(float* __restrict__ off3, const int* __restrict
     off3[3 * c + 2] = (w0[p] - w0[c]) / quant;
 }
 
-// One message: A holds the sender's cost scaled by its edge weight; the
-// result is folded into the accumulator C.  Leaves the argmin table in allinds.
+// One message: A holds the sender's cost scaled by its edge weight.  Runs the
+// three passes (the argmin table goes to allinds_c, the message to Bf) and
+// returns the message's minimum; every thread may read Bf afterwards.
 template <int LEN>
-__device__ __forceinline__ void message_into(int16_t* allinds_c, float ox, float oy, float oz, float* A, float* Bf,
-                                             float* C, int16_t* I1, int16_t* I2, float* zy, float* zx, float* zz,
-                                             float* red) {
+__device__ __forceinline__ float message(int16_t* allinds_c, float ox, float oy, float oz, float* A, float* Bf,
+                                         int16_t* I1, int16_t* I2, float* zy, float* zx, float* zz, float* red) {
     using B = BPShape<LEN>;
     const int t = threadIdx.x;
     for (int q = t; q < B::ZT; q += B::BLOCK) {
@@ -157,110 +166,142 @@ __device__ __forceinline__ void message_into(int16_t* allinds_c, float ox, float
     dt_pass<LEN, 1>(Bf, A, zx, I1, I2, nullptr);
     __syncthreads();
     float mn = dt_pass<LEN, 2>(A, Bf, zz, I2, nullptr, allinds_c);
-    mn = block_min(mn, red);  // its barriers also complete Bf
-    for (int l = t; l < B::L3; l += B::BLOCK) C[l] += (Bf[l] - mn);
-    __syncthreads();
+    return block_min(mn, red);  // its barriers also complete Bf
 }
 
-// One workgroup runs one head's program: fold the head's children into its
-// accumulator, then climb while the finished node is its parent's only child.
-// The next message's record, offsets and cost row are loaded while the
-// current message computes.
+template <int LEN>
+struct BPLds {
+    float *A, *Bf, *zy, *zx, *zz;
+    int16_t *I1, *I2;
+    __device__ explicit BPLds(float* smem) {
+        using B = BPShape<LEN>;
+        A = smem;
+        Bf = A + B::L3;
+        I1 = (int16_t*)(Bf + B::L3);
+        I2 = I1 + B::L3E;
+        zy = (float*)(I2 + B::L3E);
+        zx = zy + B::ZT;
+        zz = zx + B::ZT;
+    }
+};
+
+// Round 0: every leaf's message to its parent, stored in the leaf's cost row
+// as msg - min(msg), the term its parent folds (regularisation.h:197-232).
+template <int LEN>
+__global__ __launch_bounds__(BPShape<LEN>::BLOCK) void k_bp_leaves(BPArgs a) {
+    using B = BPShape<LEN>;
+    extern __shared__ float smem[];
+    __shared__ float red[16];
+    const BPLds<LEN> s(smem);
+    const int t = threadIdx.x;
+    const int v = a.leaves[blockIdx.x];
+    const float w = __int_as_float(a.leaf_w[blockIdx.x]);
+    float* row = a.cost + (long long)v * B::L3;
+    for (int l = t; l < B::L3; l += B::BLOCK) s.A[l] = row[l] * w;  // costall[child] *= edgemst (:197-202)
+    const float mn = message<LEN>(a.allinds + (long long)v * B::L3, a.off3[3 * v], a.off3[3 * v + 1],
+                                  a.off3[3 * v + 2], s.A, s.Bf, s.I1, s.I2, s.zy, s.zx, s.zz, red);
+    for (int l = t; l < B::L3; l += B::BLOCK) row[l] = s.Bf[l] - mn;
+}
+
+// One workgroup runs one head's program: the head's accumulator is its cost
+// plus its children's stored messages in `ordered` order (the reference's
+// fold order, :221-232); then each chain node messages its parent, which is
+// the next chain node's accumulator (its only child's message added to its
+// cost), until the chain's last node, which stores its message in its own
+// row -- or, at the root, its accumulator.  The accumulator lives in
+// registers (each thread owns fixed labels); the next chain node's cost row is
+// loaded while the current message computes.
 template <int LEN>
 __global__ __launch_bounds__(BPShape<LEN>::BLOCK) void k_bp_prog(BPArgs a, int head_begin) {
     using B = BPShape<LEN>;
     constexpr int ELEMS = (B::L3 + B::BLOCK - 1) / B::BLOCK;
     extern __shared__ float smem[];
     __shared__ float red[16];
-    float* A = smem;
-    float* Bf = A + B::L3;
-    float* C = Bf + B::L3;
-    int16_t* I1 = (int16_t*)(C + B::L3);
-    int16_t* I2 = I1 + B::L3E;
-    float* zy = (float*)(I2 + B::L3E);
-    float* zx = zy + B::ZT;
-    float* zz = zx + B::ZT;
+    const BPLds<LEN> s(smem);
     const int t = threadIdx.x;
 
     const int* pg = a.prog + a.prog_start[head_begin + blockIdx.x];
-    const int x = pg[0], k = pg[1], M = k + pg[2];
-    const int* rec = pg + 3;
-    for (int l = t; l < B::L3; l += B::BLOCK) C[l] = a.cost[(long long)x * B::L3 + l];
-    int last = x;
-    if (M > 0) {
-        // the row a message needs from HBM: the sender's cost for a child, the
-        // receiver's data cost (the new accumulator) for a climb
-        auto row_of = [&](int j, int src, int dst) { return j < k ? src : dst; };
-        int src = rec[0], dst = rec[1], wb = rec[2];
-        int nsrc = 0, ndst = 0, nwb = 0;
-        if (M > 1) { nsrc = rec[3]; ndst = rec[4]; nwb = rec[5]; }
-        float ox = a.off3[3 * src], oy = a.off3[3 * src + 1], oz = a.off3[3 * src + 2];
-        float row[ELEMS];
-        {
-            const long long rb = (long long)row_of(0, src, dst) * B::L3;
+    const int x = pg[0], k = pg[1], L = pg[2];
+    const int* kids = pg + 3;
+    const int* chain = kids + k;
+    float C[ELEMS];
+#pragma unroll
+    for (int e = 0; e < ELEMS; e++) {
+        const int l = t + e * B::BLOCK;
+        C[e] = l < B::L3 ? a.cost[(long long)x * B::L3 + l] : 0.0f;
+    }
+    for (int j = 0; j < k; j++) {
+        const float* kr = a.cost + (long long)kids[j] * B::L3;
+#pragma unroll
+        for (int e = 0; e < ELEMS; e++) {
+            const int l = t + e * B::BLOCK;
+            if (l < B::L3) C[e] += kr[l];  // parent += (msg - min)
+        }
+    }
+    int node = chain[0];
+    float w = __int_as_float(chain[1]);
+    for (int i = 0; i <= L; i++) {
+        if (node == a.root) break;  // the root keeps its accumulator
+        // prefetch the next chain node's cost row (it receives this message)
+        const int next = i < L ? chain[2 * (i + 1)] : -1;
+        float nrow[ELEMS];
+        if (next >= 0) {
 #pragma unroll
             for (int e = 0; e < ELEMS; e++) {
                 const int l = t + e * B::BLOCK;
-                row[e] = l < B::L3 ? a.cost[rb + l] : 0.0f;
+                nrow[e] = l < B::L3 ? a.cost[(long long)next * B::L3 + l] : 0.0f;
             }
         }
-        for (int j = 0; j < M; j++) {
-            // prefetch message j+1 (record j+2, offsets and row of j+1)
-            int n2src = 0, n2dst = 0, n2wb = 0;
-            if (j + 2 < M) { n2src = rec[3 * (j + 2)]; n2dst = rec[3 * (j + 2) + 1]; n2wb = rec[3 * (j + 2) + 2]; }
-            float nox = 0.f, noy = 0.f, noz = 0.f;
-            float nrow[ELEMS];
-            if (j + 1 < M) {
-                nox = a.off3[3 * nsrc];
-                noy = a.off3[3 * nsrc + 1];
-                noz = a.off3[3 * nsrc + 2];
-                const long long rb = (long long)row_of(j + 1, nsrc, ndst) * B::L3;
 #pragma unroll
-                for (int e = 0; e < ELEMS; e++) {
-                    const int l = t + e * B::BLOCK;
-                    nrow[e] = l < B::L3 ? a.cost[rb + l] : 0.0f;
-                }
-            }
-            const float w = __int_as_float(wb);
-            if (j < k) {
-#pragma unroll
-                for (int e = 0; e < ELEMS; e++) {
-                    const int l = t + e * B::BLOCK;
-                    if (l < B::L3) A[l] = row[e] * w;
-                }
-            } else {
-#pragma unroll
-                for (int e = 0; e < ELEMS; e++) {
-                    const int l = t + e * B::BLOCK;
-                    if (l < B::L3) {
-                        A[l] = C[l] * w;
-                        C[l] = row[e];
-                    }
-                }
-            }
-            message_into<LEN>(a.allinds + (long long)src * B::L3, ox, oy, oz, A, Bf, C, I1, I2, zy, zx, zz, red);
-            if (j >= k) last = dst;
-            src = nsrc; dst = ndst; wb = nwb;
-            nsrc = n2src; ndst = n2dst; nwb = n2wb;
-            ox = nox; oy = noy; oz = noz;
-#pragma unroll
-            for (int e = 0; e < ELEMS; e++) row[e] = nrow[e];
+        for (int e = 0; e < ELEMS; e++) {
+            const int l = t + e * B::BLOCK;
+            if (l < B::L3) s.A[l] = C[e] * w;  // costall[child] *= edgemst (:197-202)
         }
+        const float mn = message<LEN>(a.allinds + (long long)node * B::L3, a.off3[3 * node], a.off3[3 * node + 1],
+                                      a.off3[3 * node + 2], s.A, s.Bf, s.I1, s.I2, s.zy, s.zx, s.zz, red);
+        if (next < 0) {
+            // the chain's last node: store its message for the parent's fold
+            float* row = a.cost + (long long)node * B::L3;
+#pragma unroll
+            for (int e = 0; e < ELEMS; e++) {
+                const int l = t + e * B::BLOCK;
+                if (l < B::L3) row[l] = s.Bf[l] - mn;
+            }
+            return;
+        }
+#pragma unroll
+        for (int e = 0; e < ELEMS; e++) {
+            const int l = t + e * B::BLOCK;
+            if (l < B::L3) C[e] = nrow[e] + (s.Bf[l] - mn);  // the only child's message
+        }
+        node = next;
+        w = __int_as_float(chain[2 * (i + 1) + 1]);
     }
-    for (int l = t; l < B::L3; l += B::BLOCK) a.cost[(long long)last * B::L3 + l] = C[l];
+    // reached the root: its accumulator is what the selection reads
+#pragma unroll
+    for (int e = 0; e < ELEMS; e++) {
+        const int l = t + e * B::BLOCK;
+        if (l < B::L3) a.cost[(long long)node * B::L3 + l] = C[e];
+    }
 }
 
+// round < 0: the leaves; otherwise heads [head_begin, head_begin + count)
 template <int LEN>
-void launch_round_len(const BPArgs& a, int head_begin, int count, hipStream_t s) {
+void launch_round_len(const BPArgs& a, int head_begin, int count, bool leaves, hipStream_t s) {
     using B = BPShape<LEN>;
-    constexpr size_t smem = B::SMEM + (size_t)B::L3 * 4;  // + the accumulator C
+    constexpr size_t smem = B::SMEM;
     static bool attr_set = false;
     if (!attr_set) {
         DEEDS_HIP(hipFuncSetAttribute((const void*)k_bp_prog<LEN>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)smem));
+        DEEDS_HIP(hipFuncSetAttribute((const void*)k_bp_leaves<LEN>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)smem));
         attr_set = true;
     }
-    k_bp_prog<LEN><<<count, B::BLOCK, smem, s>>>(a, head_begin);
+    if (leaves)
+        k_bp_leaves<LEN><<<count, B::BLOCK, smem, s>>>(a);
+    else
+        k_bp_prog<LEN><<<count, B::BLOCK, smem, s>>>(a, head_begin);
     DEEDS_LAUNCH_CHECK();
 }
 
@@ -327,22 +368,23 @@ void launch_bp_rounds(float* cost, int16_t* allinds, float* off3, const BPDevTre
         k_parent_offsets<<<ceil_div(N, 256), 256, 0, s>>>(off3, t.parents, u0, v0, w0, quant, N);
         DEEDS_LAUNCH_CHECK();
     }
-    BPArgs a{cost, allinds, t.prog_start, t.prog, off3, t.root};
-    for (int r = 0; r < rounds; r++) {
-        const int b = round_start[r], n = round_start[r + 1] - b;
+    BPArgs a{cost, allinds, t.prog_start, t.prog, t.leaves, t.leaf_w, off3, t.root};
+    for (int r = -1; r < rounds; r++) {
+        const bool leaves = r < 0;
+        const int b = leaves ? 0 : round_start[r], n = leaves ? t.n_leaves : round_start[r + 1] - b;
         if (n <= 0) continue;
         switch (2 * hw + 1) {
-            case 1: launch_round_len<1>(a, b, n, s); break;
-            case 3: launch_round_len<3>(a, b, n, s); break;
-            case 5: launch_round_len<5>(a, b, n, s); break;
-            case 7: launch_round_len<7>(a, b, n, s); break;
-            case 9: launch_round_len<9>(a, b, n, s); break;
-            case 11: launch_round_len<11>(a, b, n, s); break;
-            case 13: launch_round_len<13>(a, b, n, s); break;
-            case 15: launch_round_len<15>(a, b, n, s); break;
-            case 17: launch_round_len<17>(a, b, n, s); break;
-            case 19: launch_round_len<19>(a, b, n, s); break;
-            case 21: launch_round_len<21>(a, b, n, s); break;
+            case 1: launch_round_len<1>(a, b, n, leaves, s); break;
+            case 3: launch_round_len<3>(a, b, n, leaves, s); break;
+            case 5: launch_round_len<5>(a, b, n, leaves, s); break;
+            case 7: launch_round_len<7>(a, b, n, leaves, s); break;
+            case 9: launch_round_len<9>(a, b, n, leaves, s); break;
+            case 11: launch_round_len<11>(a, b, n, leaves, s); break;
+            case 13: launch_round_len<13>(a, b, n, leaves, s); break;
+            case 15: launch_round_len<15>(a, b, n, leaves, s); break;
+            case 17: launch_round_len<17>(a, b, n, leaves, s); break;
+            case 19: launch_round_len<19>(a, b, n, leaves, s); break;
+            case 21: launch_round_len<21>(a, b, n, leaves, s); break;
             default: fail(DEEDS_ERR_ARG, "search radius outside [0, 10]");
         }
     }

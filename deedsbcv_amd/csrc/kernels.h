@@ -41,7 +41,10 @@ struct BPDevTree {
     const int* parents;      // N
     const int* depth_start;  // maxd+2
     const int* prog_start;   // heads+1 (heads in round order)
-    const int* prog;         // per head: [x, k, L] + (k+L) records (src, dst, edgemst bits)
+    const int* prog;         // per head: [x, k, L], k children, L+1 chain nodes (node, edgemst bits)
+    const int* leaves;       // n_leaves leaf nodes (round 0)
+    const int* leaf_w;       // their edgemst bits
+    int n_leaves;
     float* off3;             // 3*N floats of scratch (parent offsets)
     int root;
 };

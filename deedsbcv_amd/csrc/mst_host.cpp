@@ -309,17 +309,24 @@ void build_schedule(Tree& t) {
         int v = t.ordered[p];
         if (is_head[v]) t.heads[fill_r[round[v] - 1]++] = v;
     }
-    // the program of every head: its children, then the single-child climb
+    // Leaves message their parents in round 0 (one workgroup each).  The
+    // program of a head: fold its children's stored messages, then message
+    // its parent and climb while the finished node is its parent's only child;
+    // the chain's last node (unless it is the root) stores its message in its
+    // cost row for the multi-child parent that folds it in a later round.
     const int root = N > 0 ? t.ordered[0] : -1;
+    t.leaves.clear();
+    for (int p = 1; p < N; p++)
+        if (nch(t.ordered[p]) == 0) t.leaves.push_back(t.ordered[p]);
+    t.leaf_w.resize(t.leaves.size());
+    for (size_t i = 0; i < t.leaves.size(); i++) std::memcpy(&t.leaf_w[i], &t.edgemst[t.leaves[i]], 4);
     t.prog_start.assign(t.heads.size() + 1, 0);
     t.prog.clear();
-    t.prog.reserve(3 * t.heads.size() + 3 * (size_t)N);
-    auto record = [&](int src, int dst) {
+    t.prog.reserve(4 * t.heads.size() + 3 * (size_t)N);
+    auto wbits = [&](int v) {
         int wb;
-        std::memcpy(&wb, &t.edgemst[src], 4);
-        t.prog.push_back(src);
-        t.prog.push_back(dst);
-        t.prog.push_back(wb);
+        std::memcpy(&wb, &t.edgemst[v], 4);
+        return wb;
     };
     for (size_t h = 0; h < t.heads.size(); h++) {
         const int x = t.heads[h];
@@ -328,13 +335,17 @@ void build_schedule(Tree& t) {
         t.prog.push_back(x);
         t.prog.push_back(nch(x));
         t.prog.push_back(0);
-        for (int ci = t.child_start[x]; ci < t.child_start[x + 1]; ci++) record(t.child_list[ci], x);
+        for (int ci = t.child_start[x]; ci < t.child_start[x + 1]; ci++) t.prog.push_back(t.child_list[ci]);
+        // the chain x = v_0, v_1, ... (each the only child of the next), as (node, edgemst bits)
         int L = 0, cur = x;
+        t.prog.push_back(cur);
+        t.prog.push_back(cur == root ? 0 : wbits(cur));
         while (cur != root) {
             const int q = t.parents[cur];
             if (nch(q) != 1) break;
-            record(cur, q);
             cur = q;
+            t.prog.push_back(cur);
+            t.prog.push_back(cur == root ? 0 : wbits(cur));
             L++;
         }
         t.prog[hdr + 2] = L;
@@ -342,6 +353,6 @@ void build_schedule(Tree& t) {
     t.prog_start[t.heads.size()] = (int)t.prog.size();
 }
 
-size_t tree_device_ints(int N) { return 12 * (size_t)N + 16; }
+size_t tree_device_ints(int N) { return 16 * (size_t)N + 16; }
 
 }  // namespace deeds

@@ -28,15 +28,19 @@ struct Tree {
     std::vector<int> heads;
     std::vector<int> round_start;  // rounds + 1 offsets into heads
     int rounds = 0;
+    // Round 0: every leaf messages its parent (leaf_w = edgemst bits).
+    std::vector<int> leaves;
+    std::vector<int> leaf_w;
     // Per head (in round order) the program one workgroup executes:
-    // [x, k, L] then k + L message records (src, dst, edgemst[src] as bits):
-    // first the k children of x, then the L single-child climbs above x.
+    // [x, k, L], the k children of x (whose messages are stored in their cost
+    // rows), then the L + 1 chain nodes x = v_0 .. v_L as (node, edgemst
+    // bits); v_{i+1} is the parent of v_i and has no other child.
     std::vector<int> prog_start;  // heads + 1 offsets into prog
     std::vector<int> prog;
 };
 
 // Upper bound on the ints a device copy of a tree's BP schedule needs
-// (ordered, parents, depth_start, prog_start, prog).
+// (ordered, parents, depth_start, prog_start, prog, leaves, leaf_w).
 size_t tree_device_ints(int N);
 
 // primsMST.h:101-110: float mean and std of the image, bit-identical to the
