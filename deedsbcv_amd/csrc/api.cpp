@@ -237,12 +237,12 @@ size_t tree_ints_bound(int N) { return tree_device_ints(N); }
 size_t tree_stage_bytes(int N) { return tree_ints_bound(N) * sizeof(int) + 3 * (size_t)N * sizeof(float); }
 
 // Packs a host tree's BP schedule (ordered, parents, depth_start, prog_start,
-// prog, leaves, leaf_w) into the pinned staging area and enqueues its copy into the device
+// prog, leaves, leaf_w, sel_info) into the pinned staging area and enqueues its copy into the device
 // area d_stage of the same layout (tree_stage_bytes(N)).  The pinned memory
 // must stay untouched until the copy has executed on `s`.
 BPDevTree stage_tree(const Tree& t, void* pinned, void* d_stage, hipStream_t s) {
     size_t ints = t.ordered.size() + t.parents.size() + t.depth_start.size() + t.prog_start.size() + t.prog.size() +
-                  t.leaves.size() + t.leaf_w.size();
+                  t.leaves.size() + t.leaf_w.size() + t.sel_info.size();
     size_t bound = tree_ints_bound(t.N);
     if (ints > bound) fail(DEEDS_ERR_INTERNAL, "tree schedule larger than its staging bound");
     int* hi = static_cast<int*>(pinned);
@@ -263,6 +263,8 @@ BPDevTree stage_tree(const Tree& t, void* pinned, void* d_stage, hipStream_t s) 
     out.leaves = put(t.leaves);
     out.leaf_w = put(t.leaf_w);
     out.n_leaves = (int)t.leaves.size();
+    out.sel_info = put(t.sel_info);
+    out.max_width = t.max_width;
     out.root = t.N > 0 ? t.ordered[0] : 0;
     out.off3 = reinterpret_cast<float*>(di + bound);
     DEEDS_HIP(hipMemcpyAsync(di, hi, at * sizeof(int), hipMemcpyHostToDevice, s));

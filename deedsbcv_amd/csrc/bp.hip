@@ -305,16 +305,22 @@ void launch_round_len(const BPArgs& a, int head_begin, int count, bool leaves, h
     DEEDS_LAUNCH_CHECK();
 }
 
-// Root argmin (first strict minimum), then label selection depth by depth
-// and the displacement update u1 = label displacement + u0.
+// Root argmin (first strict minimum) of the root's accumulated cost, then
+// the label selection depth by depth, sel[c] = allinds[c][sel[parent]], with
+// the displacement update u1 = label displacement + u0 (regularisation.h:
+// 259-293).  The previous depth's labels stay in LDS (indexed by position
+// within the depth), so each depth costs one dependent load from HBM, and
+// the next depth's (node, parent index) pairs are loaded ahead.
 __global__ __launch_bounds__(1024) void k_bp_select(const float* __restrict__ cost,
                                                     const int16_t* __restrict__ allinds,
-                                                    const int* __restrict__ ordered, const int* __restrict__ parents,
-                                                    const int* __restrict__ depth_start, int N, int maxd,
+                                                    const int* __restrict__ ordered,
+                                                    const int* __restrict__ depth_start,
+                                                    const int* __restrict__ sel_info, int maxd,
                                                     const float* __restrict__ u0, const float* __restrict__ v0,
                                                     const float* __restrict__ w0, float* __restrict__ u1,
-                                                    float* __restrict__ v1, float* __restrict__ w1,
-                                                    int* __restrict__ sel, int len, int hw, float quant) {
+                                                    float* __restrict__ v1, float* __restrict__ w1, int width,
+                                                    int len, int hw, float quant) {
+    extern __shared__ int labs[];  // two buffers of `width` labels
     __shared__ float rv[16];
     __shared__ int ri[16];
     const int len3 = len * len * len;
@@ -332,30 +338,54 @@ __global__ __launch_bounds__(1024) void k_bp_select(const float* __restrict__ co
         int oi = __shfl_xor(bi, off, 64);
         if (ov < bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
     }
-    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) { rv[wid] = bv; ri[wid] = bi; }
     __syncthreads();
+    int* prev = labs;
+    int* cur = labs + width;
+    auto emit = [&](int c, int lab) {
+        const int li = lab % len, lj = (lab / len) % len, lk = lab / (len * len);
+        u1[c] = (float)(lj - hw) * quant + u0[c];
+        v1[c] = (float)(li - hw) * quant + v0[c];
+        w1[c] = (float)(lk - hw) * quant + w0[c];
+    };
     if (threadIdx.x == 0) {
         float v = rv[0];
         int id = ri[0];
         for (int w = 1; w < (int)(blockDim.x >> 6); w++)
             if (rv[w] < v || (rv[w] == v && ri[w] < id)) { v = rv[w]; id = ri[w]; }
-        sel[root] = id;
+        prev[0] = id;
+        emit(root, id);
+    }
+    int nc = -1, npp = 0;
+    if (maxd >= 1 && depth_start[1] + (int)threadIdx.x < depth_start[2]) {
+        nc = sel_info[2 * (depth_start[1] + threadIdx.x)];
+        npp = sel_info[2 * (depth_start[1] + threadIdx.x) + 1];
     }
     __syncthreads();
     for (int d = 1; d <= maxd; d++) {
-        for (int t = depth_start[d] + threadIdx.x; t < depth_start[d + 1]; t += blockDim.x) {
-            int c = ordered[t];
-            sel[c] = allinds[(long long)c * len3 + sel[parents[c]]];
+        const int b = depth_start[d], e = depth_start[d + 1];
+        const int c = nc, pp = npp;
+        nc = -1;
+        if (d < maxd && depth_start[d + 1] + (int)threadIdx.x < depth_start[d + 2]) {
+            nc = sel_info[2 * (depth_start[d + 1] + threadIdx.x)];
+            npp = sel_info[2 * (depth_start[d + 1] + threadIdx.x) + 1];
+        }
+        if (c >= 0) {
+            const int lab = allinds[(long long)c * len3 + prev[pp]];
+            cur[threadIdx.x] = lab;
+            emit(c, lab);
+        }
+        for (int p = b + (int)blockDim.x + (int)threadIdx.x; p < e; p += blockDim.x) {  // depths wider than the block
+            const int c2 = sel_info[2 * p], pp2 = sel_info[2 * p + 1];
+            const int lab = allinds[(long long)c2 * len3 + prev[pp2]];
+            cur[p - b] = lab;
+            emit(c2, lab);
         }
         __syncthreads();
-    }
-    for (int t = threadIdx.x; t < N; t += blockDim.x) {
-        int lab = sel[t];
-        int li = lab % len, lj = (lab / len) % len, lk = lab / (len * len);
-        u1[t] = (float)(lj - hw) * quant + u0[t];
-        v1[t] = (float)(li - hw) * quant + v0[t];
-        w1[t] = (float)(lk - hw) * quant + w0[t];
+        int* tmp = prev;
+        prev = cur;
+        cur = tmp;
     }
 }
 
@@ -394,9 +424,17 @@ void launch_bp_select(const float* cost, const int16_t* allinds, const BPDevTree
                       const float* u0, const float* v0, const float* w0, float* u1, float* v1, float* w1, int* sel,
                       int hw, float quant, hipStream_t s) {
     int len = 2 * hw + 1;
-    k_bp_select<<<1, 1024, 0, s>>>(cost, allinds, t.ordered, t.parents, t.depth_start, N, maxd, u0, v0, w0, u1, v1,
-                                   w1, sel, len, hw, quant);
+    const size_t smem = 2 * (size_t)std::max(1, t.max_width) * sizeof(int);
+    // beyond the default 64 KiB the kernel needs the attribute (its static LDS takes the rest)
+    if (smem > (size_t)kMaxLds - 1024)
+        fail(DEEDS_ERR_ARG, "tree too wide for the label selection (a depth with over 20000 nodes)");
+    if (smem > 65536)
+        DEEDS_HIP(hipFuncSetAttribute((const void*)k_bp_select, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    k_bp_select<<<1, 1024, smem, s>>>(cost, allinds, t.ordered, t.depth_start, t.sel_info, maxd, u0, v0, w0, u1, v1,
+                                      w1, std::max(1, t.max_width), len, hw, quant);
     DEEDS_LAUNCH_CHECK();
+    (void)sel;
+    (void)N;
 }
 
 }  // namespace deeds

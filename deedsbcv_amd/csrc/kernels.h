@@ -45,6 +45,8 @@ struct BPDevTree {
     const int* leaves;       // n_leaves leaf nodes (round 0)
     const int* leaf_w;       // their edgemst bits
     int n_leaves;
+    const int* sel_info;     // 2*N: per position of ordered, (node, parent's index within its depth)
+    int max_width;           // largest number of nodes at one depth
     float* off3;             // 3*N floats of scratch (parent offsets)
     int root;
 };

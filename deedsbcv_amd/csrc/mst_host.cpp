@@ -269,6 +269,18 @@ void build_schedule(Tree& t) {
     for (int p = 0; p < N; p++)
         if (depth[t.ordered[p]] < (p == 0 ? 0 : depth[t.ordered[p - 1]]))
             throw std::runtime_error("tree: `ordered` is not sorted by depth");
+    // top-down selection: per position, the node and its parent's index within the parent's depth
+    t.max_width = 0;
+    for (int d = 0; d <= maxd; d++) t.max_width = std::max(t.max_width, t.depth_start[d + 1] - t.depth_start[d]);
+    t.sel_info.assign(2 * (size_t)N, 0);
+    for (int p = 0; p < N; p++) {
+        const int c = t.ordered[p];
+        t.sel_info[2 * p] = c;
+        if (p > 0) {
+            const int par = t.parents[c];
+            t.sel_info[2 * p + 1] = pos[par] - t.depth_start[depth[par]];
+        }
+    }
     // children of every node in `ordered` order, and the leaves, deepest first
     t.child_start.assign(N + 1, 0);
     for (int p = 1; p < N; p++) t.child_start[t.parents[t.ordered[p]] + 1]++;
@@ -353,6 +365,6 @@ void build_schedule(Tree& t) {
     t.prog_start[t.heads.size()] = (int)t.prog.size();
 }
 
-size_t tree_device_ints(int N) { return 16 * (size_t)N + 16; }
+size_t tree_device_ints(int N) { return 18 * (size_t)N + 16; }
 
 }  // namespace deeds

@@ -17,6 +17,10 @@ struct Tree {
     // schedule: depth_start[d] = first position of depth d in `ordered` (d = 0..maxd+1)
     int maxd = 0;
     std::vector<int> depth_start;
+    // top-down selection: (node, index of its parent within the parent's
+    // depth) per position of `ordered`, and the largest depth width
+    std::vector<int> sel_info;
+    int max_width = 0;
     // children of every node in `ordered` order (CSR by node id)
     std::vector<int> child_start;  // N + 1
     std::vector<int> child_list;   // N - 1
@@ -40,7 +44,7 @@ struct Tree {
 };
 
 // Upper bound on the ints a device copy of a tree's BP schedule needs
-// (ordered, parents, depth_start, prog_start, prog, leaves, leaf_w).
+// (ordered, parents, depth_start, prog_start, prog, leaves, leaf_w, sel_info).
 size_t tree_device_ints(int N);
 
 // primsMST.h:101-110: float mean and std of the image, bit-identical to the
