@@ -49,10 +49,14 @@ class EdgeHeap {
         // child that is not "less" (comp(child, child-1) <=> child.w > (child-1).w)
         Entry* h = h_.data();
         long hole = 0, child = 0;
-        while (child < (n - 1) / 2) {
+        const long lim = (n - 1) / 2;
+        while (child < lim) {
             child = 2 * (child + 1);
-            __builtin_prefetch(h + 2 * (child + 1) - 1);
-            if (h[child].key > h[child - 1].key) child--;
+            // the children pairs two and three levels below (one line each)
+            __builtin_prefetch(h + 4 * child - 1);
+            __builtin_prefetch(h + 8 * child - 9);
+            const float a = h[child].key, b = h[child - 1].key;
+            child -= (long)(a > b);  // branch-free: the hole takes the child that is not "less"
             h[hole] = h[child];
             hole = child;
         }
@@ -183,12 +187,6 @@ void prim_tree(const float* w, int m1, int n1, int o1, Tree& t) {
     t.parents.assign(N, -1);
     t.parent_slot.assign(N, -1);
     if (N == 0) return;
-    auto neighbour = [&](int v, int nb) -> int {
-        int k = v / (m1 * n1), j = (v / m1) % n1, i = v % m1;
-        int ii = i + kNbI[nb], jj = j + kNbJ[nb], kk = k + kNbK[nb];
-        if (ii < 0 || ii >= m1 || jj < 0 || jj >= n1 || kk < 0 || kk >= o1) return -1;
-        return ii + jj * m1 + kk * m1 * n1;
-    };
     // Prim's from the grid centre (primsMST.h:123-164)
     const int root = m1 / 2 + (n1 / 2) * m1 + (o1 / 2) * m1 * n1;
     std::vector<uint8_t> in_tree(N, 0);
@@ -200,9 +198,12 @@ void prim_tree(const float* w, int m1, int n1, int o1, Tree& t) {
     const int delta[6] = {-m1, m1, -1, 1, -m1 * n1, m1 * n1};
     if ((long long)N * 8 >= (1LL << 32)) throw std::runtime_error("prims: grid too large");
     for (int it = 0; it < N - 1; it++) {
-        for (int nb = 0; nb < 6; nb++) {
-            int v = neighbour(last, nb);
-            if (v >= 0) heap.push(w[last + (size_t)nb * N], (uint32_t)last * 8u + (uint32_t)nb);
+        {
+            // the six neighbour slots of `last` in slot order (kNbI/J/K)
+            const int lk = last / (m1 * n1), rem = last - lk * m1 * n1, lj = rem / m1, li = rem - lj * m1;
+            const bool ok[6] = {lj > 0, lj < n1 - 1, li > 0, li < m1 - 1, lk > 0, lk < o1 - 1};
+            for (int nb = 0; nb < 6; nb++)
+                if (ok[nb]) heap.push(w[last + (size_t)nb * N], (uint32_t)last * 8u + (uint32_t)nb);
         }
         int from = -1, to = -1;
         while (to < 0) {

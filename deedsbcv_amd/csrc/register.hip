@@ -219,19 +219,22 @@ void register_pair(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, 
             double* spent = &build_time[2 * l + dir];
             trees[dir].push_back(std::async(std::launch::async, [ec, c, sdf, spent, sad_event]() {
                 // Prim's on the edge costs runs before the image statistics are
-                // known; the exact weights then confirm the order (else rebuild)
+                // known; the exact weights then confirm the order (else rebuild).
+                // The confirmation runs beside Prim's once the statistics arrive.
                 DEEDS_HIP(hipEventSynchronize(sad_event));
                 auto tb = std::chrono::steady_clock::now();
+                auto check = std::async(std::launch::async, [ec, c, sdf]() {
+                    return weights_distinguish_costs(ec, c.step, c.m1, c.n1, c.o1, sdf.get());
+                });
                 std::unique_ptr<Tree> t(new Tree());
                 build_tree_shape(ec, c.step, c.m1, c.n1, c.o1, *t);
-                double shape_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - tb).count();
-                float sd = sdf.get();
-                auto tf = std::chrono::steady_clock::now();
-                if (weights_distinguish_costs(ec, c.step, c.m1, c.n1, c.o1, sd))
+                const bool same_order = check.get();
+                const float sd = sdf.get();
+                if (same_order)
                     finish_tree(ec, sd, c.step, *t);
                 else
                     build_tree(ec, sd, c.step, c.m1, c.n1, c.o1, *t);
-                *spent = shape_s + std::chrono::duration<double>(std::chrono::steady_clock::now() - tf).count();
+                *spent = std::chrono::duration<double>(std::chrono::steady_clock::now() - tb).count();
                 return t;
             }));
         }
