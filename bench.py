@@ -28,6 +28,22 @@ sys.path.insert(0, str(ROOT))
 
 VALU_INT_PEAK_TOPS = 39.3  # 256 CU x 64 lanes x 2.4 GHz, 32-bit integer VALU ops
 OPS_PER_HAMMING = 4        # 2 x v_xor_b32 + 2 x v_bcnt_u32_b32 per 64-bit code pair
+TRAFFIC_JSON = ROOT / "profiles" / "registration_256x256x128_hbm_traffic.json"
+
+
+def datacost_traffic():
+    """HBM bytes per data-cost launch (mean over the launches of one registration) from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload (calibrated against a
+    1 GiB copy), or None when it is missing."""
+    try:
+        k = json.loads(TRAFFIC_JSON.read_text())["kernels"]
+    except (OSError, ValueError, KeyError):
+        return None
+    rows = [v for name, v in k.items() if "k_datacost" in name]
+    n = sum(v["dispatches"] for v in rows)
+    if not n:
+        return None
+    return sum((v["fetch_bytes"] + v["write_bytes"]) * v["dispatches"] for v in rows) / n
 
 
 def workload(name):
@@ -365,7 +381,8 @@ def main():
             "peak": VALU_INT_PEAK_TOPS,
             "unit": "TOP/s (int32 VALU)",
             "frac": achieved / VALU_INT_PEAK_TOPS,
-            "traffic": None,
+            "traffic": datacost_traffic() if a.workload == "full" else None,
+            "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
             "launches_timed": dc_n,
             "avg_launch_ms": 1e3 * dc_s / dc_n,
         },
