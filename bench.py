@@ -115,7 +115,7 @@ def atlas_bench(a):
         "value": n * a.steps / wall, "unit": "registrations/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u64 Hamming / f32", "data": "synthetic (seeded cross-modal phantom pairs)",
-        "config": {"workload": "atlas: 16 pairs, concurrency %d" % a.concurrency, "dims": dims},
+        "config": {"workload": "atlas: 16 pairs, level-synchronous groups of %d" % a.concurrency, "dims": dims},
         "sequential_registrations_per_s": n / seq, "batch_speedup_vs_sequential": (n * a.steps / wall) / (n / seq),
         "identical_to_sequential": identical}))
 
@@ -283,7 +283,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="full", choices=["full", "small", "crossmodal", "atlas", "soft"])
-    ap.add_argument("--concurrency", type=int, default=4, help="pairs in flight for --workload atlas")
+    ap.add_argument("--concurrency", type=int, default=16,
+                    help="pairs per level-synchronous group for --workload atlas")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
     ap.add_argument("--cpu-budget", type=float, default=240.0)

@@ -173,10 +173,11 @@ def register(fixed, moving, grid=(8, 7, 6, 5, 4), radius=(8, 7, 6, 5, 4), quant=
 
 
 def register_batch(pairs, grid=(8, 7, 6, 5, 4), radius=(8, 7, 6, 5, 4), quant=(5, 4, 3, 2, 1), alpha=1.6,
-                   concurrency=4, want_warped=True):
+                   concurrency=16, want_warped=True):
     """Multi-atlas batch: `pairs` is a list of (fixed, moving) or (fixed, moving, X) volumes,
-    each with its own shape.  Returns a list of (flow, warped, timing); every result equals
-    register() of that pair alone."""
+    each with its own shape, registered level-synchronously in groups of `concurrency`
+    pairs.  Returns a list of (flow, warped, timing); every result equals register() of
+    that pair alone."""
     keep, arr = [], (Pair * max(1, len(pairs)))()
     outs = []
     for i, p in enumerate(pairs):
@@ -199,7 +200,7 @@ def register_batch(pairs, grid=(8, 7, 6, 5, 4), radius=(8, 7, 6, 5, 4), quant=(5
 
 
 def register_batch_device(pairs, device=0, grid=(8, 7, 6, 5, 4), radius=(8, 7, 6, 5, 4), quant=(5, 4, 3, 2, 1),
-                          alpha=1.6, concurrency=4):
+                          alpha=1.6, concurrency=16):
     """Batch on device data: `pairs` is a list of dicts with torch CUDA tensors
     fixed, moving (o, n, m), flow (3*sz1) and optional warped, X (numpy, host)."""
     arr = (Pair * max(1, len(pairs)))()

@@ -6,6 +6,7 @@
 #include <cstring>
 #include <memory>
 
+#include "bp_host.h"
 #include "common.h"
 #include "kernels.h"
 #include "mst_host.h"
@@ -41,6 +42,31 @@ void* deeds_ctx::buf(const std::string& name, size_t bytes) {
     return b.ptr;
 }
 
+void* deeds_ctx::pinned(const std::string& name, size_t bytes) {
+    DevBuf& b = pinned_ws[name];
+    if (b.bytes >= bytes && b.ptr) return b.ptr;
+    if (b.ptr) {
+        // copies from the old block may still be queued
+        DEEDS_HIP(hipStreamSynchronize(stream));
+        if (aux_stream) DEEDS_HIP(hipStreamSynchronize(aux_stream));
+        for (hipStream_t ss : side_stream)
+            if (ss) DEEDS_HIP(hipStreamSynchronize(ss));
+        DEEDS_HIP(hipHostFree(b.ptr));
+        b.ptr = nullptr;
+        b.bytes = 0;
+    }
+    size_t want = std::max<size_t>(bytes, 256);
+    hipError_t e = hipHostMalloc(&b.ptr, want, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        b.ptr = nullptr;
+        (void)hipGetLastError();
+        fail(DEEDS_ERR_ALLOC, "hipHostMalloc of " + std::to_string(want) + " bytes for '" + name + "' failed: " +
+                                  hipGetErrorString(e));
+    }
+    b.bytes = want;
+    return b.ptr;
+}
+
 void deeds_ctx::release_all() {
     if (stream) (void)hipStreamSynchronize(stream);
     if (aux_stream) (void)hipStreamSynchronize(aux_stream);
@@ -49,6 +75,9 @@ void deeds_ctx::release_all() {
     for (auto& kv : ws)
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);
     ws.clear();
+    for (auto& kv : pinned_ws)
+        if (kv.second.ptr) (void)hipHostFree(kv.second.ptr);
+    pinned_ws.clear();
 }
 
 namespace deeds {
@@ -276,6 +305,66 @@ void run_bp(const Tree& t, const BPDevTree& dt, float* d_cost, int16_t* d_inds, 
             const float* v0, const float* w0, float* u1, float* v1, float* w1, int hw, float quant, hipStream_t s) {
     launch_bp_rounds(d_cost, d_inds, dt.off3, dt, t.N, t.round_start.data(), t.rounds, u0, v0, w0, hw, quant, s);
     launch_bp_select(d_cost, d_inds, dt, t.N, t.maxd, u0, v0, w0, u1, v1, w1, d_sel, hw, quant, s);
+}
+
+namespace {
+size_t batch_desc_bytes(int trees) { return ((size_t)trees * sizeof(BPBatchTree) + 255) & ~(size_t)255; }
+}  // namespace
+
+size_t bp_batch_bytes(int trees, long long nodes) {
+    // leaves and chain heads are disjoint sets of nodes: at most N work items per tree
+    return batch_desc_bytes(trees) + (size_t)nodes * sizeof(int2);
+}
+
+void run_bp_batch(const std::vector<BPBatchItem>& items, int hw, float quant, void* pinned, void* d_stage,
+                  size_t bytes, hipStream_t s) {
+    const int T = (int)items.size();
+    if (T == 0) return;
+    if (T == 1) {
+        // one tree: the single-tree kernels take their arguments by value (no descriptor loads)
+        const BPBatchItem& it = items[0];
+        run_bp(*it.tree, it.dt, it.cost, it.inds, nullptr, it.u0, it.v0, it.w0, it.u1, it.v1, it.w1, hw, quant, s);
+        return;
+    }
+    long long nodes = 0;
+    int max_n = 0, width = 1, rounds = 0;
+    for (const BPBatchItem& it : items) {
+        nodes += it.tree->N;
+        max_n = std::max(max_n, it.tree->N);
+        width = std::max(width, it.tree->max_width);
+        rounds = std::max(rounds, it.tree->rounds);
+    }
+    if (bp_batch_bytes(T, nodes) > bytes) fail(DEEDS_ERR_INTERNAL, "BP batch larger than its staging area");
+    const size_t desc = batch_desc_bytes(T);
+    BPBatchTree* hd = static_cast<BPBatchTree*>(pinned);
+    int2* hw2 = reinterpret_cast<int2*>(static_cast<char*>(pinned) + desc);
+    for (int t = 0; t < T; t++) {
+        const BPBatchItem& it = items[t];
+        const BPDevTree& d = it.dt;
+        hd[t] = BPBatchTree{it.cost,      it.inds,    d.prog_start, d.prog,  d.leaves, d.leaf_w, d.off3,
+                            d.parents,    d.ordered,  d.depth_start, d.sel_info, it.u0,  it.v0,    it.w0,
+                            it.u1,        it.v1,      it.w1,        d.root,  it.tree->N, it.tree->maxd};
+    }
+    // work list: the leaves of every tree, then round by round every tree's heads of that round
+    std::vector<int> round_off(1, 0);
+    long long at = 0;
+    for (int t = 0; t < T; t++)
+        for (int i = 0; i < items[t].dt.n_leaves; i++) hw2[at++] = make_int2(t, i);
+    round_off.push_back((int)at);
+    for (int r = 0; r < rounds; r++) {
+        for (int t = 0; t < T; t++) {
+            const Tree& tr = *items[t].tree;
+            if (r >= tr.rounds) continue;
+            for (int h = tr.round_start[r]; h < tr.round_start[r + 1]; h++)
+                hw2[at++] = make_int2(t, tr.prog_start[h]);
+        }
+        round_off.push_back((int)at);
+    }
+    if (at > nodes) fail(DEEDS_ERR_INTERNAL, "BP work list exceeds the node count");
+    DEEDS_HIP(hipMemcpyAsync(d_stage, pinned, desc + (size_t)at * sizeof(int2), hipMemcpyHostToDevice, s));
+    launch_bp_batch(static_cast<const BPBatchTree*>(d_stage), T, max_n,
+                    reinterpret_cast<const int2*>(static_cast<const char*>(d_stage) + desc), round_off, width, hw,
+                    quant, s);
 }
 
 }  // namespace deeds

@@ -1,16 +1,15 @@
-// Multi-atlas batch registration: many independent pairs, several at once.
-//
-// A single registration leaves the GPU idle while the host builds MST trees
-// and while its latency-bound BP rounds run with a few workgroups each.  The
-// batch runs up to `concurrency` pairs simultaneously, each on its own
-// context (own streams, own workspace, own host tree threads), so the kernels
-// of one pair fill the gaps of another.  Every pair goes through exactly the
-// same code as deeds_ctx_register, so its outputs are bit-identical to a
-// registration of that pair alone.
-#include <atomic>
+// Multi-atlas batch registration: many independent pairs, registered in
+// groups of up to `group` pairs.  The pairs of a group run level-synchronously
+// on one context (register_group in register.hip): all their pre-BP kernels
+// of a level, then the BP of all their trees in one launch sequence -- one
+// launch per round for every tree, so the latency-bound rounds are paid once
+// per level instead of once per pair -- then their post-BP kernels.  Every
+// kernel sees the inputs it sees in a registration of the pair alone, so each
+// output is bit-identical to deeds_register on that pair.
+#include <algorithm>
 #include <map>
 #include <mutex>
-#include <thread>
+#include <string>
 #include <vector>
 
 #include "common.h"
@@ -19,8 +18,18 @@ namespace deeds {
 
 namespace {
 
-// Idle contexts per device, kept for reuse across batches (their workspaces
-// stay allocated).  Like the default contexts, they live until process exit.
+void check_pairs(const deeds_pair* pairs, int count, const deeds_schedule* S, int group) {
+    DEEDS_REQUIRE(count >= 0 && (count == 0 || pairs), "pairs is NULL");
+    DEEDS_REQUIRE(group >= 1, "concurrency (pairs per group) must be >= 1");
+    for (int i = 0; i < count; i++) {
+        const deeds_pair& P = pairs[i];
+        check_schedule(S, P.m, P.n, P.o);
+        DEEDS_REQUIRE(P.fixed && P.moving && P.flow, "a pair has a null pointer");
+    }
+}
+
+// Idle contexts per device for device-pointer batches, kept for reuse (their
+// workspaces stay allocated); like the default contexts they live until exit.
 std::mutex g_pool_mu;
 std::map<int, std::vector<deeds_ctx*>> g_pool;
 
@@ -35,7 +44,7 @@ deeds_ctx* acquire_ctx(int device) {
         }
     }
     deeds_ctx* c = nullptr;
-    int rc = deeds_ctx_create(device, &c);
+    const int rc = deeds_ctx_create(device, &c);
     if (rc != DEEDS_OK) fail(rc, std::string("batch: cannot create a context: ") + deeds_last_error());
     return c;
 }
@@ -45,69 +54,9 @@ void release_ctx(int device, deeds_ctx* c) {
     g_pool[device].push_back(c);
 }
 
-void run_batch(int device, const deeds_pair* pairs, int count, const deeds_schedule* S, int concurrency,
-               deeds_timing* timings, bool host_pointers) {
-    DEEDS_REQUIRE(count >= 0 && (count == 0 || pairs), "pairs is NULL");
-    DEEDS_REQUIRE(concurrency >= 1, "concurrency must be >= 1");
-    for (int i = 0; i < count; i++) {
-        const deeds_pair& P = pairs[i];
-        check_schedule(S, P.m, P.n, P.o);
-        DEEDS_REQUIRE(P.fixed && P.moving && P.flow, "a pair has a null pointer");
-    }
-    std::atomic<int> next{0};
-    std::mutex err_mu;
-    int status = DEEDS_OK;
-    std::string message;
-    auto worker = [&]() {
-        deeds_ctx* ctx = nullptr;
-        try {
-            ctx = acquire_ctx(device);
-            DEEDS_HIP(hipSetDevice(device));
-            for (;;) {
-                {
-                    std::lock_guard<std::mutex> lk(err_mu);
-                    if (status != DEEDS_OK) break;
-                }
-                const int i = next.fetch_add(1);
-                if (i >= count) break;
-                const deeds_pair& P = pairs[i];
-                deeds_timing* t = timings ? &timings[i] : nullptr;
-                if (!host_pointers) {
-                    register_pair(ctx, P.fixed, P.moving, P.X, P.m, P.n, P.o, *S, P.flow, P.warped, t);
-                    continue;
-                }
-                const size_t sz = (size_t)P.m * P.n * P.o;
-                const int g = S->grid[S->levels - 1];
-                const size_t sz1 = (size_t)(P.m / g) * (P.n / g) * (P.o / g);
-                float* din = ctx->get<float>("b.in", 2 * sz);
-                float* dout = ctx->get<float>("b.out", sz + 3 * sz1);
-                h2d(din, P.fixed, sz, ctx->stream);
-                h2d(din + sz, P.moving, sz, ctx->stream);
-                register_pair(ctx, din, din + sz, P.X, P.m, P.n, P.o, *S, dout + sz, P.warped ? dout : nullptr, t);
-                d2h(P.flow, dout + sz, 3 * sz1, ctx->stream);
-                if (P.warped) d2h(P.warped, dout, sz, ctx->stream);
-                DEEDS_HIP(hipStreamSynchronize(ctx->stream));
-            }
-        } catch (const Error& e) {
-            std::lock_guard<std::mutex> lk(err_mu);
-            if (status == DEEDS_OK) {
-                status = e.code;
-                message = e.msg;
-            }
-        } catch (const std::exception& e) {
-            std::lock_guard<std::mutex> lk(err_mu);
-            if (status == DEEDS_OK) {
-                status = DEEDS_ERR_INTERNAL;
-                message = e.what();
-            }
-        }
-        if (ctx) release_ctx(device, ctx);
-    };
-    const int nw = std::min(concurrency, std::max(count, 1));
-    std::vector<std::thread> threads;
-    for (int w = 0; w < nw; w++) threads.emplace_back(worker);
-    for (auto& th : threads) th.join();
-    if (status != DEEDS_OK) fail(status, "batch: " + message);
+size_t flow_count(const deeds_pair& P, const deeds_schedule& S) {
+    const int g = S.grid[S.levels - 1];
+    return 3 * (size_t)(P.m / g) * (P.n / g) * (P.o / g);
 }
 
 }  // namespace
@@ -121,8 +70,34 @@ extern "C" {
 int deeds_register_batch(const deeds_pair* pairs, int count, const deeds_schedule* sched, int concurrency,
                          deeds_timing* timings) {
     return guarded(nullptr, [&] {
-        int device = default_ctx()->device;
-        run_batch(device, pairs, count, sched, concurrency, timings, true);
+        check_pairs(pairs, count, sched, concurrency);
+        deeds_ctx* c = default_ctx();
+        DEEDS_HIP(hipSetDevice(c->device));
+        for (int g0 = 0; g0 < count; g0 += concurrency) {
+            const int gn = std::min(concurrency, count - g0);
+            // device copies of the group's inputs and outputs
+            std::vector<deeds_pair> dev(gn);
+            for (int i = 0; i < gn; i++) {
+                const deeds_pair& P = pairs[g0 + i];
+                const size_t sz = (size_t)P.m * P.n * P.o, nf = flow_count(P, *sched);
+                float* din = c->get<float>("h.batch.in." + std::to_string(i), 2 * sz);
+                float* dout = c->get<float>("h.batch.out." + std::to_string(i), nf + (P.warped ? sz : 0));
+                h2d(din, P.fixed, sz, c->stream);
+                h2d(din + sz, P.moving, sz, c->stream);
+                dev[i] = P;
+                dev[i].fixed = din;
+                dev[i].moving = din + sz;
+                dev[i].flow = dout;
+                dev[i].warped = P.warped ? dout + nf : nullptr;
+            }
+            register_group(c, dev.data(), gn, *sched, timings ? timings + g0 : nullptr);
+            for (int i = 0; i < gn; i++) {
+                const deeds_pair& P = pairs[g0 + i];
+                d2h(P.flow, dev[i].flow, flow_count(P, *sched), c->stream);
+                if (P.warped) d2h(P.warped, dev[i].warped, (size_t)P.m * P.n * P.o, c->stream);
+            }
+            DEEDS_HIP(hipStreamSynchronize(c->stream));
+        }
     });
 }
 
@@ -132,7 +107,18 @@ int deeds_register_batch_device(int device, const deeds_pair* pairs, int count, 
         int n = 0;
         DEEDS_HIP(hipGetDeviceCount(&n));
         DEEDS_REQUIRE(device >= 0 && device < n, "device index out of range");
-        run_batch(device, pairs, count, sched, concurrency, timings, false);
+        check_pairs(pairs, count, sched, concurrency);
+        deeds_ctx* c = acquire_ctx(device);
+        try {
+            DEEDS_HIP(hipSetDevice(device));
+            for (int g0 = 0; g0 < count; g0 += concurrency)
+                register_group(c, pairs + g0, std::min(concurrency, count - g0), *sched,
+                               timings ? timings + g0 : nullptr);
+        } catch (...) {
+            release_ctx(device, c);
+            throw;
+        }
+        release_ctx(device, c);
     });
 }
 

@@ -26,6 +26,12 @@
 // (LEN is a template parameter).
 //
 // Top-down: one workgroup walks the BFS depths selecting labels (:259-293).
+//
+// Batched form (launch_bp_batch): the same per-workgroup bodies for several
+// trees of one label radius -- the two directions' trees of many pairs -- with
+// one launch per round covering every tree's items of that round and one
+// selection workgroup per tree; a workgroup finds its tree and item in a work
+// list, so each tree's arithmetic is exactly its single-tree arithmetic.
 #include "kernels.h"
 
 namespace deeds {
@@ -187,15 +193,14 @@ struct BPLds {
 
 // Round 0: every leaf's message to its parent, stored in the leaf's cost row
 // as msg - min(msg), the term its parent folds (regularisation.h:197-232).
+// One workgroup per leaf; `idx` is the leaf's position in a.leaves.
 template <int LEN>
-__global__ __launch_bounds__(BPShape<LEN>::BLOCK) void k_bp_leaves(BPArgs a) {
+__device__ __forceinline__ void leaf_body(const BPArgs& a, int idx, float* smem, float* red) {
     using B = BPShape<LEN>;
-    extern __shared__ float smem[];
-    __shared__ float red[16];
     const BPLds<LEN> s(smem);
     const int t = threadIdx.x;
-    const int v = a.leaves[blockIdx.x];
-    const float w = __int_as_float(a.leaf_w[blockIdx.x]);
+    const int v = a.leaves[idx];
+    const float w = __int_as_float(a.leaf_w[idx]);
     float* row = a.cost + (long long)v * B::L3;
     for (int l = t; l < B::L3; l += B::BLOCK) s.A[l] = row[l] * w;  // costall[child] *= edgemst (:197-202)
     const float mn = message<LEN>(a.allinds + (long long)v * B::L3, a.off3[3 * v], a.off3[3 * v + 1],
@@ -203,24 +208,22 @@ __global__ __launch_bounds__(BPShape<LEN>::BLOCK) void k_bp_leaves(BPArgs a) {
     for (int l = t; l < B::L3; l += B::BLOCK) row[l] = s.Bf[l] - mn;
 }
 
-// One workgroup runs one head's program: the head's accumulator is its cost
-// plus its children's stored messages in `ordered` order (the reference's
-// fold order, :221-232); then each chain node messages its parent, which is
-// the next chain node's accumulator (its only child's message added to its
-// cost), until the chain's last node, which stores its message in its own
-// row -- or, at the root, its accumulator.  The accumulator lives in
-// registers (each thread owns fixed labels); the next chain node's cost row is
-// loaded while the current message computes.
+// One workgroup runs one head's program (at a.prog + `off`): the
+// head's accumulator is its cost plus its children's stored messages in
+// `ordered` order (the reference's fold order, :221-232); then each chain
+// node messages its parent, which is the next chain node's accumulator (its
+// only child's message added to its cost), until the chain's last node, which
+// stores its message in its own row -- or, at the root, its accumulator.  The
+// accumulator lives in registers (each thread owns fixed labels); the next
+// chain node's cost row is loaded while the current message computes.
 template <int LEN>
-__global__ __launch_bounds__(BPShape<LEN>::BLOCK) void k_bp_prog(BPArgs a, int head_begin) {
+__device__ __forceinline__ void prog_body(const BPArgs& a, int off, float* smem, float* red) {
     using B = BPShape<LEN>;
     constexpr int ELEMS = (B::L3 + B::BLOCK - 1) / B::BLOCK;
-    extern __shared__ float smem[];
-    __shared__ float red[16];
     const BPLds<LEN> s(smem);
     const int t = threadIdx.x;
 
-    const int* pg = a.prog + a.prog_start[head_begin + blockIdx.x];
+    const int* pg = a.prog + off;
     const int x = pg[0], k = pg[1], L = pg[2];
     const int* kids = pg + 3;
     const int* chain = kids + k;
@@ -285,23 +288,78 @@ __global__ __launch_bounds__(BPShape<LEN>::BLOCK) void k_bp_prog(BPArgs a, int h
     }
 }
 
+template <int LEN>
+__global__ __launch_bounds__(BPShape<LEN>::BLOCK) void k_bp_leaves(BPArgs a) {
+    extern __shared__ float smem[];
+    __shared__ float red[16];
+    leaf_body<LEN>(a, blockIdx.x, smem, red);
+}
+
+template <int LEN>
+__global__ __launch_bounds__(BPShape<LEN>::BLOCK) void k_bp_prog(BPArgs a, int head_begin) {
+    extern __shared__ float smem[];
+    __shared__ float red[16];
+    prog_body<LEN>(a, a.prog_start[head_begin + blockIdx.x], smem, red);
+}
+
+__device__ __forceinline__ BPArgs args_of(const BPBatchTree& T) {
+    return BPArgs{T.cost, T.allinds, T.prog_start, T.prog, T.leaves, T.leaf_w, T.off3, T.root};
+}
+
+// Batched forms: workgroup b runs item work[b] = (tree, leaf position or
+// program offset) of one round of several trees that share the label count.
+template <int LEN>
+__global__ __launch_bounds__(BPShape<LEN>::BLOCK) void k_bp_leaves_b(const BPBatchTree* __restrict__ trees,
+                                                                     const int2* __restrict__ work) {
+    extern __shared__ float smem[];
+    __shared__ float red[16];
+    const int2 w = work[blockIdx.x];
+    leaf_body<LEN>(args_of(trees[w.x]), w.y, smem, red);
+}
+
+template <int LEN>
+__global__ __launch_bounds__(BPShape<LEN>::BLOCK) void k_bp_prog_b(const BPBatchTree* __restrict__ trees,
+                                                                   const int2* __restrict__ work) {
+    extern __shared__ float smem[];
+    __shared__ float red[16];
+    const int2 w = work[blockIdx.x];
+    prog_body<LEN>(args_of(trees[w.x]), w.y, smem, red);
+}
+
+template <int LEN>
+void set_round_attrs() {
+    constexpr int smem = (int)BPShape<LEN>::SMEM;
+    static bool attr_set = false;
+    if (attr_set) return;
+    const void* fns[4] = {(const void*)k_bp_prog<LEN>, (const void*)k_bp_leaves<LEN>, (const void*)k_bp_prog_b<LEN>,
+                          (const void*)k_bp_leaves_b<LEN>};
+    for (const void* f : fns) DEEDS_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, smem));
+    attr_set = true;
+}
+
 // round < 0: the leaves; otherwise heads [head_begin, head_begin + count)
 template <int LEN>
 void launch_round_len(const BPArgs& a, int head_begin, int count, bool leaves, hipStream_t s) {
     using B = BPShape<LEN>;
     constexpr size_t smem = B::SMEM;
-    static bool attr_set = false;
-    if (!attr_set) {
-        DEEDS_HIP(hipFuncSetAttribute((const void*)k_bp_prog<LEN>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)smem));
-        DEEDS_HIP(hipFuncSetAttribute((const void*)k_bp_leaves<LEN>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)smem));
-        attr_set = true;
-    }
+    set_round_attrs<LEN>();
     if (leaves)
         k_bp_leaves<LEN><<<count, B::BLOCK, smem, s>>>(a);
     else
         k_bp_prog<LEN><<<count, B::BLOCK, smem, s>>>(a, head_begin);
+    DEEDS_LAUNCH_CHECK();
+}
+
+// One batched round: `count` work items starting at `work`.
+template <int LEN>
+void launch_round_len_b(const BPBatchTree* trees, const int2* work, int count, bool leaves, hipStream_t s) {
+    using B = BPShape<LEN>;
+    constexpr size_t smem = B::SMEM;
+    set_round_attrs<LEN>();
+    if (leaves)
+        k_bp_leaves_b<LEN><<<count, B::BLOCK, smem, s>>>(trees, work);
+    else
+        k_bp_prog_b<LEN><<<count, B::BLOCK, smem, s>>>(trees, work);
     DEEDS_LAUNCH_CHECK();
 }
 
@@ -311,18 +369,13 @@ void launch_round_len(const BPArgs& a, int head_begin, int count, bool leaves, h
 // 259-293).  The previous depth's labels stay in LDS (indexed by position
 // within the depth), so each depth costs one dependent load from HBM, and
 // the next depth's (node, parent index) pairs are loaded ahead.
-__global__ __launch_bounds__(1024) void k_bp_select(const float* __restrict__ cost,
-                                                    const int16_t* __restrict__ allinds,
-                                                    const int* __restrict__ ordered,
-                                                    const int* __restrict__ depth_start,
-                                                    const int* __restrict__ sel_info, int maxd,
-                                                    const float* __restrict__ u0, const float* __restrict__ v0,
-                                                    const float* __restrict__ w0, float* __restrict__ u1,
-                                                    float* __restrict__ v1, float* __restrict__ w1, int width,
-                                                    int len, int hw, float quant) {
-    extern __shared__ int labs[];  // two buffers of `width` labels
-    __shared__ float rv[16];
-    __shared__ int ri[16];
+// labs: two LDS buffers of `width` labels; rv, ri: 16 entries each.
+__device__ __forceinline__ void select_body(const float* __restrict__ cost, const int16_t* __restrict__ allinds,
+                                            const int* __restrict__ ordered, const int* __restrict__ depth_start,
+                                            const int* __restrict__ sel_info, int maxd, const float* __restrict__ u0,
+                                            const float* __restrict__ v0, const float* __restrict__ w0,
+                                            float* __restrict__ u1, float* __restrict__ v1, float* __restrict__ w1,
+                                            int width, int len, int hw, float quant, int* labs, float* rv, int* ri) {
     const int len3 = len * len * len;
     const int root = ordered[0];
     const float* rc = cost + (long long)root * len3;
@@ -389,6 +442,53 @@ __global__ __launch_bounds__(1024) void k_bp_select(const float* __restrict__ co
     }
 }
 
+__global__ __launch_bounds__(1024) void k_bp_select(const float* __restrict__ cost,
+                                                    const int16_t* __restrict__ allinds,
+                                                    const int* __restrict__ ordered,
+                                                    const int* __restrict__ depth_start,
+                                                    const int* __restrict__ sel_info, int maxd,
+                                                    const float* __restrict__ u0, const float* __restrict__ v0,
+                                                    const float* __restrict__ w0, float* __restrict__ u1,
+                                                    float* __restrict__ v1, float* __restrict__ w1, int width,
+                                                    int len, int hw, float quant) {
+    extern __shared__ int labs[];
+    __shared__ float rv[16];
+    __shared__ int ri[16];
+    select_body(cost, allinds, ordered, depth_start, sel_info, maxd, u0, v0, w0, u1, v1, w1, width, len, hw, quant,
+                labs, rv, ri);
+}
+
+// One workgroup per tree; `width` is the largest depth width over the trees.
+__global__ __launch_bounds__(1024) void k_bp_select_b(const BPBatchTree* __restrict__ trees, int width, int len,
+                                                      int hw, float quant) {
+    extern __shared__ int labs[];
+    __shared__ float rv[16];
+    __shared__ int ri[16];
+    const BPBatchTree& T = trees[blockIdx.x];
+    select_body(T.cost, T.allinds, T.ordered, T.depth_start, T.sel_info, T.maxd, T.u0, T.v0, T.w0, T.u1, T.v1,
+                T.w1, width, len, hw, quant, labs, rv, ri);
+}
+
+// Parent offsets of every tree of a batch (blockIdx.y = tree).
+__global__ void k_parent_offsets_b(const BPBatchTree* __restrict__ trees, float quant) {
+    const BPBatchTree& T = trees[blockIdx.y];
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= T.N) return;
+    const int p = T.parents[c];
+    if (p < 0) return;
+    T.off3[3 * c + 0] = (T.u0[p] - T.u0[c]) / quant;
+    T.off3[3 * c + 1] = (T.v0[p] - T.v0[c]) / quant;
+    T.off3[3 * c + 2] = (T.w0[p] - T.w0[c]) / quant;
+}
+
+size_t select_smem(int width) {
+    const size_t smem = 2 * (size_t)std::max(1, width) * sizeof(int);
+    // beyond the default 64 KiB the kernel needs the attribute (its static LDS takes the rest)
+    if (smem > (size_t)kMaxLds - 1024)
+        fail(DEEDS_ERR_ARG, "tree too wide for the label selection (a depth with over 20000 nodes)");
+    return smem;
+}
+
 }  // namespace
 
 void launch_bp_rounds(float* cost, int16_t* allinds, float* off3, const BPDevTree& t, int N,
@@ -424,10 +524,7 @@ void launch_bp_select(const float* cost, const int16_t* allinds, const BPDevTree
                       const float* u0, const float* v0, const float* w0, float* u1, float* v1, float* w1, int* sel,
                       int hw, float quant, hipStream_t s) {
     int len = 2 * hw + 1;
-    const size_t smem = 2 * (size_t)std::max(1, t.max_width) * sizeof(int);
-    // beyond the default 64 KiB the kernel needs the attribute (its static LDS takes the rest)
-    if (smem > (size_t)kMaxLds - 1024)
-        fail(DEEDS_ERR_ARG, "tree too wide for the label selection (a depth with over 20000 nodes)");
+    const size_t smem = select_smem(t.max_width);
     if (smem > 65536)
         DEEDS_HIP(hipFuncSetAttribute((const void*)k_bp_select, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
     k_bp_select<<<1, 1024, smem, s>>>(cost, allinds, t.ordered, t.depth_start, t.sel_info, maxd, u0, v0, w0, u1, v1,
@@ -435,6 +532,40 @@ void launch_bp_select(const float* cost, const int16_t* allinds, const BPDevTree
     DEEDS_LAUNCH_CHECK();
     (void)sel;
     (void)N;
+}
+
+void launch_bp_batch(const BPBatchTree* d_trees, int trees, int max_n, const int2* work,
+                     const std::vector<int>& round_off, int width, int hw, float quant, hipStream_t s) {
+    if (trees <= 0) return;
+    if (max_n > 0) {
+        k_parent_offsets_b<<<dim3(ceil_div(max_n, 256), trees), 256, 0, s>>>(d_trees, quant);
+        DEEDS_LAUNCH_CHECK();
+    }
+    for (size_t r = 0; r + 1 < round_off.size(); r++) {
+        const int b = round_off[r], n = round_off[r + 1] - b;
+        if (n <= 0) continue;
+        const bool leaves = r == 0;
+        switch (2 * hw + 1) {
+            case 1: launch_round_len_b<1>(d_trees, work + b, n, leaves, s); break;
+            case 3: launch_round_len_b<3>(d_trees, work + b, n, leaves, s); break;
+            case 5: launch_round_len_b<5>(d_trees, work + b, n, leaves, s); break;
+            case 7: launch_round_len_b<7>(d_trees, work + b, n, leaves, s); break;
+            case 9: launch_round_len_b<9>(d_trees, work + b, n, leaves, s); break;
+            case 11: launch_round_len_b<11>(d_trees, work + b, n, leaves, s); break;
+            case 13: launch_round_len_b<13>(d_trees, work + b, n, leaves, s); break;
+            case 15: launch_round_len_b<15>(d_trees, work + b, n, leaves, s); break;
+            case 17: launch_round_len_b<17>(d_trees, work + b, n, leaves, s); break;
+            case 19: launch_round_len_b<19>(d_trees, work + b, n, leaves, s); break;
+            case 21: launch_round_len_b<21>(d_trees, work + b, n, leaves, s); break;
+            default: fail(DEEDS_ERR_ARG, "search radius outside [0, 10]");
+        }
+    }
+    const size_t smem = select_smem(width);
+    if (smem > 65536)
+        DEEDS_HIP(hipFuncSetAttribute((const void*)k_bp_select_b, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)smem));
+    k_bp_select_b<<<trees, 1024, smem, s>>>(d_trees, std::max(1, width), 2 * hw + 1, hw, quant);
+    DEEDS_LAUNCH_CHECK();
 }
 
 }  // namespace deeds

@@ -51,9 +51,13 @@ struct deeds_ctx {
     hipStream_t side_stream[2] = {nullptr, nullptr};  // small work issued from host helper threads (image stats)
     std::string err;
     std::map<std::string, deeds::DevBuf> ws;
+    std::map<std::string, deeds::DevBuf> pinned_ws;
 
     // Device buffer `name` of at least `bytes` (contents unspecified).
     void* buf(const std::string& name, size_t bytes);
+    // Pinned host buffer `name` of at least `bytes`, kept for reuse.  Request
+    // each name once per call: growing it waits for the context's streams.
+    void* pinned(const std::string& name, size_t bytes);
     template <class T>
     T* get(const std::string& name, size_t count) {
         return static_cast<T*>(buf(name, count * sizeof(T)));
@@ -96,6 +100,10 @@ void check_schedule(const deeds_schedule* S, int m, int n, int o);
 // One symmetric registration on ctx, device pointers (register.hip).
 void register_pair(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, const float* X, int m, int n, int o,
                    const deeds_schedule& S, float* d_flow, float* d_warped, deeds_timing* timing);
+// Level-synchronous registration of `count` pairs (device pointers) on ctx;
+// timings is optional (count entries).  Pairs are validated by the caller.
+void register_group(deeds_ctx* ctx, const deeds_pair* pairs, int count, const deeds_schedule& S,
+                    deeds_timing* timings);
 
 // Upload / download helpers on a stream.
 template <class T>

@@ -61,6 +61,33 @@ void launch_bp_select(const float* cost, const int16_t* allinds, const BPDevTree
                       const float* u0, const float* v0, const float* w0, float* u1, float* v1, float* w1,
                       int* sel, int hw, float quant, hipStream_t s);
 
+// One tree of a batched BP (device pointers; the schedule as in BPDevTree).
+struct BPBatchTree {
+    float* cost;
+    int16_t* allinds;
+    const int* prog_start;
+    const int* prog;
+    const int* leaves;
+    const int* leaf_w;
+    float* off3;
+    const int* parents;
+    const int* ordered;
+    const int* depth_start;
+    const int* sel_info;
+    const float *u0, *v0, *w0;
+    float *u1, *v1, *w1;
+    int root, N, maxd;
+};
+// BP of several trees with the same label radius in one launch sequence:
+// the parent offsets of all trees, then one launch per round holding every
+// tree's items of that round -- work[round_off[0] .. round_off[1]) are the
+// leaves (tree, leaf position), each later range a round's (tree, offset of
+// the head's program in prog) -- then one selection workgroup per tree.
+// d_trees: device array of `trees` descriptors; max_n: largest N; width:
+// largest depth width.
+void launch_bp_batch(const BPBatchTree* d_trees, int trees, int max_n, const int2* work,
+                     const std::vector<int>& round_off, int width, int hw, float quant, hipStream_t s);
+
 // ---- transforms (warp.hip) -------------------------------------------------
 struct Affine {
     float r[12];  // coefficients of y = r[0..3], x = r[4..7], z = r[8..11] (dataCostD.h:285-287)

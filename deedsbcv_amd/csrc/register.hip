@@ -4,29 +4,39 @@
 // trees depend on nothing but the fixed image, the affine-warped moving image
 // and the grid step (primsGraph is called on im1b and warped0 at every level,
 // deedsBCV0.cpp:275 and :302), so all of them are started at the beginning:
-// the GPU computes every level's edge SADs, the host threads turn them into
-// trees while the GPU runs MIND and the data costs.  Within a level the
-// forward and backward directions are independent until consistentMappingCL
-// (:311) and run on two streams.
+// the GPU computes every level's edge SADs, host threads turn them into trees
+// while the GPU runs MIND and the data costs.  Within a level the forward and
+// backward directions are independent until consistentMappingCL (:311) and
+// run on two streams, BP included.
+//
+// A registration is a PairRun: its phases (begin, per level pre-BP work, tree
+// staging, post-BP work, outputs) are what the drivers interleave.  The
+// single-pair driver runs one PairRun.  The multi-atlas driver runs many
+// level-synchronously: every pair's pre-BP work of a level, then per direction
+// the BP of all the pairs' trees in one launch sequence (one launch per round
+// for all trees -- BP rounds are latency-bound, so 16 trees cost about as
+// much as one), then every pair's post-BP work.  Every kernel sees the same inputs as in a
+// registration of the pair alone, so the outputs are bit-identical.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
+#include <functional>
 #include <future>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 
+#include "bp_host.h"
 #include "common.h"
 #include "kernels.h"
 #include "mst_host.h"
 #include "stats.h"
 
 namespace deeds {
-
-size_t tree_stage_bytes(int N);
-BPDevTree stage_tree(const Tree& t, void* pinned, void* d_stage, hipStream_t s);
-void run_bp(const Tree& t, const BPDevTree& dt, float* d_cost, int16_t* d_inds, int* d_sel, const float* u0,
-            const float* v0, const float* w0, float* u1, float* v1, float* w1, int hw, float quant, hipStream_t s);
 
 namespace {
 
@@ -62,16 +72,62 @@ struct Event {
     Event& operator=(const Event&) = delete;
 };
 
-struct PinnedBuf {
-    void* p = nullptr;
-    explicit PinnedBuf(size_t bytes) { DEEDS_HIP(hipHostMalloc(&p, std::max<size_t>(bytes, 64), hipHostMallocDefault)); }
-    ~PinnedBuf() {
-        if (p) (void)hipHostFree(p);
+double elapsed_s(hipEvent_t a, hipEvent_t b) {
+    float ms = 0;
+    DEEDS_HIP(hipEventElapsedTime(&ms, a, b));
+    return ms * 1e-3;
+}
+
+// Host threads running tasks in submission order.  The destructor runs what
+// is still queued and joins.
+class TaskPool {
+   public:
+    explicit TaskPool(int threads) {
+        for (int i = 0; i < std::max(1, threads); i++) workers_.emplace_back([this] { loop(); });
     }
-    template <class T>
-    T* as() {
-        return static_cast<T*>(p);
+    ~TaskPool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& w : workers_) w.join();
     }
+    TaskPool(const TaskPool&) = delete;
+    TaskPool& operator=(const TaskPool&) = delete;
+
+    template <class F>
+    auto submit(F f) -> std::future<decltype(f())> {
+        using R = decltype(f());
+        auto task = std::make_shared<std::packaged_task<R()>>(std::move(f));
+        std::future<R> fut = task->get_future();
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            q_.emplace_back([task] { (*task)(); });
+        }
+        cv_.notify_one();
+        return fut;
+    }
+
+   private:
+    void loop() {
+        for (;;) {
+            std::function<void()> job;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [this] { return stop_ || !q_.empty(); });
+                if (q_.empty()) return;
+                job = std::move(q_.front());
+                q_.pop_front();
+            }
+            job();
+        }
+    }
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<std::function<void()>> q_;
+    bool stop_ = false;
+    std::vector<std::thread> workers_;
 };
 
 struct LevelCfg {
@@ -80,147 +136,201 @@ struct LevelCfg {
     CostPlan plan;
 };
 
-}  // namespace
-
-void register_pair(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, const float* X, int m, int n, int o,
-                   const deeds_schedule& S, float* d_flow, float* d_warped, deeds_timing* timing) {
-    using clock = std::chrono::steady_clock;
-    const long long sz = (long long)m * n * o;
-    const int L = S.levels;
-    hipStream_t s0 = ctx->stream, s1 = ctx->aux_stream;
-
-    // ---- schedule (deedsBCV0.cpp:183-187, :222-242) ----
-    std::vector<LevelCfg> lv(L);
-    int max_qs = 0;
-    size_t max_cost = 0, max_pad = 0, max_sz1 = 0;
-    for (int l = 0; l < L; l++) {
+std::vector<LevelCfg> level_cfgs(const deeds_schedule& S, int m, int n, int o) {
+    std::vector<LevelCfg> lv(S.levels);
+    for (int l = 0; l < S.levels; l++) {
         LevelCfg& c = lv[l];
         c.step = S.grid[l];
         c.hw = S.radius[l];
         c.quant = S.quant[l];
         c.len = 2 * c.hw + 1;
         c.len3 = c.len * c.len * c.len;
-        c.m1 = m / c.step; c.n1 = n / c.step; c.o1 = o / c.step;
+        c.m1 = m / c.step;
+        c.n1 = n / c.step;
+        c.o1 = o / c.step;
         c.sz1 = c.m1 * c.n1 * c.o1;
-        c.mind_qs = (int)std::floor(0.5f * (float)c.quant + 1.0f);
+        c.mind_qs = (int)std::floor(0.5f * (float)c.quant + 1.0f);  // deedsBCV0.cpp:185
         c.new_mind = (l == 0) || (lv[l - 1].mind_qs != c.mind_qs);
         c.plan = datacost_plan(m, n, o, c.step, c.hw, (float)c.quant, S.alpha, 1);
-        max_qs = std::max(max_qs, c.mind_qs);
-        max_cost = std::max(max_cost, (size_t)c.sz1 * c.len3);
-        max_pad = std::max(max_pad, (size_t)c.plan.mp * c.plan.np * c.plan.op);
-        max_sz1 = std::max(max_sz1, (size_t)c.sz1);
     }
-    DEEDS_REQUIRE(m >= 2 * max_qs + 1 && n >= 2 * max_qs + 1 && o >= 2 * max_qs + 1,
-                  "volume too small for the MIND patch radius of the schedule");
-    DEEDS_REQUIRE(m <= mind_max_dim() && n <= mind_max_dim() && o <= mind_max_dim(),
-                  "a dimension exceeds the box filter's LDS tile limit");
+    return lv;
+}
 
-    // ---- device workspace ----
-    float* im1 = ctx->get<float>("r.im1", sz);
-    float* im1b = ctx->get<float>("r.im1b", sz);
-    float* warped0 = ctx->get<float>("r.warped0", sz);
-    uint64_t* mind_mov = ctx->get<uint64_t>("r.mind.mov", sz);
-    uint64_t* mind_fix = ctx->get<uint64_t>("r.mind.fix", sz);
-    float* warped[2] = {ctx->get<float>("r.warped.0", sz), ctx->get<float>("r.warped.1", sz)};
-    uint64_t* mind_w[2] = {ctx->get<uint64_t>("r.mindw.0", sz), ctx->get<uint64_t>("r.mindw.1", sz)};
+// Device scratch that pairs whose work is ordered on the same stream share:
+// per direction the MIND temporaries, the warped image, its codes and the
+// padded codes the data cost reads.
+struct Scratch {
     float* mw[2][2];
-    for (int d = 0; d < 2; d++)
-        for (int b = 0; b < 2; b++)
-            mw[d][b] = ctx->get<float>("r.mindtmp." + std::to_string(d) + std::to_string(b), 6 * sz);
-    uint64_t* mpad[2] = {ctx->get<uint64_t>("r.mpad.0", max_pad), ctx->get<uint64_t>("r.mpad.1", max_pad)};
-    float* cost[2] = {ctx->get<float>("r.cost.0", max_cost), ctx->get<float>("r.cost.1", max_cost)};
-    int16_t* inds[2] = {ctx->get<int16_t>("r.inds.0", max_cost), ctx->get<int16_t>("r.inds.1", max_cost)};
-    int* sel[2] = {ctx->get<int>("r.sel.0", max_sz1), ctx->get<int>("r.sel.1", max_sz1)};
-    // control-grid fields: previous level (fwd u,v,w, bwd u,v,w), this level's u0 per direction, new fields
-    size_t sz2_0 = (size_t)(m / S.grid[0]) * (n / S.grid[0]) * (o / S.grid[0]);
-    size_t fmax = std::max(max_sz1, sz2_0);
-    float* prevf = ctx->get<float>("r.prevf", 6 * fmax);
-    float* newf = ctx->get<float>("r.newf", 6 * fmax);
-    float* g0[2] = {ctx->get<float>("r.g0.0", 3 * fmax), ctx->get<float>("r.g0.1", 3 * fmax)};
-    float* cmtmp = ctx->get<float>("r.cm", 6 * fmax);
-    double* jac = ctx->get<double>("r.jac", 4 * (size_t)L);
-    double* ssd = ctx->get<double>("r.ssd", 2);
+    float* warped[2];
+    uint64_t* mind_w[2];
+    uint64_t* mpad[2];
+};
 
-    Event ev_start, ev_stop, ev_init, ev_mind, ev_bwd_done, ev_sad;
-    std::vector<std::unique_ptr<Event>> ev_lvl;
-    for (int i = 0; i < 6 * L; i++) ev_lvl.emplace_back(new Event());
-    auto EV = [&](int l, int k) { return ev_lvl[6 * l + k]->e; };
-    // events bracketing every data-cost kernel launch, on its own stream
-    std::vector<std::unique_ptr<Event>> ev_dc;
-    for (int i = 0; i < 4 * L; i++) ev_dc.emplace_back(new Event());
-    // marker on each direction's stream right before its BP is enqueued
-    std::vector<std::unique_ptr<Event>> ev_pre;
-    for (int i = 0; i < 2 * L; i++) ev_pre.emplace_back(new Event());
-    auto DC = [&](int l, int dir, int end) { return ev_dc[4 * l + 2 * dir + end]->e; };
+Scratch make_scratch(deeds_ctx* ctx, size_t sz, size_t pad) {
+    Scratch sh;
+    for (int d = 0; d < 2; d++) {
+        const std::string k = std::to_string(d);
+        for (int b = 0; b < 2; b++) sh.mw[d][b] = ctx->get<float>("r.mindtmp." + k + std::to_string(b), 6 * sz);
+        sh.warped[d] = ctx->get<float>("r.warped." + k, sz);
+        sh.mind_w[d] = ctx->get<uint64_t>("r.mindw." + k, sz);
+        sh.mpad[d] = ctx->get<uint64_t>("r.mpad." + k, pad);
+    }
+    return sh;
+}
 
-    DEEDS_HIP(hipEventRecord(ev_start.e, s0));
-    auto t0 = clock::now();
+size_t max_padded(const std::vector<LevelCfg>& lv) {
+    size_t p = 0;
+    for (const LevelCfg& c : lv) p = std::max(p, (size_t)c.plan.mp * c.plan.np * c.plan.op);
+    return p;
+}
 
-    // ---- inputs: +1024 HU offset (deedsBCV0.cpp:139-144), affine warp (:210) ----
-    const float identity[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
-    Affine A = affine_from_X(X ? X : identity);
-    launch_offset(im1b, d_fixed, -1024.0f, sz, s0);
-    launch_offset(im1, d_moving, -1024.0f, sz, s0);
-    {
-        // warpAffine with a zero field == the grid-fused warp from a zero 1x1x1 grid
-        float* zero3 = g0[0];
-        launch_zero(zero3, 3, s0);
-        launch_warp_affine_grid(warped0, im1, A, zero3, zero3 + 1, zero3 + 2, m, n, o, 1, 1, 1, s0);
+struct PairIn {
+    const float* fixed;
+    const float* moving;
+    const float* X;
+    int m, n, o;
+    float* flow;
+    float* warped;
+    deeds_timing* timing;
+};
+
+// One symmetric registration, phase by phase.  Streams: ctx->stream (s0,
+// forward direction and everything shared) and ctx->aux_stream (s1, the
+// backward direction's pre-BP work and BP).  Workspace: ctx buffers named with
+// `tag`, plus the shared scratch.
+class PairRun {
+   public:
+    PairRun(deeds_ctx* ctx, const std::string& tag, const PairIn& in, const deeds_schedule& S, const Scratch& sh)
+        : ctx_(ctx), tag_(tag), in_(in), S_(S), sh_(sh), L_(S.levels), s0_(ctx->stream), s1_(ctx->aux_stream) {
+        const int m = in.m, n = in.n, o = in.o;
+        sz_ = (long long)m * n * o;
+        lv_ = level_cfgs(S, m, n, o);
+        int max_qs = 0;
+        size_t max_cost = 0, max_sz1 = 0;
+        for (const LevelCfg& c : lv_) {
+            max_qs = std::max(max_qs, c.mind_qs);
+            max_cost = std::max(max_cost, (size_t)c.sz1 * c.len3);
+            max_sz1 = std::max(max_sz1, (size_t)c.sz1);
+        }
+        DEEDS_REQUIRE(m >= 2 * max_qs + 1 && n >= 2 * max_qs + 1 && o >= 2 * max_qs + 1,
+                      "volume too small for the MIND patch radius of the schedule");
+        DEEDS_REQUIRE(m <= mind_max_dim() && n <= mind_max_dim() && o <= mind_max_dim(),
+                      "a dimension exceeds the box filter's LDS tile limit");
+        const size_t sz = (size_t)sz_;
+        im1_ = get<float>("im1", sz);
+        im1b_ = get<float>("im1b", sz);
+        warped0_ = get<float>("warped0", sz);
+        mind_mov_ = get<uint64_t>("mind.mov", sz);
+        mind_fix_ = get<uint64_t>("mind.fix", sz);
+        for (int d = 0; d < 2; d++) {
+            cost_[d] = get<float>("cost." + std::to_string(d), max_cost);
+            inds_[d] = get<int16_t>("inds." + std::to_string(d), max_cost);
+        }
+        // control-grid fields: previous level (fwd u,v,w, bwd u,v,w), this level's u0 per direction, new fields
+        const size_t sz2_0 = (size_t)(m / S.grid[0]) * (n / S.grid[0]) * (o / S.grid[0]);
+        const size_t fmax = std::max(max_sz1, sz2_0);
+        prevf_ = get<float>("prevf", 6 * fmax);
+        newf_ = get<float>("newf", 6 * fmax);
+        for (int d = 0; d < 2; d++) g0_[d] = get<float>("g0." + std::to_string(d), 3 * fmax);
+        cmtmp_ = get<float>("cm", 6 * fmax);
+        jac_ = get<double>("jac", 4 * (size_t)L_);
+        ssd_ = get<double>("ssd", 2);
+        // MST inputs: edge SADs of every level on both images
+        ec_off_.resize(L_);
+        for (int l = 0; l < L_; l++) {
+            ec_off_[l] = ec_total_;
+            ec_total_ += 6 * (size_t)lv_[l].sz1;
+        }
+        d_ec_ = get<float>("ec", 2 * ec_total_);
+        h_ec_ = static_cast<float*>(ctx->pinned(tag + "r.ec", 2 * ec_total_ * sizeof(float)));
+        stats_bytes_ = (StatsScratch::bytes(sz_) + 255) & ~(size_t)255;
+        d_stats_ = static_cast<char*>(ctx->buf(tag + "r.stats", 2 * stats_bytes_));
+        h_stats_ = static_cast<char*>(ctx->pinned(tag + "r.stats", 2 * stats_bytes_));
+        // pinned staging for every (level, direction) tree and a device area per direction
+        stage_off_.resize(L_ + 1);
+        size_t stage_max = 0;
+        for (int l = 0; l < L_; l++) {
+            const size_t b = (tree_stage_bytes(lv_[l].sz1) + 255) & ~(size_t)255;
+            stage_off_[l] = stage_total_;
+            stage_total_ += 2 * b;
+            stage_max = std::max(stage_max, b);
+        }
+        stage_off_[L_] = stage_total_;
+        h_stage_ = static_cast<char*>(ctx->pinned(tag + "r.stage", stage_total_));
+        for (int d = 0; d < 2; d++) d_stage_[d] = ctx->buf(tag + "r.tree." + std::to_string(d), stage_max);
+        for (int i = 0; i < 6 * L_; i++) ev_lvl_.emplace_back(new Event());
+        for (int i = 0; i < 4 * L_; i++) ev_dc_.emplace_back(new Event());
+        for (int i = 0; i < 2 * L_; i++) ev_pre_.emplace_back(new Event());
+        build_time_.assign(2 * L_, 0.0);
+        wait_level_.assign(L_, 0.0);
+        trees_[0].resize(L_);
+        trees_[1].resize(L_);
     }
 
-    // ---- image statistics of both images (exact, chunks decomposed on the side streams) ----
-    Event ev_w0;
-    DEEDS_HIP(hipEventRecord(ev_w0.e, s0));
-    const size_t stats_bytes = (StatsScratch::bytes((long long)sz) + 255) & ~(size_t)255;
-    char* d_stats = static_cast<char*>(ctx->buf("r.stats", 2 * stats_bytes));
-    PinnedBuf h_stats(2 * stats_bytes);
-    double stats_time[2] = {0.0, 0.0};
-    hipEvent_t w0_event = ev_w0.e;
-    auto stats_task = [&, w0_event](int dir) {
-        auto ts = std::chrono::steady_clock::now();
-        hipStream_t ss = ctx->side_stream[dir];
-        DEEDS_HIP(hipSetDevice(ctx->device));
-        DEEDS_HIP(hipStreamWaitEvent(ss, w0_event, 0));
-        StatsScratch sc;
-        sc.dev = d_stats + dir * stats_bytes;
-        sc.host = h_stats.as<char>() + dir * stats_bytes;
-        float mean = 0, sd = 0;
-        image_stats_device(dir ? warped0 : im1b, (long long)sz, sc, ss, &mean, &sd);
-        stats_time[dir] = std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count();
-        return sd;
-    };
-    std::shared_future<float> sd_fut[2] = {std::async(std::launch::async, stats_task, 0).share(),
-                                           std::async(std::launch::async, stats_task, 1).share()};
+    int levels() const { return L_; }
+    const LevelCfg& level(int l) const { return lv_[l]; }
 
-    // ---- MST inputs: edge SADs of every level on both images ----
-    size_t ec_total = 0;
-    std::vector<size_t> ec_off(L);
-    for (int l = 0; l < L; l++) {
-        ec_off[l] = ec_total;
-        ec_total += 6 * (size_t)lv[l].sz1;
+    // Inputs, affine warp, image statistics and edge SADs (s0).
+    void begin() {
+        DEEDS_HIP(hipEventRecord(ev_start_.e, s0_));
+        // +1024 HU offset (deedsBCV0.cpp:139-144), affine warp (:210)
+        const float identity[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+        A_ = affine_from_X(in_.X ? in_.X : identity);
+        launch_offset(im1b_, in_.fixed, -1024.0f, sz_, s0_);
+        launch_offset(im1_, in_.moving, -1024.0f, sz_, s0_);
+        {
+            // warpAffine with a zero field == the grid-fused warp from a zero 1x1x1 grid
+            float* zero3 = g0_[0];
+            launch_zero(zero3, 3, s0_);
+            launch_warp_affine_grid(warped0_, im1_, A_, zero3, zero3 + 1, zero3 + 2, in_.m, in_.n, in_.o, 1, 1, 1,
+                                    s0_);
+        }
+        // image statistics of both images (exact; chunks decomposed on the side streams)
+        DEEDS_HIP(hipEventRecord(ev_w0_.e, s0_));
+        hipEvent_t w0_event = ev_w0_.e;
+        auto stats_task = [this, w0_event](int dir) {
+            auto ts = std::chrono::steady_clock::now();
+            hipStream_t ss = ctx_->side_stream[dir];
+            DEEDS_HIP(hipSetDevice(ctx_->device));
+            DEEDS_HIP(hipStreamWaitEvent(ss, w0_event, 0));
+            StatsScratch sc;
+            sc.dev = d_stats_ + dir * stats_bytes_;
+            sc.host = h_stats_ + dir * stats_bytes_;
+            float mean = 0, sd = 0;
+            image_stats_device(dir ? warped0_ : im1b_, sz_, sc, ss, &mean, &sd);
+            stats_time_[dir] = std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count();
+            return sd;
+        };
+        sd_fut_[0] = std::async(std::launch::async, stats_task, 0).share();
+        sd_fut_[1] = std::async(std::launch::async, stats_task, 1).share();
+        for (int dir = 0; dir < 2; dir++)
+            for (int l = 0; l < L_; l++)
+                launch_edge_sad(d_ec_ + dir * ec_total_ + ec_off_[l], dir ? warped0_ : im1b_, lv_[l].step, in_.m,
+                                in_.n, in_.o, s0_);
+        d2h(h_ec_, d_ec_, 2 * ec_total_, s0_);
+        DEEDS_HIP(hipEventRecord(ev_sad_.e, s0_));
+        // fields start at zero on the level-0 grid
+        m2_ = in_.m / S_.grid[0];
+        n2_ = in_.n / S_.grid[0];
+        o2_ = in_.o / S_.grid[0];
+        sz2_ = (size_t)m2_ * n2_ * o2_;
+        launch_zero(prevf_, 6 * (long long)sz2_, s0_);
+        DEEDS_HIP(hipEventRecord(ev_init_.e, s0_));
+        DEEDS_HIP(hipStreamWaitEvent(s1_, ev_init_.e, 0));
     }
-    float* d_ec = ctx->get<float>("r.ec", 2 * ec_total);
-    for (int dir = 0; dir < 2; dir++)
-        for (int l = 0; l < L; l++)
-            launch_edge_sad(d_ec + dir * ec_total + ec_off[l], dir ? warped0 : im1b, lv[l].step, m, n, o, s0);
-    PinnedBuf h_ec(2 * ec_total * sizeof(float));
-    d2h(h_ec.as<float>(), d_ec, 2 * ec_total, s0);
-    DEEDS_HIP(hipEventRecord(ev_sad.e, s0));
-    hipEvent_t sad_event = ev_sad.e;
-    std::vector<double> build_time(2 * L, 0.0);
-    double wait_time = 0.0;
-    std::vector<double> wait_level(L, 0.0);
-    std::vector<std::future<std::unique_ptr<Tree>>> trees[2];
-    for (int l = 0; l < L; l++)
+
+    // Queues the two trees of level l on the pool.  Prim's on the edge costs
+    // runs before the image statistics are known; the exact weights then
+    // confirm the order (else the tree is rebuilt from the weights).  The
+    // confirmation runs beside Prim's once the statistics arrive.
+    void submit_trees(TaskPool& pool, int l) {
         for (int dir = 0; dir < 2; dir++) {
-            const float* ec = h_ec.as<float>() + dir * ec_total + ec_off[l];
-            const LevelCfg c = lv[l];
-            std::shared_future<float> sdf = sd_fut[dir];
-            double* spent = &build_time[2 * l + dir];
-            trees[dir].push_back(std::async(std::launch::async, [ec, c, sdf, spent, sad_event]() {
-                // Prim's on the edge costs runs before the image statistics are
-                // known; the exact weights then confirm the order (else rebuild).
-                // The confirmation runs beside Prim's once the statistics arrive.
+            const float* ec = h_ec_ + dir * ec_total_ + ec_off_[l];
+            const LevelCfg c = lv_[l];
+            std::shared_future<float> sdf = sd_fut_[dir];
+            double* spent = &build_time_[2 * l + dir];
+            hipEvent_t sad_event = ev_sad_.e;
+            trees_[dir][l] = pool.submit([ec, c, sdf, spent, sad_event]() {
                 DEEDS_HIP(hipEventSynchronize(sad_event));
                 auto tb = std::chrono::steady_clock::now();
                 auto check = std::async(std::launch::async, [ec, c, sdf]() {
@@ -236,180 +346,286 @@ void register_pair(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, 
                     build_tree(ec, sd, c.step, c.m1, c.n1, c.o1, *t);
                 *spent = std::chrono::duration<double>(std::chrono::steady_clock::now() - tb).count();
                 return t;
-            }));
+            });
         }
-
-    // ---- fields start at zero on the level-0 grid ----
-    int m2 = m / S.grid[0], n2 = n / S.grid[0], o2 = o / S.grid[0];
-    size_t sz2 = (size_t)m2 * n2 * o2;
-    launch_zero(prevf, 6 * (long long)sz2, s0);
-    DEEDS_HIP(hipEventRecord(ev_init.e, s0));
-    DEEDS_HIP(hipStreamWaitEvent(s1, ev_init.e, 0));
-
-    // pinned staging for every (level, direction) tree and a device area per direction
-    std::vector<size_t> stage_off(L);
-    size_t stage_total = 0, stage_max = 0;
-    for (int l = 0; l < L; l++) {
-        size_t b = tree_stage_bytes(lv[l].sz1);
-        b = (b + 255) & ~(size_t)255;
-        stage_off[l] = stage_total;
-        stage_total += 2 * b;
-        stage_max = std::max(stage_max, b);
     }
-    PinnedBuf h_stage(stage_total);
-    void* d_stage[2] = {ctx->buf("r.tree.0", stage_max), ctx->buf("r.tree.1", stage_max)};
 
-    double hamming = 0.0;
-    std::vector<int> tree_maxd(2 * L, 0);
-    for (int l = 0; l < L; l++) {
-        const LevelCfg& c = lv[l];
-        DEEDS_HIP(hipEventRecord(EV(l, 0), s0));
+    // Pre-BP work of both directions: MIND of the images when the patch radius
+    // changes, then per direction upsample, warp, MIND, padding, data cost.
+    void pre_bp(int l) {
+        const LevelCfg& c = lv_[l];
+        DEEDS_HIP(hipEventRecord(EV(l, 0), s0_));
         if (c.new_mind) {
             // both directions' MIND of warped0 / im1b (deedsBCV0.cpp:230-236)
-            int rc = deeds_ctx_descriptor(ctx, mind_mov, warped0, m, n, o, c.mind_qs);
-            if (rc) fail(rc, ctx->err);
-            rc = deeds_ctx_descriptor(ctx, mind_fix, im1b, m, n, o, c.mind_qs);
-            if (rc) fail(rc, ctx->err);
+            launch_descriptor(mind_mov_, warped0_, in_.m, in_.n, in_.o, c.mind_qs, sh_.mw[0][0], sh_.mw[0][1], s0_);
+            launch_descriptor(mind_fix_, im1b_, in_.m, in_.n, in_.o, c.mind_qs, sh_.mw[0][0], sh_.mw[0][1], s0_);
         }
-        DEEDS_HIP(hipEventRecord(ev_mind.e, s0));
-        DEEDS_HIP(hipStreamWaitEvent(s1, ev_mind.e, 0));
-        // pre-BP work of both directions: upsample, warp, MIND, data cost
+        DEEDS_HIP(hipEventRecord(ev_mind_.e, s0_));
+        DEEDS_HIP(hipStreamWaitEvent(s1_, ev_mind_.e, 0));
         for (int dir = 0; dir < 2; dir++) {
-            hipStream_t s = dir ? s1 : s0;
-            const float* pf = prevf + dir * 3 * sz2;
-            float* u0 = g0[dir];
-            size_t n1 = c.sz1;
+            hipStream_t s = dir ? s1_ : s0_;
+            const float* pf = prevf_ + dir * 3 * sz2_;
+            float* u0 = g0_[dir];
+            const size_t n1 = c.sz1;
             // upsampleDeformationsCL to this grid (:254 / :283)
-            launch_upsample(u0, u0 + n1, u0 + 2 * n1, pf, pf + sz2, pf + 2 * sz2, c.m1, c.n1, c.o1, m2, n2, o2, s);
+            launch_upsample(u0, u0 + n1, u0 + 2 * n1, pf, pf + sz2_, pf + 2 * sz2_, c.m1, c.n1, c.o1, m2_, n2_, o2_,
+                            s);
             // upsample to full resolution fused with the warp (:255-257 / :284-285)
             if (dir == 0)
-                launch_warp_affine_grid(warped[0], im1, A, u0, u0 + n1, u0 + 2 * n1, m, n, o, c.m1, c.n1, c.o1, s);
+                launch_warp_affine_grid(sh_.warped[0], im1_, A_, u0, u0 + n1, u0 + 2 * n1, in_.m, in_.n, in_.o, c.m1,
+                                        c.n1, c.o1, s);
             else
-                launch_warp_image_grid(warped[1], im1b, u0, u0 + n1, u0 + 2 * n1, m, n, o, c.m1, c.n1, c.o1, s);
+                launch_warp_image_grid(sh_.warped[1], im1b_, u0, u0 + n1, u0 + 2 * n1, in_.m, in_.n, in_.o, c.m1,
+                                       c.n1, c.o1, s);
             if (dir == 0) DEEDS_HIP(hipEventRecord(EV(l, 1), s));
-            launch_descriptor(mind_w[dir], warped[dir], m, n, o, c.mind_qs, mw[dir][0], mw[dir][1], s);
+            launch_descriptor(sh_.mind_w[dir], sh_.warped[dir], in_.m, in_.n, in_.o, c.mind_qs, sh_.mw[dir][0],
+                              sh_.mw[dir][1], s);
             if (dir == 0) DEEDS_HIP(hipEventRecord(EV(l, 2), s));
-            launch_pad_codes(mpad[dir], mind_w[dir], m, n, o, c.plan.pad, s);
+            launch_pad_codes(sh_.mpad[dir], sh_.mind_w[dir], in_.m, in_.n, in_.o, c.plan.pad, s);
             DEEDS_HIP(hipEventRecord(DC(l, dir, 0), s));
-            launch_datacost(cost[dir], dir ? mind_mov : mind_fix, mpad[dir], m, n, o, c.step, c.hw, c.quant, c.plan,
-                            s);
+            launch_datacost(cost_[dir], dir ? mind_mov_ : mind_fix_, sh_.mpad[dir], in_.m, in_.n, in_.o, c.step,
+                            c.hw, c.quant, c.plan, s);
             DEEDS_HIP(hipEventRecord(DC(l, dir, 1), s));
             if (dir == 0) DEEDS_HIP(hipEventRecord(EV(l, 3), s));
-            hamming += (double)c.sz1 * c.len3 * (double)c.plan.samples;
+            hamming_ += (double)c.sz1 * c.len3 * (double)c.plan.samples;
         }
-        // BP of both directions (regularisationCL, :276 / :303)
-        for (int dir = 0; dir < 2; dir++) {
-            hipStream_t s = dir ? s1 : s0;
-            auto tw = clock::now();
-            std::unique_ptr<Tree> t = trees[dir][l].get();
-            double waited = std::chrono::duration<double>(clock::now() - tw).count();
-            wait_time += waited;
-            wait_level[l] += waited;
-            tree_maxd[2 * l + dir] = t->maxd;
-            DEEDS_HIP(hipEventRecord(ev_pre[2 * l + dir]->e, s));
-            size_t half = (stage_off[l + 1 < L ? l + 1 : l] - stage_off[l]) / 2;
-            if (l + 1 >= L) half = (stage_total - stage_off[l]) / 2;
-            char* slot = h_stage.as<char>() + stage_off[l] + dir * half;
-            BPDevTree dt = stage_tree(*t, slot, d_stage[dir], s);
-            float* u0 = g0[dir];
-            size_t n1 = c.sz1;
-            float* nf = newf + dir * 3 * n1;
-            run_bp(*t, dt, cost[dir], inds[dir], sel[dir], u0, u0 + n1, u0 + 2 * n1, nf, nf + n1, nf + 2 * n1, c.hw,
-                   (float)c.quant, s);
-            if (dir == 0) DEEDS_HIP(hipEventRecord(EV(l, 4), s));
-        }
-        DEEDS_HIP(hipEventRecord(ev_bwd_done.e, s1));
-        DEEDS_HIP(hipStreamWaitEvent(s0, ev_bwd_done.e, 0));
-        // consistentMappingCL (:311) on the new fields, then they become "previous"
-        {
-            size_t n1 = c.sz1;
-            float* f[6] = {newf, newf + n1, newf + 2 * n1, newf + 3 * n1, newf + 4 * n1, newf + 5 * n1};
-            launch_consistent(f, cmtmp, c.m1, c.n1, c.o1, c.step, s0);
-            launch_jacobian(jac + 4 * l, newf, newf + n1, newf + 2 * n1, c.m1, c.n1, c.o1, c.step, s0);
-            DEEDS_HIP(hipMemcpyAsync(prevf, newf, 6 * n1 * sizeof(float), hipMemcpyDeviceToDevice, s0));
-        }
-        DEEDS_HIP(hipEventRecord(EV(l, 5), s0));
-        DEEDS_HIP(hipStreamWaitEvent(s1, EV(l, 5), 0));
-        m2 = c.m1; n2 = c.n1; o2 = c.o1; sz2 = c.sz1;
     }
-    DEEDS_HIP(hipEventRecord(ev_stop.e, s0));
 
-    // ---- outputs (deedsBCV0.cpp:339-356) ----
-    DEEDS_HIP(hipMemcpyAsync(d_flow, prevf, 3 * sz2 * sizeof(float), hipMemcpyDeviceToDevice, s0));
-    float ssd_h[2] = {0, 0};
-    if (d_warped) {
-        float* wtmp = warped[0];
-        launch_warp_affine_grid(wtmp, im1, A, prevf, prevf + sz2, prevf + 2 * sz2, m, n, o, m2, n2, o2, s0);
-        launch_zero((float*)ssd, 4, s0);
-        launch_ssd(ssd, im1b, wtmp, im1, sz, s0);
-        launch_offset(d_warped, wtmp, 1024.0f, sz, s0);
+    // Waits for the level-l tree of direction `dir`, stages it on that
+    // direction's stream and appends its BP item (regularisationCL, :276 / :303).
+    void bp_item(int l, int dir, std::vector<BPBatchItem>& items) {
+        const LevelCfg& c = lv_[l];
+        hipStream_t s = dir ? s1_ : s0_;
+        const size_t half = (stage_off_[l + 1] - stage_off_[l]) / 2;
+        auto tw = std::chrono::steady_clock::now();
+        tree_[dir] = trees_[dir][l].get();
+        const double waited = std::chrono::duration<double>(std::chrono::steady_clock::now() - tw).count();
+        wait_time_ += waited;
+        wait_level_[l] += waited;
+        DEEDS_HIP(hipEventRecord(ev_pre_[2 * l + dir]->e, s));
+        char* slot = h_stage_ + stage_off_[l] + dir * half;
+        const BPDevTree dt = stage_tree(*tree_[dir], slot, d_stage_[dir], s);
+        float* u0 = g0_[dir];
+        const size_t n1 = c.sz1;
+        float* nf = newf_ + dir * 3 * n1;
+        items.push_back(BPBatchItem{tree_[dir].get(), dt, cost_[dir], inds_[dir], u0, u0 + n1, u0 + 2 * n1, nf,
+                                    nf + n1, nf + 2 * n1});
     }
-    DEEDS_HIP(hipStreamSynchronize(s0));
-    DEEDS_HIP(hipStreamSynchronize(s1));
-    auto t1 = clock::now();
-    if (timing) {
+
+    // Marks the end of the forward direction's BP of level l on s0.
+    void fwd_bp_done(int l) { DEEDS_HIP(hipEventRecord(EV(l, 4), s0_)); }
+
+    // After both directions' BP of level l (s0 waits for s1):
+    // consistentMappingCL (:311) on the new fields, which then become "previous".
+    void post_bp(int l) {
+        const LevelCfg& c = lv_[l];
+        const size_t n1 = c.sz1;
+        float* f[6] = {newf_, newf_ + n1, newf_ + 2 * n1, newf_ + 3 * n1, newf_ + 4 * n1, newf_ + 5 * n1};
+        launch_consistent(f, cmtmp_, c.m1, c.n1, c.o1, c.step, s0_);
+        launch_jacobian(jac_ + 4 * l, newf_, newf_ + n1, newf_ + 2 * n1, c.m1, c.n1, c.o1, c.step, s0_);
+        DEEDS_HIP(hipMemcpyAsync(prevf_, newf_, 6 * n1 * sizeof(float), hipMemcpyDeviceToDevice, s0_));
+        DEEDS_HIP(hipEventRecord(EV(l, 5), s0_));
+        DEEDS_HIP(hipStreamWaitEvent(s1_, EV(l, 5), 0));
+        m2_ = c.m1;
+        n2_ = c.n1;
+        o2_ = c.o1;
+        sz2_ = c.sz1;
+    }
+
+    // Outputs (deedsBCV0.cpp:339-356), on s0.
+    void finish() {
+        DEEDS_HIP(hipEventRecord(ev_stop_.e, s0_));
+        DEEDS_HIP(hipMemcpyAsync(in_.flow, prevf_, 3 * sz2_ * sizeof(float), hipMemcpyDeviceToDevice, s0_));
+        if (in_.warped) {
+            float* wtmp = sh_.warped[0];
+            launch_warp_affine_grid(wtmp, im1_, A_, prevf_, prevf_ + sz2_, prevf_ + 2 * sz2_, in_.m, in_.n, in_.o,
+                                    m2_, n2_, o2_, s0_);
+            launch_zero((float*)ssd_, 4, s0_);
+            launch_ssd(ssd_, im1b_, wtmp, im1_, sz_, s0_);
+            launch_offset(in_.warped, wtmp, 1024.0f, sz_, s0_);
+        }
+    }
+
+    // After both streams have drained: the timing record.
+    void report() {
+        deeds_timing* timing = in_.timing;
+        if (!timing) return;
         std::memset(timing, 0, sizeof(*timing));
-        timing->levels = L;
-        for (int l = 0; l < L; l++) {
-            float a = 0, b = 0, c = 0, d = 0, e = 0;
-            DEEDS_HIP(hipEventElapsedTime(&a, EV(l, 0), EV(l, 1)));
-            DEEDS_HIP(hipEventElapsedTime(&b, EV(l, 1), EV(l, 2)));
-            DEEDS_HIP(hipEventElapsedTime(&c, EV(l, 2), EV(l, 3)));
-            DEEDS_HIP(hipEventElapsedTime(&d, EV(l, 3), EV(l, 4)));
-            DEEDS_HIP(hipEventElapsedTime(&e, EV(l, 4), EV(l, 5)));
-            timing->transf[l] = (a + e) * 1e-3;
-            timing->mind[l] = b * 1e-3;
-            timing->data[l] = c * 1e-3;
-            timing->mst_reg[l] = d * 1e-3;
+        timing->levels = L_;
+        for (int l = 0; l < L_; l++) {
+            timing->transf[l] = elapsed_s(EV(l, 0), EV(l, 1)) + elapsed_s(EV(l, 4), EV(l, 5));
+            timing->mind[l] = elapsed_s(EV(l, 1), EV(l, 2));
+            timing->data[l] = elapsed_s(EV(l, 2), EV(l, 3));
+            timing->mst_reg[l] = elapsed_s(EV(l, 3), EV(l, 4));
         }
-        float tot = 0;
-        DEEDS_HIP(hipEventElapsedTime(&tot, ev_start.e, ev_stop.e));
-        timing->total = tot * 1e-3;
-        timing->hamming_evals = hamming;
-        timing->tree_wait_s = wait_time;
-        timing->stats_s = std::max(stats_time[0], stats_time[1]);
-        for (int l = 0; l < L; l++) {
+        timing->total = elapsed_s(ev_start_.e, ev_stop_.e);
+        timing->hamming_evals = hamming_;
+        timing->tree_wait_s = wait_time_;
+        timing->stats_s = std::max(stats_time_[0], stats_time_[1]);
+        for (int l = 0; l < L_; l++) {
             double lvl = 0.0;
-            for (int dir = 0; dir < 2; dir++) {
-                float ms = 0;
-                DEEDS_HIP(hipEventElapsedTime(&ms, DC(l, dir, 0), DC(l, dir, 1)));
-                lvl += ms * 1e-3;
-            }
+            for (int dir = 0; dir < 2; dir++) lvl += elapsed_s(DC(l, dir, 0), DC(l, dir, 1));
             timing->datacost_level_s[l] = lvl;
-            timing->datacost_level_evals[l] = 2.0 * lv[l].sz1 * lv[l].len3 * (double)lv[l].plan.samples;
+            timing->datacost_level_evals[l] = 2.0 * lv_[l].sz1 * lv_[l].len3 * (double)lv_[l].plan.samples;
             timing->datacost_kernel_s += lvl;
-            timing->tree_build_s += build_time[2 * l] + build_time[2 * l + 1];
-            timing->tree_build_level_s[l] = std::max(build_time[2 * l], build_time[2 * l + 1]);
-            timing->tree_wait_level_s[l] = wait_level[l];
+            timing->tree_build_s += build_time_[2 * l] + build_time_[2 * l + 1];
+            timing->tree_build_level_s[l] = std::max(build_time_[2 * l], build_time_[2 * l + 1]);
+            timing->tree_wait_level_s[l] = wait_level_[l];
             timing->datacost_launches += 2;
-            for (int dir = 0; dir < 2; dir++) {
-                float idle = 0;
-                DEEDS_HIP(hipEventElapsedTime(&idle, DC(l, dir, 1), ev_pre[2 * l + dir]->e));
-                timing->bp_idle_level_s[l] += std::max(0.0f, idle) * 1e-3;
-            }
-            float bp = 0;
-            DEEDS_HIP(hipEventElapsedTime(&bp, ev_pre[2 * l]->e, EV(l, 4)));
-            timing->bp_level_s[l] = bp * 1e-3;
+            for (int dir = 0; dir < 2; dir++)
+                timing->bp_idle_level_s[l] += std::max(0.0, elapsed_s(DC(l, dir, 1), ev_pre_[2 * l + dir]->e));
+            timing->bp_level_s[l] = elapsed_s(ev_pre_[2 * l]->e, EV(l, 4));
         }
-        std::vector<double> hj(4 * (size_t)L);
-        DEEDS_HIP(hipMemcpy(hj.data(), jac, hj.size() * sizeof(double), hipMemcpyDeviceToHost));
-        for (int l = 0; l < L; l++) {
-            double N = (double)lv[l].sz1;
-            double mean = hj[4 * l] / N;
-            double var = N > 1 ? (hj[4 * l + 1] - N * mean * mean) / (N - 1) : 0.0;
+        std::vector<double> hj(4 * (size_t)L_);
+        DEEDS_HIP(hipMemcpy(hj.data(), jac_, hj.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int l = 0; l < L_; l++) {
+            const double N = (double)lv_[l].sz1;
+            const double mean = hj[4 * l] / N;
+            const double var = N > 1 ? (hj[4 * l + 1] - N * mean * mean) / (N - 1) : 0.0;
             timing->jac_std[l] = (float)std::sqrt(std::max(var, 0.0));
             timing->jac_neg[l] = (float)(hj[4 * l + 2] / N);
         }
-        if (d_warped) {
+        if (in_.warped) {
             double hs[2];
-            DEEDS_HIP(hipMemcpy(hs, ssd, 2 * sizeof(double), hipMemcpyDeviceToHost));
-            timing->ssd_before = (float)(hs[0] / (double)sz);
-            timing->ssd_after = (float)(hs[1] / (double)sz);
+            DEEDS_HIP(hipMemcpy(hs, ssd_, 2 * sizeof(double), hipMemcpyDeviceToHost));
+            timing->ssd_before = (float)(hs[0] / (double)sz_);
+            timing->ssd_after = (float)(hs[1] / (double)sz_);
         }
     }
-    (void)t0;
-    (void)t1;
-    (void)ssd_h;
+
+   private:
+    template <class T>
+    T* get(const std::string& name, size_t count) {
+        return ctx_->get<T>(tag_ + "r." + name, count);
+    }
+    hipEvent_t EV(int l, int k) { return ev_lvl_[6 * l + k]->e; }
+    hipEvent_t DC(int l, int dir, int end) { return ev_dc_[4 * l + 2 * dir + end]->e; }
+
+    deeds_ctx* ctx_;
+    std::string tag_;
+    PairIn in_;
+    const deeds_schedule& S_;
+    Scratch sh_;
+    int L_;
+    hipStream_t s0_, s1_;
+    long long sz_ = 0;
+    std::vector<LevelCfg> lv_;
+    Affine A_{};
+    float *im1_, *im1b_, *warped0_;
+    uint64_t *mind_mov_, *mind_fix_;
+    float* cost_[2];
+    int16_t* inds_[2];
+    float *prevf_, *newf_, *cmtmp_;
+    float* g0_[2];
+    double *jac_, *ssd_;
+    size_t ec_total_ = 0;
+    std::vector<size_t> ec_off_;
+    float* d_ec_;
+    float* h_ec_;
+    size_t stats_bytes_ = 0;
+    char* d_stats_;
+    char* h_stats_;
+    std::vector<size_t> stage_off_;
+    size_t stage_total_ = 0;
+    char* h_stage_;
+    void* d_stage_[2];
+    int m2_ = 0, n2_ = 0, o2_ = 0;
+    size_t sz2_ = 0;
+    double hamming_ = 0.0;
+    Event ev_start_, ev_stop_, ev_init_, ev_mind_, ev_w0_, ev_sad_;
+    std::vector<std::unique_ptr<Event>> ev_lvl_, ev_dc_, ev_pre_;
+    double stats_time_[2] = {0.0, 0.0};
+    std::shared_future<float> sd_fut_[2];
+    std::vector<double> build_time_;
+    double wait_time_ = 0.0;
+    std::vector<double> wait_level_;
+    std::vector<std::future<std::unique_ptr<Tree>>> trees_[2];
+    std::unique_ptr<Tree> tree_[2];
+};
+
+// BP staging of one level for `trees` trees with `nodes` nodes in total,
+// 256-byte aligned.
+size_t level_bp_bytes(int trees, long long nodes) { return (bp_batch_bytes(trees, nodes) + 255) & ~(size_t)255; }
+
+// Runs the pairs level-synchronously on ctx (see the file comment).  Pairs
+// are validated by the callers.
+void run_level_sync(deeds_ctx* ctx, const std::vector<PairIn>& pairs, const deeds_schedule& S, int tree_threads) {
+    const int P = (int)pairs.size();
+    if (P == 0) return;
+    const int L = S.levels;
+    hipStream_t s0 = ctx->stream, s1 = ctx->aux_stream;
+    // shared scratch at the largest pair's size, allocated before any work is queued
+    size_t max_sz = 0, max_pad = 0;
+    std::vector<long long> level_nodes(L, 0);
+    for (const PairIn& p : pairs) {
+        const std::vector<LevelCfg> lv = level_cfgs(S, p.m, p.n, p.o);
+        max_sz = std::max(max_sz, (size_t)p.m * p.n * p.o);
+        max_pad = std::max(max_pad, max_padded(lv));
+        for (int l = 0; l < L; l++) level_nodes[l] += lv[l].sz1;
+    }
+    const Scratch sh = make_scratch(ctx, max_sz, max_pad);
+    // BP staging per (level, direction): pinned slots for the whole call, a
+    // device area per direction reused level after level on its stream
+    std::vector<size_t> bpq_off(2 * L + 1, 0);
+    size_t bpq_max = 0;
+    for (int l = 0; l < L; l++)
+        for (int dir = 0; dir < 2; dir++) {
+            const size_t b = level_bp_bytes(P, level_nodes[l]);
+            bpq_off[2 * l + dir + 1] = bpq_off[2 * l + dir] + b;
+            bpq_max = std::max(bpq_max, b);
+        }
+    char* h_bpq = static_cast<char*>(ctx->pinned("r.bpq", bpq_off[2 * L]));
+    void* d_bpq[2] = {ctx->buf("r.bpq.0", bpq_max), ctx->buf("r.bpq.1", bpq_max)};
+    std::vector<std::unique_ptr<PairRun>> runs;
+    for (int p = 0; p < P; p++)
+        runs.emplace_back(new PairRun(ctx, P == 1 ? std::string() : "b" + std::to_string(p) + ".", pairs[p], S, sh));
+    // declared after the runs: destroyed (drained and joined) before them
+    TaskPool pool(tree_threads);
+    for (auto& r : runs) r->begin();
+    // trees in the order they are needed: level by level over the pairs
+    for (int l = 0; l < L; l++)
+        for (auto& r : runs) r->submit_trees(pool, l);
+    Event ev_join;
+    for (int l = 0; l < L; l++) {
+        for (auto& r : runs) r->pre_bp(l);
+        // each direction's trees of all pairs in one BP launch sequence on that direction's stream
+        const LevelCfg& c = runs[0]->level(l);
+        for (int dir = 0; dir < 2; dir++) {
+            std::vector<BPBatchItem> items;
+            for (auto& r : runs) r->bp_item(l, dir, items);
+            const size_t off = bpq_off[2 * l + dir];
+            run_bp_batch(items, c.hw, (float)c.quant, h_bpq + off, d_bpq[dir], bpq_off[2 * l + dir + 1] - off,
+                         dir ? s1 : s0);
+            if (dir == 0)
+                for (auto& r : runs) r->fwd_bp_done(l);
+        }
+        DEEDS_HIP(hipEventRecord(ev_join.e, s1));
+        DEEDS_HIP(hipStreamWaitEvent(s0, ev_join.e, 0));
+        for (auto& r : runs) r->post_bp(l);
+    }
+    for (auto& r : runs) r->finish();
+    DEEDS_HIP(hipStreamSynchronize(s0));
+    DEEDS_HIP(hipStreamSynchronize(s1));
+    for (auto& r : runs) r->report();
+}
+
+}  // namespace
+
+void register_pair(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, const float* X, int m, int n, int o,
+                   const deeds_schedule& S, float* d_flow, float* d_warped, deeds_timing* timing) {
+    const std::vector<PairIn> one{PairIn{d_fixed, d_moving, X, m, n, o, d_flow, d_warped, timing}};
+    run_level_sync(ctx, one, S, 2 * S.levels);
+}
+
+void register_group(deeds_ctx* ctx, const deeds_pair* pairs, int count, const deeds_schedule& S,
+                    deeds_timing* timings) {
+    std::vector<PairIn> in;
+    for (int i = 0; i < count; i++) {
+        const deeds_pair& p = pairs[i];
+        in.push_back(PairIn{p.fixed, p.moving, p.X, p.m, p.n, p.o, p.flow, p.warped, timings ? &timings[i] : nullptr});
+    }
+    const int hc = (int)std::thread::hardware_concurrency();
+    run_level_sync(ctx, in, S, std::max(2, std::min(16, hc)));
 }
 
 }  // namespace deeds

@@ -156,12 +156,14 @@ int deeds_register(const float* fixed, const float* moving, const float* X, int 
                    const deeds_schedule* sched, float* flow_out, float* warped_out, deeds_timing* timing);
 
 /* Multi-atlas batch: register `count` independent pairs (each with its own
- * dimensions and affine) with the same schedule.  Up to `concurrency` pairs
- * run at once on a pool of contexts (each with its own streams and host tree
- * threads), so one pair's host MST work and latency-bound BP overlap another
- * pair's kernels.  Each result is bit-identical to a deeds_register call on
+ * dimensions and affine) with the same schedule, in groups of `concurrency`
+ * pairs that run level-synchronously: every pair's MIND / data-cost kernels
+ * of a level, then the belief propagation of all the group's trees in one
+ * launch sequence (one launch per round for all trees), then every pair's
+ * field updates.  Each result is bit-identical to a deeds_register call on
  * that pair alone (no reference counterpart: the reference registers one
- * pair per process, deedsBCV0.cpp:55).  Pointers are host memory for
+ * pair per process, deedsBCV0.cpp:55).  Device memory grows with the group
+ * (about 1.5 GB per 200x170x140 pair).  Pointers are host memory for
  * deeds_register_batch and device memory of `device` for
  * deeds_register_batch_device; timings (optional) has `count` entries. */
 typedef struct deeds_pair {
