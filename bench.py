@@ -137,8 +137,8 @@ def soft_bench(a):
     """Differentiable soft level (160^3 float32 pair, grid 6, radius 6, quantisation 2,
     T = 0.5): forward + backward of the expected displacements with respect to both
     images, against the plain GPU forward of the same level (MIND of both images,
-    Hamming cost, MST, min-sum BP).  Both build the MST on a host thread while
-    their features and cost volume run on the GPU."""
+    Hamming cost, MST, min-sum BP).  Both build the MST of the device image on a
+    host thread while their features and cost volume run on the GPU."""
     import ctypes
 
     import torch
@@ -169,9 +169,12 @@ def soft_bench(a):
     import concurrent.futures as cf
     pool = cf.ThreadPoolExecutor(max_workers=1)
 
+    mst_ctx = api.Context(0)
+
     def plain():
-        # the MST is built on a host thread while MIND and the cost volume run (as the registration driver does)
-        pending = pool.submit(api.primsGraph, f, grid)
+        # the MST (from the device image) is built on a host thread while MIND and the cost volume run,
+        # as the registration driver does
+        pending = pool.submit(mst_ctx.prims_graph, x, grid)
         _lib.ctx_call(ctx.h, "deeds_ctx_descriptor", V(mF.data_ptr()), V(x.data_ptr()), mm, n, o, qs)
         _lib.ctx_call(ctx.h, "deeds_ctx_descriptor", V(mM.data_ptr()), V(y.data_ptr()), mm, n, o, qs)
         _lib.ctx_call(ctx.h, "deeds_ctx_datacost", V(mF.data_ptr()), V(mM.data_ptr()), V(cost.data_ptr()), mm, n, o,
@@ -191,12 +194,12 @@ def soft_bench(a):
 
     def soft_fb():
         xs.grad = ys.grad = None
-        loss = (soft.soft_register(xs, ys, grid, hw, quant, alpha, T, tree_image=f) * wgt).sum()
+        loss = (soft.soft_register(xs, ys, grid, hw, quant, alpha, T) * wgt).sum()
         loss.backward()
 
     def soft_f():
         with torch.no_grad():
-            soft.soft_register(x, y, grid, hw, quant, alpha, T, tree_image=f)
+            soft.soft_register(x, y, grid, hw, quant, alpha, T)
 
     t_plain = _timed(plain, a.steps, max(1, a.warmup))
     print(f"[bench] plain forward {1e3 * t_plain:.2f} ms", file=sys.stderr, flush=True)
@@ -206,7 +209,7 @@ def soft_bench(a):
           flush=True)
 
     # stage split of the soft step (each stage timed alone)
-    tree = api.primsGraph(f, grid)
+    tree = mst_ctx.prims_graph(x, grid)
     F = soft.soft_features(xs, qs)
     M = soft.soft_features(ys, qs)
     C = soft.soft_cost(F, M, grid, hw, quant, alpha)
@@ -215,7 +218,7 @@ def soft_bench(a):
     gC = torch.autograd.grad(U, C, gU, retain_graph=True)[0]
     gF = torch.autograd.grad(C, (F, M), gC, retain_graph=True)
     st = {
-        "mst_host_ms": _timed(lambda: api.primsGraph(f, grid), a.steps, 1),
+        "mst_ms": _timed(lambda: mst_ctx.prims_graph(x, grid), a.steps, 1),
         "features_fwd_ms": _timed(lambda: soft.soft_features(x, qs), a.steps, 1) * 2,
         "cost_fwd_ms": _timed(lambda: soft.soft_cost(F.detach(), M.detach(), grid, hw, quant, alpha), a.steps, 1),
         "bp_fwd_ms": _timed(lambda: soft.soft_bp(C.detach(), torch.zeros((3, N), device=dev), tree, f.shape, grid,

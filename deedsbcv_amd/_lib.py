@@ -67,6 +67,7 @@ SIGNATURES = {
     "deeds_descriptor": [V, V, I, I, I, I],
     "deeds_datacost": [V, V, V, I, I, I, I, I, F, F, I],
     "deeds_prims_graph": [V, V, V, V, I, I, I, I],
+    "deeds_ctx_prims_graph": [V, V, V, V, V, I, I, I, I],
     "deeds_image_stats": [V, ctypes.c_longlong, V, V],
     "deeds_regularisation": [V, V, V, V, V, V, V, I, I, F, V, V, V, I, I, I],
     "deeds_upsample_deformations": [V, V, V, V, V, V, I, I, I, I, I, I],

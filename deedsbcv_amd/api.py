@@ -268,6 +268,19 @@ class Context:
     def synchronize(self):
         _lib.ctx_call(self.h, "deeds_ctx_synchronize")
 
+    def prims_graph(self, im, step):
+        """primsGraph (primsMST.h:52) of a float32 CUDA tensor image (o, n, m): host arrays
+        (ordered, parents, edgemst), the same tree as primsGraph() of the host copy.  The
+        caller makes sure the image is ready (e.g. by synchronizing torch's stream)."""
+        o, n, m = im.shape
+        sz1 = (m // step) * (n // step) * (o // step)
+        ordered = np.empty(sz1, np.int32)
+        parents = np.empty(sz1, np.int32)
+        edgemst = np.empty(sz1, np.float32)
+        _lib.ctx_call(self.h, "deeds_ctx_prims_graph", ctypes.c_void_p(im.data_ptr()), _p(ordered), _p(parents),
+                      _p(edgemst), int(step), m, n, o)
+        return ordered, parents, edgemst
+
     def register(self, fixed, moving, flow, warped=None, grid=(8, 7, 6, 5, 4), radius=(8, 7, 6, 5, 4),
                  quant=(5, 4, 3, 2, 1), alpha=1.6, X=None):
         """fixed/moving/flow/warped: torch.float32 CUDA tensors (fixed, moving shaped (o, n, m))."""

@@ -4,7 +4,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <future>
 #include <memory>
+#include <type_traits>
 
 #include "bp_host.h"
 #include "common.h"
@@ -472,6 +474,54 @@ int deeds_prims_graph(const float* im, int* ordered, int* parents, float* edgems
         DEEDS_HIP(hipStreamSynchronize(c->stream));
         Tree t;
         build_tree(ec.data(), sd, step, m1, n1, o1, t);
+        std::copy(t.ordered.begin(), t.ordered.end(), ordered);
+        std::copy(t.parents.begin(), t.parents.end(), parents);
+        std::copy(t.edgemst.begin(), t.edgemst.end(), edgemst);
+    });
+}
+
+int deeds_ctx_prims_graph(deeds_ctx* ctx, const float* d_im, int* ordered, int* parents, float* edgemst, int step,
+                          int m, int n, int o) {
+    if (!ctx) return DEEDS_ERR_ARG;
+    return guarded(ctx, [&] {
+        DEEDS_REQUIRE(d_im && ordered && parents && edgemst, "null pointer");
+        check_grid(m, n, o, step);
+        const long long sz = (long long)m * n * o;
+        const int m1 = m / step, n1 = n / step, o1 = o / step;
+        const size_t N = (size_t)m1 * n1 * o1;
+        float* d_ec = ctx->get<float>("prims.ec", 6 * N);
+        float* h_ec = static_cast<float*>(ctx->pinned("prims.ec", 6 * N * sizeof(float)));
+        launch_edge_sad(d_ec, d_im, step, m, n, o, ctx->stream);
+        d2h(h_ec, d_ec, 6 * N, ctx->stream);
+        hipEvent_t ev;
+        DEEDS_HIP(hipEventCreate(&ev));
+        std::unique_ptr<std::remove_pointer<hipEvent_t>::type, hipError_t (*)(hipEvent_t)> hold(ev, hipEventDestroy);
+        DEEDS_HIP(hipEventRecord(ev, ctx->stream));
+        // Prim's on the edge costs while the GPU computes the statistics (as the registration driver does)
+        Tree t;
+        auto shape = std::async(std::launch::async, [&] {
+            DEEDS_HIP(hipEventSynchronize(ev));
+            build_tree_shape(h_ec, step, m1, n1, o1, t);
+        });
+        float mean = 0, sd = 0;
+        StatsScratch sc;
+        const size_t bytes = StatsScratch::bytes(sz);
+        sc.dev = ctx->buf("prims.stats", bytes);
+        sc.host = ctx->pinned("prims.stats", bytes);
+        bool same_order = false;
+        try {
+            image_stats_device(d_im, sz, sc, ctx->stream, &mean, &sd);
+            // the order check runs beside Prim's
+            same_order = weights_distinguish_costs(h_ec, step, m1, n1, o1, sd);
+        } catch (...) {
+            shape.wait();
+            throw;
+        }
+        shape.get();
+        if (same_order)
+            finish_tree(h_ec, sd, step, t);
+        else
+            build_tree(h_ec, sd, step, m1, n1, o1, t);
         std::copy(t.ordered.begin(), t.ordered.end(), ordered);
         std::copy(t.parents.begin(), t.parents.end(), parents);
         std::copy(t.edgemst.begin(), t.edgemst.end(), edgemst);

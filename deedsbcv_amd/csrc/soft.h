@@ -14,7 +14,9 @@
 //   inference sum-product BP at temperature T: log-sum-exp distance
 //             transforms separable over the three label axes, upward then
 //             downward messages, marginals P_c = softmax(-g_c / T), output
-//             u1_c = u0_c + E_{P_c}[label displacement]
+//             u1_c = u0_c + E_{P_c}[label displacement]; the sweeps run on
+//             the tree re-rooted at its centre (the marginals do not depend
+//             on the root; the centre halves the sequential depth)
 //   backward  hand-written adjoints of all three stages; BP keeps the upward
 //             beliefs, upward messages and full beliefs of every node and
 //             recomputes each distance transform in the backward sweeps

@@ -1,6 +1,8 @@
 // C ABI of the differentiable soft variant (declared in include/deeds_hip.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstring>
 #include <memory>
 #include <stdexcept>
 
@@ -54,6 +56,77 @@ struct StagedTree {
     }
 };
 
+// Breadth-first distances from s over the undirected tree (CSR adjacency).
+std::vector<int> bfs_dist(const std::vector<int>& adj_start, const std::vector<int>& adj, int s) {
+    std::vector<int> d(adj_start.size() - 1, -1), q;
+    q.reserve(d.size());
+    d[s] = 0;
+    q.push_back(s);
+    for (size_t h = 0; h < q.size(); h++) {
+        const int x = q[h];
+        for (int e = adj_start[x]; e < adj_start[x + 1]; e++)
+            if (d[adj[e]] < 0) {
+                d[adj[e]] = d[x] + 1;
+                q.push_back(adj[e]);
+            }
+    }
+    return d;
+}
+
+// The same tree (same edges, each edge keeping its weight) rooted at its
+// centre, the node of least eccentricity (the smallest id among ties), in
+// BFS order sorted by (depth, id).  The sum-product marginals do not depend
+// on the root -- the pairwise term |(u0_p + q l_p) - (u0_c + q l_c)|^2 / w is
+// symmetric in the two ends of an edge -- while the number of sequential
+// depths of the sweeps drops from the height to half the diameter.
+void reroot_at_centre(Tree& t) {
+    const int N = t.N;
+    if (N <= 2) return;
+    std::vector<int> deg(N + 1, 0);
+    for (int c = 0; c < N; c++)
+        if (t.parents[c] >= 0) {
+            deg[c + 1]++;
+            deg[t.parents[c] + 1]++;
+        }
+    std::vector<int> adj_start(N + 1, 0);
+    for (int v = 0; v < N; v++) adj_start[v + 1] = adj_start[v] + deg[v + 1];
+    std::vector<int> adj(adj_start[N]), wbits(adj_start[N]), fill(adj_start.begin(), adj_start.end() - 1);
+    for (int c = 0; c < N; c++) {
+        const int p = t.parents[c];
+        if (p < 0) continue;
+        int wb;
+        std::memcpy(&wb, &t.edgemst[c], sizeof(int));
+        adj[fill[c]] = p;
+        wbits[fill[c]++] = wb;
+        adj[fill[p]] = c;
+        wbits[fill[p]++] = wb;
+    }
+    // diameter ends a, b: eccentricity(v) = max(d(a, v), d(b, v))
+    const std::vector<int> d0 = bfs_dist(adj_start, adj, t.ordered[0]);
+    const int a = (int)(std::max_element(d0.begin(), d0.end()) - d0.begin());
+    const std::vector<int> da = bfs_dist(adj_start, adj, a);
+    const int b = (int)(std::max_element(da.begin(), da.end()) - da.begin());
+    const std::vector<int> db = bfs_dist(adj_start, adj, b);
+    int centre = 0;
+    for (int v = 1; v < N; v++)
+        if (std::max(da[v], db[v]) < std::max(da[centre], db[centre])) centre = v;
+    const std::vector<int> dc = bfs_dist(adj_start, adj, centre);
+    std::vector<int> order(N);
+    for (int v = 0; v < N; v++) order[v] = v;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return dc[x] < dc[y]; });
+    std::vector<int> parents(N, -1);
+    std::vector<float> w(N, 0.0f);
+    for (int v = 0; v < N; v++)
+        for (int e = adj_start[v]; e < adj_start[v + 1]; e++)
+            if (dc[adj[e]] == dc[v] - 1) {
+                parents[v] = adj[e];
+                std::memcpy(&w[v], &wbits[e], sizeof(float));
+            }
+    t.ordered = order;
+    t.parents = parents;
+    t.edgemst = w;
+}
+
 void stage_soft_tree(deeds_ctx* ctx, const int* ordered, const int* parents, const float* edgemst, int N,
                      StagedTree& st) {
     DEEDS_REQUIRE(ordered && parents && edgemst, "null tree pointer");
@@ -64,6 +137,8 @@ void stage_soft_tree(deeds_ctx* ctx, const int* ordered, const int* parents, con
     t.edgemst.assign(edgemst, edgemst + N);
     for (int v : t.ordered) DEEDS_REQUIRE(v >= 0 && v < N, "ordered holds an out-of-range node");
     try {
+        build_schedule(t);  // validates the tree
+        reroot_at_centre(t);
         build_schedule(t);
     } catch (const std::runtime_error& e) {
         fail(DEEDS_ERR_ARG, e.what());

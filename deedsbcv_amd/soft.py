@@ -209,10 +209,37 @@ def soft_bp(cost, u0, tree, image_shape, step, hw, quant, T=0.5):
     return SoftBP.apply(cost, u0, tree, (m, n, o), step, hw, quant, T)
 
 
+_mst_contexts = {}
+
+
+def _mst_context(device):
+    """A second context per device for tree builds that overlap the soft kernels."""
+    from . import api
+    idx = device.index if device.index is not None else _torch().cuda.current_device()
+    c = _mst_contexts.get(idx)
+    if c is None:
+        c = _mst_contexts[idx] = api.Context(idx)
+    return c
+
+
+def _tree_job(fixed, step):
+    """A callable building the plain path's MST (primsMST.h:52) of `fixed` as host
+    arrays: from the device image when it is a CUDA tensor (edge costs and image
+    statistics on the GPU), else from the host array."""
+    torch = _torch()
+    if isinstance(fixed, torch.Tensor) and fixed.is_cuda:
+        im = fixed.detach().to(torch.float32).contiguous()
+        torch.cuda.current_stream(im.device).synchronize()  # the library runs on its own streams
+        ctx = _mst_context(im.device)
+        return lambda: ctx.prims_graph(im, step)
+    from . import api
+    img = fixed.detach().to("cpu", torch.float32).numpy() if isinstance(fixed, torch.Tensor) else fixed
+    return lambda: api.primsGraph(img, step)
+
+
 def tree_of(fixed, step):
     """The plain path's MST of the fixed image (primsMST.h:52) as host arrays."""
-    from . import api
-    return api.primsGraph(fixed.detach().to("cpu", _torch().float32).numpy(), step)
+    return _tree_job(fixed, step)()
 
 
 _tree_pool = None
@@ -232,9 +259,7 @@ def soft_register(fixed, moving, grid=6, radius=6, quant=2, alpha=1.6, T=0.5, tr
         import concurrent.futures as cf
         if _tree_pool is None:
             _tree_pool = cf.ThreadPoolExecutor(max_workers=1)
-        from . import api
-        img = tree_image if tree_image is not None else fixed.detach().to("cpu", torch.float32).numpy()
-        pending = _tree_pool.submit(api.primsGraph, img, grid)
+        pending = _tree_pool.submit(_tree_job(tree_image if tree_image is not None else fixed, grid))
     F = soft_features(fixed, qs)
     M = soft_features(moving, qs)
     C = soft_cost(F, M, grid, radius, quant, alpha)

@@ -189,6 +189,12 @@ int deeds_apply_float(float* out, const float* in, const float* X, const float* 
 
 /* ---- device-pointer forms (asynchronous on the context stream) ---------- */
 int deeds_ctx_descriptor(deeds_ctx* ctx, uint64_t* d_mindq, const float* d_im, int m, int n, int o, int qs);
+/* primsMST.h:52 primsGraph() on a device image: the same tree as
+ * deeds_prims_graph (edge SADs and the exact image statistics on the GPU,
+ * Prim's algorithm on the host); ordered/parents/edgemst are host arrays of
+ * (m/step)(n/step)(o/step) entries.  Synchronous. */
+int deeds_ctx_prims_graph(deeds_ctx* ctx, const float* d_im, int* ordered, int* parents, float* edgemst, int step,
+                          int m, int n, int o);
 /* primsMST.h:101-110 on a device image: the same bit-identical mean and
  * standard deviation as deeds_image_stats, the per-chunk work on the GPU.
  * Returns when the result is complete. */

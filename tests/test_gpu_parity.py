@@ -82,6 +82,22 @@ def test_prims_vs_oracle(oracle, step):
             _eq(a, b, f"{name} step={step}")
 
 
+@pytest.mark.parametrize("step", [4, 6, 7])
+def test_prims_device_image_vs_oracle(oracle, step):
+    """deeds_ctx_prims_graph (device image, statistics on the GPU) builds the oracle's tree."""
+    torch = pytest.importorskip("torch")
+    print(f"case: device-image prims vs oracle step={step}", flush=True)
+    ctx = api.Context(0)
+    f, mv = phantom.make_pair((61, 54, 47), 43)
+    for im in (f.astype(np.float32) + 1024, mv.astype(np.float32) * 0.61 + 999.5):
+        t = torch.from_numpy(im).cuda()
+        torch.cuda.synchronize()
+        got = ctx.prims_graph(t, step)
+        want = oracle.prims(im, step)
+        for a, b, name in zip(got, want, ("ordered", "parents", "edgemst")):
+            _eq(a, b, f"{name} step={step}")
+
+
 # ----------------------------------------------------------------- BP -------
 
 @pytest.mark.parametrize("case", [(5, 2, 2), (4, 2, 1), (6, 1, 3)])
