@@ -80,6 +80,7 @@ void deeds_ctx::release_all() {
     for (auto& kv : pinned_ws)
         if (kv.second.ptr) (void)hipHostFree(kv.second.ptr);
     pinned_ws.clear();
+    soft_cache.reset();  // it points into the released buffers
 }
 
 namespace deeds {

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -52,6 +53,7 @@ struct deeds_ctx {
     std::string err;
     std::map<std::string, deeds::DevBuf> ws;
     std::map<std::string, deeds::DevBuf> pinned_ws;
+    std::shared_ptr<void> soft_cache;  // last staged soft-variant tree (soft_api.cpp)
 
     // Device buffer `name` of at least `bytes` (contents unspecified).
     void* buf(const std::string& name, size_t bytes);

@@ -242,7 +242,7 @@ void prim_tree(const float* w, int m1, int n1, int o1, Tree& t) {
 
 }  // namespace
 
-void build_schedule(Tree& t) {
+void build_schedule(Tree& t, bool bp_programs) {
     const int N = t.N;
     if ((int)t.ordered.size() != N || (int)t.parents.size() != N)
         throw std::runtime_error("tree: ordered/parents must hold one entry per node");
@@ -273,16 +273,7 @@ void build_schedule(Tree& t) {
     // top-down selection: per position, the node and its parent's index within the parent's depth
     t.max_width = 0;
     for (int d = 0; d <= maxd; d++) t.max_width = std::max(t.max_width, t.depth_start[d + 1] - t.depth_start[d]);
-    t.sel_info.assign(2 * (size_t)N, 0);
-    for (int p = 0; p < N; p++) {
-        const int c = t.ordered[p];
-        t.sel_info[2 * p] = c;
-        if (p > 0) {
-            const int par = t.parents[c];
-            t.sel_info[2 * p + 1] = pos[par] - t.depth_start[depth[par]];
-        }
-    }
-    // children of every node in `ordered` order, and the leaves, deepest first
+    // children of every node in `ordered` order
     t.child_start.assign(N + 1, 0);
     for (int p = 1; p < N; p++) t.child_start[t.parents[t.ordered[p]] + 1]++;
     for (int v = 0; v < N; v++) t.child_start[v + 1] += t.child_start[v];
@@ -291,6 +282,16 @@ void build_schedule(Tree& t) {
     for (int p = 1; p < N; p++) {
         int c = t.ordered[p];
         t.child_list[fill[t.parents[c]]++] = c;
+    }
+    if (!bp_programs) return;
+    t.sel_info.assign(2 * (size_t)N, 0);
+    for (int p = 0; p < N; p++) {
+        const int c = t.ordered[p];
+        t.sel_info[2 * p] = c;
+        if (p > 0) {
+            const int par = t.parents[c];
+            t.sel_info[2 * p + 1] = pos[par] - t.depth_start[depth[par]];
+        }
     }
     // rounds of chain heads (a node's round is final once its children's are)
     auto nch = [&](int v) { return t.child_start[v + 1] - t.child_start[v]; };

@@ -65,8 +65,10 @@ void build_tree_shape(const float* ec, int step, int m1, int n1, int o1, Tree& t
 bool weights_distinguish_costs(const float* ec, int step, int m1, int n1, int o1, float sd);
 void finish_tree(const float* ec, float sd, int step, Tree& t);
 
-// Fills depth_start and the bottom-up groups from ordered/parents (any tree
-// in BFS order, e.g. one handed in by a caller).
-void build_schedule(Tree& t);
+// Fills depth_start, the child lists and (with bp_programs) the selection
+// info and the bottom-up rounds and programs from ordered/parents (any tree in
+// BFS order, e.g. one handed in by a caller); throws std::runtime_error when
+// ordered/parents do not describe one.
+void build_schedule(Tree& t, bool bp_programs = true);
 
 }  // namespace deeds

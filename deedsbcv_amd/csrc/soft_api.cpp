@@ -42,18 +42,14 @@ void check_soft_grid(int m, int n, int o, int step, int hw, float quant) {
     DEEDS_REQUIRE((long long)m * n * o <= (1LL << 31) / 12, "volume too large for the soft cost");
 }
 
-// A host tree's depth schedule and children, copied to the device (ctx buffer
-// "soft.tree") through a pinned block that lives until the copy has run.
-struct StagedTree {
+// The last tree staged on a context: the caller's arrays (the cache key) and
+// the re-rooted schedule whose arrays live in the ctx buffer "soft.tree".  The
+// forward and backward BP of one step pass the same tree, so the backward
+// finds it here.
+struct SoftTreeCache {
+    std::vector<int> ordered, parents;
+    std::vector<float> edgemst;
     SoftTree t;
-    void* pinned = nullptr;
-    hipStream_t stream = nullptr;
-    ~StagedTree() {
-        if (pinned) {
-            (void)hipStreamSynchronize(stream);
-            (void)hipHostFree(pinned);
-        }
-    }
 };
 
 // Breadth-first distances from s over the undirected tree (CSR adjacency).
@@ -127,9 +123,15 @@ void reroot_at_centre(Tree& t) {
     t.edgemst = w;
 }
 
-void stage_soft_tree(deeds_ctx* ctx, const int* ordered, const int* parents, const float* edgemst, int N,
-                     StagedTree& st) {
+const SoftTree& stage_soft_tree(deeds_ctx* ctx, const int* ordered, const int* parents, const float* edgemst,
+                                int N) {
     DEEDS_REQUIRE(ordered && parents && edgemst, "null tree pointer");
+    auto* cache = static_cast<SoftTreeCache*>(ctx->soft_cache.get());
+    if (cache && (int)cache->ordered.size() == N && std::equal(ordered, ordered + N, cache->ordered.begin()) &&
+        std::equal(parents, parents + N, cache->parents.begin()) &&
+        std::memcmp(edgemst, cache->edgemst.data(), (size_t)N * sizeof(float)) == 0)
+        return cache->t;
+    ctx->soft_cache.reset();
     Tree t;
     t.N = N;
     t.ordered.assign(ordered, ordered + N);
@@ -137,18 +139,19 @@ void stage_soft_tree(deeds_ctx* ctx, const int* ordered, const int* parents, con
     t.edgemst.assign(edgemst, edgemst + N);
     for (int v : t.ordered) DEEDS_REQUIRE(v >= 0 && v < N, "ordered holds an out-of-range node");
     try {
-        build_schedule(t);  // validates the tree
+        build_schedule(t, false);  // validates the tree
         reroot_at_centre(t);
-        build_schedule(t);
+        build_schedule(t, false);
     } catch (const std::runtime_error& e) {
         fail(DEEDS_ERR_ARG, e.what());
     }
     const size_t ints = (size_t)N + N + (N + 1) + (N > 0 ? N - 1 : 0);
     const size_t bytes = ints * sizeof(int) + (size_t)N * sizeof(float);
-    st.stream = ctx->stream;
-    DEEDS_HIP(hipHostMalloc(&st.pinned, bytes, hipHostMallocDefault));
-    int* h = static_cast<int*>(st.pinned);
+    // every soft entry point drains its stream before returning, so the
+    // previous copy out of the pinned block has completed
+    int* h = static_cast<int*>(ctx->pinned("soft.tree", bytes));
     int* d = static_cast<int*>(ctx->buf("soft.tree", bytes));
+    std::shared_ptr<SoftTreeCache> c = std::make_shared<SoftTreeCache>();
     size_t at = 0;
     auto put = [&](const std::vector<int>& v) {
         const int* p = d + at;
@@ -156,17 +159,23 @@ void stage_soft_tree(deeds_ctx* ctx, const int* ordered, const int* parents, con
         at += v.size();
         return p;
     };
-    st.t.N = N;
-    st.t.maxd = t.maxd;
-    st.t.depth_start = t.depth_start;
-    st.t.d_ordered = put(t.ordered);
-    st.t.d_parents = put(t.parents);
-    st.t.d_child_start = put(t.child_start);
-    st.t.d_child_list = put(t.child_list);
+    SoftTree& st = c->t;
+    st.N = N;
+    st.maxd = t.maxd;
+    st.depth_start = t.depth_start;
+    st.d_ordered = put(t.ordered);
+    st.d_parents = put(t.parents);
+    st.d_child_start = put(t.child_start);
+    st.d_child_list = put(t.child_list);
     DEEDS_REQUIRE(at == ints, "tree arrays have unexpected sizes");
     std::copy(t.edgemst.begin(), t.edgemst.end(), reinterpret_cast<float*>(h + at));
-    st.t.d_edgemst = reinterpret_cast<const float*>(d + at);
+    st.d_edgemst = reinterpret_cast<const float*>(d + at);
     DEEDS_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, ctx->stream));
+    c->ordered.assign(ordered, ordered + N);
+    c->parents.assign(parents, parents + N);
+    c->edgemst.assign(edgemst, edgemst + N);
+    ctx->soft_cache = c;
+    return c->t;
 }
 
 void check_bp_args(int m, int n, int o, int step, int hw, float quant, float T, int real) {
@@ -288,17 +297,16 @@ int deeds_ctx_soft_bp(deeds_ctx* ctx, int real, void* d_u1, void* d_saved, const
         check_bp_args(m, n, o, step, hw, quant, T, real);
         DEEDS_REQUIRE(d_u1 && d_saved && d_cost && d_u0, "null pointer");
         const int N = (m / step) * (n / step) * (o / step);
-        StagedTree st;
-        stage_soft_tree(ctx, ordered, parents, edgemst, N, st);
+        const SoftTree& st = stage_soft_tree(ctx, ordered, parents, edgemst, N);
         if (real == DEEDS_REAL_F32)
-            soft_bp_forward<float>(st.t, static_cast<const float*>(d_cost), static_cast<const float*>(d_u0),
+            soft_bp_forward<float>(st, static_cast<const float*>(d_cost), static_cast<const float*>(d_u0),
                                    static_cast<float*>(d_u1), static_cast<float*>(d_saved), hw, quant, T,
                                    ctx->stream);
         else
-            soft_bp_forward<double>(st.t, static_cast<const double*>(d_cost), static_cast<const double*>(d_u0),
+            soft_bp_forward<double>(st, static_cast<const double*>(d_cost), static_cast<const double*>(d_u0),
                                     static_cast<double*>(d_u1), static_cast<double*>(d_saved), hw, quant, T,
                                     ctx->stream);
-        // the tree copy must complete before the pinned staging block is released
+        // drained before returning: the next staging may rewrite the pinned tree block
         DEEDS_HIP(hipStreamSynchronize(ctx->stream));
     });
 }
@@ -312,16 +320,15 @@ int deeds_ctx_soft_bp_backward(deeds_ctx* ctx, int real, void* d_gcost, const vo
         DEEDS_REQUIRE(d_gcost && d_gu1 && d_saved && d_cost && d_u0, "null pointer");
         const int N = (m / step) * (n / step) * (o / step);
         const size_t L = 2 * (size_t)hw + 1;
-        StagedTree st;
-        stage_soft_tree(ctx, ordered, parents, edgemst, N, st);
+        const SoftTree& st = stage_soft_tree(ctx, ordered, parents, edgemst, N);
         const size_t scratch = 2 * (size_t)N * L * L * L;
         if (real == DEEDS_REAL_F32)
-            soft_bp_backward<float>(st.t, static_cast<const float*>(d_cost), static_cast<const float*>(d_u0),
+            soft_bp_backward<float>(st, static_cast<const float*>(d_cost), static_cast<const float*>(d_u0),
                                     static_cast<const float*>(d_saved), static_cast<const float*>(d_gu1),
                                     static_cast<float*>(d_gcost), ctx->get<float>("soft.bp.scratch", scratch), hw,
                                     quant, T, ctx->stream);
         else
-            soft_bp_backward<double>(st.t, static_cast<const double*>(d_cost), static_cast<const double*>(d_u0),
+            soft_bp_backward<double>(st, static_cast<const double*>(d_cost), static_cast<const double*>(d_u0),
                                      static_cast<const double*>(d_saved), static_cast<const double*>(d_gu1),
                                      static_cast<double*>(d_gcost), ctx->get<double>("soft.bp.scratch", scratch), hw,
                                      quant, T, ctx->stream);

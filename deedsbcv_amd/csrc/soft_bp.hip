@@ -13,10 +13,10 @@
 // the table in registers and produce OUT outputs each, so a pass costs no
 // index arithmetic or LDS traffic beyond one read of the line.
 //
-// Backward sweeps: deepest first, A_c = d/dg_c (the marginal's own term plus
-// the adjoints Dt_k of the children's downward transforms) and
-// Dt_c = DTdown_c^T(A_c); then root side first, dH_c = A_c + DTup_c^T(dH_p - Dt_c),
-// which is d/dC_c.  Each transform is recomputed from the saved beliefs in LDS
+// Backward sweeps: the marginals' own terms of A_c = d/dg_c for all nodes in
+// one launch; then deepest first, A_c += the adjoints Dt_k of the children's
+// downward transforms and Dt_c = DTdown_c^T(A_c); then root side first,
+// dH_c = A_c + DTup_c^T(dH_p - Dt_c), which is d/dC_c.  Each transform is recomputed from the saved beliefs in LDS
 // (all four pass buffers kept) and differentiated pass by pass in reverse.
 // The minimum subtracted from each message only shifts beliefs by constants
 // the marginals ignore, so it has no adjoint term.
@@ -299,6 +299,30 @@ __global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_marginals(BPState<R>
     }
 }
 
+// The marginal's own term of A_c = d/dg_c for every node (one workgroup per
+// node, all nodes at once): the gradient of u1_c through P_c.
+template <typename R, int LEN>
+__global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_bwd_marg(BPState<R> a, BPGrad<R> b) {
+    using H = SS<R, LEN>;
+    extern __shared__ __align__(16) unsigned char sp_smem[];
+    const Cubes<R, LEN> s(reinterpret_cast<R*>(sp_smem));
+    const int c = blockIdx.x;
+    const long long cb = (long long)c * H::L3;
+    R e[3];
+    marginal<R, LEN>(a, c, s.X3, s.red, e);
+    const R gu = b.gu1[c], gv = b.gu1[a.N + c], gw = b.gu1[2 * a.N + c];
+    const R iT = (R)1 / a.T;
+    for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) {
+        const int li = l % LEN, lj = (l / LEN) % LEN, lk = l / H::L2;
+        const R du = (R)(lj - H::HW) * a.quant - e[0];
+        const R dv = (R)(li - H::HW) * a.quant - e[1];
+        const R dw = (R)(lk - H::HW) * a.quant - e[2];
+        b.A[cb + l] = -(s.X3[l] * iT) * (gu * du + gv * dv + gw * dw);
+    }
+}
+
+// Deepest depth first: A_c = own term + the children's Dt (in child order),
+// then Dt_c = DTdown_c^T(A_c).
 template <typename R, int LEN>
 __global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_bwd_down(BPState<R> a, BPGrad<R> b, int pos0) {
     using H = SS<R, LEN>;
@@ -306,17 +330,9 @@ __global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_bwd_down(BPState<R> 
     const Cubes<R, LEN> s(reinterpret_cast<R*>(sp_smem));
     const int c = a.t.ordered[pos0 + blockIdx.x];
     const long long cb = (long long)c * H::L3;
-    R e[3];
-    marginal<R, LEN>(a, c, s.X3, s.red, e);
-    const R gu = b.gu1[c], gv = b.gu1[a.N + c], gw = b.gu1[2 * a.N + c];
-    const R iT = (R)1 / a.T;
     const int c0 = a.t.child_start[c], c1 = a.t.child_start[c + 1];
     for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) {
-        const int li = l % LEN, lj = (l / LEN) % LEN, lk = l / H::L2;
-        const R du = (R)(lj - H::HW) * a.quant - e[0];
-        const R dv = (R)(li - H::HW) * a.quant - e[1];
-        const R dw = (R)(lk - H::HW) * a.quant - e[2];
-        R g = -(s.X3[l] * iT) * (gu * du + gv * dv + gw * dw);
+        R g = b.A[cb + l];
         for (int ci = c0; ci < c1; ci++) g += b.Dt[(long long)a.t.child_list[ci] * H::L3 + l];
         b.A[cb + l] = g;
         s.Ga[l] = g;
@@ -392,8 +408,11 @@ struct Sweeps {
         DEEDS_LAUNCH_CHECK();
     }
     static void backward(const SoftTree& t, const BPState<R>& a, const BPGrad<R>& g, hipStream_t s) {
+        set_smem((const void*)k_sp_bwd_marg<R, LEN>, H::SMEM);
         set_smem((const void*)k_sp_bwd_down<R, LEN>, H::SMEM);
         set_smem((const void*)k_sp_bwd_up<R, LEN>, H::SMEM);
+        k_sp_bwd_marg<R, LEN><<<t.N, H::BLOCK, H::SMEM, s>>>(a, g);
+        DEEDS_LAUNCH_CHECK();
         for (int d = t.maxd; d >= 0; d--) {
             const int b = t.depth_start[d], n = t.depth_start[d + 1] - b;
             k_sp_bwd_down<R, LEN><<<n, H::BLOCK, H::SMEM, s>>>(a, g, b);

@@ -3,13 +3,15 @@
 // v <- RN32(RN64(v + d*d)), d = (double)(x - mean), for the variance -- but
 // without a dependent chain of eight million additions.
 //
-// While v stays in one binade [2^(E+23), 2^(E+24)) its ulp is u = 2^E and
-// v = A*u with A an integer in [2^23, 2^24).  For an addend t write
-// t/u = B + r, B = floor(t/u), 0 <= r < 1.  If A + B is in [2^23, 2^24 - 2]
-// the exact sum stays in the binade and one rounded step gives
+// While |v| stays in one binade [2^(E+23), 2^(E+24)) its ulp is u = 2^E and
+// v = A*u with A an integer, 2^23 <= |A| < 2^24.  For an addend t write
+// t/u = B + r, B = floor(t/u), 0 <= r < 1.  If A + B and A + B + 1 stay in
+// that range (of A's sign) the exact sum keeps the ulp, and one rounded step
+// gives
 //   A + B            r below one half,
 //   A + B + 1        r above one half,
-//   A + B + (A+B)&1  r exactly one half (round to even).
+//   A + B + (A+B)&1  r exactly one half (round to even),
+// for either sign of A (round to nearest even is symmetric under negation).
 // (For the variance the sum is first rounded to double, i.e. r to a multiple
 // of 2^-29 with ties to even, before the comparison.)  Only the exact halves
 // depend on the running value.  So a chunk of the image decomposes in
@@ -19,7 +21,7 @@
 // the exact running A, resolving each half with its true parity.  u is
 // guessed per chunk from a double prefix sum of the addends and checked in
 // the sequential pass; a chunk whose guess or bounds fail there (the first
-// chunks, binade crossings, negative sums) runs as the plain sequential loop.
+// chunks, binade crossings, sign changes) runs as the plain sequential loop.
 // Either way the result is bit-identical to the sequential loops.
 #include <algorithm>
 #include <climits>
@@ -41,17 +43,17 @@ constexpr long long kA0 = 1LL << 23, kA1 = 1LL << 24;
 
 struct Chunk {
     int E = 0;            // log2 of the ulp the chunk was decomposed with
-    bool planned = false;  // decomposed (the guessed running value was a positive normal)
+    bool planned = false;  // decomposed (the guessed running value was a normal float)
     double addend_sum = 0;  // double sum of the chunk's addends (for the next chunks' guesses)
     long long sum = 0;     // sum of the increments, every half rounded down
     long long lo = 0, hi = 0;  // min of (P_k + B_k), max of (P_k + B_k + 1) over the steps, P_k = prefix of increments
     std::vector<long long> halves;  // P_k + B_k at each exact half
 };
 
-// v = A * 2^E with A in [2^23, 2^24): E for a positive normal float v.
+// v = A * 2^E with 2^23 <= |A| < 2^24: E for a normal float v of either sign.
 inline int ulp_exp(float v) {
     int x;
-    std::frexp(v, &x);  // v in [2^(x-1), 2^x)
+    std::frexp(v, &x);  // |v| in [2^(x-1), 2^x)
     return x - 24;
 }
 
@@ -104,7 +106,7 @@ struct VarStep {
 
 template <class Step>
 void plan_chunk(const Step& st, long long k0, long long k1, float guess, Chunk& c) {
-    c.planned = guess > 0.0f && std::isnormal(guess);
+    c.planned = std::isnormal(guess);
     if (!c.planned) return;
     c.E = ulp_exp(guess);
     const double inv_u = std::ldexp(1.0, -c.E);
@@ -127,9 +129,12 @@ void plan_chunk(const Step& st, long long k0, long long k1, float guess, Chunk& 
 // Applies a chunk decomposed for ulp 2^E to v; false when the decomposition
 // does not hold for this v (then the chunk runs as the sequential loop).
 bool apply_plan(int E, long long sum, long long lo, long long hi, const long long* halves, long long nh, float& v) {
-    if (!(v > 0.0f) || !std::isnormal(v) || ulp_exp(v) != E) return false;
+    if (!std::isnormal(v) || ulp_exp(v) != E) return false;
     const long long A = (long long)std::ldexp((double)v, -E);
-    if (A + lo < kA0 || A + nh + hi > kA1 - 1) return false;
+    // every step's integer part (rounded halves included, at most nh of them)
+    // and its successor stay in A's binade: [2^23, 2^24 - 1] or [-(2^24 - 1), -2^23]
+    if (A > 0 ? (A + lo < kA0 || A + nh + hi > kA1 - 1) : (A + lo < -(kA1 - 1) || A + nh + hi > -kA0))
+        return false;
     long long T = 0;
     for (long long h = 0; h < nh; h++) T += (A + T + halves[h]) & 1;
     v = (float)std::ldexp((double)(A + sum + T), E);
@@ -205,7 +210,7 @@ float accumulate_device(const Step& st, int mode, const float* d_im, long long c
     double run = 0;
     for (size_t c = 0; c < nch; c++) {
         const float g = (float)run;
-        h_E[c] = g > 0.0f && std::isnormal(g) ? ulp_exp(g) : kStatsNoPlan;
+        h_E[c] = std::isnormal(g) ? ulp_exp(g) : kStatsNoPlan;
         run += h_sums[c];
     }
     DEEDS_HIP(hipMemcpyAsync(d_E, h_E, nch * sizeof(int), hipMemcpyHostToDevice, s));

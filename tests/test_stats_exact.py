@@ -2,7 +2,7 @@
 oracle's sequential loops (primsMST.h:101-110 restated), bit for bit, on the
 CPU: the chunked ulp-unit emulation must reproduce every rounding of the
 sequential float accumulation, including exact halfway cases, binade
-crossings, negative partial sums and tiny inputs."""
+crossings, negative running sums, sign changes and tiny inputs."""
 import ctypes
 import time
 
@@ -28,6 +28,13 @@ def _cases():
     yield "fractional", (rng.random(2_000_000) * 3000).astype(np.float32)
     yield "signed", rng.normal(0, 1000, 1_000_000).astype(np.float32)
     yield "negative_mean", rng.normal(-700, 50, 500_000).astype(np.float32)
+    # raw HU (no +1024): mostly air, so the running sum is negative throughout
+    yield "ct_raw_hu", np.where(rng.random(3_000_000) < 0.6, -1000,
+                                rng.integers(-200, 300, 3_000_000)).astype(np.float32)
+    yield "negative_fractional", (-rng.random(1_500_000) * 2500 - 3.25).astype(np.float32)
+    yield "negative_halves", (-(rng.integers(0, 8, 1_000_000) * 512 + 512)).astype(np.float32)
+    yield "sign_change", np.concatenate([rng.normal(-900, 30, 400_000), rng.normal(1500, 30, 600_000)]).astype(
+        np.float32)
     yield "halves_512", (rng.integers(0, 8, 2_000_000) * 512 + 512).astype(np.float32)
     yield "powers_of_two", np.ldexp(1.0, rng.integers(-4, 14, 1_000_000)).astype(np.float32)
     yield "constant", np.full(1_000_003, 1024.0, np.float32)
