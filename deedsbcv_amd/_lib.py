@@ -69,6 +69,8 @@ SIGNATURES = {
     "deeds_prims_graph": [V, V, V, V, I, I, I, I],
     "deeds_ctx_prims_graph": [V, V, V, V, V, I, I, I, I],
     "deeds_image_stats": [V, ctypes.c_longlong, V, V],
+    "deeds_dice": [V, V, ctypes.c_longlong, I, V, V, V],
+    "deeds_ctx_dice": [V, V, V, ctypes.c_longlong, I, V, V, V],
     "deeds_regularisation": [V, V, V, V, V, V, V, I, I, F, V, V, V, I, I, I],
     "deeds_upsample_deformations": [V, V, V, V, V, V, I, I, I, I, I, I],
     "deeds_consistent_mapping": [V, V, V, V, V, V, I, I, I, I],

@@ -16,6 +16,7 @@ runs on the GPU through libdeeds_hip.so; there is no CPU path.
     warpImageCL           dataCostD.h:213
     warpAffineS           dataCostD.h:241
     register              deedsBCV0.cpp:55 (the full symmetric registration)
+    dice                  diceMulti.cpp:17 (per-label Dice overlap)
 """
 from __future__ import annotations
 
@@ -217,6 +218,21 @@ def register_batch_device(pairs, device=0, grid=(8, 7, 6, 5, 4), radius=(8, 7, 6
     _lib.call("deeds_register_batch_device", int(device), arr, len(pairs), ctypes.byref(sched), int(concurrency),
               times)
     return [times[i].as_dict() for i in range(len(pairs))]
+
+
+def dice(seg1, seg2):
+    """diceMulti (diceMulti.cpp:17-79): (labels, dice) for every label of seg1 except its
+    smallest value, in increasing order; dice[i] = 2|A & B| / (|A| + |B|) in float."""
+    a = np.ascontiguousarray(seg1, np.int16)
+    b = np.ascontiguousarray(seg2, np.int16)
+    if a.shape != b.shape:
+        raise ValueError("Inconsistent image sizes (must have same dimensions)")
+    count = ctypes.c_int()
+    _lib.call("deeds_dice", _p(a), _p(b), a.size, 0, None, None, ctypes.byref(count))
+    labels = np.empty(count.value, np.int16)
+    values = np.empty(count.value, np.float32)
+    _lib.call("deeds_dice", _p(a), _p(b), a.size, count.value, _p(labels), _p(values), ctypes.byref(count))
+    return labels, values
 
 
 def apply_seg(seg, flow, grid_step, X=None):

@@ -187,8 +187,19 @@ int deeds_apply_seg(int16_t* out, const int16_t* in, const float* X, const float
 int deeds_apply_float(float* out, const float* in, const float* X, const float* flow, int grid_step,
                       int m, int n, int o);
 
+/* diceMulti.cpp:17-79: Dice overlap 2|A_l & B_l| / (|A_l| + |B_l|) of every
+ * label l of seg1 except its smallest value (the background), in increasing
+ * label order, computed in float as the reference does.  labels / dice
+ * (either may be NULL) receive up to `capacity` entries; *count is the number
+ * of labels, which may exceed capacity. */
+int deeds_dice(const int16_t* seg1, const int16_t* seg2, long long voxels, int capacity, int16_t* labels,
+               float* dice, int* count);
+
 /* ---- device-pointer forms (asynchronous on the context stream) ---------- */
 int deeds_ctx_descriptor(deeds_ctx* ctx, uint64_t* d_mindq, const float* d_im, int m, int n, int o, int qs);
+/* diceMulti.cpp:17-79 on device volumes (synchronous): as deeds_dice. */
+int deeds_ctx_dice(deeds_ctx* ctx, const int16_t* d_seg1, const int16_t* d_seg2, long long voxels, int capacity,
+                   int16_t* labels, float* dice, int* count);
 /* primsMST.h:52 primsGraph() on a device image: the same tree as
  * deeds_prims_graph (edge SADs and the exact image statistics on the GPU,
  * Prim's algorithm on the host); ordered/parents/edgemst are host arrays of

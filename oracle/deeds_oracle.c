@@ -757,3 +757,36 @@ int orc_register(const float* fixed, const float* moving, const float* X, int m,
     free(mind_mov); free(mind_fix); free(mind_w); free(f);
     return sz1;
 }
+
+/* ------------------------------------------------------------------------- */
+/* Evaluation                                                                 */
+/* ------------------------------------------------------------------------- */
+
+/* diceMulti.cpp:38-55: the distinct values of seg1 in increasing order
+ * (sort + unique, :38-40), all but the smallest (:44), each with
+ * 2*|A & B| / (|A| + |B|) from counts converted to float (:46-55). */
+int orc_dice(const short* seg1, const short* seg2, long long sz, int capacity, short* labels, float* dice) {
+    long long* c1 = (long long*)calloc(65536, sizeof(long long));
+    long long* c2 = (long long*)calloc(65536, sizeof(long long));
+    long long* both = (long long*)calloc(65536, sizeof(long long));
+    if (!c1 || !c2 || !both) { fprintf(stderr, "oracle: out of memory\n"); abort(); }
+    for (long long p = 0; p < sz; p++) {
+        int a = seg1[p] + 32768, b = seg2[p] + 32768;
+        c1[a]++;
+        c2[b]++;
+        if (a == b) both[a]++;
+    }
+    int total = 0, seen = 0;
+    for (int u = 0; u < 65536; u++) {
+        if (!c1[u]) continue;
+        if (seen++ == 0) continue; /* the smallest label (background) */
+        if (total < capacity) {
+            float unionval = (float)both[u], card1 = (float)c1[u], card2 = (float)c2[u];
+            labels[total] = (short)(u - 32768);
+            dice[total] = 2.0f * unionval / (card1 + card2);
+        }
+        total++;
+    }
+    free(c1); free(c2); free(both);
+    return total;
+}

@@ -56,6 +56,11 @@ void orc_warp_image(float* out, const float* im1, const float* im1b,
 void orc_warp_affine_s(short* out, const short* in, const float* X,
                        const float* u, const float* v, const float* w, int m, int n, int o);
 
+/* diceMulti.cpp:38-55: Dice of every label of seg1 but its smallest, in
+ * increasing label order; returns the number of labels (entries written up to
+ * capacity). */
+int orc_dice(const short* seg1, const short* seg2, long long sz, int capacity, short* labels, float* dice);
+
 /* the symmetric multi-level registration loop (deedsBCV0.cpp:183-349).
  * fixed/moving are HU volumes (the +1024 offset of deedsBCV0.cpp:139-144 is
  * applied inside).  flow_out receives the final control-grid field [u;v;w]

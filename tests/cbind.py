@@ -168,6 +168,8 @@ class Oracle(_Lib):
         L.orc_register.argtypes = [V, V, V, I, I, I, I, V, V, V, F, V, V, I]
         L.orc_register.restype = I
         L.orc_ssd_channel.argtypes = [V, V, I, I, I, I, I]
+        L.orc_dice.argtypes = [V, V, ctypes.c_longlong, I, V, V]
+        L.orc_dice.restype = I
         for k in ("boxfilter", "message_dt", "regularisation", "mind_thresholds", "edge_sad",
                   "image_stats", "ssd_channel"):
             getattr(L, "orc_" + k).restype = None
@@ -235,6 +237,15 @@ class Oracle(_Lib):
         neg = np.zeros(1, np.float32)
         s = self.lib.orc_jacobian(P(u), P(v), P(w), m, n, o, factor, P(neg))
         return np.float32(s), neg[0]
+
+    def dice(self, seg1, seg2):
+        a = np.ascontiguousarray(seg1, np.int16)
+        b = np.ascontiguousarray(seg2, np.int16)
+        k = self.lib.orc_dice(P(a), P(b), a.size, 0, None, None)
+        labels = np.zeros(k, np.int16)
+        dice = np.zeros(k, np.float32)
+        self.lib.orc_dice(P(a), P(b), a.size, k, P(labels), P(dice))
+        return labels, dice
 
     def register(self, fixed, moving, grid, radius, quant, alpha=1.6, X=None, want_warped=True):
         fixed, moving = _f32(fixed), _f32(moving)

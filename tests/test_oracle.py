@@ -10,6 +10,8 @@ The oracle is pinned three ways:
      brute-force MAP labelings on tiny trees), so that a dropped term, a wrong
      sign or a transposed axis fails even where 1 and 2 cannot see it.
 """
+from pathlib import Path
+
 import numpy as np
 import pytest
 from scipy.sparse import coo_matrix
@@ -294,3 +296,65 @@ def test_regularisation_is_map_on_chains(oracle):
              (edgemst[2] * cost[2][None, None, :] + pair(1, 2)[:, None, :]))
         s = sel
         assert abs(E[s[1], s[0], s[2]] - E.min()) < 1e-4
+
+
+# ------------------------------------------------------------ diceMulti -----
+
+def _label_pair(shape, seed, n_labels=6, offset=0):
+    """Two overlapping label volumes: random boxes of labels on background."""
+    rng = np.random.default_rng(seed)
+    a = np.full(shape, offset, np.int16)
+    b = np.full(shape, offset, np.int16)
+    for lab in range(1, n_labels + 1):
+        for v in (a, b):
+            lo = [rng.integers(0, s // 2) for s in shape]
+            hi = [l + rng.integers(2, s // 2) for l, s in zip(lo, shape)]
+            v[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] = lab + offset
+    return a, b
+
+
+def test_dice_closed_forms(oracle):
+    a, b = _label_pair((20, 18, 16), 5)
+    labels, d = oracle.dice(a, a)
+    assert labels.tolist() == sorted(set(np.unique(a).tolist()) - {int(a.min())})
+    assert np.all(d == 1.0)
+    # disjoint label sets: every Dice is 0
+    labels, d = oracle.dice(a, (a + 100).astype(np.int16))
+    assert np.all(d == 0.0)
+    # brute force with numpy counts on a small pair (labels of seg1 but the smallest)
+    labels, d = oracle.dice(a, b)
+    want = []
+    for lab in np.unique(a)[1:]:
+        inter = np.float32(np.count_nonzero((a == lab) & (b == lab)))
+        c1, c2 = np.float32(np.count_nonzero(a == lab)), np.float32(np.count_nonzero(b == lab))
+        want.append(np.float32(2.0) * inter / (c1 + c2))
+    assert np.array_equal(d, np.array(want, np.float32))
+
+
+def _dice_text(labels_dice):
+    """The reference's two output lines (diceMulti.cpp:58-64) for (labels, dice)."""
+    d = labels_dice[1]
+    mean = np.float32(0.0)
+    for x in d:
+        mean = np.float32(mean + x)
+    mean = np.float32(mean / np.float32(len(d)))
+    return "Dice (for each label): " + "".join(" %4.3f, " % x for x in d) + "\nAVERAGE DICE: %f\n" % mean
+
+
+@pytest.mark.parametrize("case", [(7, 6, 0), (8, 12, -3), (9, 3, 200)])
+def test_ref_dice_binary(oracle, ref, tmp_path, case):
+    """The oracle's Dice, printed as diceMulti prints it, equals the unmodified
+    reference binary's output on the same files."""
+    import subprocess
+    from deedsbcv_amd import nifti
+    seed, k, offset = case
+    a, b = _label_pair((33, 29, 25), seed, k, offset)
+    nifti.write_nifti(tmp_path / "a.nii.gz", a)
+    nifti.write_nifti(tmp_path / "b.nii.gz", b)
+    exe = Path(__file__).resolve().parents[1] / "oracle" / "_ref" / "diceMulti"
+    if not exe.exists():
+        pytest.skip("reference diceMulti not built")
+    r = subprocess.run([str(exe), str(tmp_path / "a.nii.gz"), str(tmp_path / "b.nii.gz")], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 0
+    assert r.stdout == _dice_text(oracle.dice(a, b))
