@@ -790,3 +790,362 @@ int orc_dice(const short* seg1, const short* seg2, long long sz, int capacity, s
     free(c1); free(c2); free(both);
     return total;
 }
+
+/* ------------------------------------------------------------------------- */
+/* Affine pre-alignment (linearBCV)                                           */
+/* ------------------------------------------------------------------------- */
+
+/* affineLTS1.h:39-75: trilinear warp of `in` by the affine X (no field), zero
+ * where the 2x2x2 neighbourhood leaves the volume; the weights are double,
+ * except dx*dy (terms 5, 8) and dx*dy*dz*in (term 8), which are float. */
+void orc_lin_warp(float* out, const float* in, const float* X, int m, int n, int o) {
+    for (int k = 0; k < o; k++)
+        for (int j = 0; j < n; j++)
+            for (int i = 0; i < m; i++) {
+                float y1 = (float)i * X[0] + (float)j * X[1] + (float)k * X[2] + X[3];
+                float x1 = (float)i * X[4] + (float)j * X[5] + (float)k * X[6] + X[7];
+                float z1 = (float)i * X[8] + (float)j * X[9] + (float)k * X[10] + X[11];
+                int x = (int)floorf(x1), y = (int)floorf(y1), z = (int)floorf(z1);
+                float dx = x1 - (float)x, dy = y1 - (float)y, dz = z1 - (float)z;
+                long long p = IDX(i, j, k, m, n);
+                if (y < 0 || y >= m - 1 || x < 0 || x >= n - 1 || z < 0 || z >= o - 1) {
+                    out[p] = 0.0f;
+                    continue;
+                }
+                double ddx = dx, ddy = dy, ddz = dz;
+                double s = (1.0 - ddx) * (1.0 - ddy) * (1.0 - ddz) * in[IDX(y, x, z, m, n)];
+                s += (1.0 - ddx) * ddy * (1.0 - ddz) * in[IDX(y + 1, x, z, m, n)];
+                s += ddx * (1.0 - ddy) * (1.0 - ddz) * in[IDX(y, x + 1, z, m, n)];
+                s += (1.0 - ddx) * (1.0 - ddy) * ddz * in[IDX(y, x, z + 1, m, n)];
+                s += (double)(dx * dy) * (1.0 - ddz) * in[IDX(y + 1, x + 1, z, m, n)];
+                s += (1.0 - ddx) * ddy * ddz * in[IDX(y + 1, x, z + 1, m, n)];
+                s += ddx * (1.0 - ddy) * ddz * in[IDX(y, x + 1, z + 1, m, n)];
+                s += (double)(dx * dy * dz * in[IDX(y + 1, x + 1, z + 1, m, n)]);
+                out[p] = (float)s;
+            }
+}
+
+/* QRsolve.h:1-7: float accumulation of the exact double squares (pow(float,2)). */
+static float lin_norm(const float* v, int len) {
+    float s = 0;
+    for (int i = 0; i < len; i++) s = (float)((double)s + (double)v[i] * (double)v[i]);
+    return sqrtf(s);
+}
+
+/* QRsolve.h:9-15 */
+static float lin_dot(const float* a, const float* b, int len) {
+    float d = 0;
+    for (int i = 0; i < len; i++) d = d + a[i] * b[i];
+    return d;
+}
+
+/* QRsolve.h:17-70: Gram-Schmidt QR of the len x 4 matrix A (columns A + c*len)
+ * and back substitution for the len2 right-hand sides b + j*len -> X + j*4. */
+void orc_qrsolve(float* X, const float* A, const float* b, int len, int len2) {
+    float* Q = (float*)xmalloc(sizeof(float) * 4 * (size_t)len);
+    float r11 = lin_norm(A, len);
+    for (int i = 0; i < len; i++) Q[i] = A[i] / r11;
+    float r12 = lin_dot(Q, A + len, len), r13 = lin_dot(Q, A + 2 * len, len), r14 = lin_dot(Q, A + 3 * len, len);
+    for (int i = 0; i < len; i++) {
+        Q[i + len] = A[i + len] - Q[i] * r12;
+        Q[i + 2 * len] = A[i + 2 * len] - Q[i] * r13;
+        Q[i + 3 * len] = A[i + 3 * len] - Q[i] * r14;
+    }
+    float r22 = lin_norm(Q + len, len);
+    for (int i = 0; i < len; i++) Q[i + len] = Q[i + len] / r22;
+    float r23 = lin_dot(Q + len, A + 2 * len, len), r24 = lin_dot(Q + len, A + 3 * len, len);
+    for (int i = 0; i < len; i++) {
+        Q[i + 2 * len] = A[i + 2 * len] - Q[i] * r13 - Q[i + len] * r23;
+        Q[i + 3 * len] = A[i + 3 * len] - Q[i] * r14 - Q[i + len] * r24;
+    }
+    float r33 = lin_norm(Q + 2 * len, len);
+    for (int i = 0; i < len; i++) Q[i + 2 * len] = Q[i + 2 * len] / r33;
+    float r34 = lin_dot(Q + 2 * len, A + 3 * len, len);
+    for (int i = 0; i < len; i++) Q[i + 3 * len] = A[i + 3 * len] - Q[i] * r14 - Q[i + len] * r24 - Q[i + 2 * len] * r34;
+    float r44 = lin_norm(Q + 3 * len, len);
+    for (int i = 0; i < len; i++) Q[i + 3 * len] = Q[i + 3 * len] / r44;
+    for (int j = 0; j < len2; j++) {
+        const float* bj = b + (size_t)j * len;
+        float d1 = lin_dot(Q, bj, len), d2 = lin_dot(Q + len, bj, len);
+        float d3 = lin_dot(Q + 2 * len, bj, len), d4 = lin_dot(Q + 3 * len, bj, len);
+        float x4 = d4 / r44;
+        float x3 = (d3 - r34 * x4) / r33;
+        float x2 = (d2 - r23 * x3 - r24 * x4) / r22;
+        float x1 = (d1 - r12 * x2 - r13 * x3 - r14 * x4) / r11;
+        X[0 + j * 4] = x1; X[1 + j * 4] = x2; X[2 + j * 4] = x3; X[3 + j * 4] = x4;
+    }
+    free(Q);
+}
+
+/* QRsolve.h:72-122: four sweeps of one-sided Jacobi rotations on the columns
+ * of the 3x3 A (U <- A rotated, V <- the rotations), then unit columns of U. */
+static void lin_jacobi_svd3(const float* A, float* U, float* V) {
+    for (int i = 0; i < 9; i++) { U[i] = A[i]; V[i] = 0; }
+    V[0] = 1; V[4] = 1; V[8] = 1;
+    float T[3];
+    for (int iter = 0; iter < 4; iter++)
+        for (int j = 1; j < 3; j++)
+            for (int i = 0; i < j; i++) {
+                float alpha = lin_dot(U + i * 3, U + i * 3, 3);
+                float beta = lin_dot(U + j * 3, U + j * 3, 3);
+                float gamma = lin_dot(U + i * 3, U + j * 3, 3);
+                float zeta = (float)((double)(beta - alpha) / (2.0 * (double)gamma));
+                float t = (float)((zeta > 0 ? 1.0 : -1.0) / ((double)fabsf(zeta) + sqrt(1.0 + (double)(zeta * zeta))));
+                float c = (float)(1.0 / (double)sqrtf(1.0f + t * t));
+                float s = c * t;
+                for (int k = 0; k < 3; k++) T[k] = U[k + i * 3];
+                for (int k = 0; k < 3; k++) U[k + i * 3] = c * T[k] - s * U[k + j * 3];
+                for (int k = 0; k < 3; k++) U[k + j * 3] = s * T[k] + c * U[k + j * 3];
+                for (int k = 0; k < 3; k++) T[k] = V[k + i * 3];
+                for (int k = 0; k < 3; k++) V[k + i * 3] = c * T[k] - s * V[k + j * 3];
+                for (int k = 0; k < 3; k++) V[k + j * 3] = s * T[k] + c * V[k + j * 3];
+            }
+    for (int j = 0; j < 3; j++) {
+        float sv = lin_norm(U + j * 3, 3);
+        for (int i = 0; i < 3; i++) U[i + j * 3] /= sv;
+    }
+}
+
+/* QRsolve.h:124-254: rigid fit (rotation from the SVD of the centred
+ * cross-covariance, translation from the mean residual). */
+void orc_find_rigid(float* RT, const float* pts1t, const float* pts2t, int len) {
+    float* p1 = (float*)xmalloc(sizeof(float) * 4 * (size_t)len);
+    float* p2 = (float*)xmalloc(sizeof(float) * 4 * (size_t)len);
+    for (int i = 0; i < 4; i++)
+        for (int k = 0; k < len; k++) {
+            p1[i + k * 4] = pts1t[k + (size_t)i * len];
+            p2[i + k * 4] = pts2t[k + (size_t)i * len];
+        }
+    float m1[3] = {0, 0, 0}, m2[3] = {0, 0, 0};
+    for (int k = 0; k < len; k++)
+        for (int i = 0; i < 3; i++) { m1[i] += p1[i + k * 4]; m2[i] += p2[i + k * 4]; }
+    for (int i = 0; i < 3; i++) { m1[i] /= (float)len; m2[i] /= (float)len; }
+    float K[9] = {0}, U[9], V[9], R[16];
+    for (int k = 0; k < len; k++)
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) K[i + j * 3] += (p1[j + k * 4] - m1[j]) * (p2[i + k * 4] - m2[i]);
+    lin_jacobi_svd3(K, U, V);
+    for (int i = 0; i < 16; i++) { R[i] = 0.0f; RT[i] = 0.0f; }
+    R[15] = 1.0f;
+    RT[15] = 1.0f;
+    for (int k = 0; k < 3; k++)
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) R[j + i * 4] += U[j + k * 3] * V[i + k * 3];
+    float* Rp2 = (float*)xmalloc(sizeof(float) * 4 * (size_t)len);
+    orc_qrsolve(Rp2, R, p2, 4, len);
+    float t1[3] = {0, 0, 0};
+    for (int i = 0; i < len; i++) { t1[0] += Rp2[i * 4]; t1[1] += Rp2[1 + i * 4]; t1[2] += Rp2[2 + i * 4]; }
+    for (int i = 0; i < 3; i++) t1[i] = t1[i] / (float)len - m1[i];
+    float t0[3] = {0, 0, 0};
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) t0[i] += R[i + j * 4] * t1[j];
+    for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 3; j++) RT[i + j * 4] += R[j + i * 4];
+        RT[3 + i * 4] = t0[i];
+    }
+    free(p1); free(p2); free(Rp2);
+}
+
+static int lin_cmp_float(const void* a, const void* b) {
+    float x = *(const float*)a, y = *(const float*)b;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+/* the k-th smallest of v[0..len) (the value std::nth_element places at k) */
+static float lin_kth(const float* v, int len, int k) {
+    float* c = (float*)xmalloc(sizeof(float) * (size_t)len);
+    memcpy(c, v, sizeof(float) * (size_t)len);
+    qsort(c, (size_t)len, sizeof(float), lin_cmp_float);
+    float r = c[k];
+    free(c);
+    return r;
+}
+
+/* QRsolve.h:258-296: least trimmed squares -- fit to all points, then 15
+ * refits to the half of the points whose residual is at most the median. */
+void orc_affine_robust(float* RT, const float* pts1, const float* pts2, int len, int rigid) {
+    if (rigid) orc_find_rigid(RT, pts1, pts2, len);
+    else orc_qrsolve(RT, pts1, pts2, len, 4);
+    int lenh = len / 2;
+    float* err = (float*)xmalloc(sizeof(float) * (size_t)len);
+    float* b1 = (float*)xmalloc(sizeof(float) * 4 * (size_t)(lenh > 0 ? lenh : 1));
+    float* b2 = (float*)xmalloc(sizeof(float) * 4 * (size_t)(lenh > 0 ? lenh : 1));
+    for (int i = 0; i < 4 * lenh; i++) { b1[i] = 1; b2[i] = 1; }
+    for (int iter = 0; iter < 15; iter++) {
+        for (int l = 0; l < len; l++) {
+            float x = 0, y = 0, z = 0;
+            for (int i = 0; i < 4; i++) {
+                y += pts1[l + i * len] * RT[i];
+                x += pts1[l + i * len] * RT[i + 4];
+                z += pts1[l + i * len] * RT[i + 8];
+            }
+            double ey = y - pts2[l], ex = x - pts2[l + len], ez = z - pts2[l + 2 * len];
+            err[l] = (float)(ey * ey + ex * ex + ez * ez);
+        }
+        float median = lin_kth(err, len, len / 2);
+        int count = 0;
+        for (int i = 0; i < len; i++) {
+            int l = count < lenh - 1 ? count : lenh - 1;
+            if (err[i] <= median) {
+                for (int d = 0; d < 3; d++) {
+                    b1[l + d * lenh] = pts1[i + d * len];
+                    b2[l + d * lenh] = pts2[i + d * len];
+                }
+                count++;
+            }
+        }
+        if (rigid) orc_find_rigid(RT, b1, b2, lenh);
+        else orc_qrsolve(RT, b1, b2, lenh, 4);
+    }
+    free(err); free(b1); free(b2);
+}
+
+/* affineLTS1.h:1-12: C = A * B (4x4, column-major, float accumulation). */
+static void lin_matmult(const float* A, const float* B, float* C) {
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            C[i + j * 4] = 0.0f;
+            for (int k = 0; k < 4; k++) C[i + j * 4] += A[i + k * 4] * B[k + j * 4];
+        }
+}
+
+/* affineLTS1.h:122-148: in-place smoothing along the node axis with index
+ * stride `st` (1: y, m1: x, m1*n1: z): c[p] += 0.5*c[p+1] + 0.5*c[p-1] in
+ * double, nodes visited in index order (p-1 already smoothed), clamped. */
+static void lin_smooth(float* c, int len3, int m1, int n1, int o1, int axis) {
+    int sz1 = m1 * n1 * o1;
+    for (int i = 0; i < sz1; i++) {
+        int z1 = i / (m1 * n1), x1 = (i - z1 * m1 * n1) / m1, y1 = i - z1 * m1 * n1 - x1 * m1;
+        int i1, i2;
+        if (axis == 0) { i1 = (y1 + 1 < m1 - 1 ? y1 + 1 : m1 - 1) + x1 * m1 + z1 * m1 * n1; i2 = (y1 - 1 > 0 ? y1 - 1 : 0) + x1 * m1 + z1 * m1 * n1; }
+        else if (axis == 1) { i1 = y1 + (x1 + 1 < n1 - 1 ? x1 + 1 : n1 - 1) * m1 + z1 * m1 * n1; i2 = y1 + (x1 - 1 > 0 ? x1 - 1 : 0) * m1 + z1 * m1 * n1; }
+        else { i1 = y1 + x1 * m1 + (z1 + 1 < o1 - 1 ? z1 + 1 : o1 - 1) * m1 * n1; i2 = y1 + x1 * m1 + (z1 - 1 > 0 ? z1 - 1 : 0) * m1 * n1; }
+        for (int l = 0; l < len3; l++) {
+            double add = 0.5 * (double)c[(size_t)i1 * len3 + l] + 0.5 * (double)c[(size_t)i2 * len3 + l];
+            c[(size_t)i * len3 + l] = (float)((double)c[(size_t)i * len3 + l] + add);
+        }
+    }
+}
+
+/* affineLTS1.h:81-236 (estimateAffine2): smooth both cost volumes, take each
+ * node's first minimum, drop background (block mean < 5) and border nodes,
+ * keep the nodes whose minimum is below the quantile, and fit the affine (or
+ * rigid) map between node centres and their best displacements robustly;
+ * X = fit * Xprev.  cost / cost2 are smoothed in place. */
+void orc_estimate_affine(float* X, const float* Xprev, const float* im1, const float* im2, float* cost, float* cost2,
+                         int m, int n, int o, int step, float quant, int hw, int rigid, int* used) {
+    int m1 = m / step, n1 = n / step, o1 = o / step, sz1 = m1 * n1 * o1;
+    int len = 2 * hw + 1, len3 = len * len * len, stephw = (step - 1) / 2;
+    for (int axis = 0; axis < 3; axis++) {
+        lin_smooth(cost, len3, m1, n1, o1, axis);
+        lin_smooth(cost2, len3, m1, n1, o1, axis);
+    }
+    float* mv1 = (float*)xmalloc(sizeof(float) * (size_t)sz1);
+    float* mv2 = (float*)xmalloc(sizeof(float) * (size_t)sz1);
+    int* mi1 = (int*)xmalloc(sizeof(int) * (size_t)sz1);
+    int* mi2 = (int*)xmalloc(sizeof(int) * (size_t)sz1);
+    int allcount = 0;
+    for (int i = 0; i < sz1; i++) {
+        const float* c1 = cost + (size_t)i * len3;
+        const float* c2 = cost2 + (size_t)i * len3;
+        int b1 = 0, b2 = 0;
+        for (int l = 1; l < len3; l++) {
+            if (c1[l] < c1[b1]) b1 = l;
+            if (c2[l] < c2[b2]) b2 = l;
+        }
+        mv1[i] = c1[b1]; mi1[i] = b1;
+        mv2[i] = c2[b2]; mi2[i] = b2;
+        int z1 = i / (m1 * n1), x1 = (i - z1 * m1 * n1) / m1, y1 = i - z1 * m1 * n1 - x1 * m1;
+        float mean1 = 0.0f, mean2 = 0.0f;
+        for (int k1 = 0; k1 < step; k1++)
+            for (int j1 = 0; j1 < step; j1++)
+                for (int i1 = 0; i1 < step; i1++) {
+                    long long p = IDX(y1 * step + i1, x1 * step + j1, z1 * step + k1, m, n);
+                    mean1 += im1[p];
+                    mean2 += im2[p];
+                }
+        mean1 /= (float)(step * step * step);
+        mean2 /= (float)(step * step * step);
+        if (mean1 < 5.0f) mv1[i] = 1e20f;
+        if (mean2 < 5.0f) mv2[i] = 1e20f;
+        if (z1 < 2 || z1 > o1 - 2 || x1 < 2 || x1 > n1 - 2 || y1 < 2 || y1 > m1 - 2) { mv1[i] = 1e20f; mv2[i] = 1e20f; }
+        if (mv1[i] < 1e19f) allcount++;
+        if (mv2[i] < 1e19f) allcount++;
+    }
+    /* quantile(values, sz1, 0.5*allcount/(float)sz1/2.0f), linearBCV.cpp:36-46 */
+    float q = (float)(0.5 * allcount / (double)(float)sz1 / 2.0);
+    float qc = q < 0.01f ? 0.01f : (q > 0.99f ? 0.99f : q);
+    int qi = (int)((float)sz1 * qc);
+    float med1 = lin_kth(mv1, sz1, qi), med2 = lin_kth(mv2, sz1, qi);
+    int valsz = 0;
+    for (int i = 0; i < sz1; i++) valsz += (mv1[i] < med1) + (mv2[i] < med2);
+    if (used) *used = valsz;
+    float* p1 = (float*)xmalloc(sizeof(float) * 4 * (size_t)(valsz > 0 ? valsz : 1));
+    float* p2 = (float*)xmalloc(sizeof(float) * 4 * (size_t)(valsz > 0 ? valsz : 1));
+    int cur = 0;
+    for (int i = 0; i < sz1; i++) {
+        int k1 = i / (m1 * n1), j1 = (i - k1 * m1 * n1) / m1, i1 = i - j1 * m1 - k1 * m1 * n1;
+        int y1 = i1 * step + stephw, x1 = j1 * step + stephw, z1 = k1 * step + stephw;
+        for (int dir = 0; dir < 2; dir++) {
+            if (!(dir ? mv2[i] < med2 : mv1[i] < med1)) continue;
+            int ind = dir ? mi2[i] : mi1[i];
+            int li = ind % len, lj = (ind / len) % len, lk = ind / (len * len);
+            float ys = (float)(li - hw) * quant, xs = (float)(lj - hw) * quant, zs = (float)(lk - hw) * quant;
+            float* a = dir ? p2 : p1; /* the node centre */
+            float* b = dir ? p1 : p2; /* the centre displaced by the best label */
+            a[cur] = (float)y1; a[cur + valsz] = (float)x1; a[cur + 2 * valsz] = (float)z1;
+            b[cur] = (float)y1 + ys; b[cur + valsz] = (float)x1 + xs; b[cur + 2 * valsz] = (float)z1 + zs;
+            p1[cur + 3 * valsz] = 1.0f; p2[cur + 3 * valsz] = 1.0f;
+            cur++;
+        }
+    }
+    float X1[16];
+    orc_affine_robust(X1, p1, p2, valsz, rigid);
+    lin_matmult(X1, Xprev, X);
+    free(mv1); free(mv2); free(mi1); free(mi2); free(p1); free(p2);
+}
+
+/* linearBCV.cpp:144-263: the multi-level affine (rigid) estimation on HU
+ * volumes (+1024 inside); X_out receives the 16 floats column-wise. */
+void orc_linear_register(const float* fixed, const float* moving, int m, int n, int o, int levels, const int* grid,
+                         const int* radius, const int* quant, int rigid, float* X_out) {
+    long long sz = (long long)m * n * o;
+    float* im1b = (float*)xmalloc(sizeof(float) * sz);
+    float* im1 = (float*)xmalloc(sizeof(float) * sz);
+    float* w1 = (float*)xmalloc(sizeof(float) * sz);
+    float* w2 = (float*)xmalloc(sizeof(float) * sz);
+    uint64_t* mind_m = (uint64_t*)xmalloc(sizeof(uint64_t) * sz);
+    uint64_t* mind_f = (uint64_t*)xmalloc(sizeof(uint64_t) * sz);
+    uint64_t* mind_w = (uint64_t*)xmalloc(sizeof(uint64_t) * sz);
+    for (long long p = 0; p < sz; p++) { im1b[p] = fixed[p] - (-1024.0f); im1[p] = moving[p] - (-1024.0f); }
+    float Xprev[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1}, X[16];
+    const float ident[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    memcpy(X, Xprev, sizeof(X));
+    for (int level = 0; level < levels; level++) {
+        float q1 = (float)quant[level];
+        int step = grid[level], hw = radius[level];
+        float Xinv[16];
+        orc_qrsolve(Xinv, Xprev, ident, 4, 4);
+        orc_lin_warp(w2, im1b, Xinv, m, n, o);
+        orc_lin_warp(w1, im1, Xprev, m, n, o);
+        int prev = (int)floorf(0.5f * (float)quant[level > 0 ? level - 1 : 0] + 1.0f);
+        int curr = (int)floorf(0.5f * (float)quant[level] + 1.0f);
+        if (level == 0 || prev != curr) {
+            orc_descriptor(mind_m, im1, m, n, o, curr);
+            orc_descriptor(mind_f, im1b, m, n, o, curr);
+        }
+        int len = 2 * hw + 1, len3 = len * len * len;
+        int sz1 = (m / step) * (n / step) * (o / step);
+        float* c1 = (float*)xmalloc(sizeof(float) * (size_t)sz1 * len3);
+        float* c2 = (float*)xmalloc(sizeof(float) * (size_t)sz1 * len3);
+        orc_descriptor(mind_w, w1, m, n, o, curr);
+        orc_datacost(mind_f, mind_w, c1, m, n, o, step, hw, q1, 1.0f, 1);
+        orc_descriptor(mind_w, w2, m, n, o, curr);
+        orc_datacost(mind_m, mind_w, c2, m, n, o, step, hw, q1, 1.0f, 1);
+        orc_estimate_affine(X, Xprev, im1b, im1, c1, c2, m, n, o, step, q1, hw, rigid, NULL);
+        memcpy(Xprev, X, sizeof(X));
+        free(c1); free(c2);
+    }
+    memcpy(X_out, X, sizeof(X));
+    free(im1b); free(im1); free(w1); free(w2); free(mind_m); free(mind_f); free(mind_w);
+}

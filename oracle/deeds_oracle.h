@@ -61,6 +61,16 @@ void orc_warp_affine_s(short* out, const short* in, const float* X,
  * capacity). */
 int orc_dice(const short* seg1, const short* seg2, long long sz, int capacity, short* labels, float* dice);
 
+/* affine pre-alignment (linearBCV.cpp, affineLTS1.h, QRsolve.h) */
+void orc_lin_warp(float* out, const float* in, const float* X, int m, int n, int o);
+void orc_qrsolve(float* X, const float* A, const float* b, int len, int len2);
+void orc_find_rigid(float* RT, const float* pts1t, const float* pts2t, int len);
+void orc_affine_robust(float* RT, const float* pts1, const float* pts2, int len, int rigid);
+void orc_estimate_affine(float* X, const float* Xprev, const float* im1, const float* im2, float* cost, float* cost2,
+                         int m, int n, int o, int step, float quant, int hw, int rigid, int* used);
+void orc_linear_register(const float* fixed, const float* moving, int m, int n, int o, int levels, const int* grid,
+                         const int* radius, const int* quant, int rigid, float* X_out);
+
 /* the symmetric multi-level registration loop (deedsBCV0.cpp:183-349).
  * fixed/moving are HU volumes (the +1024 offset of deedsBCV0.cpp:139-144 is
  * applied inside).  flow_out receives the final control-grid field [u;v;w]

@@ -148,7 +148,60 @@ class _Lib:
         return out
 
 
-class Oracle(_Lib):
+class _LinearFns:
+    """lin_warp / qrsolve / affine_robust / estimate_affine shared by the oracle and
+    the reference harness (same signatures up to the prefix)."""
+
+    def _lin_setup(self):
+        I, F, V = c_int, c_float, c_void_p
+        p = self.prefix
+        for name, args in (("lin_warp", [V, V, V, I, I, I]), ("qrsolve", [V, V, V, I, I]),
+                           ("affine_robust", [V, V, V, I, I])):
+            fn = getattr(self.lib, p + name)
+            fn.argtypes = args
+            fn.restype = None
+        fn = getattr(self.lib, p + "estimate_affine")
+        fn.argtypes = [V, V, V, V, V, V, I, I, I, I, F, I, I] + ([V] if p == "orc_" else [])
+        fn.restype = None
+
+    def lin_warp(self, im, X):
+        im = _f32(im)
+        m, n, o = dims_of(im)
+        out = np.zeros(im.shape, np.float32)
+        getattr(self.lib, self.prefix + "lin_warp")(P(out), P(im), P(_f32(X)), m, n, o)
+        return out
+
+    def qrsolve(self, A, b, length, len2):
+        X = np.zeros(4 * len2, np.float32)
+        getattr(self.lib, self.prefix + "qrsolve")(P(X), P(_f32(A)), P(_f32(b)), length, len2)
+        return X
+
+    def affine_robust(self, pts1, pts2, length, rigid):
+        RT = np.zeros(16, np.float32)
+        getattr(self.lib, self.prefix + "affine_robust")(P(RT), P(_f32(pts1)), P(_f32(pts2)), length, int(rigid))
+        return RT
+
+    def estimate_affine(self, Xprev, im1, im2, cost, cost2, step, quant, hw, rigid):
+        im1, im2 = _f32(im1), _f32(im2)
+        m, n, o = dims_of(im1)
+        c1, c2 = _f32(cost).copy(), _f32(cost2).copy()
+        X = np.zeros(16, np.float32)
+        extra = [None] if self.prefix == "orc_" else []
+        getattr(self.lib, self.prefix + "estimate_affine")(P(X), P(_f32(Xprev)), P(im1), P(im2), P(c1), P(c2), m, n,
+                                                           o, step, float(quant), hw, int(rigid), *extra)
+        return X, c1, c2
+
+
+class RefLinear(_LinearFns):
+    """oracle/_ref/libdeedsreflin.so: the unmodified linearBCV.cpp functions."""
+    prefix = "ref_"
+
+    def __init__(self, path):
+        self.lib = ctypes.CDLL(str(path))
+        self._lin_setup()
+
+
+class Oracle(_Lib, _LinearFns):
     prefix = "orc_"
 
     def _setup(self):
@@ -170,9 +223,14 @@ class Oracle(_Lib):
         L.orc_ssd_channel.argtypes = [V, V, I, I, I, I, I]
         L.orc_dice.argtypes = [V, V, ctypes.c_longlong, I, V, V]
         L.orc_dice.restype = I
+        L.orc_linear_register.argtypes = [V, V, I, I, I, I, V, V, V, I, V]
+        L.orc_linear_register.restype = None
+        L.orc_estimate_affine.argtypes = [V, V, V, V, V, V, I, I, I, I, F, I, I, V]
+        L.orc_estimate_affine.restype = None
         for k in ("boxfilter", "message_dt", "regularisation", "mind_thresholds", "edge_sad",
                   "image_stats", "ssd_channel"):
             getattr(L, "orc_" + k).restype = None
+        self._lin_setup()
 
     def boxfilter(self, vol, hw):
         v = _f32(vol).copy()
@@ -237,6 +295,15 @@ class Oracle(_Lib):
         neg = np.zeros(1, np.float32)
         s = self.lib.orc_jacobian(P(u), P(v), P(w), m, n, o, factor, P(neg))
         return np.float32(s), neg[0]
+
+    def linear_register(self, fixed, moving, grid, radius, quant, rigid=False):
+        fixed, moving = _f32(fixed), _f32(moving)
+        m, n, o = dims_of(fixed)
+        X = np.zeros(16, np.float32)
+        self.lib.orc_linear_register(P(fixed), P(moving), m, n, o, len(grid), P(np.asarray(grid, np.int32)),
+                                     P(np.asarray(radius, np.int32)), P(np.asarray(quant, np.int32)), int(rigid),
+                                     P(X))
+        return X
 
     def dice(self, seg1, seg2):
         a = np.ascontiguousarray(seg1, np.int16)
@@ -320,3 +387,10 @@ def load_oracle():
 
 def load_ref():
     return Ref(REF_SO) if REF_SO.exists() else None
+
+
+REF_LIN_SO = ROOT / "oracle" / "_ref" / "libdeedsreflin.so"
+
+
+def load_ref_linear():
+    return RefLinear(REF_LIN_SO) if REF_LIN_SO.exists() else None

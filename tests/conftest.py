@@ -30,6 +30,15 @@ def ref():
 
 
 @pytest.fixture(scope="session")
+def ref_linear():
+    from cbind import load_ref_linear
+    r = load_ref_linear()
+    if r is None:
+        pytest.skip("reference build oracle/_ref not present (built only where the reference checkout exists)")
+    return r
+
+
+@pytest.fixture(scope="session")
 def golden():
     import numpy as np
     p = Path(__file__).resolve().parent / "golden" / "golden_small.npz"

@@ -358,3 +358,112 @@ def test_ref_dice_binary(oracle, ref, tmp_path, case):
                        text=True, timeout=60)
     assert r.returncode == 0
     assert r.stdout == _dice_text(oracle.dice(a, b))
+
+
+# ------------------------------------------------------------- linearBCV ----
+
+def _affine_X(seed, scale=0.03, shift=2.0):
+    """A column-wise 4x4 affine near identity (the layout linearBCV keeps)."""
+    rng = np.random.default_rng(seed)
+    A = np.eye(4, dtype=np.float32)
+    A[:3, :3] += rng.normal(0, scale, (3, 3)).astype(np.float32)
+    A[:3, 3] = rng.normal(0, shift, 3).astype(np.float32)
+    return A.T.ravel().copy()  # X[i + 4*j] = A[i, j]
+
+
+def _points(seed, n, X, noise=0.3, outliers=0.2):
+    """pts1 (4 x n, homogeneous) and pts2 = X applied to pts1 plus noise and outliers."""
+    rng = np.random.default_rng(seed)
+    p1 = np.ones((4, n), np.float32)
+    p1[:3] = rng.uniform(0, 60, (3, n)).astype(np.float32)
+    A = np.asarray(X, np.float32).reshape(4, 4).T
+    p2 = (A @ p1).astype(np.float32)
+    p2[:3] += rng.normal(0, noise, (3, n)).astype(np.float32)
+    bad = rng.random(n) < outliers
+    p2[:3, bad] += rng.uniform(-15, 15, (3, int(bad.sum()))).astype(np.float32)
+    p2[3] = 1
+    return p1.ravel().copy(), p2.ravel().copy()
+
+
+def test_ref_lin_warp(oracle, ref_linear):
+    f, _ = phantom.make_pair((37, 31, 27), 8)
+    im = f.astype(np.float32) + 1024.5
+    for seed in (1, 2, 3):
+        X = _affine_X(seed)
+        assert np.array_equal(oracle.lin_warp(im, X), ref_linear.lin_warp(im, X))
+
+
+def test_ref_qrsolve_and_robust_fit(oracle, ref_linear):
+    rng = np.random.default_rng(4)
+    A = rng.normal(0, 1, (4, 4)).astype(np.float32).ravel()
+    b = np.eye(4, dtype=np.float32).ravel()
+    inv = oracle.qrsolve(A, b, 4, 4)
+    assert np.array_equal(inv, ref_linear.qrsolve(A, b, 4, 4))
+    # closed form: the QR solve of A X = I inverts A
+    assert np.abs(A.reshape(4, 4).T @ inv.reshape(4, 4).T - np.eye(4)).max() < 1e-4
+    for seed, n in ((5, 400), (6, 1001), (7, 57)):
+        X = _affine_X(seed)
+        p1, p2 = _points(seed, n, X)
+        for rigid in (0, 1):
+            got = oracle.affine_robust(p1, p2, n, rigid)
+            assert np.array_equal(got, ref_linear.affine_robust(p1, p2, n, rigid)), (seed, rigid)
+        # noise-free points: the trimmed fit recovers the affine map (to float QR
+        # accuracy over coordinates up to 60; a transposed or dropped term is off by O(1))
+        q1, q2 = _points(seed, n, X, noise=0.0, outliers=0.0)
+        assert np.abs(oracle.affine_robust(q1, q2, n, 0) - np.asarray(X).reshape(4, 4).T.ravel()).max() < 1e-2
+
+
+def test_ref_rigid_recovers_rotation(oracle):
+    """Closed form: points rotated and translated exactly give that rigid map."""
+    rng = np.random.default_rng(12)
+    th = 0.1
+    R = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1]], np.float64)
+    p1 = np.ones((4, 300), np.float32)
+    p1[:3] = rng.uniform(0, 50, (3, 300))
+    p2 = p1.copy()
+    p2[:3] = (R @ p1[:3].astype(np.float64) + np.array([[1.5], [-2.0], [0.5]])).astype(np.float32)
+    RT = oracle.affine_robust(p1.ravel(), p2.ravel(), 300, 1).reshape(4, 4)
+    assert np.abs(RT[:3, :3] - R).max() < 1e-3
+
+
+def test_ref_estimate_affine(oracle, ref_linear):
+    f, mv = phantom.make_pair((48, 44, 40), 13)
+    im1, im2 = f.astype(np.float32) + 1024, mv.astype(np.float32) + 1024
+    F, M = oracle.descriptor(im1, 2), oracle.descriptor(im2, 2)
+    for step, hw, q in ((5, 3, 2), (4, 2, 1)):
+        c1 = oracle.datacost(F, M, step, hw, q, 1.0)
+        c2 = oracle.datacost(M, F, step, hw, q, 1.0)
+        Xprev = _affine_X(step, 0.01, 0.5)
+        for rigid in (0, 1):
+            got = oracle.estimate_affine(Xprev, im1, im2, c1, c2, step, q, hw, rigid)
+            want = ref_linear.estimate_affine(Xprev, im1, im2, c1, c2, step, q, hw, rigid)
+            for a, b in zip(got, want):
+                assert np.array_equal(a, b), (step, hw, q, rigid)
+
+
+def _cpp_float(x):
+    """C++ ostream default formatting of a float (6 significant digits, %g)."""
+    return "%g" % np.float32(x)
+
+
+def test_ref_linear_binary(oracle, tmp_path):
+    """orc_linear_register reproduces the reference linearBCV's _matrix.txt."""
+    import subprocess
+    from deedsbcv_amd import nifti
+    exe = Path(__file__).resolve().parents[1] / "oracle" / "_ref" / "linearBCV"
+    if not exe.exists():
+        pytest.skip("reference linearBCV not built")
+    f, mv = phantom.make_pair((64, 56, 48), 17)
+    nifti.write_nifti(tmp_path / "f.nii.gz", f)
+    nifti.write_nifti(tmp_path / "m.nii.gz", mv)
+    for rigid in (0, 1):
+        args = [str(exe), "-F", str(tmp_path / "f.nii.gz"), "-M", str(tmp_path / "m.nii.gz"), "-O",
+                str(tmp_path / f"o{rigid}"), "-G", "6x5x4", "-L", "3x2x2", "-Q", "2x2x1"]
+        if rigid:
+            args += ["-R", "1"]
+        r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout + r.stderr
+        X = oracle.linear_register(f.astype(np.float32), mv.astype(np.float32), (6, 5, 4), (3, 2, 2), (2, 2, 1),
+                                   rigid)
+        want = "".join("  ".join(_cpp_float(X[i + 4 * c]) for c in range(4)) + "\n" for i in range(4))
+        assert (tmp_path / f"o{rigid}_matrix.txt").read_text() == want
