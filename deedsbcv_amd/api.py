@@ -17,6 +17,7 @@ runs on the GPU through libdeeds_hip.so; there is no CPU path.
     warpAffineS           dataCostD.h:241
     register              deedsBCV0.cpp:55 (the full symmetric registration)
     dice                  diceMulti.cpp:17 (per-label Dice overlap)
+    linear_register       linearBCV.cpp:144 (affine / rigid pre-alignment)
 """
 from __future__ import annotations
 
@@ -218,6 +219,21 @@ def register_batch_device(pairs, device=0, grid=(8, 7, 6, 5, 4), radius=(8, 7, 6
     _lib.call("deeds_register_batch_device", int(device), arr, len(pairs), ctypes.byref(sched), int(concurrency),
               times)
     return [times[i].as_dict() for i in range(len(pairs))]
+
+
+def linear_register(fixed, moving, grid=(7, 6, 5, 4), radius=(5, 4, 3, 2), quant=(4, 3, 2, 1), rigid=False):
+    """linearBCV (linearBCV.cpp:144-263): the 16-float affine (or rigid) matrix, column-wise,
+    aligning moving to fixed (HU volumes), and the point pairs fitted per level."""
+    fixed, moving = _f32(fixed), _f32(moving)
+    if fixed.shape != moving.shape:
+        raise ValueError("Inconsistent image sizes (must have same dimensions)")
+    m, n, o = _dims(fixed)
+    sched = make_schedule(grid, radius, quant, 1.0)
+    X = np.zeros(16, np.float32)
+    used = np.zeros(len(grid), np.int32)
+    _lib.call("deeds_linear_register", _p(fixed), _p(moving), m, n, o, ctypes.byref(sched), int(bool(rigid)), _p(X),
+              _p(used))
+    return X, used
 
 
 def dice(seg1, seg2):

@@ -5,7 +5,7 @@
 
 Outputs:
     deedsbcv_amd/lib/libdeeds_hip.so      the C-ABI library (include/deeds_hip.h)
-    deedsbcv_amd/bin/deedsBCV, applyBCV, applyBCVfloat, diceMulti   drop-in command lines
+    deedsbcv_amd/bin/deedsBCV, applyBCV, applyBCVfloat, diceMulti, linearBCV   drop-in command lines
 Objects go to deedsbcv_amd/_build/.  Every translation unit is compiled with
 -ffp-contract=off (the reference's arithmetic has no fused multiply-adds) and
 without packed-FP32 instruction selection.
@@ -35,9 +35,9 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-p
 
 LIB_SOURCES = ["mind.hip", "datacost.hip", "mst.hip", "bp.hip", "warp.hip", "register.hip", "api.cpp",
                "mst_host.cpp", "stats.hip", "stats_host.cpp", "batch.cpp", "soft_features.hip", "soft_cost.hip",
-               "soft_bp.hip", "soft_api.cpp", "dice.hip"]
+               "soft_bp.hip", "soft_api.cpp", "dice.hip", "linear.hip", "linear_host.cpp"]
 TOOLS = {"deedsBCV": "tool_deeds.cpp", "applyBCV": "tool_apply.cpp", "applyBCVfloat": "tool_apply.cpp",
-         "diceMulti": "tool_dice.cpp"}
+         "diceMulti": "tool_dice.cpp", "linearBCV": "tool_linear.cpp"}
 
 
 def _run(cmd):

@@ -102,6 +102,9 @@ void check_schedule(const deeds_schedule* S, int m, int n, int o);
 // One symmetric registration on ctx, device pointers (register.hip).
 void register_pair(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, const float* X, int m, int n, int o,
                    const deeds_schedule& S, float* d_flow, float* d_warped, deeds_timing* timing);
+// linearBCV's multi-level affine / rigid estimation on ctx (linear_host.cpp).
+void linear_register(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, int m, int n, int o,
+                     const deeds_schedule& S, bool rigid, float* X_out, int* points_used);
 // Level-synchronous registration of `count` pairs (device pointers) on ctx;
 // timings is optional (count entries).  Pairs are validated by the caller.
 void register_group(deeds_ctx* ctx, const deeds_pair* pairs, int count, const deeds_schedule& S,

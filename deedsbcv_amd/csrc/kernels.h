@@ -116,4 +116,18 @@ void launch_ssd(double* acc, const float* ref, const float* warped, const float*
 void launch_jacobian(double* scratch, const float* u, const float* v, const float* w, int m, int n, int o,
                      int factor, hipStream_t s);
 
+// ---- affine pre-alignment (linear.hip) ------------------------------------
+// out = in - sub (the +1024 HU offset of linearBCV.cpp:146-154 with sub = -1024).
+void launch_lin_offset(float* out, const float* in, float sub, long long n, hipStream_t s);
+// affineLTS1.h:39-75: trilinear warp of `in` by the affine X (16 floats,
+// column-wise), zero where the neighbourhood leaves the volume.
+void launch_lin_warp(float* out, const float* in, const float* X, int m, int n, int o, hipStream_t s);
+// affineLTS1.h:122-148: the three in-place smoothing passes of one cost volume.
+void launch_lin_smooth(float* cost, int m1, int n1, int o1, int len3, hipStream_t s);
+// affineLTS1.h:150-190: per node the first minimum of c1 / c2 (masked to 1e20
+// for background and border nodes) into minval[0..N) / [N..2N), the label
+// index into minind.
+void launch_lin_nodes(const float* c1, const float* c2, const float* im1, const float* im2, int m, int n, int step,
+                      int m1, int n1, int o1, int len3, float* minval, int* minind, hipStream_t s);
+
 }  // namespace deeds

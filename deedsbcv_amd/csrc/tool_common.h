@@ -144,14 +144,15 @@ struct Args {
     int grid[10] = {8, 7, 6, 5, 4, 3, 2, 2, 2, 2};
     int radius[10] = {8, 7, 6, 5, 4, 3, 2, 1, 1, 1};
     int quant[10] = {5, 4, 3, 2, 1, 1, 1, 1, 1, 1};
+    int rigid = 0;
     int required = 0;
     bool ok = true;
 };
 
 // parseArguments.h:12-174 (same flags and AxBxC level strings; an unknown flag
-// is reported and ignored instead of being treated as -F).
-inline Args parse_args(int argc, char* const argv[]) {
-    Args a;
+// is reported and ignored instead of being treated as -F).  `a` holds the
+// program's defaults (linearBCV starts from 7x6x5x4 / 5x4x3x2 / 4x3x2x1).
+inline Args parse_args(int argc, char* const argv[], Args a = Args()) {
     int num = a.levels, num2 = a.levels, num3 = a.levels;
     bool s_set = false;
     const char* lv = "%dx%dx%dx%dx%dx%dx%dx%dx%dx%d";
@@ -183,7 +184,7 @@ inline Args parse_args(int argc, char* const argv[]) {
                 break;
             case 'S': if (need()) { a.seg = val; a.has_seg = true; } break;
             case 'A': if (need()) { a.affine = val; a.has_affine = true; } break;
-            case 'R': break;  // rigid flag: parsed by the reference, unused by deeds
+            case 'R': if (need()) a.rigid = atoi(val); break;  // linearBCV only (deedsBCV ignores it)
             case 'D': if (need()) { a.deformed = val; a.required++; } break;
             default: printf("Invalid option: %s use -h for help\n", argv[k]); break;
         }

@@ -195,8 +195,21 @@ int deeds_apply_float(float* out, const float* in, const float* X, const float* 
 int deeds_dice(const int16_t* seg1, const int16_t* seg2, long long voxels, int capacity, int16_t* labels,
                float* dice, int* count);
 
+/* linearBCV.cpp:144-263 (linearBCV's estimation): multi-level affine (or,
+ * with rigid != 0, rigid) alignment of moving to fixed (HU volumes; the +1024
+ * offset is applied inside) using sched's grid / radius / quant per level
+ * (alpha is not used: the data cost runs with alpha 1).  X receives the 16
+ * floats column-wise, as linearBCV writes them to _matrix.txt and deedsBCV
+ * reads them with -A.  points_used (optional, sched->levels entries) receives
+ * the number of point pairs fitted per level (the "# points used" figure). */
+int deeds_linear_register(const float* fixed, const float* moving, int m, int n, int o,
+                          const deeds_schedule* sched, int rigid, float* X, int* points_used);
+
 /* ---- device-pointer forms (asynchronous on the context stream) ---------- */
 int deeds_ctx_descriptor(deeds_ctx* ctx, uint64_t* d_mindq, const float* d_im, int m, int n, int o, int qs);
+/* linearBCV.cpp:144-263 on device volumes (synchronous): as deeds_linear_register. */
+int deeds_ctx_linear_register(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, int m, int n, int o,
+                              const deeds_schedule* sched, int rigid, float* X, int* points_used);
 /* diceMulti.cpp:17-79 on device volumes (synchronous): as deeds_dice. */
 int deeds_ctx_dice(deeds_ctx* ctx, const int16_t* d_seg1, const int16_t* d_seg2, long long voxels, int capacity,
                    int16_t* labels, float* dice, int* count);

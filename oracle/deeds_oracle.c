@@ -1069,8 +1069,8 @@ void orc_estimate_affine(float* X, const float* Xprev, const float* im1, const f
         if (mean1 < 5.0f) mv1[i] = 1e20f;
         if (mean2 < 5.0f) mv2[i] = 1e20f;
         if (z1 < 2 || z1 > o1 - 2 || x1 < 2 || x1 > n1 - 2 || y1 < 2 || y1 > m1 - 2) { mv1[i] = 1e20f; mv2[i] = 1e20f; }
-        if (mv1[i] < 1e19f) allcount++;
-        if (mv2[i] < 1e19f) allcount++;
+        if ((double)mv1[i] < 1e19) allcount++;
+        if ((double)mv2[i] < 1e19) allcount++;
     }
     /* quantile(values, sz1, 0.5*allcount/(float)sz1/2.0f), linearBCV.cpp:36-46 */
     float q = (float)(0.5 * allcount / (double)(float)sz1 / 2.0);
