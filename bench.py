@@ -243,6 +243,103 @@ def soft_bench(a):
     }))
 
 
+def linear_bench(a):
+    """linearBCV (linearBCV.cpp:144-263) on the 256x256x128 pair with its default
+    schedule (grid 7,6,5,4; radius 5,4,3,2; quantisation 4,3,2,1; affine), inputs in
+    HBM; the reference linearBCV binary on the same pair is the CPU baseline."""
+    import ctypes
+
+    import torch
+
+    from deedsbcv_amd import _lib, api
+    from deedsbcv_amd import phantom as ph
+    print("[bench] generating the 256x256x128 pair", file=sys.stderr, flush=True)
+    f, m, _ = workload("full")
+    o, n, mm = f.shape
+    grid, radius, quant = (7, 6, 5, 4), (5, 4, 3, 2), (4, 3, 2, 1)
+    dev = torch.device("cuda:0")
+    x = torch.from_numpy(f.astype(np.float32)).to(dev)
+    y = torch.from_numpy(m.astype(np.float32)).to(dev)
+    ctx = api.Context(0)
+    sched = _lib.make_schedule(grid, radius, quant, 1.0)
+    X = np.zeros(16, np.float32)
+    used = np.zeros(4, np.int32)
+    V = ctypes.c_void_p
+
+    def step():
+        _lib.ctx_call(ctx.h, "deeds_ctx_linear_register", V(x.data_ptr()), V(y.data_ptr()), mm, n, o,
+                      ctypes.byref(sched), 0, X.ctypes.data_as(V), used.ctypes.data_as(V))
+
+    t = _timed(step, a.steps, max(1, a.warmup))
+    cb = None
+    ref = ROOT / "oracle" / "_ref" / "linearBCV"
+    if not a.no_cpu_baseline and ref.exists():
+        from deedsbcv_amd import nifti
+        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        with tempfile.TemporaryDirectory() as d:
+            nifti.write_nifti(f"{d}/f.nii.gz", f)
+            nifti.write_nifti(f"{d}/m.nii.gz", m)
+            r = subprocess.run([str(ref), "-F", f"{d}/f.nii.gz", "-M", f"{d}/m.nii.gz", "-O", f"{d}/o"],
+                               capture_output=True, text=True, env=dict(os.environ, OMP_NUM_THREADS=str(threads)),
+                               timeout=a.cpu_budget)
+            mt = re.search(r"Total time: ([0-9.eE+-]+) sec", r.stdout)
+            if r.returncode == 0 and mt:
+                total = float(mt.group(1))
+                ref_X = (Path(d) / "o_matrix.txt").read_text()
+                ours = "".join("  ".join("%g" % X[i + 4 * c] for c in range(4)) + "\n" for i in range(4))
+                cb = {"value": 1.0 / total, "unit": "alignments/s", "cores": threads, "kind": "reference",
+                      "sample": "one affine alignment of the same pair by the unmodified reference linearBCV "
+                                "(its 'Total time' span, linearBCV.cpp:187-263)",
+                      "total_s": total, "same_matrix_text": ref_X == ours}
+    print(json.dumps({
+        "metric": "affine pre-alignments/s @256x256x128 (linearBCV, 4 levels)",
+        "value": 1.0 / t, "unit": "alignments/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": 1e3 * t, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64 Hamming / f32", "data": "synthetic (seeded CT-like phantom pair, B-spline warp)",
+        "config": {"workload": "linear: 256x256x128 pair, grid (7, 6, 5, 4), radius (5, 4, 3, 2), "
+                               "quant (4, 3, 2, 1), affine", "dims": [mm, n, o]},
+        "points_used_per_level": used.tolist(), "cpu_baseline": cb}))
+
+
+def dice_bench(a):
+    """diceMulti (diceMulti.cpp:17-79) on two 256x256x128 int16 label volumes with
+    14 labels, inputs in HBM: 6 bytes per voxel (presence pass over the first
+    volume, count pass over both) against the 8 TB/s HBM peak."""
+    import ctypes
+
+    import torch
+
+    from deedsbcv_amd import _lib, api
+    rng = np.random.default_rng(5)
+    shape = (128, 256, 256)
+    a_lab = np.zeros(shape, np.int16)
+    for lab in range(1, 15):
+        lo = [rng.integers(0, s // 2) for s in shape]
+        a_lab[lo[0]:lo[0] + shape[0] // 3, lo[1]:lo[1] + shape[1] // 3, lo[2]:lo[2] + shape[2] // 3] = lab
+    b_lab = np.roll(a_lab, 3, axis=2)
+    dev = torch.device("cuda:0")
+    A = torch.from_numpy(a_lab).to(dev)
+    B = torch.from_numpy(b_lab).to(dev)
+    ctx = api.Context(0)
+    cnt = ctypes.c_int()
+    V = ctypes.c_void_p
+
+    def step():
+        _lib.ctx_call(ctx.h, "deeds_ctx_dice", V(A.data_ptr()), V(B.data_ptr()), a_lab.size, 0, None, None,
+                      ctypes.byref(cnt))
+
+    t = _timed(step, a.steps, max(1, a.warmup))
+    gbs = 6.0 * a_lab.size / t / 1e9
+    print(json.dumps({
+        "metric": "Dice evaluations/s @256x256x128 int16 (14 labels)", "value": 1.0 / t, "unit": "evaluations/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * t, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int16", "data": "synthetic label volumes",
+        "config": {"workload": "dice: 256x256x128 int16, 14 labels"}, "labels": cnt.value,
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": 8000.0, "unit": "GB/s", "frac": gbs / 8000.0,
+                     "traffic": None, "note": "whole call (two kernels, two small host round trips)"},
+        "cpu_baseline": None}))
+
+
 def cpu_baseline(f, m, sched, threads, budget_s):
     """The reference's own CPU path on the host cores, on the same pair."""
     ref = ROOT / "oracle" / "_ref" / "deedsBCV"
@@ -285,7 +382,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="full", choices=["full", "small", "crossmodal", "atlas", "soft"])
+    ap.add_argument("--workload", default="full",
+                    choices=["full", "small", "crossmodal", "atlas", "soft", "linear", "dice"])
     ap.add_argument("--concurrency", type=int, default=16,
                     help="pairs per level-synchronous group for --workload atlas")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -304,6 +402,10 @@ def main():
         return atlas_bench(a)
     if a.workload == "soft":
         return soft_bench(a)
+    if a.workload == "linear":
+        return linear_bench(a)
+    if a.workload == "dice":
+        return dice_bench(a)
 
     import torch
 
