@@ -2,9 +2,11 @@
 //
 // cost[node*len3 + l] = scale * sum_{block samples s} popcount(F[s] ^ M[s + d(l)])
 // with the moving codes clamp-padded by quant*hw (:126-138) so the inner loop
-// needs no bounds logic.  One workgroup per control point; its block of fixed
-// codes is wave-uniform (scalar loads), each lane owns labels l, l+256, ...
-// and accumulates an exact integer sum with v_xor + v_bcnt (two 32-bit halves).
+// needs no bounds logic.  Two forms: per control point (a workgroup per node,
+// its fixed codes wave-uniform, each lane owning labels l, l+256, ...), and
+// row-tiled (a workgroup per row of nodes and chunk of label planes, fixed and
+// moving sample rows staged in LDS), chosen by the launcher.  Both accumulate
+// exact integer sums with v_xor + v_bcnt (two 32-bit halves).
 #include "kernels.h"
 
 #include <cmath>
@@ -101,6 +103,92 @@ __global__ __launch_bounds__(256) void k_datacost(float* __restrict__ cost, cons
     }
 }
 
+// Row-tiled form for full sampling along i (skip_i == 1, every schedule of
+// deedsBCV).  A workgroup owns one row of nodes along i -- fixed (x, z), all
+// m1 nodes -- and a chunk of label planes (lx, lz).  The fixed codes of the
+// row's R = ceil(STEP/SJ) * ceil(STEP/SK) sample rows are staged in LDS once;
+// for every plane the matching moving rows (the same sample rows displaced by
+// (lx*q, lz*q), over the i-range the row's blocks reach under every ys) are
+// staged with coalesced loads.  Thread t owns position t of the row: it keeps
+// the fixed code in a register and accumulates popcount(F ^ M[t + ys*q]) for
+// every ys over the R rows, so a wave reads consecutive LDS words (no bank
+// conflicts for any q) and one fixed read serves all ys; with the label count
+// and quantisation compile-time, every read has an immediate offset and an
+// evaluation is xor, xor, bcnt, bcnt.  The per-position sums are then added
+// over each node's STEP positions through LDS (reusing the moving-row buffer).
+// Sums are exact integers: the result equals the per-node kernels'.
+template <int STEP, int SJ, int SK, int LEN, int Q>
+__global__ __launch_bounds__(1024) void k_datacost_rows(float* __restrict__ cost, const uint64_t* __restrict__ F,
+                                                        const uint64_t* __restrict__ Mp, int m, int n, int m1,
+                                                        int n1, float scale, int mp, int np, int chunk_planes,
+                                                        int chunks, int work) {
+    constexpr int RJ = (STEP + SJ - 1) / SJ, RK = (STEP + SK - 1) / SK, R = RJ * RK;
+    constexpr int len = LEN, q = Q;
+    extern __shared__ uint64_t lds_rows[];
+    // contiguous work ranges per XCD (blocks are dispatched round-robin over the 8 XCDs),
+    // so the planes of one row and the rows next to it share an L2
+    const int per = (work + 7) / 8;
+    const int b = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (b >= work) return;
+    const int row = b / chunks, chunk = b - row * chunks;
+    const int x = row % n1, z = row / n1;
+    const int W = m1 * STEP, WM = W + (len - 1) * q;
+    uint64_t* sF = lds_rows;
+    uint64_t* sM = lds_rows + R * W;              // moving rows, then the per-position sums
+    unsigned* sD = reinterpret_cast<unsigned*>(sM);
+    const long long mn = (long long)m * n, mpnp = (long long)mp * np;
+    const int t = threadIdx.x, T = blockDim.x;  // T >= W, and WM < W + T
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const uint64_t* src = F + (long long)(x * STEP + (r % RJ) * SJ) * m + (long long)(z * STEP + (r / RJ) * SK) * mn;
+        if (t < W) sF[r * W + t] = src[t];
+    }
+    const int len2 = len * len, len3 = len2 * len;
+    const int p0 = chunk * chunk_planes, p1 = min(len2, p0 + chunk_planes);
+    const int pairs = m1 * len;
+    for (int p = p0; p < p1; p++) {
+        const int lx = p % len, lz = p / len;
+        __syncthreads();  // the previous plane's sums have been read
+        const uint64_t* mbase = Mp + (long long)(x * STEP + lx * q) * mp + (long long)(z * STEP + lz * q) * mpnp;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const uint64_t* src = mbase + (long long)((r % RJ) * SJ) * mp + (long long)((r / RJ) * SK) * mpnp;
+            const uint64_t a = t < WM ? src[t] : 0;
+            const uint64_t b2 = t + T < WM ? src[t + T] : 0;
+            if (t < WM) sM[r * WM + t] = a;
+            if (t + T < WM) sM[r * WM + t + T] = b2;
+        }
+        __syncthreads();
+        unsigned acc[LEN];
+#pragma unroll
+        for (int ys = 0; ys < LEN; ys++) acc[ys] = 0;
+        if (t < W) {
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const uint64_t f = sF[r * W + t];
+                const uint64_t* mr = sM + r * WM + t;
+#pragma unroll
+                for (int ys = 0; ys < LEN; ys++) acc[ys] += __builtin_popcountll(f ^ mr[ys * Q]);
+            }
+        }
+        __syncthreads();  // every moving row has been read: the buffer takes the sums
+        if (t < W) {
+#pragma unroll
+            for (int ys = 0; ys < LEN; ys++) sD[t * LEN + ys] = acc[ys];
+        }
+        __syncthreads();
+        for (int pr = t; pr < pairs; pr += blockDim.x) {
+            const int yi = pr / len, ys = pr - yi * len;
+            const unsigned* d = sD + yi * STEP * LEN + ys;  // position-major: lanes read consecutive words
+            unsigned sum = 0;
+#pragma unroll
+            for (int i = 0; i < STEP; i++) sum += d[i * LEN];
+            const long long node = yi + (long long)x * m1 + (long long)z * m1 * n1;
+            cost[node * len3 + (long long)p * len + ys] = (float)sum * scale;
+        }
+    }
+}
+
 }  // namespace
 
 CostPlan datacost_plan(int m, int n, int o, int step, int hw, float quant, float alpha, int randnum) {
@@ -139,6 +227,40 @@ void launch_datacost(float* cost, const uint64_t* F, const uint64_t* Mp, int m, 
     int len = 2 * hw + 1;
     const unsigned grid = (unsigned)(8 * ((nodes + 7) / 8));
     const int N = (int)nodes;
+    // row-tiled kernel when one thread per row position suffices and the row's
+    // fixed and moving sample rows fit in LDS
+    {
+        const int W = m1 * step, WM = W + (len - 1) * q;
+        const int RJ = (step + plan.skip_j - 1) / plan.skip_j, RK = (step + plan.skip_k - 1) / plan.skip_k;
+        const size_t moving = std::max((size_t)RJ * RK * WM * sizeof(uint64_t), (size_t)len * W * sizeof(unsigned));
+        const size_t smem = (size_t)RJ * RK * W * sizeof(uint64_t) + moving;
+        const int threads = std::min(1024, ((W + 63) / 64) * 64);
+        const int len2 = len * len;
+        const int chunk_planes = len;  // one lz slice of planes per workgroup
+        const int chunks = (len2 + chunk_planes - 1) / chunk_planes;
+        const long long work = (long long)n1 * o1 * chunks;
+        // (two staging loads per thread and row cover WM = W + (len-1)q when (len-1)q <= threads)
+        if (plan.skip_i == 1 && W <= 1024 && (len - 1) * q <= threads && smem <= 150 * 1024 &&
+            work < (1LL << 31) - 8) {
+            const unsigned blocks = (unsigned)(8 * ((work + 7) / 8));
+#define DEEDS_DCR(ST, SJ, SK, LEN, QQ)                                                                        \
+    if (step == ST && plan.skip_j == SJ && plan.skip_k == SK && len == LEN && q == QQ) {                      \
+        if (smem > 65536)                                                                                     \
+            DEEDS_HIP(hipFuncSetAttribute((const void*)k_datacost_rows<ST, SJ, SK, LEN, QQ>,                  \
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));            \
+        k_datacost_rows<ST, SJ, SK, LEN, QQ><<<blocks, threads, smem, s>>>(                                   \
+            cost, F, Mp, m, n, m1, n1, plan.scale, plan.mp, plan.np, chunk_planes, chunks, (int)work);         \
+        DEEDS_LAUNCH_CHECK();                                                                                 \
+        return;                                                                                               \
+    }
+            // the levels of deedsBCV's default schedule with quantisation >= 2 (grid 8..5,
+            // radius 8..5, quantisation 5..2).  At quantisation 1 the per-node kernel's
+            // label-major loads are already coalesced and it is the faster of the two.
+            DEEDS_DCR(8, 2, 2, 17, 5) DEEDS_DCR(7, 2, 2, 15, 4) DEEDS_DCR(6, 2, 2, 13, 3)
+            DEEDS_DCR(5, 2, 2, 11, 2)
+#undef DEEDS_DCR
+        }
+    }
 #define DEEDS_DC(ST, SI, SJ, SK)                                                                              \
     if (step == ST && plan.skip_i == SI && plan.skip_j == SJ && plan.skip_k == SK) {                        \
         k_datacost_t<ST, SI, SJ, SK><<<grid, 256, 0, s>>>(cost, F, Mp, m, n, m1, n1, N, len, q, plan.scale, \
