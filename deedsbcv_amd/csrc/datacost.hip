@@ -4,8 +4,8 @@
 // with the moving codes clamp-padded by quant*hw (:126-138) so the inner loop
 // needs no bounds logic.  Two forms: per control point (a workgroup per node,
 // its fixed codes wave-uniform, each lane owning labels l, l+256, ...), and
-// row-tiled (a workgroup per row of nodes and chunk of label planes, fixed and
-// moving sample rows staged in LDS), chosen by the launcher.  Both accumulate
+// row-tiled (a workgroup per row of nodes and chunk of label planes, moving
+// sample rows staged in LDS), chosen by the launcher.  Both accumulate
 // exact integer sums with v_xor + v_bcnt (two 32-bit halves).
 #include "kernels.h"
 
@@ -106,8 +106,9 @@ __global__ __launch_bounds__(256) void k_datacost(float* __restrict__ cost, cons
 // Row-tiled form for full sampling along i (skip_i == 1, every schedule of
 // deedsBCV).  A workgroup owns one row of nodes along i -- fixed (x, z), all
 // m1 nodes -- and a chunk of label planes (lx, lz).  The fixed codes of the
-// row's R = ceil(STEP/SJ) * ceil(STEP/SK) sample rows are staged in LDS once;
-// for every plane the matching moving rows (the same sample rows displaced by
+// row's R = ceil(STEP/SJ) * ceil(STEP/SK) sample rows are loaded once (each
+// thread keeps its position's R codes in registers); for every plane the
+// matching moving rows (the same sample rows displaced by
 // (lx*q, lz*q), over the i-range the row's blocks reach under every ys) are
 // staged with coalesced loads.  Thread t owns position t of the row: it keeps
 // the fixed code in a register and accumulates popcount(F ^ M[t + ys*q]) for
@@ -133,15 +134,16 @@ __global__ __launch_bounds__(1024) void k_datacost_rows(float* __restrict__ cost
     const int row = b / chunks, chunk = b - row * chunks;
     const int x = row % n1, z = row / n1;
     const int W = m1 * STEP, WM = W + (len - 1) * q;
-    uint64_t* sF = lds_rows;
-    uint64_t* sM = lds_rows + R * W;              // moving rows, then the per-position sums
+    uint64_t* sM = lds_rows;  // moving rows, then the per-position sums
     unsigned* sD = reinterpret_cast<unsigned*>(sM);
     const long long mn = (long long)m * n, mpnp = (long long)mp * np;
     const int t = threadIdx.x, T = blockDim.x;  // T >= W, and WM < W + T
+    // this thread's fixed codes of the R sample rows stay in registers for every plane
+    uint64_t fr[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const uint64_t* src = F + (long long)(x * STEP + (r % RJ) * SJ) * m + (long long)(z * STEP + (r / RJ) * SK) * mn;
-        if (t < W) sF[r * W + t] = src[t];
+        fr[r] = t < W ? src[t] : 0;
     }
     const int len2 = len * len, len3 = len2 * len;
     const int p0 = chunk * chunk_planes, p1 = min(len2, p0 + chunk_planes);
@@ -165,7 +167,7 @@ __global__ __launch_bounds__(1024) void k_datacost_rows(float* __restrict__ cost
         if (t < W) {
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                const uint64_t f = sF[r * W + t];
+                const uint64_t f = fr[r];
                 const uint64_t* mr = sM + r * WM + t;
 #pragma unroll
                 for (int ys = 0; ys < LEN; ys++) acc[ys] += __builtin_popcountll(f ^ mr[ys * Q]);
@@ -228,12 +230,11 @@ void launch_datacost(float* cost, const uint64_t* F, const uint64_t* Mp, int m, 
     const unsigned grid = (unsigned)(8 * ((nodes + 7) / 8));
     const int N = (int)nodes;
     // row-tiled kernel when one thread per row position suffices and the row's
-    // fixed and moving sample rows fit in LDS
+    // moving sample rows fit in LDS
     {
         const int W = m1 * step, WM = W + (len - 1) * q;
         const int RJ = (step + plan.skip_j - 1) / plan.skip_j, RK = (step + plan.skip_k - 1) / plan.skip_k;
-        const size_t moving = std::max((size_t)RJ * RK * WM * sizeof(uint64_t), (size_t)len * W * sizeof(unsigned));
-        const size_t smem = (size_t)RJ * RK * W * sizeof(uint64_t) + moving;
+        const size_t smem = std::max((size_t)RJ * RK * WM * sizeof(uint64_t), (size_t)len * W * sizeof(unsigned));
         const int threads = std::min(1024, ((W + 63) / 64) * 64);
         const int len2 = len * len;
         const int chunk_planes = len;  // one lz slice of planes per workgroup
