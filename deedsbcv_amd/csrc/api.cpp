@@ -306,7 +306,8 @@ BPDevTree stage_tree(const Tree& t, void* pinned, void* d_stage, hipStream_t s) 
 // Bottom-up rounds, then the top-down selection.
 void run_bp(const Tree& t, const BPDevTree& dt, float* d_cost, int16_t* d_inds, int* d_sel, const float* u0,
             const float* v0, const float* w0, float* u1, float* v1, float* w1, int hw, float quant, hipStream_t s) {
-    launch_bp_rounds(d_cost, d_inds, dt.off3, dt, t.N, t.round_start.data(), t.rounds, u0, v0, w0, hw, quant, s);
+    launch_bp_rounds(d_cost, d_inds, dt.off3, dt, t.N, t.round_start.data(), t.rounds, t.tail_round, u0, v0, w0, hw,
+                     quant, s);
     launch_bp_select(d_cost, d_inds, dt, t.N, t.maxd, u0, v0, w0, u1, v1, w1, d_sel, hw, quant, s);
 }
 
@@ -315,8 +316,9 @@ size_t batch_desc_bytes(int trees) { return ((size_t)trees * sizeof(BPBatchTree)
 }  // namespace
 
 size_t bp_batch_bytes(int trees, long long nodes) {
-    // leaves and chain heads are disjoint sets of nodes: at most N work items per tree
-    return batch_desc_bytes(trees) + (size_t)nodes * sizeof(int2);
+    // leaves and chain heads are disjoint sets of nodes: at most N work items
+    // per tree, plus two items per tree for its spine
+    return batch_desc_bytes(trees) + ((size_t)nodes + 2 * (size_t)trees) * sizeof(int2);
 }
 
 void run_bp_batch(const std::vector<BPBatchItem>& items, int hw, float quant, void* pinned, void* d_stage,
@@ -335,7 +337,7 @@ void run_bp_batch(const std::vector<BPBatchItem>& items, int hw, float quant, vo
         nodes += it.tree->N;
         max_n = std::max(max_n, it.tree->N);
         width = std::max(width, it.tree->max_width);
-        rounds = std::max(rounds, it.tree->rounds);
+        rounds = std::max(rounds, it.tree->tail_round);  // the rounds launched one by one
     }
     if (bp_batch_bytes(T, nodes) > bytes) fail(DEEDS_ERR_INTERNAL, "BP batch larger than its staging area");
     const size_t desc = batch_desc_bytes(T);
@@ -357,13 +359,21 @@ void run_bp_batch(const std::vector<BPBatchItem>& items, int hw, float quant, vo
     for (int r = 0; r < rounds; r++) {
         for (int t = 0; t < T; t++) {
             const Tree& tr = *items[t].tree;
-            if (r >= tr.rounds) continue;
+            if (r >= tr.tail_round) continue;
             for (int h = tr.round_start[r]; h < tr.round_start[r + 1]; h++)
                 hw2[at++] = make_int2(t, tr.prog_start[h]);
         }
         round_off.push_back((int)at);
     }
-    if (at > nodes) fail(DEEDS_ERR_INTERNAL, "BP work list exceeds the node count");
+    // spines: ((tree, first head), (end head, 0)) per tree with one
+    for (int t = 0; t < T; t++) {
+        const Tree& tr = *items[t].tree;
+        if (tr.tail_round >= tr.rounds) continue;
+        hw2[at++] = make_int2(t, tr.round_start[tr.tail_round]);
+        hw2[at++] = make_int2(tr.round_start[tr.rounds], 0);
+    }
+    round_off.push_back((int)at);
+    if (at > nodes + 2LL * T) fail(DEEDS_ERR_INTERNAL, "BP work list exceeds its staging bound");
     DEEDS_HIP(hipMemcpyAsync(d_stage, pinned, desc + (size_t)at * sizeof(int2), hipMemcpyHostToDevice, s));
     launch_bp_batch(static_cast<const BPBatchTree*>(d_stage), T, max_n,
                     reinterpret_cast<const int2*>(static_cast<const char*>(d_stage) + desc), round_off, width, hw,

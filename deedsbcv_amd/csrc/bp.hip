@@ -302,6 +302,19 @@ __global__ __launch_bounds__(BPShape<LEN>::BLOCK) void k_bp_prog(BPArgs a, int h
     prog_body<LEN>(a, a.prog_start[head_begin + blockIdx.x], smem, red);
 }
 
+// The spine: heads [head_begin, head_end), one per round, run in order by one
+// workgroup (each program reads messages the previous ones stored: the
+// barrier makes them visible inside the workgroup).
+template <int LEN>
+__global__ __launch_bounds__(BPShape<LEN>::BLOCK) void k_bp_tail(BPArgs a, int head_begin, int head_end) {
+    extern __shared__ float smem[];
+    __shared__ float red[16];
+    for (int h = head_begin; h < head_end; h++) {
+        prog_body<LEN>(a, a.prog_start[h], smem, red);
+        __syncthreads();
+    }
+}
+
 __device__ __forceinline__ BPArgs args_of(const BPBatchTree& T) {
     return BPArgs{T.cost, T.allinds, T.prog_start, T.prog, T.leaves, T.leaf_w, T.off3, T.root};
 }
@@ -326,40 +339,62 @@ __global__ __launch_bounds__(BPShape<LEN>::BLOCK) void k_bp_prog_b(const BPBatch
     prog_body<LEN>(args_of(trees[w.x]), w.y, smem, red);
 }
 
+// Spines of several trees: workgroup b runs tree tails[2b].x's programs at the
+// offsets prog[tails[2b].y .. tails[2b+1].x) of its head list, in order.
+template <int LEN>
+__global__ __launch_bounds__(BPShape<LEN>::BLOCK) void k_bp_tail_b(const BPBatchTree* __restrict__ trees,
+                                                                   const int2* __restrict__ tails) {
+    extern __shared__ float smem[];
+    __shared__ float red[16];
+    const int2 a0 = tails[2 * blockIdx.x], a1 = tails[2 * blockIdx.x + 1];
+    const BPArgs a = args_of(trees[a0.x]);
+    for (int h = a0.y; h < a1.x; h++) {
+        prog_body<LEN>(a, a.prog_start[h], smem, red);
+        __syncthreads();
+    }
+}
+
 template <int LEN>
 void set_round_attrs() {
     constexpr int smem = (int)BPShape<LEN>::SMEM;
     static bool attr_set = false;
     if (attr_set) return;
-    const void* fns[4] = {(const void*)k_bp_prog<LEN>, (const void*)k_bp_leaves<LEN>, (const void*)k_bp_prog_b<LEN>,
-                          (const void*)k_bp_leaves_b<LEN>};
+    const void* fns[6] = {(const void*)k_bp_prog<LEN>,   (const void*)k_bp_leaves<LEN>,
+                          (const void*)k_bp_prog_b<LEN>, (const void*)k_bp_leaves_b<LEN>,
+                          (const void*)k_bp_tail<LEN>,   (const void*)k_bp_tail_b<LEN>};
     for (const void* f : fns) DEEDS_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, smem));
     attr_set = true;
 }
 
-// round < 0: the leaves; otherwise heads [head_begin, head_begin + count)
+// kind 0: the leaves; 1: heads [head_begin, head_begin + count), one
+// workgroup each; 2: the same heads run in order by one workgroup (the spine)
 template <int LEN>
-void launch_round_len(const BPArgs& a, int head_begin, int count, bool leaves, hipStream_t s) {
+void launch_round_len(const BPArgs& a, int head_begin, int count, int kind, hipStream_t s) {
     using B = BPShape<LEN>;
     constexpr size_t smem = B::SMEM;
     set_round_attrs<LEN>();
-    if (leaves)
+    if (kind == 0)
         k_bp_leaves<LEN><<<count, B::BLOCK, smem, s>>>(a);
-    else
+    else if (kind == 1)
         k_bp_prog<LEN><<<count, B::BLOCK, smem, s>>>(a, head_begin);
+    else
+        k_bp_tail<LEN><<<1, B::BLOCK, smem, s>>>(a, head_begin, head_begin + count);
     DEEDS_LAUNCH_CHECK();
 }
 
-// One batched round: `count` work items starting at `work`.
+// One batched round: `count` work items starting at `work` (kind 0: leaves,
+// 1: heads, 2: `count` tree spines described by pairs of items).
 template <int LEN>
-void launch_round_len_b(const BPBatchTree* trees, const int2* work, int count, bool leaves, hipStream_t s) {
+void launch_round_len_b(const BPBatchTree* trees, const int2* work, int count, int kind, hipStream_t s) {
     using B = BPShape<LEN>;
     constexpr size_t smem = B::SMEM;
     set_round_attrs<LEN>();
-    if (leaves)
+    if (kind == 0)
         k_bp_leaves_b<LEN><<<count, B::BLOCK, smem, s>>>(trees, work);
-    else
+    else if (kind == 1)
         k_bp_prog_b<LEN><<<count, B::BLOCK, smem, s>>>(trees, work);
+    else
+        k_bp_tail_b<LEN><<<count, B::BLOCK, smem, s>>>(trees, work);
     DEEDS_LAUNCH_CHECK();
 }
 
@@ -492,29 +527,37 @@ size_t select_smem(int width) {
 }  // namespace
 
 void launch_bp_rounds(float* cost, int16_t* allinds, float* off3, const BPDevTree& t, int N,
-                      const int* round_start, int rounds, const float* u0, const float* v0, const float* w0, int hw,
-                      float quant, hipStream_t s) {
+                      const int* round_start, int rounds, int tail_round, const float* u0, const float* v0,
+                      const float* w0, int hw, float quant, hipStream_t s) {
     if (N > 0) {
         k_parent_offsets<<<ceil_div(N, 256), 256, 0, s>>>(off3, t.parents, u0, v0, w0, quant, N);
         DEEDS_LAUNCH_CHECK();
     }
     BPArgs a{cost, allinds, t.prog_start, t.prog, t.leaves, t.leaf_w, off3, t.root};
-    for (int r = -1; r < rounds; r++) {
-        const bool leaves = r < 0;
-        const int b = leaves ? 0 : round_start[r], n = leaves ? t.n_leaves : round_start[r + 1] - b;
+    tail_round = std::max(0, std::min(tail_round, rounds));
+    // r = -1: the leaves; r < tail_round: one launch per round; then the spine in one launch
+    for (int r = -1; r <= tail_round; r++) {
+        int kind, b, n;
+        if (r < 0) {
+            kind = 0, b = 0, n = t.n_leaves;
+        } else if (r < tail_round) {
+            kind = 1, b = round_start[r], n = round_start[r + 1] - b;
+        } else {
+            kind = 2, b = round_start[tail_round], n = round_start[rounds] - b;
+        }
         if (n <= 0) continue;
         switch (2 * hw + 1) {
-            case 1: launch_round_len<1>(a, b, n, leaves, s); break;
-            case 3: launch_round_len<3>(a, b, n, leaves, s); break;
-            case 5: launch_round_len<5>(a, b, n, leaves, s); break;
-            case 7: launch_round_len<7>(a, b, n, leaves, s); break;
-            case 9: launch_round_len<9>(a, b, n, leaves, s); break;
-            case 11: launch_round_len<11>(a, b, n, leaves, s); break;
-            case 13: launch_round_len<13>(a, b, n, leaves, s); break;
-            case 15: launch_round_len<15>(a, b, n, leaves, s); break;
-            case 17: launch_round_len<17>(a, b, n, leaves, s); break;
-            case 19: launch_round_len<19>(a, b, n, leaves, s); break;
-            case 21: launch_round_len<21>(a, b, n, leaves, s); break;
+            case 1: launch_round_len<1>(a, b, n, kind, s); break;
+            case 3: launch_round_len<3>(a, b, n, kind, s); break;
+            case 5: launch_round_len<5>(a, b, n, kind, s); break;
+            case 7: launch_round_len<7>(a, b, n, kind, s); break;
+            case 9: launch_round_len<9>(a, b, n, kind, s); break;
+            case 11: launch_round_len<11>(a, b, n, kind, s); break;
+            case 13: launch_round_len<13>(a, b, n, kind, s); break;
+            case 15: launch_round_len<15>(a, b, n, kind, s); break;
+            case 17: launch_round_len<17>(a, b, n, kind, s); break;
+            case 19: launch_round_len<19>(a, b, n, kind, s); break;
+            case 21: launch_round_len<21>(a, b, n, kind, s); break;
             default: fail(DEEDS_ERR_ARG, "search radius outside [0, 10]");
         }
     }
@@ -542,21 +585,24 @@ void launch_bp_batch(const BPBatchTree* d_trees, int trees, int max_n, const int
         DEEDS_LAUNCH_CHECK();
     }
     for (size_t r = 0; r + 1 < round_off.size(); r++) {
-        const int b = round_off[r], n = round_off[r + 1] - b;
+        const int b = round_off[r];
+        // the last range holds the spines as pairs of items (one workgroup per pair)
+        const bool spines = r + 2 == round_off.size();
+        const int n = spines ? (round_off[r + 1] - b) / 2 : round_off[r + 1] - b;
         if (n <= 0) continue;
-        const bool leaves = r == 0;
+        const int kind = r == 0 ? 0 : spines ? 2 : 1;
         switch (2 * hw + 1) {
-            case 1: launch_round_len_b<1>(d_trees, work + b, n, leaves, s); break;
-            case 3: launch_round_len_b<3>(d_trees, work + b, n, leaves, s); break;
-            case 5: launch_round_len_b<5>(d_trees, work + b, n, leaves, s); break;
-            case 7: launch_round_len_b<7>(d_trees, work + b, n, leaves, s); break;
-            case 9: launch_round_len_b<9>(d_trees, work + b, n, leaves, s); break;
-            case 11: launch_round_len_b<11>(d_trees, work + b, n, leaves, s); break;
-            case 13: launch_round_len_b<13>(d_trees, work + b, n, leaves, s); break;
-            case 15: launch_round_len_b<15>(d_trees, work + b, n, leaves, s); break;
-            case 17: launch_round_len_b<17>(d_trees, work + b, n, leaves, s); break;
-            case 19: launch_round_len_b<19>(d_trees, work + b, n, leaves, s); break;
-            case 21: launch_round_len_b<21>(d_trees, work + b, n, leaves, s); break;
+            case 1: launch_round_len_b<1>(d_trees, work + b, n, kind, s); break;
+            case 3: launch_round_len_b<3>(d_trees, work + b, n, kind, s); break;
+            case 5: launch_round_len_b<5>(d_trees, work + b, n, kind, s); break;
+            case 7: launch_round_len_b<7>(d_trees, work + b, n, kind, s); break;
+            case 9: launch_round_len_b<9>(d_trees, work + b, n, kind, s); break;
+            case 11: launch_round_len_b<11>(d_trees, work + b, n, kind, s); break;
+            case 13: launch_round_len_b<13>(d_trees, work + b, n, kind, s); break;
+            case 15: launch_round_len_b<15>(d_trees, work + b, n, kind, s); break;
+            case 17: launch_round_len_b<17>(d_trees, work + b, n, kind, s); break;
+            case 19: launch_round_len_b<19>(d_trees, work + b, n, kind, s); break;
+            case 21: launch_round_len_b<21>(d_trees, work + b, n, kind, s); break;
             default: fail(DEEDS_ERR_ARG, "search radius outside [0, 10]");
         }
     }

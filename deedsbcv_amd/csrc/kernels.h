@@ -52,11 +52,13 @@ struct BPDevTree {
 };
 // Largest label radius the LDS-resident BP kernel supports (3 len^3 floats + 2 len^3 int16 in LDS).
 constexpr int kMaxBPRadius = 10;
-// Bottom-up pass: one launch per round; round r covers heads round_start[r] .. round_start[r+1]
-// (host array).  off3: 3*N floats of scratch for the parent offsets.
+// Bottom-up pass: one launch per round below tail_round (round r covers heads
+// round_start[r] .. round_start[r+1], host array), then the rounds from
+// tail_round on (one head each) in one single-workgroup launch.  off3: 3*N
+// floats of scratch for the parent offsets.
 void launch_bp_rounds(float* cost, int16_t* allinds, float* off3, const BPDevTree& t, int N,
-                      const int* round_start, int rounds, const float* u0, const float* v0, const float* w0, int hw,
-                      float quant, hipStream_t s);
+                      const int* round_start, int rounds, int tail_round, const float* u0, const float* v0,
+                      const float* w0, int hw, float quant, hipStream_t s);
 void launch_bp_select(const float* cost, const int16_t* allinds, const BPDevTree& t, int N, int maxd,
                       const float* u0, const float* v0, const float* w0, float* u1, float* v1, float* w1,
                       int* sel, int hw, float quant, hipStream_t s);
@@ -82,7 +84,9 @@ struct BPBatchTree {
 // the parent offsets of all trees, then one launch per round holding every
 // tree's items of that round -- work[round_off[0] .. round_off[1]) are the
 // leaves (tree, leaf position), each later range a round's (tree, offset of
-// the head's program in prog) -- then one selection workgroup per tree.
+// the head's program in prog), the last range the trees' spines as item pairs
+// ((tree, first head), (end head, 0)) run by one workgroup each -- then one
+// selection workgroup per tree.
 // d_trees: device array of `trees` descriptors; max_n: largest N; width:
 // largest depth width.
 void launch_bp_batch(const BPBatchTree* d_trees, int trees, int max_n, const int2* work,

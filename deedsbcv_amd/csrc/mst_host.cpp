@@ -317,6 +317,8 @@ void build_schedule(Tree& t, bool bp_programs) {
         if (is_head[v]) t.round_start[round[v]]++;  // count of round r stored at index r (r >= 1)
     // exclusive prefix: heads of round r (1-based) occupy [round_start[r-1], round_start[r])
     for (int r = 1; r <= t.rounds; r++) t.round_start[r] += t.round_start[r - 1];
+    t.tail_round = t.rounds;
+    while (t.tail_round > 0 && t.round_start[t.tail_round] - t.round_start[t.tail_round - 1] == 1) t.tail_round--;
     t.heads.assign(t.round_start[t.rounds], 0);
     std::vector<int> fill_r(t.round_start.begin(), t.round_start.end());
     for (int p = 0; p < N; p++) {

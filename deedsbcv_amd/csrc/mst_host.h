@@ -32,6 +32,9 @@ struct Tree {
     std::vector<int> heads;
     std::vector<int> round_start;  // rounds + 1 offsets into heads
     int rounds = 0;
+    // Rounds tail_round .. rounds-1 hold one head each (the tree's spine):
+    // one workgroup runs them in order in a single launch.
+    int tail_round = 0;
     // Round 0: every leaf messages its parent (leaf_w = edgemst bits).
     std::vector<int> leaves;
     std::vector<int> leaf_w;
