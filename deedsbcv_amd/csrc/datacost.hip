@@ -254,9 +254,8 @@ void launch_datacost(float* cost, const uint64_t* F, const uint64_t* Mp, int m, 
         DEEDS_LAUNCH_CHECK();                                                                                 \
         return;                                                                                               \
     }
-            // the levels of deedsBCV's default schedule with quantisation >= 2 (grid 8..5,
-            // radius 8..5, quantisation 5..2).  At quantisation 1 the per-node kernel's
-            // label-major loads are already coalesced and it is the faster of the two.
+            // the five levels of deedsBCV's default schedule (grid 8..4, radius 8..4,
+            // quantisation 5..1); other schedules take the per-node kernels below
             DEEDS_DCR(8, 2, 2, 17, 5) DEEDS_DCR(7, 2, 2, 15, 4) DEEDS_DCR(6, 2, 2, 13, 3)
             DEEDS_DCR(5, 2, 2, 11, 2) DEEDS_DCR(4, 1, 1, 9, 1)
 #undef DEEDS_DCR

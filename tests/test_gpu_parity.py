@@ -59,6 +59,21 @@ def test_cost_vs_oracle(oracle, case):
         f"cost {case}")
 
 
+# The five levels of the default schedule have dedicated row kernels; the
+# 256-wide shape gives rows as long as the 256x256x128 workload's (one wave
+# of position runs nearly full), the 45-wide one short rows.
+@pytest.mark.parametrize("case", [(8, 8, 5), (7, 7, 4), (6, 6, 3), (5, 5, 2), (4, 4, 1)])
+@pytest.mark.parametrize("shape", [(45, 41, 38), (256, 20, 18)])
+def test_cost_schedule_levels_vs_oracle(oracle, case, shape):
+    step, hw, q = case
+    print(f"case: datacost schedule level {case} {shape}", flush=True)
+    f, mv = phantom.make_pair(shape, 32)
+    F = oracle.descriptor(f.astype(np.float32) + 1024, 2)
+    M = oracle.descriptor(mv.astype(np.float32) + 1024, 2)
+    _eq(api.dataCostCL(F, M, step, hw, q, 1.6, 1), oracle.datacost(F, M, step, hw, q, 1.6, 1),
+        f"cost {case} {shape}")
+
+
 # ---------------------------------------------------------------- MST -------
 
 @pytest.mark.parametrize("case", [(5, 2, 2), (4, 2, 1), (6, 1, 3)])
