@@ -3,16 +3,18 @@
 // Pipeline per call (6 float channels in HBM):
 //   k_ssd_box_i   squared difference to the shifted image (:79-90, :113-117)
 //                 fused with the box pass along i (:2-29)
-//   k_box_cols    box pass along j, then along k (:31-73)
+//   k_box_lines   box pass along j, then along k (:31-73), box radius 1..3
+//                 (k_box_cols for larger radii)
 //   k_mind_code   12 SSC values -> min/noise normalisation -> 6-level
 //                 thermometer, packed into a 64-bit code (:172-222)
 // The box filter must reproduce the reference's float rounding: every line is
 // one sequential float prefix sum (the sums exceed 2^24, so a tree scan would
-// change bits) followed by the clamped window difference.  Each workgroup
-// stages TILE lines in LDS with coalesced loads, one thread per line runs the
-// sequential prefix in LDS (the loads are independent of the add chain, so
-// they stay in flight), and all threads write the window differences back
-// coalesced.
+// change bits) followed by the clamped window difference.  Along i (lines
+// contiguous in memory) a workgroup stages TILE lines in LDS with coalesced
+// loads, one thread per line runs the sequential prefix in LDS, and all
+// threads write the window differences back coalesced.  Along j and k the
+// lanes of a wave are consecutive i, so one thread per line streams its line
+// straight from HBM with coalesced loads and keeps the prefix in registers.
 #include "kernels.h"
 
 #include <cmath>
@@ -69,8 +71,12 @@ __global__ __launch_bounds__(BOX_THREADS) void k_ssd_box_i(float* __restrict__ o
     const int ld = m + 1;
     const long long mn = (long long)m * n;
     const int di = c_ssd_shift[ch][0] * qs, dj = c_ssd_shift[ch][1] * qs, dk = c_ssd_shift[ch][2] * qs;
-    for (int idx = threadIdx.x; idx < rows * m; idx += blockDim.x) {
-        const int r = idx / m, i = idx - r * m, j = j0 + r;
+    // (r, i) walk the tile row-major without a division per element
+    int r = (int)threadIdx.x / m, i = (int)threadIdx.x - r * m;
+    const int dr = BOX_THREADS / m, di_step = BOX_THREADS - dr * m;
+    for (; r < rows; r += dr, i += di_step) {
+        if (i >= m) { i -= m; r++; if (r >= rows) break; }
+        const int j = j0 + r;
         const long long p = i + (long long)j * m + k * mn;
         const float c = im[p];
         const int ii = i + di, jj = j + dj, kk = k + dk;
@@ -83,8 +89,10 @@ __global__ __launch_bounds__(BOX_THREADS) void k_ssd_box_i(float* __restrict__ o
     if ((int)threadIdx.x < rows) prefix_in_place(tile + threadIdx.x * ld, m, 1);
     __syncthreads();
     float* dst = out + (long long)ch * mn * o;
-    for (int idx = threadIdx.x; idx < rows * m; idx += blockDim.x) {
-        const int r = idx / m, i = idx - r * m;
+    r = (int)threadIdx.x / m;
+    i = (int)threadIdx.x - r * m;
+    for (; r < rows; r += dr, i += di_step) {
+        if (i >= m) { i -= m; r++; if (r >= rows) break; }
         dst[i + (long long)(j0 + r) * m + k * mn] = window(tile + r * ld, i, m, qs, 1);
     }
 }
@@ -114,18 +122,78 @@ __global__ __launch_bounds__(BOX_THREADS) void k_box_cols(float* __restrict__ ds
     }
 }
 
+// j pass (axis 1) or k pass (axis 2) with one thread per line, streaming:
+// lanes are consecutive i, so every load and store is coalesced, and the
+// line's prefix sums never leave registers -- a running float sum (the
+// reference's sequential order) and the last 2*HW + 2 prefix values, enough
+// for the clamped window difference of the output HW steps behind.
+template <int HW>
+__global__ __launch_bounds__(256) void k_box_lines(float* __restrict__ dst, const float* __restrict__ src, int m,
+                                                   int n, int o, int axis) {
+    const long long mn = (long long)m * n, sz = mn * o;
+    const int other_count = axis == 1 ? o : n;
+    const long long lines = (long long)m * other_count * 6;
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= lines) return;
+    const int i = (int)(id % m);
+    const long long rest = id / m;
+    const int other = (int)(rest % other_count), ch = (int)(rest / other_count);
+    const int L = axis == 1 ? n : o;
+    const long long stride = axis == 1 ? (long long)m : mn;
+    const long long base = ch * sz + i + (axis == 1 ? other * mn : (long long)other * m);
+    const float* sp = src + base;
+    float* dp = dst + base;
+    constexpr int H = 2 * HW + 2;
+    float hist[H];  // after step t: hist[q] = P[t - 2*HW - 1 + q]
+#pragma unroll
+    for (int q = 0; q < H; q++) hist[q] = 0.0f;
+    float acc = 0.0f;
+    constexpr int U = 8;
+    for (int tb = 0; tb < L; tb += U) {
+        float x[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) x[u] = tb + u < L ? sp[(tb + u) * stride] : 0.0f;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = tb + u;
+            if (t < L) {
+                acc = t == 0 ? x[u] : acc + x[u];
+#pragma unroll
+                for (int q = 0; q + 1 < H; q++) hist[q] = hist[q + 1];
+                hist[H - 1] = acc;
+                // output t - HW: P[min(t, L-1)] - P[t - 2HW - 1] (MINDSSCbox.h:17-27)
+                const int so = t - HW;
+                if (so >= 0) {
+                    float v = acc;
+                    if (so > HW) v = v - hist[0];
+                    dp[so * stride] = v;
+                }
+            }
+        }
+    }
+    // the last HW outputs clamp the window's upper end at L - 1
+#pragma unroll
+    for (int e = 0; e < HW; e++) {
+        const int so = L - HW + e;
+        float v = hist[H - 1];
+        if (so > HW) v = v - hist[e + 1];
+        dp[so * stride] = v;
+    }
+}
+
 struct Thr {
     float t[5];
 };
 
-__global__ void k_mind_code(uint64_t* __restrict__ out, const float* __restrict__ d, int m, int n, int o, int qs,
-                            Thr thr) {
-    long long sz = (long long)m * n * o;
-    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= sz) return;
-    int i = (int)(p % m);
-    int j = (int)((p / m) % n);
-    int k = (int)(p / ((long long)m * n));
+// Grid (ceil(m / 256), n, o): one thread per voxel, coordinates from the
+// launch shape (no divisions).
+__global__ __launch_bounds__(256) void k_mind_code(uint64_t* __restrict__ out, const float* __restrict__ d, int m,
+                                                   int n, int o, int qs, Thr thr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int j = blockIdx.y, k = blockIdx.z;
+    const long long sz = (long long)m * n * o;
+    const long long p = i + (long long)j * m + (long long)k * m * n;
     float v[12];
 #pragma unroll
     for (int l = 0; l < 12; l++) {
@@ -179,17 +247,33 @@ void launch_descriptor(uint64_t* out, const float* im, int m, int n, int o, int 
         DEEDS_HIP(hipFuncSetAttribute((const void*)k_box_cols, hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds));
         attr = true;
     }
-    long long sz = (long long)m * n * o;
     // i pass (with the SSD): im -> w0; j pass: w0 -> w1; k pass: w1 -> w0
     k_ssd_box_i<<<dim3(ceil_div(n, TILE), o, 6), BOX_THREADS, rows_smem(m), s>>>(w0, im, m, n, o, qs);
     DEEDS_LAUNCH_CHECK();
-    k_box_cols<<<dim3(ceil_div(m, TILE), o, 6), BOX_THREADS, cols_smem(n), s>>>(w1, w0, m, n, o, 1, qs);
-    DEEDS_LAUNCH_CHECK();
-    k_box_cols<<<dim3(ceil_div(m, TILE), n, 6), BOX_THREADS, cols_smem(o), s>>>(w0, w1, m, n, o, 2, qs);
-    DEEDS_LAUNCH_CHECK();
+    if (qs >= 1 && qs <= 3) {
+        const long long lines_j = (long long)m * o * 6, lines_k = (long long)m * n * 6;
+        const unsigned gj = (unsigned)ceil_div(lines_j, 256), gk = (unsigned)ceil_div(lines_k, 256);
+        if (qs == 1) {
+            k_box_lines<1><<<gj, 256, 0, s>>>(w1, w0, m, n, o, 1);
+            k_box_lines<1><<<gk, 256, 0, s>>>(w0, w1, m, n, o, 2);
+        } else if (qs == 2) {
+            k_box_lines<2><<<gj, 256, 0, s>>>(w1, w0, m, n, o, 1);
+            k_box_lines<2><<<gk, 256, 0, s>>>(w0, w1, m, n, o, 2);
+        } else {
+            k_box_lines<3><<<gj, 256, 0, s>>>(w1, w0, m, n, o, 1);
+            k_box_lines<3><<<gk, 256, 0, s>>>(w0, w1, m, n, o, 2);
+        }
+        DEEDS_LAUNCH_CHECK();
+    } else {
+        k_box_cols<<<dim3(ceil_div(m, TILE), o, 6), BOX_THREADS, cols_smem(n), s>>>(w1, w0, m, n, o, 1, qs);
+        DEEDS_LAUNCH_CHECK();
+        k_box_cols<<<dim3(ceil_div(m, TILE), n, 6), BOX_THREADS, cols_smem(o), s>>>(w0, w1, m, n, o, 2, qs);
+        DEEDS_LAUNCH_CHECK();
+    }
     Thr thr;
     mind_thresholds(thr.t);
-    k_mind_code<<<ceil_div(sz, 256), 256, 0, s>>>(out, w0, m, n, o, qs, thr);
+    const int ct = std::min(256, 64 * (int)ceil_div(m, 64));
+    k_mind_code<<<dim3(ceil_div(m, ct), n, o), ct, 0, s>>>(out, w0, m, n, o, qs, thr);
     DEEDS_LAUNCH_CHECK();
 }
 
