@@ -23,7 +23,8 @@ namespace deeds {
 
 namespace {
 
-constexpr int TILE = 32;      // lines per workgroup
+constexpr int TILE = 32;      // lines per workgroup (j, k passes of k_box_cols)
+constexpr int TILE_I = 16;    // lines per workgroup of the i pass
 constexpr int BOX_THREADS = 256;
 
 // shift of the six SSD channels in units of qs, as (di, dj, dk)
@@ -66,34 +67,44 @@ __device__ __forceinline__ float window(const float* P, int t, int L, int hw, in
 __global__ __launch_bounds__(BOX_THREADS) void k_ssd_box_i(float* __restrict__ out, const float* __restrict__ im,
                                                             int m, int n, int o, int qs) {
     extern __shared__ float tile[];
-    const int j0 = blockIdx.x * TILE, k = blockIdx.y, ch = blockIdx.z;
-    const int rows = min(TILE, n - j0);
+    const int j0 = blockIdx.x * TILE_I, k = blockIdx.y, ch = blockIdx.z;
+    const int rows = min(TILE_I, n - j0);
     const int ld = m + 1;
     const long long mn = (long long)m * n;
     const int di = c_ssd_shift[ch][0] * qs, dj = c_ssd_shift[ch][1] * qs, dk = c_ssd_shift[ch][2] * qs;
-    // (r, i) walk the tile row-major without a division per element
-    int r = (int)threadIdx.x / m, i = (int)threadIdx.x - r * m;
-    const int dr = BOX_THREADS / m, di_step = BOX_THREADS - dr * m;
-    for (; r < rows; r += dr, i += di_step) {
-        if (i >= m) { i -= m; r++; if (r >= rows) break; }
-        const int j = j0 + r;
-        const long long p = i + (long long)j * m + k * mn;
-        const float c = im[p];
-        const int ii = i + di, jj = j + dj, kk = k + dk;
-        const bool in = ii >= 0 && ii < m && jj >= 0 && jj < n && kk >= 0 && kk < o;
-        const float s = in ? im[ii + (long long)jj * m + kk * mn] : c;
-        const float d = s - c;
-        tile[r * ld + i] = d * d;
+    const int kk = k + dk;
+    const bool k_in = kk >= 0 && kk < o;
+    // thread = column i; the TILE_I rows' loads are unrolled so they are all in flight together
+    for (int ib = 0; ib < m; ib += BOX_THREADS) {
+        const int i = ib + (int)threadIdx.x, ii = i + di;
+        const bool i_ok = i < m, ii_in = ii >= 0 && ii < m;
+        float cv[TILE_I], sv[TILE_I];
+#pragma unroll
+        for (int r = 0; r < TILE_I; r++) {
+            const int j = j0 + r, jj = j + dj;
+            const bool ok = i_ok && r < rows;
+            const bool in = ok && ii_in && jj >= 0 && jj < n && k_in;
+            cv[r] = ok ? im[i + (long long)j * m + k * mn] : 0.0f;
+            sv[r] = in ? im[ii + (long long)jj * m + kk * mn] : cv[r];
+        }
+#pragma unroll
+        for (int r = 0; r < TILE_I; r++) {
+            if (i_ok && r < rows) {
+                const float d = sv[r] - cv[r];
+                tile[r * ld + i] = d * d;
+            }
+        }
     }
     __syncthreads();
     if ((int)threadIdx.x < rows) prefix_in_place(tile + threadIdx.x * ld, m, 1);
     __syncthreads();
     float* dst = out + (long long)ch * mn * o;
-    r = (int)threadIdx.x / m;
-    i = (int)threadIdx.x - r * m;
-    for (; r < rows; r += dr, i += di_step) {
-        if (i >= m) { i -= m; r++; if (r >= rows) break; }
-        dst[i + (long long)(j0 + r) * m + k * mn] = window(tile + r * ld, i, m, qs, 1);
+    for (int ib = 0; ib < m; ib += BOX_THREADS) {
+        const int i = ib + (int)threadIdx.x;
+        if (i >= m) break;
+#pragma unroll
+        for (int r = 0; r < TILE_I; r++)
+            if (r < rows) dst[i + (long long)(j0 + r) * m + k * mn] = window(tile + r * ld, i, m, qs, 1);
     }
 }
 
@@ -225,7 +236,7 @@ __global__ __launch_bounds__(256) void k_mind_code(uint64_t* __restrict__ out, c
     out[p] = code;
 }
 
-size_t rows_smem(int m) { return (size_t)TILE * (m + 1) * sizeof(float); }
+size_t rows_smem(int m) { return (size_t)TILE_I * (m + 1) * sizeof(float); }
 size_t cols_smem(int L) { return (size_t)L * (TILE + 1) * sizeof(float); }
 
 }  // namespace
@@ -248,7 +259,7 @@ void launch_descriptor(uint64_t* out, const float* im, int m, int n, int o, int 
         attr = true;
     }
     // i pass (with the SSD): im -> w0; j pass: w0 -> w1; k pass: w1 -> w0
-    k_ssd_box_i<<<dim3(ceil_div(n, TILE), o, 6), BOX_THREADS, rows_smem(m), s>>>(w0, im, m, n, o, qs);
+    k_ssd_box_i<<<dim3(ceil_div(n, TILE_I), o, 6), BOX_THREADS, rows_smem(m), s>>>(w0, im, m, n, o, qs);
     DEEDS_LAUNCH_CHECK();
     if (qs >= 1 && qs <= 3) {
         const long long lines_j = (long long)m * o * 6, lines_k = (long long)m * n * 6;
