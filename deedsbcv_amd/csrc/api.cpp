@@ -272,7 +272,7 @@ size_t tree_stage_bytes(int N) { return tree_ints_bound(N) * sizeof(int) + 3 * (
 // prog, leaves, leaf_w, sel_info) into the pinned staging area and enqueues its copy into the device
 // area d_stage of the same layout (tree_stage_bytes(N)).  The pinned memory
 // must stay untouched until the copy has executed on `s`.
-BPDevTree stage_tree(const Tree& t, void* pinned, void* d_stage, hipStream_t s) {
+BPDevTree pack_tree(const Tree& t, void* pinned, void* d_stage, size_t* bytes) {
     size_t ints = t.ordered.size() + t.parents.size() + t.depth_start.size() + t.prog_start.size() + t.prog.size() +
                   t.leaves.size() + t.leaf_w.size() + t.sel_info.size();
     size_t bound = tree_ints_bound(t.N);
@@ -299,7 +299,14 @@ BPDevTree stage_tree(const Tree& t, void* pinned, void* d_stage, hipStream_t s) 
     out.max_width = t.max_width;
     out.root = t.N > 0 ? t.ordered[0] : 0;
     out.off3 = reinterpret_cast<float*>(di + bound);
-    DEEDS_HIP(hipMemcpyAsync(di, hi, at * sizeof(int), hipMemcpyHostToDevice, s));
+    *bytes = at * sizeof(int);
+    return out;
+}
+
+BPDevTree stage_tree(const Tree& t, void* pinned, void* d_stage, hipStream_t s) {
+    size_t bytes = 0;
+    const BPDevTree out = pack_tree(t, pinned, d_stage, &bytes);
+    DEEDS_HIP(hipMemcpyAsync(d_stage, pinned, bytes, hipMemcpyHostToDevice, s));
     return out;
 }
 

@@ -11,8 +11,11 @@ namespace deeds {
 
 // Bytes of a tree's staging area (schedule ints, then 3*N floats of scratch).
 size_t tree_stage_bytes(int N);
-// Packs t's schedule into `pinned` and enqueues its copy into d_stage on s;
-// `pinned` must stay untouched until the copy has executed.
+// Packs t's schedule into `pinned` (host work only) and returns the device
+// view of it at d_stage; *bytes receives the size the copy must move.
+BPDevTree pack_tree(const Tree& t, void* pinned, void* d_stage, size_t* bytes);
+// pack_tree, then the copy into d_stage enqueued on s; `pinned` must stay
+// untouched until the copy has executed.
 BPDevTree stage_tree(const Tree& t, void* pinned, void* d_stage, hipStream_t s);
 // Bottom-up rounds, then the top-down selection, of one tree.
 void run_bp(const Tree& t, const BPDevTree& dt, float* d_cost, int16_t* d_inds, int* d_sel, const float* u0,

@@ -330,7 +330,11 @@ class PairRun {
             std::shared_future<float> sdf = sd_fut_[dir];
             double* spent = &build_time_[2 * l + dir];
             hipEvent_t sad_event = ev_sad_.e;
-            trees_[dir][l] = pool.submit([ec, c, sdf, spent, sad_event]() {
+            // the schedule is packed into this (level, direction)'s pinned slot by the
+            // building thread, so the launching thread only enqueues the copy
+            char* slot = h_stage_ + stage_off_[l] + dir * ((stage_off_[l + 1] - stage_off_[l]) / 2);
+            void* d_stage = d_stage_[dir];
+            trees_[dir][l] = pool.submit([ec, c, sdf, spent, sad_event, slot, d_stage]() {
                 DEEDS_HIP(hipEventSynchronize(sad_event));
                 auto tb = std::chrono::steady_clock::now();
                 auto check = std::async(std::launch::async, [ec, c, sdf]() {
@@ -344,8 +348,11 @@ class PairRun {
                     finish_tree(ec, sd, c.step, *t);
                 else
                     build_tree(ec, sd, c.step, c.m1, c.n1, c.o1, *t);
+                StagedTree st;
+                st.dt = pack_tree(*t, slot, d_stage, &st.bytes);
+                st.tree = std::move(t);
                 *spent = std::chrono::duration<double>(std::chrono::steady_clock::now() - tb).count();
-                return t;
+                return st;
             });
         }
     }
@@ -398,13 +405,15 @@ class PairRun {
         hipStream_t s = dir ? s1_ : s0_;
         const size_t half = (stage_off_[l + 1] - stage_off_[l]) / 2;
         auto tw = std::chrono::steady_clock::now();
-        tree_[dir] = trees_[dir][l].get();
+        StagedTree st = trees_[dir][l].get();
         const double waited = std::chrono::duration<double>(std::chrono::steady_clock::now() - tw).count();
         wait_time_ += waited;
         wait_level_[l] += waited;
         DEEDS_HIP(hipEventRecord(ev_pre_[2 * l + dir]->e, s));
         char* slot = h_stage_ + stage_off_[l] + dir * half;
-        const BPDevTree dt = stage_tree(*tree_[dir], slot, d_stage_[dir], s);
+        DEEDS_HIP(hipMemcpyAsync(d_stage_[dir], slot, st.bytes, hipMemcpyHostToDevice, s));
+        tree_[dir] = std::move(st.tree);
+        const BPDevTree dt = st.dt;
         float* u0 = g0_[dir];
         const size_t n1 = c.sz1;
         float* nf = newf_ + dir * 3 * n1;
@@ -539,7 +548,13 @@ class PairRun {
     std::vector<double> build_time_;
     double wait_time_ = 0.0;
     std::vector<double> wait_level_;
-    std::vector<std::future<std::unique_ptr<Tree>>> trees_[2];
+    // a level's tree with its schedule already packed into the pinned slot
+    struct StagedTree {
+        std::unique_ptr<Tree> tree;
+        BPDevTree dt{};
+        size_t bytes = 0;
+    };
+    std::vector<std::future<StagedTree>> trees_[2];
     std::unique_ptr<Tree> tree_[2];
 };
 
