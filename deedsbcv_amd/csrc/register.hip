@@ -261,6 +261,7 @@ class PairRun {
         for (int i = 0; i < 6 * L_; i++) ev_lvl_.emplace_back(new Event());
         for (int i = 0; i < 4 * L_; i++) ev_dc_.emplace_back(new Event());
         for (int i = 0; i < 2 * L_; i++) ev_pre_.emplace_back(new Event());
+        for (int i = 0; i < L_; i++) ev_sad_.emplace_back(new Event());
         build_time_.assign(2 * L_, 0.0);
         wait_level_.assign(L_, 0.0);
         trees_[0].resize(L_);
@@ -303,12 +304,16 @@ class PairRun {
         };
         sd_fut_[0] = std::async(std::launch::async, stats_task, 0).share();
         sd_fut_[1] = std::async(std::launch::async, stats_task, 1).share();
-        for (int dir = 0; dir < 2; dir++)
-            for (int l = 0; l < L_; l++)
-                launch_edge_sad(d_ec_ + dir * ec_total_ + ec_off_[l], dir ? warped0_ : im1b_, lv_[l].step, in_.m,
-                                in_.n, in_.o, s0_);
-        d2h(h_ec_, d_ec_, 2 * ec_total_, s0_);
-        DEEDS_HIP(hipEventRecord(ev_sad_.e, s0_));
+        // finest level first: its trees take longest to build, so they start first
+        for (int l = L_ - 1; l >= 0; l--) {
+            const size_t cnt = 6 * (size_t)lv_[l].sz1;
+            for (int dir = 0; dir < 2; dir++) {
+                float* d = d_ec_ + dir * ec_total_ + ec_off_[l];
+                launch_edge_sad(d, dir ? warped0_ : im1b_, lv_[l].step, in_.m, in_.n, in_.o, s0_);
+                d2h(h_ec_ + dir * ec_total_ + ec_off_[l], d, cnt, s0_);
+            }
+            DEEDS_HIP(hipEventRecord(ev_sad_[l]->e, s0_));
+        }
         // fields start at zero on the level-0 grid
         m2_ = in_.m / S_.grid[0];
         n2_ = in_.n / S_.grid[0];
@@ -329,7 +334,7 @@ class PairRun {
             const LevelCfg c = lv_[l];
             std::shared_future<float> sdf = sd_fut_[dir];
             double* spent = &build_time_[2 * l + dir];
-            hipEvent_t sad_event = ev_sad_.e;
+            hipEvent_t sad_event = ev_sad_[l]->e;
             // the schedule is packed into this (level, direction)'s pinned slot by the
             // building thread, so the launching thread only enqueues the copy
             char* slot = h_stage_ + stage_off_[l] + dir * ((stage_off_[l + 1] - stage_off_[l]) / 2);
@@ -541,7 +546,8 @@ class PairRun {
     int m2_ = 0, n2_ = 0, o2_ = 0;
     size_t sz2_ = 0;
     double hamming_ = 0.0;
-    Event ev_start_, ev_stop_, ev_init_, ev_mind_, ev_w0_, ev_sad_;
+    Event ev_start_, ev_stop_, ev_init_, ev_mind_, ev_w0_;
+    std::vector<std::unique_ptr<Event>> ev_sad_;  // per level: its edge SADs are on the host
     std::vector<std::unique_ptr<Event>> ev_lvl_, ev_dc_, ev_pre_;
     double stats_time_[2] = {0.0, 0.0};
     std::shared_future<float> sd_fut_[2];
