@@ -23,66 +23,80 @@ namespace {
 // containers the reference's Prim's uses, ordered by the reference's
 // Edge::operator< (primsMST.h:24-26: a < b iff a.weight > b.weight, so the
 // top is the lightest edge).  Ties among equal weights resolve exactly as
-// in the reference only because the sequence of sifts is the same.  An entry
-// is 8 bytes, the float key (the reference's double weight holds a float, so
-// comparisons are identical) and the edge coded as from*8 + neighbour slot,
-// which keeps the sifts in cache for grids of 10^5 nodes.
+// in the reference only because the sequence of sifts is the same, and that
+// sequence depends only on how keys compare: any key type whose order and
+// ties are those of the reference's weights gives the same tree.  Keys and
+// edge codes (from*8 + neighbour slot) are separate arrays, so the hole walk
+// of a pop, which compares keys only, touches as little memory as possible:
+// float keys, or 16-bit order-preserving ranks of them (about a third faster
+// on grids of 10^5 nodes, whose heap then stays in L2).
+template <typename Key>
 class EdgeHeap {
    public:
-    struct Entry {
-        float key;
-        uint32_t code;
-    };
-    explicit EdgeHeap(size_t reserve) { h_.reserve(reserve); }
-    bool empty() const { return h_.empty(); }
-    void push(float w, uint32_t code) {
-        h_.push_back(Entry{w, code});
-        sift_up((long)h_.size() - 1, 0, h_.back());
+    explicit EdgeHeap(size_t reserve) : k_(reserve + 16), c_(reserve + 16) {}
+    bool empty() const { return n_ == 0; }
+    void push(Key w, uint32_t code) {
+        if (n_ == k_.size()) {
+            k_.resize(2 * n_);
+            c_.resize(2 * n_);
+        }
+        sift_up((long)n_++, w, code);
     }
     // removes the top (lightest) edge and returns its code
     uint32_t pop() {
-        const uint32_t top = h_[0].code;
-        const long n = (long)h_.size() - 1;
-        const Entry v = h_[n];
-        h_[n] = h_[0];
+        Key* K = k_.data();
+        uint32_t* C = c_.data();
+        const uint32_t top = C[0];
+        const long n = (long)n_ - 1;
+        const Key vk = K[n];
+        const uint32_t vc = C[n];
+        K[n] = K[0];
+        C[n] = C[0];
         // __adjust_heap(first, 0, n, value): the hole walks down along the
         // child that is not "less" (comp(child, child-1) <=> child.w > (child-1).w)
-        Entry* h = h_.data();
         long hole = 0, child = 0;
         const long lim = (n - 1) / 2;
         while (child < lim) {
             child = 2 * (child + 1);
-            // the children pairs two and three levels below (one line each)
-            __builtin_prefetch(h + 4 * child - 1);
-            __builtin_prefetch(h + 8 * child - 9);
-            const float a = h[child].key, b = h[child - 1].key;
+            // the children pairs two, three and four levels below
+            __builtin_prefetch(K + 4 * child - 1);
+            __builtin_prefetch(K + 8 * child - 9);
+            __builtin_prefetch(K + 16 * child - 25);
+            const Key a = K[child], b = K[child - 1];
             child -= (long)(a > b);  // branch-free: the hole takes the child that is not "less"
-            h[hole] = h[child];
+            K[hole] = K[child];
+            C[hole] = C[child];
             hole = child;
         }
         if ((n & 1) == 0 && child == (n - 2) / 2) {
             child = 2 * (child + 1);
-            h[hole] = h[child - 1];
+            K[hole] = K[child - 1];
+            C[hole] = C[child - 1];
             hole = child - 1;
         }
-        sift_up(hole, 0, v);
-        h_.pop_back();
+        sift_up(hole, vk, vc);
+        n_--;
         return top;
     }
 
    private:
-    // __push_heap(first, hole, top, value): move parents down while comp(parent, value)
-    void sift_up(long hole, long top, Entry v) {
-        Entry* h = h_.data();
+    // __push_heap(first, hole, 0, value): move parents down while comp(parent, value)
+    void sift_up(long hole, Key w, uint32_t code) {
+        Key* K = k_.data();
+        uint32_t* C = c_.data();
         long parent = (hole - 1) / 2;
-        while (hole > top && h[parent].key > v.key) {
-            h[hole] = h[parent];
+        while (hole > 0 && K[parent] > w) {
+            K[hole] = K[parent];
+            C[hole] = C[parent];
             hole = parent;
             parent = (hole - 1) / 2;
         }
-        h[hole] = v;
+        K[hole] = w;
+        C[hole] = code;
     }
-    std::vector<Entry> h_;
+    std::vector<Key> k_;
+    std::vector<uint32_t> c_;
+    size_t n_ = 0;
 };
 
 const int kNbI[6] = {0, 0, -1, 1, 0, 0};
@@ -107,7 +121,41 @@ inline float edge_weight(float c, float two_sd) { return -std::exp(-c / two_sd);
 // Prim's with the given keys (smaller key = lighter edge), then the BFS
 // order; records the neighbour slot of every node's edge to its parent with
 // the reference's last-match scan (primsMST.h:198-211).
-void prim_tree(const float* key, int m1, int n1, int o1, Tree& t);
+template <typename Key>
+void prim_tree(const Key* key, int m1, int n1, int o1, Tree& t);
+
+// 16-bit keys with the order and ties of the costs c = ec / step^3, when the
+// SADs are integers below 2^22 (integer-valued images: HU volumes, an identity
+// affine) and there are at most 65536 distinct costs.  c is non-decreasing in
+// the integer SAD, so equal costs come from runs of consecutive present SADs:
+// walking the present SADs in increasing order and advancing the rank when c
+// changes numbers the distinct costs densely.
+bool cost_ranks(const float* ec, float vol, size_t count, std::vector<uint16_t>& rank) {
+    float mx = 0.0f;
+    for (size_t e = 0; e < count; e++) {
+        const float x = ec[e];
+        if (!(x >= 0.0f && x < 4194304.0f) || x != std::floor(x)) return false;
+        mx = std::max(mx, x);
+    }
+    const size_t top = (size_t)mx + 1;
+    std::vector<uint8_t> present(top, 0);
+    for (size_t e = 0; e < count; e++) present[(size_t)ec[e]] = 1;
+    std::vector<uint16_t> of_sad(top, 0);
+    int r = -1;
+    float prev = 0.0f;
+    for (size_t v = 0; v < top; v++) {
+        if (!present[v]) continue;
+        const float c = (float)v / vol;
+        if (r < 0 || c != prev) {
+            if (++r > 65535) return false;
+            prev = c;
+        }
+        of_sad[v] = (uint16_t)r;
+    }
+    rank.resize(count);
+    for (size_t e = 0; e < count; e++) rank[e] = of_sad[(size_t)ec[e]];
+    return true;
+}
 
 }  // namespace
 
@@ -152,7 +200,13 @@ bool weights_distinguish_costs(const float* ec, int step, int m1, int n1, int o1
 }
 
 void build_tree_shape(const float* ec, int step, int m1, int n1, int o1, Tree& t) {
-    std::vector<float> c = edge_costs(ec, step, (size_t)6 * m1 * n1 * o1);
+    const size_t count = (size_t)6 * m1 * n1 * o1;
+    std::vector<uint16_t> rank;
+    if (cost_ranks(ec, (float)((double)step * step * step), count, rank)) {
+        prim_tree(rank.data(), m1, n1, o1, t);
+        return;
+    }
+    std::vector<float> c = edge_costs(ec, step, count);
     prim_tree(c.data(), m1, n1, o1, t);
 }
 
@@ -180,7 +234,8 @@ void build_tree(const float* ec, float sd, int step, int m1, int n1, int o1, Tre
 
 namespace {
 
-void prim_tree(const float* w, int m1, int n1, int o1, Tree& t) {
+template <typename Key>
+void prim_tree(const Key* w, int m1, int n1, int o1, Tree& t) {
     const int N = m1 * n1 * o1;
     t.m1 = m1; t.n1 = n1; t.o1 = o1; t.N = N;
     t.ordered.assign(N, 0);
@@ -191,7 +246,7 @@ void prim_tree(const float* w, int m1, int n1, int o1, Tree& t) {
     const int root = m1 / 2 + (n1 / 2) * m1 + (o1 / 2) * m1 * n1;
     std::vector<uint8_t> in_tree(N, 0);
     std::vector<int> depth(N, 0);
-    EdgeHeap heap((size_t)4 * N + 16);
+    EdgeHeap<Key> heap((size_t)4 * N + 16);
     in_tree[root] = 1;
     int last = root;
     // linear offset of each neighbour slot (valid wherever the neighbour exists)
