@@ -266,6 +266,10 @@ class PairRun {
         wait_level_.assign(L_, 0.0);
         trees_[0].resize(L_);
         trees_[1].resize(L_);
+        for (int d = 0; d < 2; d++) {
+            got_[d].resize(L_);
+            have_[d].assign(L_, 0);
+        }
     }
 
     int levels() const { return L_; }
@@ -364,7 +368,35 @@ class PairRun {
 
     // Pre-BP work of both directions: MIND of the images when the patch radius
     // changes, then per direction upsample, warp, MIND, padding, data cost.
-    void pre_bp(int l) {
+    // Chain steps on the bottom-up pass's critical path as the rounds run it
+    // (each round costs its longest program), or -1 while the tree is not built.
+    long long bp_steps(int l, int dir) {
+        if (!have_[dir][l]) {
+            if (trees_[dir][l].wait_for(std::chrono::seconds(0)) != std::future_status::ready) return -1;
+            got_[dir][l] = trees_[dir][l].get();
+            have_[dir][l] = true;
+        }
+        const Tree& t = *got_[dir][l].tree;
+        long long steps = 0;
+        for (int r = 0; r < t.rounds; r++) {
+            int longest = 0;
+            for (int h = t.round_start[r]; h < t.round_start[r + 1]; h++)
+                longest = std::max(longest, t.prog[t.prog_start[h] + 2] + 1);
+            steps += longest;
+        }
+        return steps;
+    }
+
+    // The direction whose BP is expected to take longer, when both trees of
+    // level l are built: its data cost then runs first, alone on the GPU, so
+    // its BP starts half a data-cost phase earlier (-1: no order).
+    int first_direction(int l) {
+        const long long a = bp_steps(l, 0), b = bp_steps(l, 1);
+        if (a < 0 || b < 0 || a == b) return -1;
+        return a > b ? 0 : 1;
+    }
+
+    void pre_bp(int l, int first = -1) {
         const LevelCfg& c = lv_[l];
         DEEDS_HIP(hipEventRecord(EV(l, 0), s0_));
         if (c.new_mind) {
@@ -374,7 +406,8 @@ class PairRun {
         }
         DEEDS_HIP(hipEventRecord(ev_mind_.e, s0_));
         DEEDS_HIP(hipStreamWaitEvent(s1_, ev_mind_.e, 0));
-        for (int dir = 0; dir < 2; dir++) {
+        for (int di = 0; di < 2; di++) {
+            const int dir = first == 1 ? 1 - di : di;  // the first direction's work is queued first
             hipStream_t s = dir ? s1_ : s0_;
             const float* pf = prevf_ + dir * 3 * sz2_;
             float* u0 = g0_[dir];
@@ -394,6 +427,8 @@ class PairRun {
                               sh_.mw[dir][1], s);
             if (dir == 0) DEEDS_HIP(hipEventRecord(EV(l, 2), s));
             launch_pad_codes(sh_.mpad[dir], sh_.mind_w[dir], in_.m, in_.n, in_.o, c.plan.pad, s);
+            // the second direction's data cost waits for the first's
+            if (first >= 0 && dir != first) DEEDS_HIP(hipStreamWaitEvent(s, DC(l, first, 1), 0));
             DEEDS_HIP(hipEventRecord(DC(l, dir, 0), s));
             launch_datacost(cost_[dir], dir ? mind_mov_ : mind_fix_, sh_.mpad[dir], in_.m, in_.n, in_.o, c.step,
                             c.hw, c.quant, c.plan, s);
@@ -410,7 +445,8 @@ class PairRun {
         hipStream_t s = dir ? s1_ : s0_;
         const size_t half = (stage_off_[l + 1] - stage_off_[l]) / 2;
         auto tw = std::chrono::steady_clock::now();
-        StagedTree st = trees_[dir][l].get();
+        StagedTree st = have_[dir][l] ? std::move(got_[dir][l]) : trees_[dir][l].get();
+        have_[dir][l] = false;
         const double waited = std::chrono::duration<double>(std::chrono::steady_clock::now() - tw).count();
         wait_time_ += waited;
         wait_level_[l] += waited;
@@ -561,6 +597,8 @@ class PairRun {
         size_t bytes = 0;
     };
     std::vector<std::future<StagedTree>> trees_[2];
+    std::vector<StagedTree> got_[2];  // trees taken early by bp_steps
+    std::vector<char> have_[2];
     std::unique_ptr<Tree> tree_[2];
 };
 
@@ -608,7 +646,11 @@ void run_level_sync(deeds_ctx* ctx, const std::vector<PairIn>& pairs, const deed
         for (auto& r : runs) r->submit_trees(pool, l);
     Event ev_join;
     for (int l = 0; l < L; l++) {
-        for (auto& r : runs) r->pre_bp(l);
+        // one pair: the direction with the longer expected BP computes its data cost first
+        if (P == 1)
+            runs[0]->pre_bp(l, runs[0]->first_direction(l));
+        else
+            for (auto& r : runs) r->pre_bp(l);
         // each direction's trees of all pairs in one BP launch sequence on that direction's stream
         const LevelCfg& c = runs[0]->level(l);
         for (int dir = 0; dir < 2; dir++) {
