@@ -97,16 +97,21 @@ __device__ __forceinline__ float dt_pass(const float* __restrict__ src, float* _
     for (int r = 0; r < B::OUT; r++) {
         const int i = i0 + r;
         if (i < LEN) {
-            float best = v[0] + zw[r + LEN];
-            int bj = 0;
+            // The strict-< scan in j order keeps the first minimum: the same
+            // value as the minimum of all candidates and the lowest j holding
+            // it.  Minimum by three-input min, then a reverse scan for the
+            // index (candidates are non-negative and finite: no signed zeros
+            // or NaNs to tell apart).
+            float c[LEN];
 #pragma unroll
-            for (int j = 0; j < LEN; j++) {
-                float cnd = v[j] + zw[r - j + LEN];
-                if (cnd < best) {
-                    best = cnd;
-                    bj = j;
-                }
-            }
+            for (int j = 0; j < LEN; j++) c[j] = v[j] + zw[r - j + LEN];
+            float best = c[0];
+#pragma unroll
+            for (int j = 1; j + 1 < LEN; j += 2) best = __builtin_fminf(__builtin_fminf(best, c[j]), c[j + 1]);
+            if ((LEN & 1) == 0) best = __builtin_fminf(best, c[LEN - 1]);
+            int bj = LEN - 1;
+#pragma unroll
+            for (int j = LEN - 2; j >= 0; j--) bj = c[j] == best ? j : bj;
             const int o = base + i * STRIDE;
             dst[o] = best;
             mn = best < mn ? best : mn;
