@@ -387,13 +387,15 @@ class PairRun {
         return steps;
     }
 
-    // The direction whose BP is expected to take longer, when both trees of
-    // level l are built: its data cost then runs first, alone on the GPU, so
-    // its BP starts half a data-cost phase earlier (-1: no order).
+    // The direction whose data cost runs first, alone on the GPU (the other
+    // waits for it): its BP then starts half a data-cost phase earlier while
+    // the total data-cost time stays the same.  When both trees of level l are
+    // built, the one whose BP is expected to take longer; otherwise the
+    // forward direction.
     int first_direction(int l) {
         const long long a = bp_steps(l, 0), b = bp_steps(l, 1);
-        if (a < 0 || b < 0 || a == b) return -1;
-        return a > b ? 0 : 1;
+        if (a < 0 || b < 0) return 0;
+        return a >= b ? 0 : 1;
     }
 
     void pre_bp(int l, int first = -1) {
