@@ -366,8 +366,6 @@ class PairRun {
         }
     }
 
-    // Pre-BP work of both directions: MIND of the images when the patch radius
-    // changes, then per direction upsample, warp, MIND, padding, data cost.
     // Chain steps on the bottom-up pass's critical path as the rounds run it
     // (each round costs its longest program), or -1 while the tree is not built.
     long long bp_steps(int l, int dir) {
@@ -398,6 +396,9 @@ class PairRun {
         return a >= b ? 0 : 1;
     }
 
+    // Pre-BP work of both directions: MIND of the images when the patch radius
+    // changes, then per direction upsample, warp, MIND, padding, data cost
+    // (the `first` direction's data cost before the other's; -1: concurrent).
     void pre_bp(int l, int first = -1) {
         const LevelCfg& c = lv_[l];
         DEEDS_HIP(hipEventRecord(EV(l, 0), s0_));
