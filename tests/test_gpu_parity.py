@@ -97,6 +97,18 @@ def test_prims_vs_oracle(oracle, step):
             _eq(a, b, f"{name} step={step}")
 
 
+def test_prims_many_distinct_costs_vs_oracle(oracle):
+    """Integer image whose edge costs take more than 65536 distinct values:
+    Prim's keys fall back from 16-bit ranks to floats, same tree."""
+    print("case: prims with >65536 distinct integer costs", flush=True)
+    rng = np.random.default_rng(7)
+    im = rng.integers(0, 60000, size=(96, 160, 160)).astype(np.float32)
+    got = api.primsGraph(im, 4)
+    want = oracle.prims(im, 4)
+    for a, b, name in zip(got, want, ("ordered", "parents", "edgemst")):
+        _eq(a, b, f"{name} many distinct costs")
+
+
 @pytest.mark.parametrize("step", [4, 6, 7])
 def test_prims_device_image_vs_oracle(oracle, step):
     """deeds_ctx_prims_graph (device image, statistics on the GPU) builds the oracle's tree."""
