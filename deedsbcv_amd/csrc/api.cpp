@@ -323,7 +323,7 @@ size_t batch_desc_bytes(int trees) { return ((size_t)trees * sizeof(BPBatchTree)
 }  // namespace
 
 size_t bp_batch_bytes(int trees, long long nodes) {
-    // leaves and chain heads are disjoint sets of nodes: at most N work items
+    // leaves and segment heads are disjoint sets of nodes: at most N work items
     // per tree, plus two items per tree for its spine
     return batch_desc_bytes(trees) + ((size_t)nodes + 2 * (size_t)trees) * sizeof(int2);
 }
@@ -357,7 +357,7 @@ void run_bp_batch(const std::vector<BPBatchItem>& items, int hw, float quant, vo
                             d.parents,    d.ordered,  d.depth_start, d.sel_info, it.u0,  it.v0,    it.w0,
                             it.u1,        it.v1,      it.w1,        d.root,  it.tree->N, it.tree->maxd};
     }
-    // work list: the leaves of every tree, then round by round every tree's heads of that round
+    // work list: the leaves of every tree, then round by round every tree's segments of that round
     std::vector<int> round_off(1, 0);
     long long at = 0;
     for (int t = 0; t < T; t++)
@@ -372,7 +372,7 @@ void run_bp_batch(const std::vector<BPBatchItem>& items, int hw, float quant, vo
         }
         round_off.push_back((int)at);
     }
-    // spines: ((tree, first head), (end head, 0)) per tree with one
+    // spines: ((tree, first segment), (end segment, 0)) per tree with one
     for (int t = 0; t < T; t++) {
         const Tree& tr = *items[t].tree;
         if (tr.tail_round >= tr.rounds) continue;

@@ -1,24 +1,24 @@
 // Tree belief propagation on gfx950 (reference: regularisation.h:6-303).
 //
-// Bottom-up pass: one launch per "round" of the tree schedule, one workgroup
-// per chain head.  A chain head is an internal node with several children or
-// with a single leaf child; its children are leaves or chain ends finished in
-// earlier rounds.  The workgroup
-//   - accumulates the head's cost in LDS, and for each child c in `ordered`
-//     order stages cost[c] * edgemst[c] (:197-202), runs the three separable
+// Bottom-up pass: the leaves in one launch, then one launch per "round" of
+// the tree schedule -- a window of 16 node steps -- with one workgroup per
+// segment, a path v_0 .. v_L up the tree whose nodes' other children finished
+// in earlier rounds.  The workgroup, for each segment node in turn,
+//   - accumulates the node's cost plus its children's messages in `ordered`
+//     order (:221-232): the previous segment node's from LDS, the others
+//     stored in their cost rows;
+//   - stages acc * edgemst (:197-202), runs the three separable
 //     min-convolution passes y, x, z with the quadratic ((t - len) + offset)^2
-//     in label units and the reference's strict-< lowest-index tie rule,
-//     writing the composite argmin table of c (:6-113), and folds
-//     acc += msg - min(msg) (:221-232);
-//   - then, while the parent of the finished node has that node as its only
-//     child, messages it straight from LDS into the parent (the reference's
-//     fold with a single child), so a chain of single-child links costs one
-//     round instead of one launch per BFS depth;
-//   - writes the last finished node's cost back to HBM.
+//     in label units and the reference's lowest-index tie rule, writing the
+//     node's composite argmin table (:6-113), and keeps msg - min(msg) in LDS
+//     for the parent;
+//   - at the segment's last node stores that message in the node's cost row.
 // Rounds replace the reference's loop over BFS depths (:190-235) with fewer
-// global synchronisation points (the kernel boundary makes the chain ends of
-// one round visible to the next on every XCD).  The arithmetic per node is the
-// reference's, in its order, so messages and argmin tables are bit-identical.
+// global synchronisation points (the kernel boundary makes the messages stored
+// in one round visible to the next on every XCD), and the segments follow the
+// tree's critical path, so the bottom-up pass takes little more than the
+// tree's height in node steps.  The arithmetic per node is the reference's,
+// in its order, so messages and argmin tables are bit-identical.
 //
 // Each 1-D pass keeps the reference's brute-force O(len) loop per output, but
 // register-blocked: a thread loads a whole line of the label cube and a window
@@ -221,51 +221,77 @@ __device__ __forceinline__ void leaf_body(const BPArgs& a, int idx, float* smem,
     for (int l = t; l < B::L3; l += B::BLOCK) row[l] = s.Bf[l] - mn;
 }
 
-// One workgroup runs one head's program (at a.prog + `off`): the
-// head's accumulator is its cost plus its children's stored messages in
-// `ordered` order (the reference's fold order, :221-232); then each chain
-// node messages its parent, which is the next chain node's accumulator (its
-// only child's message added to its cost), until the chain's last node, which
-// stores its message in its own row -- or, at the root, its accumulator.  The
-// accumulator lives in registers (each thread owns fixed labels); the next
-// chain node's cost row is loaded while the current message computes.
+// One workgroup runs one segment's program (at a.prog + `off`): the first
+// node's accumulator is its cost plus its children's stored messages in
+// `ordered` order (the reference's fold order, :221-232); then each segment
+// node messages its parent, the next segment node, whose accumulator is its
+// cost plus its children's messages in the same order -- the one just
+// computed taken from LDS, the others (finished in earlier rounds) from their
+// cost rows -- until the segment's last node, which stores its message in its
+// own row -- or, at the root, its accumulator.  The accumulator lives in
+// registers (each thread owns fixed labels); the next node's cost row and up
+// to PF of its stored child messages are loaded while the current message
+// computes.
 template <int LEN>
 __device__ __forceinline__ void prog_body(const BPArgs& a, int off, float* smem, float* red) {
     using B = BPShape<LEN>;
     constexpr int ELEMS = (B::L3 + B::BLOCK - 1) / B::BLOCK;
+    // stored child messages prefetched per node: what the registers left
+    // beside the passes hold without spilling (none from 17 labels on)
+    constexpr int PF = ELEMS >= 8 ? 0 : 2;
     const BPLds<LEN> s(smem);
     const int t = threadIdx.x;
 
     const int* pg = a.prog + off;
-    const int x = pg[0], k = pg[1], L = pg[2];
-    const int* kids = pg + 3;
-    const int* chain = kids + k;
+    const int L = pg[0];
+    const int* rec = pg + 1;  // [node, edgemst bits, k, pp, k children]
+    int node = rec[0];
+    float w = __int_as_float(rec[1]);
     float C[ELEMS];
 #pragma unroll
     for (int e = 0; e < ELEMS; e++) {
         const int l = t + e * B::BLOCK;
-        C[e] = l < B::L3 ? a.cost[(long long)x * B::L3 + l] : 0.0f;
+        C[e] = l < B::L3 ? a.cost[(long long)node * B::L3 + l] : 0.0f;
     }
-    for (int j = 0; j < k; j++) {
-        const float* kr = a.cost + (long long)kids[j] * B::L3;
+    for (int j = 0; j < rec[2]; j++) {
+        const float* kr = a.cost + (long long)rec[4 + j] * B::L3;
 #pragma unroll
         for (int e = 0; e < ELEMS; e++) {
             const int l = t + e * B::BLOCK;
             if (l < B::L3) C[e] += kr[l];  // parent += (msg - min)
         }
     }
-    int node = chain[0];
-    float w = __int_as_float(chain[1]);
     for (int i = 0; i <= L; i++) {
         if (node == a.root) break;  // the root keeps its accumulator
-        // prefetch the next chain node's cost row (it receives this message)
-        const int next = i < L ? chain[2 * (i + 1)] : -1;
-        float nrow[ELEMS];
+        // prefetch the next node's cost row and its first PF stored child messages
+        const int* nrec = rec + 4 + rec[2];
+        const int next = i < L ? nrec[0] : -1;
+        const int nk = next >= 0 ? nrec[2] : 0, npp = next >= 0 ? nrec[3] : -1;
+        const int* nkids = nrec + 4;
+        const int o0 = npp == 0 ? 1 : 0;          // first stored child
+        const int o1 = o0 + 1 + (npp == o0 + 1);  // second stored child
+        float nrow[ELEMS], p0[ELEMS], p1[ELEMS];
         if (next >= 0) {
 #pragma unroll
             for (int e = 0; e < ELEMS; e++) {
                 const int l = t + e * B::BLOCK;
                 nrow[e] = l < B::L3 ? a.cost[(long long)next * B::L3 + l] : 0.0f;
+            }
+            if (PF > 0 && o0 < nk) {
+                const float* kr = a.cost + (long long)nkids[o0] * B::L3;
+#pragma unroll
+                for (int e = 0; e < ELEMS; e++) {
+                    const int l = t + e * B::BLOCK;
+                    p0[e] = l < B::L3 ? kr[l] : 0.0f;
+                }
+            }
+            if (PF > 1 && o1 < nk) {
+                const float* kr = a.cost + (long long)nkids[o1] * B::L3;
+#pragma unroll
+                for (int e = 0; e < ELEMS; e++) {
+                    const int l = t + e * B::BLOCK;
+                    p1[e] = l < B::L3 ? kr[l] : 0.0f;
+                }
             }
         }
 #pragma unroll
@@ -276,7 +302,7 @@ __device__ __forceinline__ void prog_body(const BPArgs& a, int off, float* smem,
         const float mn = message<LEN>(a.allinds + (long long)node * B::L3, a.off3[3 * node], a.off3[3 * node + 1],
                                       a.off3[3 * node + 2], s.A, s.Bf, s.I1, s.I2, s.zy, s.zx, s.zz, red);
         if (next < 0) {
-            // the chain's last node: store its message for the parent's fold
+            // the segment's last node: store its message for the parent's fold
             float* row = a.cost + (long long)node * B::L3;
 #pragma unroll
             for (int e = 0; e < ELEMS; e++) {
@@ -285,13 +311,34 @@ __device__ __forceinline__ void prog_body(const BPArgs& a, int off, float* smem,
             }
             return;
         }
+        // the next node's accumulator: its cost, then its children in order
 #pragma unroll
-        for (int e = 0; e < ELEMS; e++) {
-            const int l = t + e * B::BLOCK;
-            if (l < B::L3) C[e] = nrow[e] + (s.Bf[l] - mn);  // the only child's message
+        for (int e = 0; e < ELEMS; e++) C[e] = nrow[e];
+        for (int j = 0; j < nk; j++) {
+            if (j == npp) {
+#pragma unroll
+                for (int e = 0; e < ELEMS; e++) {
+                    const int l = t + e * B::BLOCK;
+                    if (l < B::L3) C[e] += s.Bf[l] - mn;
+                }
+            } else if (PF > 0 && j == o0) {
+#pragma unroll
+                for (int e = 0; e < ELEMS; e++) C[e] += p0[e];
+            } else if (PF > 1 && j == o1) {
+#pragma unroll
+                for (int e = 0; e < ELEMS; e++) C[e] += p1[e];
+            } else {
+                const float* kr = a.cost + (long long)nkids[j] * B::L3;
+#pragma unroll
+                for (int e = 0; e < ELEMS; e++) {
+                    const int l = t + e * B::BLOCK;
+                    if (l < B::L3) C[e] += kr[l];
+                }
+            }
         }
+        rec = nrec;
         node = next;
-        w = __int_as_float(chain[2 * (i + 1) + 1]);
+        w = __int_as_float(rec[1]);
     }
     // reached the root: its accumulator is what the selection reads
 #pragma unroll

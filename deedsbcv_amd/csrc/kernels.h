@@ -40,8 +40,8 @@ struct BPDevTree {
     const int* ordered;      // N
     const int* parents;      // N
     const int* depth_start;  // maxd+2
-    const int* prog_start;   // heads+1 (heads in round order)
-    const int* prog;         // per head: [x, k, L], k children, L+1 chain nodes (node, edgemst bits)
+    const int* prog_start;   // segments+1 (segments in round order)
+    const int* prog;         // per segment: [L], then L+1 records [node, edgemst bits, k, pp, k children]
     const int* leaves;       // n_leaves leaf nodes (round 0)
     const int* leaf_w;       // their edgemst bits
     int n_leaves;
@@ -52,9 +52,9 @@ struct BPDevTree {
 };
 // Largest label radius the LDS-resident BP kernel supports (3 len^3 floats + 2 len^3 int16 in LDS).
 constexpr int kMaxBPRadius = 10;
-// Bottom-up pass: one launch per round below tail_round (round r covers heads
-// round_start[r] .. round_start[r+1], host array), then the rounds from
-// tail_round on (one head each) in one single-workgroup launch.  off3: 3*N
+// Bottom-up pass: one launch per round below tail_round (round r covers
+// segments round_start[r] .. round_start[r+1], host array), then the rounds
+// from tail_round on (one segment each) in one single-workgroup launch.  off3: 3*N
 // floats of scratch for the parent offsets.
 void launch_bp_rounds(float* cost, int16_t* allinds, float* off3, const BPDevTree& t, int N,
                       const int* round_start, int rounds, int tail_round, const float* u0, const float* v0,
@@ -84,8 +84,8 @@ struct BPBatchTree {
 // the parent offsets of all trees, then one launch per round holding every
 // tree's items of that round -- work[round_off[0] .. round_off[1]) are the
 // leaves (tree, leaf position), each later range a round's (tree, offset of
-// the head's program in prog), the last range the trees' spines as item pairs
-// ((tree, first head), (end head, 0)) run by one workgroup each -- then one
+// the segment's program in prog), the last range the trees' spines as item
+// pairs ((tree, first segment), (end segment, 0)) run by one workgroup each -- then one
 // selection workgroup per tree.
 // d_trees: device array of `trees` descriptors; max_n: largest N; width:
 // largest depth width.

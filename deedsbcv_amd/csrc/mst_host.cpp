@@ -56,6 +56,27 @@ class EdgeHeap {
         // child that is not "less" (comp(child, child-1) <=> child.w > (child-1).w)
         long hole = 0, child = 0;
         const long lim = (n - 1) / 2;
+        // Two levels per step while both children of the hole are below lim
+        // (so the walk takes both): the four grandchildren are loaded with
+        // the two children, so one load latency covers two levels.  The same
+        // comparisons in the same order as the one-level walk below.
+        while (2 * (child + 1) < lim) {
+            const long r = 2 * (child + 1);
+            __builtin_prefetch(K + 16 * child + 16);
+            const Key a = K[r], b = K[r - 1];
+            const Key g0 = K[2 * r - 1], g1 = K[2 * r], g2 = K[2 * r + 1], g3 = K[2 * r + 2];
+            const bool left = a > b;  // the hole takes the left child
+            const long c1 = r - (long)left;
+            const Key k1 = left ? b : a;
+            const Key ra = left ? g1 : g3, rb = left ? g0 : g2;  // c1's right and left children
+            const bool left2 = ra > rb;
+            const long c2 = 2 * (c1 + 1) - (long)left2;
+            K[hole] = k1;
+            C[hole] = C[c1];
+            K[c1] = left2 ? rb : ra;
+            C[c1] = C[c2];
+            hole = child = c2;
+        }
         while (child < lim) {
             child = 2 * (child + 1);
             // the children pairs two, three and four levels below
@@ -164,6 +185,44 @@ bool weights_distinguish_costs(const float* ec, int step, int m1, int n1, int o1
     // with c, so ordering by c equals ordering by w (ties included) exactly
     // when distinct costs of the pushed (valid) edges get distinct weights.
     const int N = m1 * n1 * o1;
+    const float two_sd = 2.0f * sd;
+    {
+        // Integer SADs below 2^22 (as cost_ranks takes them): the distinct
+        // costs in increasing order come from the present SADs in increasing
+        // order (c = SAD / step^3 is exact in the SAD and non-decreasing), so
+        // a table of present SADs replaces the sort.
+        bool integral = true;
+        float mx = 0.0f;
+        for (size_t e = 0; e < (size_t)6 * N && integral; e++) {
+            const float x = ec[e];
+            integral = x >= 0.0f && x < 4194304.0f && x == std::floor(x);
+            mx = std::max(mx, x);
+        }
+        if (integral) {
+            std::vector<uint8_t> present((size_t)mx + 1, 0);
+            for (int k = 0, v = 0; k < o1; k++)
+                for (int j = 0; j < n1; j++)
+                    for (int i = 0; i < m1; i++, v++) {
+                        const bool ok[6] = {j > 0, j < n1 - 1, i > 0, i < m1 - 1, k > 0, k < o1 - 1};
+                        for (int nb = 0; nb < 6; nb++)
+                            if (ok[nb]) present[(size_t)ec[v + (size_t)nb * N]] = 1;
+                    }
+            const float vol = (float)((double)step * step * step);
+            bool first = true;
+            float prev_c = 0.0f, prev_w = 0.0f;
+            for (size_t s = 0; s < present.size(); s++) {
+                if (!present[s]) continue;
+                const float cv = (float)s / vol;
+                if (!first && cv == prev_c) continue;
+                const float wv = edge_weight(cv, two_sd);
+                if (!first && !(wv > prev_w)) return false;  // two costs, one weight
+                first = false;
+                prev_c = cv;
+                prev_w = wv;
+            }
+            return true;
+        }
+    }
     std::vector<float> c = edge_costs(ec, step, (size_t)6 * N);
     std::vector<uint32_t> bits;
     bits.reserve((size_t)6 * N);
@@ -186,7 +245,6 @@ bool weights_distinguish_costs(const float* ec, int step, int m1, int n1, int o1
         for (uint32_t b : bits) tmp[cnt[(b >> shift) & 2047u]++] = b;
         bits.swap(tmp);
     }
-    const float two_sd = 2.0f * sd;
     float prev_w = 0.0f;
     for (size_t e = 0; e < bits.size(); e++) {
         if (e > 0 && bits[e] == bits[e - 1]) continue;
@@ -297,7 +355,7 @@ void prim_tree(const Key* w, int m1, int n1, int o1, Tree& t) {
 
 }  // namespace
 
-void build_schedule(Tree& t, bool bp_programs) {
+void build_schedule(Tree& t, bool bp_programs, int window) {
     const int N = t.N;
     if ((int)t.ordered.size() != N || (int)t.parents.size() != N)
         throw std::runtime_error("tree: ordered/parents must hold one entry per node");
@@ -348,80 +406,122 @@ void build_schedule(Tree& t, bool bp_programs) {
             t.sel_info[2 * p + 1] = pos[par] - t.depth_start[depth[par]];
         }
     }
-    // rounds of chain heads (a node's round is final once its children's are)
-    auto nch = [&](int v) { return t.child_start[v + 1] - t.child_start[v]; };
-    std::vector<int> round(N, 0);
-    std::vector<char> is_head(N, 0);
+    // Steps and segments (bottom-up, so every child is placed before its
+    // parent).  For each choice of the child c that continues into v, v's
+    // earliest step is max(t(c) + 1, start of the window after the latest
+    // window among the other children); v takes the smallest, preferring a
+    // choice that continues c's segment inside its window, then the child
+    // latest in fold order (so fewer stored messages are folded after it).
+    if (window < 1) throw std::runtime_error("tree: BP window must be positive");
+    if ((long long)N * (window + 1) >= (1LL << 31)) throw std::runtime_error("tree: too many nodes for the BP schedule");
+    // per node: step, window (-1: leaf) and the child whose segment it
+    // continues (-1: none), side by side for the random accesses by child
+    struct Slot {
+        int step = -1, wn = -1, pred = -1;
+    };
+    std::vector<Slot> sl(N);
+    std::vector<int> rev_heads, rev_leaves;  // in descending `ordered` position
+    std::vector<int> per_round;              // segment heads per round
+    rev_leaves.reserve(N);
+    int internal = 0;
     for (int p = N - 1; p >= 0; p--) {
-        int v = t.ordered[p];
-        int k = nch(v);
-        if (k == 0) continue;
-        int c0 = t.child_list[t.child_start[v]];
-        if (k == 1 && nch(c0) > 0) {
-            round[v] = round[c0];  // continues the chain of its only child
-        } else {
-            int r = 0;
-            for (int ci = t.child_start[v]; ci < t.child_start[v + 1]; ci++) r = std::max(r, round[t.child_list[ci]]);
-            round[v] = r + 1;
-            is_head[v] = 1;
+        const int v = t.ordered[p];
+        const int b = t.child_start[v], k = t.child_start[v + 1] - b;
+        if (k == 0) {
+            if (p > 0) rev_leaves.push_back(v);
+            continue;
+        }
+        const int* ch = t.child_list.data() + b;
+        // the two latest child windows, for "latest among the others"
+        int w1 = -2, w2 = -2, i1 = -1;
+        for (int i = 0; i < k; i++) {
+            const int w = sl[ch[i]].wn;
+            if (w > w1) {
+                w2 = w1, w1 = w, i1 = i;
+            } else if (w > w2) {
+                w2 = w;
+            }
+        }
+        int best = -1, best_c = -1;
+        bool best_same = false;
+        for (int i = k - 1; i >= 0; i--) {
+            const int c = ch[i], sc = sl[c].step;
+            const int other = i == i1 ? w2 : w1;
+            const int cand = std::max(sc + 1, std::max(0, (other + 1) * window));
+            // continuing: the very next step, still inside c's window
+            const bool same = sc >= 0 && cand == sc + 1 && cand % window != 0;
+            if (best < 0 || cand < best || (cand == best && same && !best_same)) {
+                best = cand, best_same = same, best_c = c;
+            }
+        }
+        Slot& s = sl[v];
+        s.step = best;
+        s.wn = best / window;
+        s.pred = best_same ? best_c : -1;
+        internal++;
+        if (s.pred < 0) {
+            rev_heads.push_back(v);
+            if ((int)per_round.size() <= s.wn) per_round.resize(s.wn + 1, 0);
+            per_round[s.wn]++;
         }
     }
-    t.rounds = N > 0 ? round[t.ordered[0]] : 0;
+    const int root = N > 0 ? t.ordered[0] : -1;
+    t.rounds = N > 0 ? sl[root].wn + 1 : 0;  // the root is the last node placed
+    per_round.resize(t.rounds, 0);
     t.round_start.assign(t.rounds + 1, 0);
-    for (int v = 0; v < N; v++)
-        if (is_head[v]) t.round_start[round[v]]++;  // count of round r stored at index r (r >= 1)
-    // exclusive prefix: heads of round r (1-based) occupy [round_start[r-1], round_start[r])
-    for (int r = 1; r <= t.rounds; r++) t.round_start[r] += t.round_start[r - 1];
+    for (int r = 0; r < t.rounds; r++) t.round_start[r + 1] = t.round_start[r] + per_round[r];
     t.tail_round = t.rounds;
     while (t.tail_round > 0 && t.round_start[t.tail_round] - t.round_start[t.tail_round - 1] == 1) t.tail_round--;
+    // segment heads by round, in `ordered` order within one
     t.heads.assign(t.round_start[t.rounds], 0);
-    std::vector<int> fill_r(t.round_start.begin(), t.round_start.end());
-    for (int p = 0; p < N; p++) {
-        int v = t.ordered[p];
-        if (is_head[v]) t.heads[fill_r[round[v] - 1]++] = v;
+    std::vector<int> fill_r(t.round_start.begin(), t.round_start.end() - 1);
+    for (size_t i = rev_heads.size(); i-- > 0;) {
+        const int v = rev_heads[i];
+        t.heads[fill_r[sl[v].wn]++] = v;
     }
-    // Leaves message their parents in round 0 (one workgroup each).  The
-    // program of a head: fold its children's stored messages, then message
-    // its parent and climb while the finished node is its parent's only child;
-    // the chain's last node (unless it is the root) stores its message in its
-    // cost row for the multi-child parent that folds it in a later round.
-    const int root = N > 0 ? t.ordered[0] : -1;
-    t.leaves.clear();
-    for (int p = 1; p < N; p++)
-        if (nch(t.ordered[p]) == 0) t.leaves.push_back(t.ordered[p]);
+    // Leaves message their parents before round 0 (one workgroup each).  A
+    // segment's last node (unless it is the root) stores its message in its
+    // cost row for the parent that folds it in a later round.
+    t.leaves.assign(rev_leaves.rbegin(), rev_leaves.rend());
     t.leaf_w.resize(t.leaves.size());
     for (size_t i = 0; i < t.leaves.size(); i++) std::memcpy(&t.leaf_w[i], &t.edgemst[t.leaves[i]], 4);
-    t.prog_start.assign(t.heads.size() + 1, 0);
-    t.prog.clear();
-    t.prog.reserve(4 * t.heads.size() + 3 * (size_t)N);
-    auto wbits = [&](int v) {
-        int wb;
-        std::memcpy(&wb, &t.edgemst[v], 4);
-        return wb;
-    };
-    for (size_t h = 0; h < t.heads.size(); h++) {
+    // programs: [L] per segment, [node, edgemst bits, k, pp, children] per
+    // node; every non-root node is listed once as a child
+    const size_t heads = t.heads.size();
+    t.prog_start.assign(heads + 1, 0);
+    t.prog.assign(heads + 4 * (size_t)internal + (N > 0 ? N - 1 : 0), 0);
+    int* out = t.prog.data();
+    size_t at = 0;
+    std::vector<int> longest(t.rounds, 0);
+    for (size_t h = 0; h < heads; h++) {
         const int x = t.heads[h];
-        t.prog_start[h] = (int)t.prog.size();
-        const size_t hdr = t.prog.size();
-        t.prog.push_back(x);
-        t.prog.push_back(nch(x));
-        t.prog.push_back(0);
-        for (int ci = t.child_start[x]; ci < t.child_start[x + 1]; ci++) t.prog.push_back(t.child_list[ci]);
-        // the chain x = v_0, v_1, ... (each the only child of the next), as (node, edgemst bits)
-        int L = 0, cur = x;
-        t.prog.push_back(cur);
-        t.prog.push_back(cur == root ? 0 : wbits(cur));
-        while (cur != root) {
-            const int q = t.parents[cur];
-            if (nch(q) != 1) break;
-            cur = q;
-            t.prog.push_back(cur);
-            t.prog.push_back(cur == root ? 0 : wbits(cur));
+        t.prog_start[h] = (int)at;
+        const size_t hdr = at++;
+        int L = -1, prev = -1, cur = x;
+        while (true) {
+            const int b = t.child_start[cur], k = t.child_start[cur + 1] - b;
+            const int* ch = t.child_list.data() + b;
+            int pp = -1;
+            for (int i = 0; i < k; i++)
+                if (ch[i] == prev) pp = i;
+            out[at] = cur;
+            if (cur != root) std::memcpy(&out[at + 1], &t.edgemst[cur], 4);
+            out[at + 2] = k;
+            out[at + 3] = pp;
+            std::copy(ch, ch + k, out + at + 4);
+            at += 4 + (size_t)k;
             L++;
+            if (cur == root || sl[t.parents[cur]].pred != cur) break;
+            prev = cur;
+            cur = t.parents[cur];
         }
-        t.prog[hdr + 2] = L;
+        out[hdr] = L;
+        longest[sl[x].wn] = std::max(longest[sl[x].wn], L + 1);
     }
-    t.prog_start[t.heads.size()] = (int)t.prog.size();
+    if (at != t.prog.size()) throw std::runtime_error("tree: BP programs do not cover the tree");
+    t.prog_start[heads] = (int)at;
+    t.bp_steps = 0;
+    for (int r = 0; r < t.rounds; r++) t.bp_steps += longest[r];
 }
 
 size_t tree_device_ints(int N) { return 18 * (size_t)N + 16; }

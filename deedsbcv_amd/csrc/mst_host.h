@@ -24,27 +24,39 @@ struct Tree {
     // children of every node in `ordered` order (CSR by node id)
     std::vector<int> child_start;  // N + 1
     std::vector<int> child_list;   // N - 1
-    // Bottom-up rounds.  A chain head is an internal node with several
-    // children or with one leaf child; from a head the chain climbs while the
-    // parent has a single child.  round(head) = 1 + max round of the chains
-    // ending at its children (leaves: 0), so a round only needs chain ends of
-    // earlier rounds.  heads are grouped by round (ordered order within one).
+    // Bottom-up rounds.  Every internal node gets a step t(v) >= 0 (leaves:
+    // the launch before round 0); round r is the window of steps
+    // [r*W, (r+1)*W), W = kBPWindow, one launch.  A segment is a path
+    // v_0 .. v_L, v_{i+1} the parent of v_i, at consecutive steps of one
+    // window, run by one workgroup with v_i's message kept on chip for
+    // v_{i+1}; every other child of a segment node finished in an earlier
+    // window (its message is stored in its cost row).  t(v) is the earliest
+    // step this allows: the later of t(c)+1 for the child c that continues
+    // into v and the start of the window after every other child's.
+    // heads are the segments' first nodes, grouped by round (ordered order
+    // within one).
     std::vector<int> heads;
     std::vector<int> round_start;  // rounds + 1 offsets into heads
     int rounds = 0;
-    // Rounds tail_round .. rounds-1 hold one head each (the tree's spine):
+    // Rounds tail_round .. rounds-1 hold one segment each (the tree's spine):
     // one workgroup runs them in order in a single launch.
     int tail_round = 0;
-    // Round 0: every leaf messages its parent (leaf_w = edgemst bits).
+    // Bottom-up critical path in node steps: the longest segment of each
+    // round, summed over the rounds.
+    long long bp_steps = 0;
+    // Before round 0: every leaf messages its parent (leaf_w = edgemst bits).
     std::vector<int> leaves;
     std::vector<int> leaf_w;
-    // Per head (in round order) the program one workgroup executes:
-    // [x, k, L], the k children of x (whose messages are stored in their cost
-    // rows), then the L + 1 chain nodes x = v_0 .. v_L as (node, edgemst
-    // bits); v_{i+1} is the parent of v_i and has no other child.
+    // Per segment (in round order) the program one workgroup executes: [L],
+    // then for v_0 .. v_L the record [node, edgemst bits, k, pp, the k
+    // children in `ordered` order]; pp is the position of v_{i-1} among the
+    // children of v_i (-1 for v_0, whose children are all stored).
     std::vector<int> prog_start;  // heads + 1 offsets into prog
     std::vector<int> prog;
 };
+
+// Steps per bottom-up round (one launch each, below the spine).
+constexpr int kBPWindow = 16;
 
 // Upper bound on the ints a device copy of a tree's BP schedule needs
 // (ordered, parents, depth_start, prog_start, prog, leaves, leaf_w, sel_info).
@@ -69,9 +81,10 @@ bool weights_distinguish_costs(const float* ec, int step, int m1, int n1, int o1
 void finish_tree(const float* ec, float sd, int step, Tree& t);
 
 // Fills depth_start, the child lists and (with bp_programs) the selection
-// info and the bottom-up rounds and programs from ordered/parents (any tree in
+// info and the bottom-up rounds and programs (rounds of `window` steps) from
+// ordered/parents (any tree in
 // BFS order, e.g. one handed in by a caller); throws std::runtime_error when
 // ordered/parents do not describe one.
-void build_schedule(Tree& t, bool bp_programs = true);
+void build_schedule(Tree& t, bool bp_programs = true, int window = kBPWindow);
 
 }  // namespace deeds

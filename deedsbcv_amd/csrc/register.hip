@@ -366,23 +366,15 @@ class PairRun {
         }
     }
 
-    // Chain steps on the bottom-up pass's critical path as the rounds run it
-    // (each round costs its longest program), or -1 while the tree is not built.
+    // Node steps on the bottom-up pass's critical path as the rounds run it
+    // (each round costs its longest segment), or -1 while the tree is not built.
     long long bp_steps(int l, int dir) {
         if (!have_[dir][l]) {
             if (trees_[dir][l].wait_for(std::chrono::seconds(0)) != std::future_status::ready) return -1;
             got_[dir][l] = trees_[dir][l].get();
             have_[dir][l] = true;
         }
-        const Tree& t = *got_[dir][l].tree;
-        long long steps = 0;
-        for (int r = 0; r < t.rounds; r++) {
-            int longest = 0;
-            for (int h = t.round_start[r]; h < t.round_start[r + 1]; h++)
-                longest = std::max(longest, t.prog[t.prog_start[h] + 2] + 1);
-            steps += longest;
-        }
-        return steps;
+        return got_[dir][l].tree->bp_steps;
     }
 
     // The direction whose data cost runs first, alone on the GPU (the other
