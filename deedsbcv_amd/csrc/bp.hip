@@ -138,6 +138,16 @@ __device__ __forceinline__ float block_min(float v, float* red) {
     return r;
 }
 
+// The schedule (programs, leaves) and the parent offsets are written before
+// a BP launch (host copies, k_parent_offsets) and only read inside it.  Read
+// through the constant address space, their uniform loads become scalar
+// loads: a vector load would wait (vmcnt) behind the node's outstanding
+// argmin and message stores before the next node's addresses are known.
+using cint = const int __attribute__((address_space(4)));
+using cfloat = const float __attribute__((address_space(4)));
+__device__ __forceinline__ cint* ro(const int* p) { return (cint*)p; }
+__device__ __forceinline__ cfloat* ro(const float* p) { return (cfloat*)p; }
+
 struct BPArgs {
     float* cost;
     int16_t* allinds;
@@ -212,12 +222,13 @@ __device__ __forceinline__ void leaf_body(const BPArgs& a, int idx, float* smem,
     using B = BPShape<LEN>;
     const BPLds<LEN> s(smem);
     const int t = threadIdx.x;
-    const int v = a.leaves[idx];
-    const float w = __int_as_float(a.leaf_w[idx]);
+    const int v = ro(a.leaves)[idx];
+    const float w = __int_as_float(ro(a.leaf_w)[idx]);
     float* row = a.cost + (long long)v * B::L3;
     for (int l = t; l < B::L3; l += B::BLOCK) s.A[l] = row[l] * w;  // costall[child] *= edgemst (:197-202)
-    const float mn = message<LEN>(a.allinds + (long long)v * B::L3, a.off3[3 * v], a.off3[3 * v + 1],
-                                  a.off3[3 * v + 2], s.A, s.Bf, s.I1, s.I2, s.zy, s.zx, s.zz, red);
+    cfloat* off3 = ro(a.off3);
+    const float mn = message<LEN>(a.allinds + (long long)v * B::L3, off3[3 * v], off3[3 * v + 1], off3[3 * v + 2],
+                                  s.A, s.Bf, s.I1, s.I2, s.zy, s.zx, s.zz, red);
     for (int l = t; l < B::L3; l += B::BLOCK) row[l] = s.Bf[l] - mn;
 }
 
@@ -242,9 +253,10 @@ __device__ __forceinline__ void prog_body(const BPArgs& a, int off, float* smem,
     const BPLds<LEN> s(smem);
     const int t = threadIdx.x;
 
-    const int* pg = a.prog + off;
+    cint* pg = ro(a.prog) + off;
+    cfloat* off3 = ro(a.off3);
     const int L = pg[0];
-    const int* rec = pg + 1;  // [node, edgemst bits, k, pp, k children]
+    cint* rec = pg + 1;  // [node, edgemst bits, k, pp, k children]
     int node = rec[0];
     float w = __int_as_float(rec[1]);
     float C[ELEMS];
@@ -264,10 +276,10 @@ __device__ __forceinline__ void prog_body(const BPArgs& a, int off, float* smem,
     for (int i = 0; i <= L; i++) {
         if (node == a.root) break;  // the root keeps its accumulator
         // prefetch the next node's cost row and its first PF stored child messages
-        const int* nrec = rec + 4 + rec[2];
+        cint* nrec = rec + 4 + rec[2];
         const int next = i < L ? nrec[0] : -1;
         const int nk = next >= 0 ? nrec[2] : 0, npp = next >= 0 ? nrec[3] : -1;
-        const int* nkids = nrec + 4;
+        cint* nkids = nrec + 4;
         const int o0 = npp == 0 ? 1 : 0;          // first stored child
         const int o1 = o0 + 1 + (npp == o0 + 1);  // second stored child
         float nrow[ELEMS], p0[ELEMS], p1[ELEMS];
@@ -299,8 +311,8 @@ __device__ __forceinline__ void prog_body(const BPArgs& a, int off, float* smem,
             const int l = t + e * B::BLOCK;
             if (l < B::L3) s.A[l] = C[e] * w;  // costall[child] *= edgemst (:197-202)
         }
-        const float mn = message<LEN>(a.allinds + (long long)node * B::L3, a.off3[3 * node], a.off3[3 * node + 1],
-                                      a.off3[3 * node + 2], s.A, s.Bf, s.I1, s.I2, s.zy, s.zx, s.zz, red);
+        const float mn = message<LEN>(a.allinds + (long long)node * B::L3, off3[3 * node], off3[3 * node + 1],
+                                      off3[3 * node + 2], s.A, s.Bf, s.I1, s.I2, s.zy, s.zx, s.zz, red);
         if (next < 0) {
             // the segment's last node: store its message for the parent's fold
             float* row = a.cost + (long long)node * B::L3;
@@ -359,7 +371,7 @@ template <int LEN>
 __global__ __launch_bounds__(BPShape<LEN>::BLOCK) void k_bp_prog(BPArgs a, int head_begin) {
     extern __shared__ float smem[];
     __shared__ float red[16];
-    prog_body<LEN>(a, a.prog_start[head_begin + blockIdx.x], smem, red);
+    prog_body<LEN>(a, ro(a.prog_start)[head_begin + blockIdx.x], smem, red);
 }
 
 // The spine: heads [head_begin, head_end), one per round, run in order by one
@@ -370,7 +382,7 @@ __global__ __launch_bounds__(BPShape<LEN>::BLOCK) void k_bp_tail(BPArgs a, int h
     extern __shared__ float smem[];
     __shared__ float red[16];
     for (int h = head_begin; h < head_end; h++) {
-        prog_body<LEN>(a, a.prog_start[h], smem, red);
+        prog_body<LEN>(a, ro(a.prog_start)[h], smem, red);
         __syncthreads();
     }
 }
@@ -409,7 +421,7 @@ __global__ __launch_bounds__(BPShape<LEN>::BLOCK) void k_bp_tail_b(const BPBatch
     const int2 a0 = tails[2 * blockIdx.x], a1 = tails[2 * blockIdx.x + 1];
     const BPArgs a = args_of(trees[a0.x]);
     for (int h = a0.y; h < a1.x; h++) {
-        prog_body<LEN>(a, a.prog_start[h], smem, red);
+        prog_body<LEN>(a, ro(a.prog_start)[h], smem, red);
         __syncthreads();
     }
 }
