@@ -260,17 +260,42 @@ __device__ __forceinline__ void prog_body(const BPArgs& a, int off, float* smem,
     int node = rec[0];
     float w = __int_as_float(rec[1]);
     float C[ELEMS];
-#pragma unroll
-    for (int e = 0; e < ELEMS; e++) {
-        const int l = t + e * B::BLOCK;
-        C[e] = l < B::L3 ? a.cost[(long long)node * B::L3 + l] : 0.0f;
-    }
-    for (int j = 0; j < rec[2]; j++) {
-        const float* kr = a.cost + (long long)rec[4 + j] * B::L3;
+    {
+        // the first node's cost and stored child messages: all loads issued
+        // before the first add (a grid node has at most six children)
+        constexpr int KMAX = 6;
+        const int k0 = rec[2];
+        float kv[KMAX][ELEMS];
 #pragma unroll
         for (int e = 0; e < ELEMS; e++) {
             const int l = t + e * B::BLOCK;
-            if (l < B::L3) C[e] += kr[l];  // parent += (msg - min)
+            C[e] = l < B::L3 ? a.cost[(long long)node * B::L3 + l] : 0.0f;
+        }
+#pragma unroll
+        for (int j = 0; j < KMAX; j++) {
+            if (j < k0) {
+                const float* kr = a.cost + (long long)rec[4 + j] * B::L3;
+#pragma unroll
+                for (int e = 0; e < ELEMS; e++) {
+                    const int l = t + e * B::BLOCK;
+                    kv[j][e] = l < B::L3 ? kr[l] : 0.0f;
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < KMAX; j++) {
+            if (j < k0) {
+#pragma unroll
+                for (int e = 0; e < ELEMS; e++) C[e] += kv[j][e];  // parent += (msg - min), in `ordered` order
+            }
+        }
+        for (int j = KMAX; j < k0; j++) {
+            const float* kr = a.cost + (long long)rec[4 + j] * B::L3;
+#pragma unroll
+            for (int e = 0; e < ELEMS; e++) {
+                const int l = t + e * B::BLOCK;
+                if (l < B::L3) C[e] += kr[l];
+            }
         }
     }
     for (int i = 0; i <= L; i++) {

@@ -522,6 +522,32 @@ void build_schedule(Tree& t, bool bp_programs, int window) {
     t.prog_start[heads] = (int)at;
     t.bp_steps = 0;
     for (int r = 0; r < t.rounds; r++) t.bp_steps += longest[r];
+    // Within a launched round, the longest segments first: workgroups are
+    // dispatched about in index order, so a round with more segments than
+    // the GPU holds at once starts its critical ones in the first wave.
+    // Segments of one round are independent, and a program is found through
+    // prog_start, so only prog_start and heads are permuted (stably, by
+    // counting sort on the length, at most `window` nodes).
+    std::vector<int> cnt(window + 2), ps, hd;
+    for (int r = 0; r < t.tail_round; r++) {
+        const int b = t.round_start[r], e = t.round_start[r + 1];
+        std::fill(cnt.begin(), cnt.end(), 0);
+        for (int h = b; h < e; h++) {
+            const int L = t.prog[t.prog_start[h]];  // L + 1 nodes, inside one window
+            if (L < 0 || L >= window) throw std::runtime_error("tree: BP segment longer than its window");
+            cnt[window - L]++;
+        }
+        for (int i = 1; i < window + 2; i++) cnt[i] += cnt[i - 1];
+        ps.assign(t.prog_start.begin() + b, t.prog_start.begin() + e);
+        hd.assign(t.heads.begin() + b, t.heads.begin() + e);
+        std::vector<int> pos(window + 2, 0);
+        for (int i = 1; i < window + 2; i++) pos[i] = cnt[i - 1];
+        for (int i = 0; i < e - b; i++) {
+            const int slot = b + pos[window - t.prog[ps[i]]]++;
+            t.prog_start[slot] = ps[i];
+            t.heads[slot] = hd[i];
+        }
+    }
 }
 
 size_t tree_device_ints(int N) { return 18 * (size_t)N + 16; }
