@@ -377,6 +377,25 @@ class PairRun {
         return got_[dir][l].tree->bp_steps;
     }
 
+    // The direction whose tree of level l is built first (the forward one
+    // when both are, or once both are); the host's wait counts as tree wait.
+    int first_built(int l) {
+        auto tw = std::chrono::steady_clock::now();
+        int dir = 0;
+        while (true) {
+            if (bp_steps(l, 0) >= 0) break;
+            if (bp_steps(l, 1) >= 0) {
+                dir = 1;
+                break;
+            }
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
+        }
+        const double waited = std::chrono::duration<double>(std::chrono::steady_clock::now() - tw).count();
+        wait_time_ += waited;
+        wait_level_[l] += waited;
+        return dir;
+    }
+
     // The direction whose data cost runs first, alone on the GPU (the other
     // waits for it): its BP then starts half a data-cost phase earlier while
     // the total data-cost time stays the same.  When both trees of level l are
@@ -648,7 +667,11 @@ void run_level_sync(deeds_ctx* ctx, const std::vector<PairIn>& pairs, const deed
             for (auto& r : runs) r->pre_bp(l);
         // each direction's trees of all pairs in one BP launch sequence on that direction's stream
         const LevelCfg& c = runs[0]->level(l);
-        for (int dir = 0; dir < 2; dir++) {
+        // one pair: the BP whose tree is built first is launched first (the
+        // two directions' BPs run on their own streams)
+        const int d0 = P == 1 ? runs[0]->first_built(l) : 0;
+        for (int k = 0; k < 2; k++) {
+            const int dir = k ? 1 - d0 : d0;
             std::vector<BPBatchItem> items;
             for (auto& r : runs) r->bp_item(l, dir, items);
             const size_t off = bpq_off[2 * l + dir];
