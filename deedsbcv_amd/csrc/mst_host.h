@@ -33,8 +33,8 @@ struct Tree {
     // window (its message is stored in its cost row).  t(v) is the earliest
     // step this allows: the later of t(c)+1 for the child c that continues
     // into v and the start of the window after every other child's.
-    // heads are the segments' first nodes, grouped by round (ordered order
-    // within one).
+    // heads are the segments' first nodes, grouped by round; below the spine
+    // the longest segments come first within a round (then `ordered` order).
     std::vector<int> heads;
     std::vector<int> round_start;  // rounds + 1 offsets into heads
     int rounds = 0;
