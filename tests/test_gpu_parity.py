@@ -156,6 +156,52 @@ def test_bp_vs_oracle(oracle, case):
     _eq(np.stack(got), np.stack(want), f"bp {case}")
 
 
+def _bfs_tree(par, rng):
+    """A tree given by parent links (par[0] = -1) as (ordered, parents) with
+    shuffled node ids, `ordered` sorted by (depth, id) as primsGraph returns it."""
+    n = len(par)
+    depth = np.zeros(n, np.int64)
+    for v in range(1, n):
+        depth[v] = depth[par[v]] + 1
+    lab = np.concatenate([[0], 1 + rng.permutation(n - 1)])
+    parents = np.full(n, -1, np.int32)
+    parents[lab[1:]] = lab[np.asarray(par[1:])]
+    nd = np.empty(n, np.int64)
+    nd[lab] = depth
+    ordered = np.lexsort((np.arange(n), nd)).astype(np.int32)
+    return ordered, parents
+
+
+@pytest.mark.parametrize("kind", ["star", "bushy", "chain", "binary"])
+def test_bp_caller_trees_vs_oracle(oracle, kind):
+    """regularisationCL on trees a caller hands in (not grid MSTs): nodes with
+    up to 40 children (more than a grid node has), long single-child chains
+    and bushy trees exercise every fold path of the segment programs."""
+    print(f"case: bp caller tree {kind}", flush=True)
+    rng = np.random.default_rng(17)
+    step, hw, q = 4, 2, 2
+    o1, n1, m1 = 5, 6, 7
+    n = o1 * n1 * m1
+    if kind == "star":
+        par = [-1] + [0] * 40 + [int(rng.integers(1, 41)) for _ in range(n - 41)]
+        par = par[:41] + [int(rng.integers(0, v)) if v > 120 else par[v] for v in range(41, n)]
+    elif kind == "bushy":
+        par = [-1] + [int(rng.integers(0, v)) for v in range(1, n)]
+    elif kind == "chain":
+        par = [-1] + [v - 1 if v % 37 else int(rng.integers(0, v)) for v in range(1, n)]
+    else:
+        par = [-1] + [(v - 1) // 2 for v in range(1, n)]
+    ordered, parents = _bfs_tree(par, rng)
+    edgemst = rng.uniform(0.1, 1.0, n).astype(np.float32)
+    len3 = (2 * hw + 1) ** 3
+    cost = rng.uniform(0.0, 50.0, (n, len3)).astype(np.float32)
+    u0 = (rng.normal(size=(3, o1, n1, m1)) * 2).astype(np.float32)
+    shape = (o1 * step, n1 * step, m1 * step)
+    got = api.regularisationCL(cost, u0[0], u0[1], u0[2], hw, step, q, ordered, parents, edgemst, shape)
+    want = oracle.regularisation(cost, u0[0], u0[1], u0[2], hw, step, q, ordered, parents, edgemst, shape[::-1])
+    _eq(np.stack(got), np.stack(want), f"bp caller tree {kind}")
+
+
 # ---------------------------------------------------------- transforms ------
 
 def test_transforms_golden(golden):
