@@ -91,6 +91,14 @@ SIGNATURES = {
     "deeds_ctx_datacost": [V, V, V, V, I, I, I, I, I, F, F, I],
     "deeds_ctx_regularisation": [V, V, V, V, V, V, V, V, I, I, F, V, V, V, I, I, I, V],
     "deeds_ctx_register": [V, V, V, V, I, I, I, ctypes.POINTER(Schedule), V, V, ctypes.POINTER(Timing)],
+    "deeds_ctx_upsample_deformations": [V, V, V, V, V, V, V, I, I, I, I, I, I],
+    "deeds_ctx_consistent_mapping": [V, V, V, V, V, V, V, I, I, I, I],
+    "deeds_ctx_jacobian": [V, V, V, V, I, I, I, I, V, V],
+    "deeds_ctx_warp_affine": [V, V, V, V, V, V, V, V, I, I, I, V],
+    "deeds_ctx_warp_image": [V, V, V, V, V, V, V, I, I, I, V],
+    "deeds_ctx_warp_affine_s": [V, V, V, V, V, V, V, I, I, I],
+    "deeds_ctx_apply_seg": [V, V, V, V, V, I, I, I, I],
+    "deeds_ctx_apply_float": [V, V, V, V, V, I, I, I, I],
     "deeds_soft_saved_count": [I, I],
     "deeds_ctx_soft_features": [V, I, V, V, I, I, I, I],
     "deeds_ctx_soft_features_backward": [V, I, V, V, V, I, I, I, I],

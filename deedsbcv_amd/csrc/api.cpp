@@ -264,9 +264,12 @@ namespace deeds {
 
 size_t tree_ints_bound(int N) { return tree_device_ints(N); }
 
-// Bytes of a tree's staging area: the schedule ints, then 3*N floats of
-// device scratch for the BP parent offsets (not copied).
-size_t tree_stage_bytes(int N) { return tree_ints_bound(N) * sizeof(int) + 3 * (size_t)N * sizeof(float); }
+// Bytes of a tree's staging area: the schedule ints, then device scratch
+// that is not copied: 3*N floats for the BP parent offsets and 2*N ints for
+// the label selection of trees with a depth too wide for LDS.
+size_t tree_stage_bytes(int N) {
+    return tree_ints_bound(N) * sizeof(int) + 3 * (size_t)N * sizeof(float) + 2 * (size_t)N * sizeof(int);
+}
 
 // Packs a host tree's BP schedule (ordered, parents, depth_start, prog_start,
 // prog, leaves, leaf_w, sel_info) into the pinned staging area and enqueues its copy into the device
@@ -299,6 +302,7 @@ BPDevTree pack_tree(const Tree& t, void* pinned, void* d_stage, size_t* bytes) {
     out.max_width = t.max_width;
     out.root = t.N > 0 ? t.ordered[0] : 0;
     out.off3 = reinterpret_cast<float*>(di + bound);
+    out.sel_labs = di + bound + 3 * (size_t)t.N;
     *bytes = at * sizeof(int);
     return out;
 }
@@ -311,11 +315,11 @@ BPDevTree stage_tree(const Tree& t, void* pinned, void* d_stage, hipStream_t s) 
 }
 
 // Bottom-up rounds, then the top-down selection.
-void run_bp(const Tree& t, const BPDevTree& dt, float* d_cost, int16_t* d_inds, int* d_sel, const float* u0,
-            const float* v0, const float* w0, float* u1, float* v1, float* w1, int hw, float quant, hipStream_t s) {
+void run_bp(const Tree& t, const BPDevTree& dt, float* d_cost, int16_t* d_inds, const float* u0, const float* v0,
+            const float* w0, float* u1, float* v1, float* w1, int hw, float quant, hipStream_t s) {
     launch_bp_rounds(d_cost, d_inds, dt.off3, dt, t.N, t.round_start.data(), t.rounds, t.tail_round, u0, v0, w0, hw,
                      quant, s);
-    launch_bp_select(d_cost, d_inds, dt, t.N, t.maxd, u0, v0, w0, u1, v1, w1, d_sel, hw, quant, s);
+    launch_bp_select(d_cost, d_inds, dt, t.N, t.maxd, u0, v0, w0, u1, v1, w1, hw, quant, s);
 }
 
 namespace {
@@ -335,7 +339,7 @@ void run_bp_batch(const std::vector<BPBatchItem>& items, int hw, float quant, vo
     if (T == 1) {
         // one tree: the single-tree kernels take their arguments by value (no descriptor loads)
         const BPBatchItem& it = items[0];
-        run_bp(*it.tree, it.dt, it.cost, it.inds, nullptr, it.u0, it.v0, it.w0, it.u1, it.v1, it.w1, hw, quant, s);
+        run_bp(*it.tree, it.dt, it.cost, it.inds, it.u0, it.v0, it.w0, it.u1, it.v1, it.w1, hw, quant, s);
         return;
     }
     long long nodes = 0;
@@ -355,7 +359,8 @@ void run_bp_batch(const std::vector<BPBatchItem>& items, int hw, float quant, vo
         const BPDevTree& d = it.dt;
         hd[t] = BPBatchTree{it.cost,      it.inds,    d.prog_start, d.prog,  d.leaves, d.leaf_w, d.off3,
                             d.parents,    d.ordered,  d.depth_start, d.sel_info, it.u0,  it.v0,    it.w0,
-                            it.u1,        it.v1,      it.w1,        d.root,  it.tree->N, it.tree->maxd};
+                            it.u1,        it.v1,      it.w1,        d.sel_labs, d.root, it.tree->N,
+                            it.tree->maxd};
     }
     // work list: the leaves of every tree, then round by round every tree's segments of that round
     std::vector<int> round_off(1, 0);
@@ -421,8 +426,9 @@ int deeds_ctx_regularisation(deeds_ctx* ctx, float* d_cost, const float* d_u0, c
         void* d_stage = ctx->buf("bp.tree", stage_bytes);
         BPDevTree dt = stage_tree(t, pinned, d_stage, ctx->stream);
         int16_t* inds = d_allinds ? d_allinds : ctx->get<int16_t>("bp.inds", (size_t)t.N * len3);
-        int* sel = ctx->get<int>("bp.sel", t.N);
-        run_bp(t, dt, d_cost, inds, sel, d_u0, d_v0, d_w0, d_u1, d_v1, d_w1, hw, quant, ctx->stream);
+        run_bp(t, dt, d_cost, inds, d_u0, d_v0, d_w0, d_u1, d_v1, d_w1, hw, quant, ctx->stream);
+        // the caller's table gets the root's row as well (regularisation.h:273-274)
+        if (d_allinds) launch_root_identity(d_allinds, t.ordered[0], (int)len3, ctx->stream);
         // the staging copy must complete before the pinned block is released
         DEEDS_HIP(hipStreamSynchronize(ctx->stream));
     });
@@ -596,6 +602,111 @@ int deeds_regularisation(float* cost, const float* u0, const float* v0, const fl
     });
 }
 
+// ---------------------------------------------------------------------------
+// field operations: device forms, and the host forms over them
+// ---------------------------------------------------------------------------
+
+namespace {
+const float kIdentityX[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+
+// Logging SSD of dataCostD.h:300-312 (mean squared difference to im1b before
+// and after the warp), reduced on the context stream; waits for the result.
+void warp_ssd(deeds_ctx* ctx, const float* d_im1b, const float* d_warped, const float* d_before, size_t sz,
+              float* ssd01) {
+    double* acc = ctx->get<double>("warp.ssd", 2);
+    DEEDS_HIP(hipMemsetAsync(acc, 0, 2 * sizeof(double), ctx->stream));
+    launch_ssd(acc, d_im1b, d_warped, d_before, (long long)sz, ctx->stream);
+    double h[2];
+    d2h(h, acc, 2, ctx->stream);
+    DEEDS_HIP(hipStreamSynchronize(ctx->stream));
+    ssd01[0] = (float)(h[0] / (double)sz);
+    ssd01[1] = (float)(h[1] / (double)sz);
+}
+}  // namespace
+
+int deeds_ctx_upsample_deformations(deeds_ctx* ctx, float* d_u1, float* d_v1, float* d_w1, const float* d_u0,
+                                    const float* d_v0, const float* d_w0, int m, int n, int o, int m2, int n2,
+                                    int o2) {
+    if (!ctx) return DEEDS_ERR_ARG;
+    return guarded(ctx, [&] {
+        DEEDS_REQUIRE(d_u1 && d_v1 && d_w1 && d_u0 && d_v0 && d_w0, "null pointer");
+        check_dims(m, n, o);
+        check_dims(m2, n2, o2);
+        launch_upsample(d_u1, d_v1, d_w1, d_u0, d_v0, d_w0, m, n, o, m2, n2, o2, ctx->stream);
+    });
+}
+
+int deeds_ctx_consistent_mapping(deeds_ctx* ctx, float* d_u, float* d_v, float* d_w, float* d_u2, float* d_v2,
+                                 float* d_w2, int m, int n, int o, int factor) {
+    if (!ctx) return DEEDS_ERR_ARG;
+    return guarded(ctx, [&] {
+        DEEDS_REQUIRE(d_u && d_v && d_w && d_u2 && d_v2 && d_w2, "null pointer");
+        check_dims(m, n, o);
+        DEEDS_REQUIRE(factor != 0, "factor must be nonzero");
+        const size_t sz = (size_t)m * n * o;
+        float* f[6] = {d_u, d_v, d_w, d_u2, d_v2, d_w2};
+        launch_consistent(f, ctx->get<float>("cm.tmp", 6 * sz), m, n, o, factor, ctx->stream);
+    });
+}
+
+int deeds_ctx_jacobian(deeds_ctx* ctx, const float* d_u, const float* d_v, const float* d_w, int m, int n, int o,
+                       int factor, float* std_out, float* neg_frac_out) {
+    if (!ctx) return DEEDS_ERR_ARG;
+    return guarded(ctx, [&] {
+        DEEDS_REQUIRE(d_u && d_v && d_w, "null pointer");
+        check_dims(m, n, o);
+        DEEDS_REQUIRE(factor != 0, "factor must be nonzero");
+        const size_t sz = (size_t)m * n * o;
+        double* acc = ctx->get<double>("jac.acc", 4);
+        launch_jacobian(acc, d_u, d_v, d_w, m, n, o, factor, ctx->stream);
+        double h[3];
+        d2h(h, acc, 3, ctx->stream);
+        DEEDS_HIP(hipStreamSynchronize(ctx->stream));
+        const double N = (double)sz;
+        const double mean = h[0] / N;
+        const double var = sz > 1 ? (h[1] - N * mean * mean) / (N - 1) : 0.0;
+        if (std_out) *std_out = (float)std::sqrt(std::max(var, 0.0));
+        if (neg_frac_out) *neg_frac_out = (float)(h[2] / N);
+    });
+}
+
+int deeds_ctx_warp_affine(deeds_ctx* ctx, float* d_out, const float* d_in, const float* d_im1b, const float* X,
+                          const float* d_u, const float* d_v, const float* d_w, int m, int n, int o, float* ssd01) {
+    if (!ctx) return DEEDS_ERR_ARG;
+    return guarded(ctx, [&] {
+        DEEDS_REQUIRE(d_out && d_in && d_u && d_v && d_w, "null pointer");
+        DEEDS_REQUIRE(d_out != d_in, "out must not alias in");
+        check_dims(m, n, o);
+        const Affine A = affine_from_X(X ? X : kIdentityX);
+        launch_warp_affine(d_out, d_in, A, d_u, d_v, d_w, m, n, o, ctx->stream);
+        if (ssd01 && d_im1b) warp_ssd(ctx, d_im1b, d_out, d_in, (size_t)m * n * o, ssd01);
+    });
+}
+
+int deeds_ctx_warp_image(deeds_ctx* ctx, float* d_out, const float* d_im1, const float* d_im1b, const float* d_u,
+                         const float* d_v, const float* d_w, int m, int n, int o, float* ssd01) {
+    if (!ctx) return DEEDS_ERR_ARG;
+    return guarded(ctx, [&] {
+        DEEDS_REQUIRE(d_out && d_im1 && d_u && d_v && d_w, "null pointer");
+        DEEDS_REQUIRE(d_out != d_im1, "out must not alias im1");
+        check_dims(m, n, o);
+        launch_interp3(d_out, d_im1, d_u, d_v, d_w, m, n, o, m, n, o, 1, ctx->stream);
+        if (ssd01 && d_im1b) warp_ssd(ctx, d_im1b, d_out, d_im1, (size_t)m * n * o, ssd01);
+    });
+}
+
+int deeds_ctx_warp_affine_s(deeds_ctx* ctx, int16_t* d_out, const int16_t* d_in, const float* X, const float* d_u,
+                            const float* d_v, const float* d_w, int m, int n, int o) {
+    if (!ctx) return DEEDS_ERR_ARG;
+    return guarded(ctx, [&] {
+        DEEDS_REQUIRE(d_out && d_in && d_u && d_v && d_w, "null pointer");
+        DEEDS_REQUIRE(d_out != d_in, "out must not alias in");
+        check_dims(m, n, o);
+        const Affine A = affine_from_X(X ? X : kIdentityX);
+        launch_warp_affine_s(d_out, d_in, A, d_u, d_v, d_w, m, n, o, ctx->stream);
+    });
+}
+
 int deeds_upsample_deformations(float* u1, float* v1, float* w1, const float* u0, const float* v0, const float* w0,
                                 int m, int n, int o, int m2, int n2, int o2) {
     return guarded(nullptr, [&] {
@@ -610,7 +721,9 @@ int deeds_upsample_deformations(float* u1, float* v1, float* w1, const float* u0
         h2d(din, u0, sz2, c->stream);
         h2d(din + sz2, v0, sz2, c->stream);
         h2d(din + 2 * sz2, w0, sz2, c->stream);
-        launch_upsample(dout, dout + sz, dout + 2 * sz, din, din + sz2, din + 2 * sz2, m, n, o, m2, n2, o2, c->stream);
+        int rc = deeds_ctx_upsample_deformations(c, dout, dout + sz, dout + 2 * sz, din, din + sz2, din + 2 * sz2, m,
+                                                 n, o, m2, n2, o2);
+        if (rc) fail(rc, c->err);
         d2h(u1, dout, sz, c->stream);
         d2h(v1, dout + sz, sz, c->stream);
         d2h(w1, dout + 2 * sz, sz, c->stream);
@@ -627,14 +740,15 @@ int deeds_consistent_mapping(float* u, float* v, float* w, float* u2, float* v2,
         deeds_ctx* c = default_ctx();
         DEEDS_HIP(hipSetDevice(c->device));
         size_t sz = (size_t)m * n * o;
-        float* df = c->get<float>("h.cm", 12 * sz);
+        float* df = c->get<float>("h.cm", 6 * sz);
         float* hf[6] = {u, v, w, u2, v2, w2};
         float* f[6];
         for (int i = 0; i < 6; i++) {
             f[i] = df + i * sz;
             h2d(f[i], hf[i], sz, c->stream);
         }
-        launch_consistent(f, df + 6 * sz, m, n, o, factor, c->stream);
+        int rc = deeds_ctx_consistent_mapping(c, f[0], f[1], f[2], f[3], f[4], f[5], m, n, o, factor);
+        if (rc) fail(rc, c->err);
         for (int i = 0; i < 6; i++) d2h(hf[i], f[i], sz, c->stream);
         DEEDS_HIP(hipStreamSynchronize(c->stream));
     });
@@ -650,24 +764,12 @@ int deeds_jacobian(const float* u, const float* v, const float* w, int m, int n,
         DEEDS_HIP(hipSetDevice(c->device));
         size_t sz = (size_t)m * n * o;
         float* df = c->get<float>("h.jac", 3 * sz);
-        double* acc = c->get<double>("h.jac.acc", 4);
         h2d(df, u, sz, c->stream);
         h2d(df + sz, v, sz, c->stream);
         h2d(df + 2 * sz, w, sz, c->stream);
-        launch_jacobian(acc, df, df + sz, df + 2 * sz, m, n, o, factor, c->stream);
-        double h[3];
-        d2h(h, acc, 3, c->stream);
-        DEEDS_HIP(hipStreamSynchronize(c->stream));
-        double N = (double)sz;
-        double mean = h[0] / N;
-        double var = sz > 1 ? (h[1] - N * mean * mean) / (N - 1) : 0.0;
-        if (std_out) *std_out = (float)std::sqrt(std::max(var, 0.0));
-        if (neg_frac_out) *neg_frac_out = (float)(h[2] / N);
+        int rc = deeds_ctx_jacobian(c, df, df + sz, df + 2 * sz, m, n, o, factor, std_out, neg_frac_out);
+        if (rc) fail(rc, c->err);
     });
-}
-
-namespace {
-const float kIdentityX[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
 }
 
 int deeds_warp_affine(float* out, const float* in, const float* im1b, const float* X, const float* u, const float* v,
@@ -683,19 +785,11 @@ int deeds_warp_affine(float* out, const float* in, const float* im1b, const floa
         h2d(d + sz, u, sz, c->stream);
         h2d(d + 2 * sz, v, sz, c->stream);
         h2d(d + 3 * sz, w, sz, c->stream);
-        Affine A = affine_from_X(X ? X : kIdentityX);
-        launch_warp_affine(d + 4 * sz, d, A, d + sz, d + 2 * sz, d + 3 * sz, m, n, o, c->stream);
-        if (ssd01 && im1b) {
-            double* acc = c->get<double>("h.ssd", 2);
-            DEEDS_HIP(hipMemsetAsync(acc, 0, 2 * sizeof(double), c->stream));
-            h2d(d + 5 * sz, im1b, sz, c->stream);
-            launch_ssd(acc, d + 5 * sz, d + 4 * sz, d, (long long)sz, c->stream);
-            double h[2];
-            d2h(h, acc, 2, c->stream);
-            DEEDS_HIP(hipStreamSynchronize(c->stream));
-            ssd01[0] = (float)(h[0] / (double)sz);
-            ssd01[1] = (float)(h[1] / (double)sz);
-        }
+        const bool ssd = ssd01 && im1b;
+        if (ssd) h2d(d + 5 * sz, im1b, sz, c->stream);
+        int rc = deeds_ctx_warp_affine(c, d + 4 * sz, d, ssd ? d + 5 * sz : nullptr, X, d + sz, d + 2 * sz, d + 3 * sz,
+                                       m, n, o, ssd ? ssd01 : nullptr);
+        if (rc) fail(rc, c->err);
         d2h(out, d + 4 * sz, sz, c->stream);
         DEEDS_HIP(hipStreamSynchronize(c->stream));
     });
@@ -714,18 +808,11 @@ int deeds_warp_image(float* out, const float* im1, const float* im1b, const floa
         h2d(d + sz, u, sz, c->stream);
         h2d(d + 2 * sz, v, sz, c->stream);
         h2d(d + 3 * sz, w, sz, c->stream);
-        launch_interp3(d + 4 * sz, d, d + sz, d + 2 * sz, d + 3 * sz, m, n, o, m, n, o, 1, c->stream);
-        if (ssd01 && im1b) {
-            double* acc = c->get<double>("h.ssd", 2);
-            DEEDS_HIP(hipMemsetAsync(acc, 0, 2 * sizeof(double), c->stream));
-            h2d(d + 5 * sz, im1b, sz, c->stream);
-            launch_ssd(acc, d + 5 * sz, d + 4 * sz, d, (long long)sz, c->stream);
-            double h[2];
-            d2h(h, acc, 2, c->stream);
-            DEEDS_HIP(hipStreamSynchronize(c->stream));
-            ssd01[0] = (float)(h[0] / (double)sz);
-            ssd01[1] = (float)(h[1] / (double)sz);
-        }
+        const bool ssd = ssd01 && im1b;
+        if (ssd) h2d(d + 5 * sz, im1b, sz, c->stream);
+        int rc = deeds_ctx_warp_image(c, d + 4 * sz, d, ssd ? d + 5 * sz : nullptr, d + sz, d + 2 * sz, d + 3 * sz, m,
+                                      n, o, ssd ? ssd01 : nullptr);
+        if (rc) fail(rc, c->err);
         d2h(out, d + 4 * sz, sz, c->stream);
         DEEDS_HIP(hipStreamSynchronize(c->stream));
     });
@@ -745,8 +832,8 @@ int deeds_warp_affine_s(int16_t* out, const int16_t* in, const float* X, const f
         h2d(d, u, sz, c->stream);
         h2d(d + sz, v, sz, c->stream);
         h2d(d + 2 * sz, w, sz, c->stream);
-        Affine A = affine_from_X(X ? X : kIdentityX);
-        launch_warp_affine_s(s + sz, s, A, d, d + sz, d + 2 * sz, m, n, o, c->stream);
+        int rc = deeds_ctx_warp_affine_s(c, s + sz, s, X, d, d + sz, d + 2 * sz, m, n, o);
+        if (rc) fail(rc, c->err);
         d2h(out, s + sz, sz, c->stream);
         DEEDS_HIP(hipStreamSynchronize(c->stream));
     });

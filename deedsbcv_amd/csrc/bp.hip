@@ -574,6 +574,11 @@ __device__ __forceinline__ void select_body(const float* __restrict__ cost, cons
     }
 }
 
+// GLOBAL: the two label rows live in the tree's scratch (sel_labs) instead
+// of LDS, for trees with a depth wider than LDS holds.  Only this workgroup
+// touches them, and the barrier between depths orders its global stores and
+// loads as it does for LDS.
+template <bool GLOBAL>
 __global__ __launch_bounds__(1024) void k_bp_select(const float* __restrict__ cost,
                                                     const int16_t* __restrict__ allinds,
                                                     const int* __restrict__ ordered,
@@ -582,23 +587,30 @@ __global__ __launch_bounds__(1024) void k_bp_select(const float* __restrict__ co
                                                     const float* __restrict__ u0, const float* __restrict__ v0,
                                                     const float* __restrict__ w0, float* __restrict__ u1,
                                                     float* __restrict__ v1, float* __restrict__ w1, int width,
-                                                    int len, int hw, float quant) {
+                                                    int len, int hw, float quant, int* sel_labs) {
     extern __shared__ int labs[];
     __shared__ float rv[16];
     __shared__ int ri[16];
     select_body(cost, allinds, ordered, depth_start, sel_info, maxd, u0, v0, w0, u1, v1, w1, width, len, hw, quant,
-                labs, rv, ri);
+                GLOBAL ? sel_labs : labs, rv, ri);
 }
 
 // One workgroup per tree; `width` is the largest depth width over the trees.
+template <bool GLOBAL>
 __global__ __launch_bounds__(1024) void k_bp_select_b(const BPBatchTree* __restrict__ trees, int width, int len,
                                                       int hw, float quant) {
     extern __shared__ int labs[];
     __shared__ float rv[16];
     __shared__ int ri[16];
     const BPBatchTree& T = trees[blockIdx.x];
+    // in the tree's own scratch the second row starts N ints in (no depth of this tree is wider)
     select_body(T.cost, T.allinds, T.ordered, T.depth_start, T.sel_info, T.maxd, T.u0, T.v0, T.w0, T.u1, T.v1,
-                T.w1, width, len, hw, quant, labs, rv, ri);
+                T.w1, GLOBAL ? max(1, T.N) : width, len, hw, quant, GLOBAL ? T.sel_labs : labs, rv, ri);
+}
+
+__global__ void k_root_identity(int16_t* __restrict__ allinds, int root, int len3) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < len3) allinds[(long long)root * len3 + l] = (int16_t)l;
 }
 
 // Parent offsets of every tree of a batch (blockIdx.y = tree).
@@ -613,12 +625,12 @@ __global__ void k_parent_offsets_b(const BPBatchTree* __restrict__ trees, float 
     T.off3[3 * c + 2] = (T.w0[p] - T.w0[c]) / quant;
 }
 
+// LDS bytes of the selection's two label rows; 0 when they do not fit (the
+// kernel then keeps them in the tree's global scratch).
 size_t select_smem(int width) {
     const size_t smem = 2 * (size_t)std::max(1, width) * sizeof(int);
     // beyond the default 64 KiB the kernel needs the attribute (its static LDS takes the rest)
-    if (smem > (size_t)kMaxLds - 1024)
-        fail(DEEDS_ERR_ARG, "tree too wide for the label selection (a depth with over 20000 nodes)");
-    return smem;
+    return smem > (size_t)kMaxLds - 1024 ? 0 : smem;
 }
 
 }  // namespace
@@ -661,17 +673,29 @@ void launch_bp_rounds(float* cost, int16_t* allinds, float* off3, const BPDevTre
 }
 
 void launch_bp_select(const float* cost, const int16_t* allinds, const BPDevTree& t, int N, int maxd,
-                      const float* u0, const float* v0, const float* w0, float* u1, float* v1, float* w1, int* sel,
-                      int hw, float quant, hipStream_t s) {
+                      const float* u0, const float* v0, const float* w0, float* u1, float* v1, float* w1, int hw,
+                      float quant, hipStream_t s) {
     int len = 2 * hw + 1;
-    const size_t smem = select_smem(t.max_width);
-    if (smem > 65536)
-        DEEDS_HIP(hipFuncSetAttribute((const void*)k_bp_select, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-    k_bp_select<<<1, 1024, smem, s>>>(cost, allinds, t.ordered, t.depth_start, t.sel_info, maxd, u0, v0, w0, u1, v1,
-                                      w1, std::max(1, t.max_width), len, hw, quant);
+    const int width = std::max(1, t.max_width);
+    const size_t smem = select_smem(width);
+    if (smem == 0) {
+        if (!t.sel_labs || 2LL * width > 2LL * std::max(1, N))
+            fail(DEEDS_ERR_INTERNAL, "label selection: no scratch for a depth too wide for LDS");
+        k_bp_select<true><<<1, 1024, 0, s>>>(cost, allinds, t.ordered, t.depth_start, t.sel_info, maxd, u0, v0, w0,
+                                             u1, v1, w1, width, len, hw, quant, t.sel_labs);
+    } else {
+        if (smem > 65536)
+            DEEDS_HIP(hipFuncSetAttribute((const void*)k_bp_select<false>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+        k_bp_select<false><<<1, 1024, smem, s>>>(cost, allinds, t.ordered, t.depth_start, t.sel_info, maxd, u0, v0,
+                                                 w0, u1, v1, w1, width, len, hw, quant, nullptr);
+    }
     DEEDS_LAUNCH_CHECK();
-    (void)sel;
-    (void)N;
+}
+
+void launch_root_identity(int16_t* allinds, int root, int len3, hipStream_t s) {
+    k_root_identity<<<ceil_div(len3, 256), 256, 0, s>>>(allinds, root, len3);
+    DEEDS_LAUNCH_CHECK();
 }
 
 void launch_bp_batch(const BPBatchTree* d_trees, int trees, int max_n, const int2* work,
@@ -703,11 +727,16 @@ void launch_bp_batch(const BPBatchTree* d_trees, int trees, int max_n, const int
             default: fail(DEEDS_ERR_ARG, "search radius outside [0, 10]");
         }
     }
+    // each tree's scratch holds 2*N ints, and a depth is at most N wide
     const size_t smem = select_smem(width);
-    if (smem > 65536)
-        DEEDS_HIP(hipFuncSetAttribute((const void*)k_bp_select_b, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)smem));
-    k_bp_select_b<<<trees, 1024, smem, s>>>(d_trees, std::max(1, width), 2 * hw + 1, hw, quant);
+    if (smem == 0) {
+        k_bp_select_b<true><<<trees, 1024, 0, s>>>(d_trees, std::max(1, width), 2 * hw + 1, hw, quant);
+    } else {
+        if (smem > 65536)
+            DEEDS_HIP(hipFuncSetAttribute((const void*)k_bp_select_b<false>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+        k_bp_select_b<false><<<trees, 1024, smem, s>>>(d_trees, std::max(1, width), 2 * hw + 1, hw, quant);
+    }
     DEEDS_LAUNCH_CHECK();
 }
 

@@ -18,8 +18,8 @@ BPDevTree pack_tree(const Tree& t, void* pinned, void* d_stage, size_t* bytes);
 // untouched until the copy has executed.
 BPDevTree stage_tree(const Tree& t, void* pinned, void* d_stage, hipStream_t s);
 // Bottom-up rounds, then the top-down selection, of one tree.
-void run_bp(const Tree& t, const BPDevTree& dt, float* d_cost, int16_t* d_inds, int* d_sel, const float* u0,
-            const float* v0, const float* w0, float* u1, float* v1, float* w1, int hw, float quant, hipStream_t s);
+void run_bp(const Tree& t, const BPDevTree& dt, float* d_cost, int16_t* d_inds, const float* u0, const float* v0,
+            const float* w0, float* u1, float* v1, float* w1, int hw, float quant, hipStream_t s);
 
 // One tree of a batched BP: its host schedule, its staged device copy, its
 // cost volume / argmin tables and its fields.

@@ -48,6 +48,7 @@ struct BPDevTree {
     const int* sel_info;     // 2*N: per position of ordered, (node, parent's index within its depth)
     int max_width;           // largest number of nodes at one depth
     float* off3;             // 3*N floats of scratch (parent offsets)
+    int* sel_labs;           // 2*N ints of scratch: the selection's label rows when a depth is too wide for LDS
     int root;
 };
 // Largest label radius the LDS-resident BP kernel supports (3 len^3 floats + 2 len^3 int16 in LDS).
@@ -61,7 +62,9 @@ void launch_bp_rounds(float* cost, int16_t* allinds, float* off3, const BPDevTre
                       const float* w0, int hw, float quant, hipStream_t s);
 void launch_bp_select(const float* cost, const int16_t* allinds, const BPDevTree& t, int N, int maxd,
                       const float* u0, const float* v0, const float* w0, float* u1, float* v1, float* w1,
-                      int* sel, int hw, float quant, hipStream_t s);
+                      int hw, float quant, hipStream_t s);
+// regularisation.h:273-274: the root's argmin row is the identity (allinds[root][l] = l).
+void launch_root_identity(int16_t* allinds, int root, int len3, hipStream_t s);
 
 // One tree of a batched BP (device pointers; the schedule as in BPDevTree).
 struct BPBatchTree {
@@ -78,6 +81,7 @@ struct BPBatchTree {
     const int* sel_info;
     const float *u0, *v0, *w0;
     float *u1, *v1, *w1;
+    int* sel_labs;  // 2*N ints of scratch (BPDevTree::sel_labs)
     int root, N, maxd;
 };
 // BP of several trees with the same label radius in one launch sequence:

@@ -762,6 +762,47 @@ int deeds_register(const float* fixed, const float* moving, const float* X, int 
     });
 }
 
+int deeds_ctx_apply_seg(deeds_ctx* ctx, int16_t* d_out, const int16_t* d_in, const float* X, const float* d_flow,
+                        int grid_step, int m, int n, int o) {
+    if (!ctx) return DEEDS_ERR_ARG;
+    return guarded(ctx, [&] {
+        DEEDS_REQUIRE(d_out && d_in && d_flow, "null pointer");
+        DEEDS_REQUIRE(d_out != d_in, "out must not alias in");
+        DEEDS_REQUIRE(m > 0 && n > 0 && o > 0 && (long long)m * n * o < (1LL << 31), "bad volume dimensions");
+        DEEDS_REQUIRE(grid_step >= 1 && m / grid_step >= 1 && n / grid_step >= 1 && o / grid_step >= 1,
+                      "bad grid step");
+        const size_t sz = (size_t)m * n * o;
+        const int m1 = m / grid_step, n1 = n / grid_step, o1 = o / grid_step;
+        const size_t sz1 = (size_t)m1 * n1 * o1;
+        float* full = ctx->get<float>("apply.full", 3 * sz);
+        // applyBCV.cpp:161-180: the full-resolution field, then the nearest-neighbour warp
+        launch_upsample(full, full + sz, full + 2 * sz, d_flow, d_flow + sz1, d_flow + 2 * sz1, m, n, o, m1, n1, o1,
+                        ctx->stream);
+        const float identity[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+        const Affine A = affine_from_X(X ? X : identity);
+        launch_warp_affine_s(d_out, d_in, A, full, full + sz, full + 2 * sz, m, n, o, ctx->stream);
+    });
+}
+
+int deeds_ctx_apply_float(deeds_ctx* ctx, float* d_out, const float* d_in, const float* X, const float* d_flow,
+                          int grid_step, int m, int n, int o) {
+    if (!ctx) return DEEDS_ERR_ARG;
+    return guarded(ctx, [&] {
+        DEEDS_REQUIRE(d_out && d_in && d_flow, "null pointer");
+        DEEDS_REQUIRE(d_out != d_in, "out must not alias in");
+        DEEDS_REQUIRE(m > 0 && n > 0 && o > 0 && (long long)m * n * o < (1LL << 31), "bad volume dimensions");
+        DEEDS_REQUIRE(grid_step >= 1 && m / grid_step >= 1 && n / grid_step >= 1 && o / grid_step >= 1,
+                      "bad grid step");
+        const int m1 = m / grid_step, n1 = n / grid_step, o1 = o / grid_step;
+        const size_t sz1 = (size_t)m1 * n1 * o1;
+        const float identity[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+        const Affine A = affine_from_X(X ? X : identity);
+        // applyBCVfloat.cpp:186-198: the field upsampled in registers and the trilinear warp, one pass
+        launch_warp_affine_grid(d_out, d_in, A, d_flow, d_flow + sz1, d_flow + 2 * sz1, m, n, o, m1, n1, o1,
+                                ctx->stream);
+    });
+}
+
 int deeds_apply_seg(int16_t* out, const int16_t* in, const float* X, const float* flow, int grid_step, int m, int n,
                     int o) {
     return guarded(nullptr, [&] {
@@ -770,18 +811,14 @@ int deeds_apply_seg(int16_t* out, const int16_t* in, const float* X, const float
                       "bad grid step");
         deeds_ctx* c = default_ctx();
         DEEDS_HIP(hipSetDevice(c->device));
-        size_t sz = (size_t)m * n * o;
-        int m1 = m / grid_step, n1 = n / grid_step, o1 = o / grid_step;
-        size_t sz1 = (size_t)m1 * n1 * o1;
-        float* f = c->get<float>("h.apply.f", 3 * sz1 + 3 * sz);
+        const size_t sz = (size_t)m * n * o;
+        const size_t sz1 = (size_t)(m / grid_step) * (n / grid_step) * (o / grid_step);
+        float* f = c->get<float>("h.apply.f", 3 * sz1);
         int16_t* s = c->get<int16_t>("h.apply.s", 2 * sz);
         h2d(f, flow, 3 * sz1, c->stream);
         h2d(s, in, sz, c->stream);
-        float* full = f + 3 * sz1;
-        launch_upsample(full, full + sz, full + 2 * sz, f, f + sz1, f + 2 * sz1, m, n, o, m1, n1, o1, c->stream);
-        const float identity[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
-        Affine A = affine_from_X(X ? X : identity);
-        launch_warp_affine_s(s + sz, s, A, full, full + sz, full + 2 * sz, m, n, o, c->stream);
+        int rc = deeds_ctx_apply_seg(c, s + sz, s, X, f, grid_step, m, n, o);
+        if (rc) fail(rc, c->err);
         d2h(out, s + sz, sz, c->stream);
         DEEDS_HIP(hipStreamSynchronize(c->stream));
     });
@@ -795,16 +832,14 @@ int deeds_apply_float(float* out, const float* in, const float* X, const float* 
                       "bad grid step");
         deeds_ctx* c = default_ctx();
         DEEDS_HIP(hipSetDevice(c->device));
-        size_t sz = (size_t)m * n * o;
-        int m1 = m / grid_step, n1 = n / grid_step, o1 = o / grid_step;
-        size_t sz1 = (size_t)m1 * n1 * o1;
+        const size_t sz = (size_t)m * n * o;
+        const size_t sz1 = (size_t)(m / grid_step) * (n / grid_step) * (o / grid_step);
         float* f = c->get<float>("h.apply.f", 3 * sz1);
         float* im = c->get<float>("h.apply.im", 2 * sz);
         h2d(f, flow, 3 * sz1, c->stream);
         h2d(im, in, sz, c->stream);
-        const float identity[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
-        Affine A = affine_from_X(X ? X : identity);
-        launch_warp_affine_grid(im + sz, im, A, f, f + sz1, f + 2 * sz1, m, n, o, m1, n1, o1, c->stream);
+        int rc = deeds_ctx_apply_float(c, im + sz, im, X, f, grid_step, m, n, o);
+        if (rc) fail(rc, c->err);
         d2h(out, im + sz, sz, c->stream);
         DEEDS_HIP(hipStreamSynchronize(c->stream));
     });

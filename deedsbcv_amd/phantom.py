@@ -26,13 +26,13 @@ BODY = 40.0
 
 def _body_and_organs(shape_mno, n_organs, rng):
     m, n, o = shape_mno
-    zz, yy, xx = np.meshgrid(np.arange(o, dtype=np.float64), np.arange(n, dtype=np.float64),
-                             np.arange(m, dtype=np.float64), indexing="ij")
     vol = np.full((o, n, m), AIR, dtype=np.float64)
     cx, cy = (m - 1) / 2.0, (n - 1) / 2.0
     ax, ay = 0.44 * m, 0.36 * n
-    body = ((xx - cx) / ax) ** 2 + ((yy - cy) / ay) ** 2 <= 1.0
-    body &= (zz >= 0.04 * o) & (zz <= 0.96 * o)
+    zz = np.arange(o, dtype=np.float64)[:, None, None]
+    yy = np.arange(n, dtype=np.float64)[None, :, None]
+    xx = np.arange(m, dtype=np.float64)[None, None, :]
+    body = (((xx - cx) / ax) ** 2 + ((yy - cy) / ay) ** 2 <= 1.0) & (zz >= 0.04 * o) & (zz <= 0.96 * o)
     vol[body] = BODY
     for _ in range(n_organs):
         # centre inside the body's inner part, random radii and orientation
@@ -41,9 +41,18 @@ def _body_and_organs(shape_mno, n_organs, rng):
         c = np.array([cx + r * ax * np.cos(t), cy + r * ay * np.sin(t), rng.uniform(0.2, 0.8) * o])
         radii = rng.uniform(0.05, 0.16, size=3) * np.array([m, n, o])
         q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
-        d = np.stack([xx - c[0], yy - c[1], zz - c[2]], axis=-1) @ q
+        # the ellipsoid lies within its largest radius of the centre: evaluate that box only
+        # (elementwise, so every voxel gets the value a whole-volume evaluation would give)
+        R = radii.max() + 2
+        lo = [max(0, int(np.floor(c[a] - R))) for a in range(3)]
+        hi = [min(s, int(np.ceil(c[a] + R)) + 1) for a, s in enumerate((m, n, o))]
+        sl = (slice(lo[2], hi[2]), slice(lo[1], hi[1]), slice(lo[0], hi[0]))
+        Z, Y, X = np.meshgrid(np.arange(lo[2], hi[2], dtype=np.float64), np.arange(lo[1], hi[1], dtype=np.float64),
+                              np.arange(lo[0], hi[0], dtype=np.float64), indexing="ij")
+        d = np.stack([X - c[0], Y - c[1], Z - c[2]], axis=-1) @ q
         inside = ((d / radii) ** 2).sum(-1) <= 1.0
-        vol[inside & body] = rng.uniform(-200.0, 300.0)
+        sub = vol[sl]
+        sub[inside & body[sl]] = rng.uniform(-200.0, 300.0)
     return vol
 
 

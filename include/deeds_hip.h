@@ -227,14 +227,47 @@ int deeds_ctx_datacost(deeds_ctx* ctx, const uint64_t* d_fixed_mind, const uint6
                        int m, int n, int o, int step, int hw, float quant, float alpha, int randnum);
 /* BP over a tree given on the host (ordered/parents/edgemst as returned by
  * deeds_prims_graph); d_cost is consumed.  d_allinds (optional, sz1*len3
- * int16) receives the per-node argmin tables. */
+ * int16) receives the per-node argmin tables of messageDT
+ * (regularisation.h:6-113), the root's row being the identity
+ * (regularisation.h:273-274).  Returns when the work is complete. */
 int deeds_ctx_regularisation(deeds_ctx* ctx, float* d_cost, const float* d_u0, const float* d_v0, const float* d_w0,
                              float* d_u1, float* d_v1, float* d_w1, int hw, int step, float quant,
                              const int* ordered, const int* parents, const float* edgemst,
                              int m, int n, int o, int16_t* d_allinds);
+/* Field operations on device arrays (asynchronous on the context stream,
+ * except where a host result is returned).  Arguments as in the host forms
+ * above; X (16 floats or NULL for identity) and ssd01 are host memory; out
+ * must not alias the input volume. */
+/* transformations.h:245 upsampleDeformationsCL() */
+int deeds_ctx_upsample_deformations(deeds_ctx* ctx, float* d_u1, float* d_v1, float* d_w1, const float* d_u0,
+                                    const float* d_v0, const float* d_w0, int m, int n, int o, int m2, int n2,
+                                    int o2);
+/* transformations.h:189 consistentMappingCL(): in place on the six fields */
+int deeds_ctx_consistent_mapping(deeds_ctx* ctx, float* d_u, float* d_v, float* d_w, float* d_u2, float* d_v2,
+                                 float* d_w2, int m, int n, int o, int factor);
+/* transformations.h:84 jacobian() (synchronous: returns the two figures) */
+int deeds_ctx_jacobian(deeds_ctx* ctx, const float* d_u, const float* d_v, const float* d_w, int m, int n, int o,
+                       int factor, float* std_out, float* neg_frac_out);
+/* dataCostD.h:267 warpAffine(); with ssd01 and d_im1b given it waits for the logging SSD */
+int deeds_ctx_warp_affine(deeds_ctx* ctx, float* d_out, const float* d_in, const float* d_im1b, const float* X,
+                          const float* d_u, const float* d_v, const float* d_w, int m, int n, int o, float* ssd01);
+/* dataCostD.h:213 warpImageCL() */
+int deeds_ctx_warp_image(deeds_ctx* ctx, float* d_out, const float* d_im1, const float* d_im1b, const float* d_u,
+                         const float* d_v, const float* d_w, int m, int n, int o, float* ssd01);
+/* dataCostD.h:241 warpAffineS() */
+int deeds_ctx_warp_affine_s(deeds_ctx* ctx, int16_t* d_out, const int16_t* d_in, const float* X, const float* d_u,
+                            const float* d_v, const float* d_w, int m, int n, int o);
 int deeds_ctx_register(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, const float* X,
                        int m, int n, int o, const deeds_schedule* sched,
                        float* d_flow, float* d_warped, deeds_timing* timing);
+
+/* applyBCV.cpp:161-198 / applyBCVfloat.cpp:186-198 on device volumes: d_flow
+ * is the 3*sz1-float _displacements.dat payload on grid `grid_step`; X host
+ * memory (16 floats or NULL).  Asynchronous; d_out must not alias d_in. */
+int deeds_ctx_apply_seg(deeds_ctx* ctx, int16_t* d_out, const int16_t* d_in, const float* X, const float* d_flow,
+                        int grid_step, int m, int n, int o);
+int deeds_ctx_apply_float(deeds_ctx* ctx, float* d_out, const float* d_in, const float* X, const float* d_flow,
+                          int grid_step, int m, int n, int o);
 
 /* ---- differentiable soft variant (no reference counterpart) --------------
  * A smooth relaxation of one registration level beside the plain path, for

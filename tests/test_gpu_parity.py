@@ -259,7 +259,8 @@ def test_register_golden(golden):
 
 
 def test_register_small_workload_vs_oracle(oracle):
-    """The 96x96x80 three-level workload: GPU == oracle == reference binary."""
+    """The 96x96x80 three-level workload: GPU == oracle (the oracle == the reference
+    binary on this workload is tests/test_oracle.py::test_oracle_register_small_workload_vs_ref_binary)."""
     print("case: register small workload vs oracle", flush=True)
     f, mv = phantom.workload_small(0)
     sch = phantom.SCHEDULE_SMALL
@@ -308,3 +309,151 @@ def test_errors_are_statuses_not_faults():
                              np.ones(8, np.float32), (10, 10, 10))
     # the library still works afterwards
     assert api.descriptor(im, 1).shape == im.shape
+
+
+# ----------------------------------------------------- argmin tables --------
+
+def _tree_case(oracle, shape, step, hw, q, seed):
+    rng = np.random.default_rng(seed)
+    f, mv = phantom.make_pair(shape, seed)
+    im = f.astype(np.float32) + 1024
+    F = oracle.descriptor(im, 2)
+    M = oracle.descriptor(mv.astype(np.float32) + 1024, 2)
+    cost = oracle.datacost(F, M, step, hw, q)
+    tree = oracle.prims(im, step)
+    m, n, o = shape
+    u0 = (rng.normal(size=(3, o // step, n // step, m // step)) * 2).astype(np.float32)
+    return cost, tree, u0
+
+
+@pytest.mark.parametrize("case", [(8, 8, 5), (6, 4, 3), (4, 2, 1), (5, 3, 2)])
+def test_bp_argmin_tables_vs_oracle(oracle, case):
+    """deeds_ctx_regularisation's d_allinds equals the oracle's messageDT argmin
+    tables for every node and label (not only the selected labels), the root's
+    row being the identity (regularisation.h:273-274).  The Hamming costs are
+    multiples of one float scale, so equal sums tie exactly and the lowest-index
+    rule of regularisation.h:45-51 decides many entries."""
+    torch = pytest.importorskip("torch")
+    step, hw, q = case
+    print(f"case: bp argmin tables {case}", flush=True)
+    shape = (48, 44, 40)
+    cost, tree, u0 = _tree_case(oracle, shape, step, hw, q, 55 + step)
+    want = oracle.regularisation(cost, u0[0], u0[1], u0[2], hw, step, q, *tree, shape, want_inds=True)
+    ctx = api.Context(0)
+    dcost = torch.from_numpy(cost).cuda()
+    du0 = torch.from_numpy(u0.reshape(3, -1)).cuda()
+    inds = torch.full(cost.shape, -1, dtype=torch.int16, device="cuda")
+    u1 = ctx.regularisation(dcost, du0, hw, step, q, tree, shape[::-1], allinds=inds)
+    torch.cuda.synchronize()
+    _eq(u1.cpu().numpy(), np.stack(want[:3]).reshape(3, -1), f"u1 {case}")
+    _eq(inds.cpu().numpy(), want[3], f"allinds {case}")
+    root = int(tree[0][0])
+    _eq(inds[root].cpu().numpy(), np.arange(cost.shape[1], dtype=np.int16), "root row")
+
+
+def test_bp_wide_depth_vs_oracle(oracle):
+    """A caller tree with 25 000 nodes at one depth (more label rows than LDS
+    holds): the selection keeps them in the tree's global scratch and gives the
+    oracle's labels and tables."""
+    torch = pytest.importorskip("torch")
+    print("case: bp wide depth (25000 nodes at depth 2)", flush=True)
+    rng = np.random.default_rng(23)
+    step, hw, q = 1, 1, 1
+    o1, n1, m1 = 9, 50, 57  # 25 650 nodes: root, 200 children, 25 449 grandchildren
+    n = o1 * n1 * m1
+    par = [-1] + [0] * 200 + [1 + (v - 201) % 200 for v in range(201, n)]
+    ordered, parents = _bfs_tree(par, rng)
+    width = np.bincount(np.r_[0, np.ones(200, int), 2 * np.ones(n - 201, int)]).max()
+    assert width > 20000
+    edgemst = rng.uniform(0.1, 1.0, n).astype(np.float32)
+    cost = rng.integers(0, 40, (n, 27)).astype(np.float32) * np.float32(0.25)
+    u0 = (rng.normal(size=(3, o1, n1, m1)) * 2).astype(np.float32)
+    shape = (o1 * step, n1 * step, m1 * step)
+    want = oracle.regularisation(cost, u0[0], u0[1], u0[2], hw, step, q, ordered, parents, edgemst, shape[::-1],
+                                 want_inds=True)
+    ctx = api.Context(0)
+    inds = torch.full(cost.shape, -1, dtype=torch.int16, device="cuda")
+    u1 = ctx.regularisation(torch.from_numpy(cost).cuda(), torch.from_numpy(u0.reshape(3, -1)).cuda(), hw, step, q,
+                            (ordered, parents, edgemst), shape, allinds=inds)
+    torch.cuda.synchronize()
+    _eq(u1.cpu().numpy(), np.stack(want[:3]).reshape(3, -1), "u1 wide")
+    _eq(inds.cpu().numpy(), want[3], "allinds wide")
+
+
+# ------------------------------------------------ device field forms --------
+
+@pytest.mark.parametrize("on", ["default stream", "side stream"])
+def test_device_field_forms_vs_oracle(oracle, on):
+    """deeds_ctx_upsample_deformations / consistent_mapping / warp_affine /
+    warp_image / warp_affine_s / apply_float / apply_seg / jacobian on device
+    tensors give the oracle's arrays bit for bit (the logging figures to 1e-4),
+    with torch's work on its default (null) stream and on a stream of its own."""
+    torch = pytest.importorskip("torch")
+    print(f"case: device field forms vs oracle ({on})", flush=True)
+    if on == "side stream":
+        with torch.cuda.stream(torch.cuda.Stream()):
+            _device_field_forms(oracle, torch)
+    else:
+        _device_field_forms(oracle, torch)
+
+
+def _device_field_forms(oracle, torch):
+    rng = np.random.default_rng(81)
+    ctx = api.Context(0)
+    m2, n2, o2, m, n, o = 9, 8, 7, 45, 40, 36
+    g = (rng.normal(size=(6, o2, n2, m2)) * 2.5).astype(np.float32)
+    up = ctx.upsample_deformations(torch.from_numpy(g[:3].copy()).cuda(), (o, n, m))
+    want_up = np.stack(oracle.upsample(g[0], g[1], g[2], (m, n, o)))
+    _eq(up.cpu().numpy(), want_up, "device upsample")
+    cm = torch.from_numpy(g.copy()).cuda()
+    ctx.consistent_mapping(cm, 5)
+    _eq(cm.cpu().numpy(), np.stack(oracle.consistent(*g, 5)), "device consistent mapping")
+    f, mv = phantom.make_pair((m, n, o), 82)
+    im = mv.astype(np.float32) + 1024
+    im1b = f.astype(np.float32) + 1024
+    X = np.eye(4, dtype=np.float32)
+    X[0, 1], X[1, 3], X[2, 0], X[0, 3] = 0.02, 1.3, -0.015, -0.4
+    Xc = X.T.ravel().copy()
+    d_im, d_b = torch.from_numpy(im).cuda(), torch.from_numpy(im1b).cuda()
+    w, ssd = ctx.warp_affine(d_im, Xc, up, im1b=d_b)
+    wo, ssdo = oracle.warp_affine(im, im1b, Xc, *want_up)
+    _eq(w.cpu().numpy(), wo, "device warpAffine")
+    np.testing.assert_allclose(ssd, ssdo, rtol=1e-4)
+    w2, _ = ctx.warp_image(d_im, up)
+    _eq(w2.cpu().numpy(), oracle.warp_image(im, None, *want_up)[0], "device warpImageCL")
+    seg = ((f > 60).astype(np.int16) + (f > 160)).astype(np.int16)
+    ws = ctx.warp_affine_s(torch.from_numpy(seg).cuda(), Xc, up)
+    _eq(ws.cpu().numpy(), oracle.warp_affine_s(seg, Xc, *want_up), "device warpAffineS")
+    s, neg = ctx.jacobian(torch.from_numpy(g[:3].copy()).cuda(), 4)
+    s0, neg0 = oracle.jacobian(g[0], g[1], g[2], 4)
+    assert abs(s - float(s0)) <= 1e-4 * max(1.0, float(s0)) and abs(neg - float(neg0)) < 1e-6
+    # applyBCVfloat / applyBCV on device data: the grid field upsampled and warped in one call
+    step = 5
+    flow = (rng.normal(size=3 * (m // step) * (n // step) * (o // step)) * 2).astype(np.float32)
+    d_flow = torch.from_numpy(flow).cuda()
+    _eq(ctx.apply_float(d_im, d_flow, step, Xc).cpu().numpy(), api.apply_float(im, flow, step, Xc),
+        "device applyBCVfloat vs host form")
+    sz1 = flow.size // 3
+    fu = [flow[i * sz1:(i + 1) * sz1].reshape(o // step, n // step, m // step) for i in range(3)]
+    full = oracle.upsample(fu[0], fu[1], fu[2], (m, n, o))
+    _eq(ctx.apply_float(d_im, d_flow, step, Xc).cpu().numpy(), oracle.warp_affine(im, im1b, Xc, *full)[0],
+        "device applyBCVfloat vs oracle")
+    _eq(ctx.apply_seg(torch.from_numpy(seg).cuda(), d_flow, step, Xc).cpu().numpy(),
+        oracle.warp_affine_s(seg, Xc, *full), "device applyBCV vs oracle")
+    torch.cuda.synchronize()
+
+
+def test_context_rejects_bad_tensors():
+    """The Python mirror checks dtype, device, contiguity and size before any launch."""
+    torch = pytest.importorskip("torch")
+    ctx = api.Context(0)
+    f = torch.zeros((20, 24, 28), device="cuda")
+    flow = torch.zeros(3 * 5 * 6 * 7, device="cuda")
+    with pytest.raises(ValueError):
+        ctx.register(f.to(torch.int16), f.to(torch.int16), flow, grid=(4,), radius=(1,), quant=(1,))
+    with pytest.raises(ValueError):
+        ctx.register(f, f.transpose(0, 1).contiguous().transpose(0, 1), flow, grid=(4,), radius=(1,), quant=(1,))
+    with pytest.raises(ValueError):
+        ctx.register(f, f, flow[:10], grid=(4,), radius=(1,), quant=(1,))
+    with pytest.raises(ValueError):
+        ctx.descriptor(f.cpu(), 1)

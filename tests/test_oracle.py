@@ -467,3 +467,20 @@ def test_ref_linear_binary(oracle, tmp_path):
                                    rigid)
         want = "".join("  ".join(_cpp_float(X[i + 4 * c]) for c in range(4)) + "\n" for i in range(4))
         assert (tmp_path / f"o{rigid}_matrix.txt").read_text() == want
+
+
+def test_oracle_register_small_workload_vs_ref_binary(oracle, tmp_path):
+    """End to end on the 96x96x80 three-level workload (grid 6,5,4; radius 6,5,4;
+    quantisation 3,2,1; alpha 1.6, symmetric): the oracle's flow and deformed
+    image equal the unmodified reference deedsBCV binary's _displacements.dat
+    and _deformed.nii.gz bit for bit (deedsBCV0.cpp:339-356)."""
+    import refrun
+    if not refrun.available():
+        pytest.skip("reference deedsBCV not built")
+    f, mv = phantom.workload_small(0)
+    s = phantom.SCHEDULE_SMALL
+    run = refrun.RefRun(f, mv, s, tmp_path)
+    flow, warped = oracle.register(f, mv, s["grid"], s["radius"], s["quant"], s["alpha"])
+    rflow, rwarped, _, _ = run.result()
+    refrun.assert_same(flow, rflow, "flow vs reference _displacements.dat")
+    refrun.assert_same(warped, rwarped, "warped vs reference _deformed.nii.gz")
