@@ -73,6 +73,7 @@ SIGNATURES = {
     "deeds_linear_register": [V, V, I, I, I, V, I, V, V],
     "deeds_ctx_linear_register": [V, V, V, I, I, I, V, I, V, V],
     "deeds_ctx_dice": [V, V, V, ctypes.c_longlong, I, V, V, V],
+    "deeds_ctx_dice_noalloc": [V, V, V, ctypes.c_longlong, I, V, V, V],
     "deeds_regularisation": [V, V, V, V, V, V, V, I, I, F, V, V, V, I, I, I],
     "deeds_upsample_deformations": [V, V, V, V, V, V, I, I, I, I, I, I],
     "deeds_consistent_mapping": [V, V, V, V, V, V, I, I, I, I],

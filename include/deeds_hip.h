@@ -213,6 +213,24 @@ int deeds_ctx_linear_register(deeds_ctx* ctx, const float* d_fixed, const float*
 /* diceMulti.cpp:17-79 on device volumes (synchronous): as deeds_dice. */
 int deeds_ctx_dice(deeds_ctx* ctx, const int16_t* d_seg1, const int16_t* d_seg2, long long voxels, int capacity,
                    int16_t* labels, float* dice, int* count);
+/* Allocation-free Dice.  No counterpart in the reference: a variant of
+ * deeds_ctx_dice for callers that cannot allocate device or pinned memory
+ * (code under HIP graph capture, tight serving loops).  Every intermediate
+ * (presence bitmap, value->label table, per-label counters) lives in a fixed
+ * file-scope device array of about 1 MiB per GPU, held until the process ends;
+ * the call makes no hipMalloc/hipFree/hipHostMalloc and no host round trip:
+ * it only enqueues kernels on ctx's stream and returns.  Outputs are DEVICE
+ * memory: d_labels / d_dice (capacity entries; may be NULL when capacity is 0)
+ * receive the first `capacity` labels after the smallest and their Dice, as
+ * deeds_ctx_dice returns them, and *d_count (device int) their number.
+ * Limits: voxels < 2^32 (32-bit counters), capacity <= 65536; larger values
+ * are rejected with DEEDS_ERR_ARG.  All calls on one GPU share the array, so
+ * calls that overlap in time on one GPU (two streams, two threads) race and
+ * must be serialised by the caller; each call clears what it needs first.
+ * The first call on a device resolves the array's address (before any graph
+ * capture). */
+int deeds_ctx_dice_noalloc(deeds_ctx* ctx, const int16_t* d_seg1, const int16_t* d_seg2, long long voxels,
+                           int capacity, int16_t* d_labels, float* d_dice, int* d_count);
 /* primsMST.h:52 primsGraph() on a device image: the same tree as
  * deeds_prims_graph (edge SADs and the exact image statistics on the GPU,
  * Prim's algorithm on the host); ordered/parents/edgemst are host arrays of
