@@ -14,6 +14,7 @@ _deformed.nii.gz (deedsBCV0.cpp:339-356).
 
 The reference runs in the background on the host cores while the GPU works."""
 import concurrent.futures as cf
+import multiprocessing as mp
 
 import numpy as np
 import pytest
@@ -51,16 +52,12 @@ def test_crossmodal_192x160x192_equals_reference(tmp_path):
     _check_against_reference(f, mv, phantom.SCHEDULE_FULL, tmp_path, "192x160x192")
 
 
-def _atlas_pair(args):
-    seed, dims = args
-    return phantom.workload_crossmodal(seed, dims)
-
-
 def test_atlas_batch_16_pairs_equals_sequential_and_reference(tmp_path):
     print("case: 16-pair multi-atlas batch at drawn sizes", flush=True)
     dims = phantom.atlas_dims(100, 16)
-    with cf.ProcessPoolExecutor(max_workers=min(16, refrun.host_threads())) as ex:
-        pairs = list(ex.map(_atlas_pair, [(100 + i, d) for i, d in enumerate(dims)]))
+    # workers forked from a fresh server process, not from this one (its GPU runtime has threads)
+    with cf.ProcessPoolExecutor(max_workers=refrun.host_threads(), mp_context=mp.get_context("forkserver")) as ex:
+        pairs = list(ex.map(phantom.workload_crossmodal, [100 + i for i in range(16)], dims))
     s = phantom.SCHEDULE_FULL
     # the two smallest pairs also go to the reference binary, in the background
     small = sorted(range(16), key=lambda i: dims[i][0] * dims[i][1] * dims[i][2])[:2]
