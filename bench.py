@@ -115,11 +115,20 @@ def _timed(fn, steps, warmup):
 
 # ------------------------------------------------------- reference (CPU) ---
 
-def reference_run(f, m, sched, threads, budget_s):
+def _deeds_cmd(exe, d, tag, sched):
+    return [str(exe), "-F", f"{d}/f.nii.gz", "-M", f"{d}/m.nii.gz", "-O", f"{d}/{tag}",
+            "-G", "x".join(map(str, sched["grid"])), "-L", "x".join(map(str, sched["radius"])),
+            "-Q", "x".join(map(str, sched["quant"])), "-a", str(sched["alpha"])]
+
+
+def reference_run(f, m, sched, threads, budget_s, cli=False):
     """The unmodified reference deedsBCV (oracle/_ref, compiled from the reference
     sources) on the same pair with `threads` OpenMP threads: its 'Total time' span,
-    its per-level data-cost seconds, and its _displacements.dat / _deformed.nii.gz.
-    None where that build is absent."""
+    its per-level data-cost seconds, its process wall time and its
+    _displacements.dat / _deformed.nii.gz.  With `cli`, this library's deedsBCV
+    command line (deedsbcv_amd/bin) then runs on the same files: its wall time
+    and whether its output files decompress to the reference's bytes.  None
+    where the reference build is absent."""
     ref = REF_BIN / "deedsBCV"
     if not ref.exists():
         return None
@@ -127,20 +136,32 @@ def reference_run(f, m, sched, threads, budget_s):
     with tempfile.TemporaryDirectory() as d:
         nifti.write_nifti(f"{d}/f.nii.gz", f)
         nifti.write_nifti(f"{d}/m.nii.gz", m)
-        cmd = [str(ref), "-F", f"{d}/f.nii.gz", "-M", f"{d}/m.nii.gz", "-O", f"{d}/o",
-               "-G", "x".join(map(str, sched["grid"])), "-L", "x".join(map(str, sched["radius"])),
-               "-Q", "x".join(map(str, sched["quant"])), "-a", str(sched["alpha"])]
         t0 = time.perf_counter()
-        r = subprocess.run(cmd, capture_output=True, text=True, env=dict(os.environ, OMP_NUM_THREADS=str(threads)),
-                           timeout=budget_s)
+        r = subprocess.run(_deeds_cmd(ref, d, "o", sched), capture_output=True, text=True,
+                           env=dict(os.environ, OMP_NUM_THREADS=str(threads)), timeout=budget_s)
         wall = time.perf_counter() - t0
         mt = re.search(r"Total time: ([0-9.eE+-]+) sec", r.stdout)
         if r.returncode != 0 or not mt:
             return None
-        return {"total_s": float(mt.group(1)), "wall_s": wall,
-                "datacost_s": sum(float(x) for x in re.findall(r"data=([0-9.eE+-]+)", r.stdout)),
-                "flow": nifti.read_displacements(f"{d}/o_displacements.dat"),
-                "warped": nifti.read_nifti(f"{d}/o_deformed.nii.gz")[0]}
+        out = {"total_s": float(mt.group(1)), "wall_s": wall,
+               "datacost_s": sum(float(x) for x in re.findall(r"data=([0-9.eE+-]+)", r.stdout)),
+               "flow": nifti.read_displacements(f"{d}/o_displacements.dat"),
+               "warped": nifti.read_nifti(f"{d}/o_deformed.nii.gz")[0]}
+        exe = ROOT / "deedsbcv_amd" / "bin" / "deedsBCV"
+        if cli and exe.exists():
+            import gzip
+            t0 = time.perf_counter()
+            c = subprocess.run(_deeds_cmd(exe, d, "g", sched), capture_output=True, text=True, timeout=budget_s)
+            cwall = time.perf_counter() - t0
+            if c.returncode == 0:
+                same = (Path(f"{d}/g_displacements.dat").read_bytes() == Path(f"{d}/o_displacements.dat").read_bytes()
+                        and gzip.open(f"{d}/g_deformed.nii.gz").read() == gzip.open(f"{d}/o_deformed.nii.gz").read())
+                out["cli"] = {"deeds_hip_wall_s": cwall, "reference_wall_s": wall, "speedup": wall / cwall,
+                              "identical_files": same,
+                              "note": "process wall time of the deedsBCV command line on the same .nii.gz pair: "
+                                      "start-up, NIfTI read, registration, .dat and deformed .nii.gz written "
+                                      "(decompressed bytes compared)"}
+        return out
 
 
 def _same(a, b):
@@ -233,7 +254,9 @@ def plain_bench(a, name, steps, warmup, cpu=True):
     if cpu:
         threads = a.cpu_threads or host_threads()
         log(f"{name}: reference deedsBCV with {threads} OpenMP threads")
-        r = reference_run(f, m, sched, threads, a.cpu_budget)
+        r = reference_run(f, m, sched, threads, a.cpu_budget, cli=name == "full")
+        if r is not None and "cli" in r:
+            out["cli_end_to_end"] = r["cli"]
         out["cpu_baseline"] = cpu_baseline_entry(
             [r], threads, evals, "one full symmetric registration of the same pair by the unmodified reference "
                                  "(its 'Total time' span, deedsBCV0.cpp:217-336)")

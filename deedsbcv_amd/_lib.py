@@ -64,6 +64,7 @@ SIGNATURES = {
     "deeds_ctx_set_stream": [V, V],
     "deeds_ctx_synchronize": [V],
     "deeds_ctx_trim": [V],
+    "deeds_inject_fault": [I, ctypes.c_longlong],
     "deeds_descriptor": [V, V, I, I, I, I],
     "deeds_datacost": [V, V, V, I, I, I, I, I, F, F, I],
     "deeds_prims_graph": [V, V, V, V, I, I, I, I],

@@ -2,7 +2,11 @@
 // forms, one per reference function (see include/deeds_hip.h for the
 // reference file:line each replaces).
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <mutex>
 #include <cstring>
 #include <future>
 #include <memory>
@@ -33,6 +37,7 @@ void* deeds_ctx::buf(const std::string& name, size_t bytes) {
         b.bytes = 0;
     }
     size_t want = std::max<size_t>(bytes, 256);
+    if (fault_now(1)) fail(DEEDS_ERR_ALLOC, "injected device allocation failure for '" + name + "'");
     hipError_t e = hipMalloc(&b.ptr, want);
     if (e != hipSuccess) {
         b.ptr = nullptr;
@@ -58,6 +63,7 @@ void* deeds_ctx::pinned(const std::string& name, size_t bytes) {
         b.bytes = 0;
     }
     size_t want = std::max<size_t>(bytes, 256);
+    if (fault_now(1)) fail(DEEDS_ERR_ALLOC, "injected pinned allocation failure for '" + name + "'");
     hipError_t e = hipHostMalloc(&b.ptr, want, hipHostMallocDefault);
     if (e != hipSuccess) {
         b.ptr = nullptr;
@@ -67,6 +73,14 @@ void* deeds_ctx::pinned(const std::string& name, size_t bytes) {
     }
     b.bytes = want;
     return b.ptr;
+}
+
+void deeds_ctx::drain() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (aux_stream) (void)hipStreamSynchronize(aux_stream);
+    for (hipStream_t ss : side_stream)
+        if (ss) (void)hipStreamSynchronize(ss);
+    (void)hipGetLastError();
 }
 
 void deeds_ctx::release_all() {
@@ -84,6 +98,31 @@ void deeds_ctx::release_all() {
 }
 
 namespace deeds {
+
+namespace {
+// Injected faults: per kind, the number of events still to pass before one
+// fails (-1: none armed).  One-shot: the failing event disarms its kind.
+std::atomic<long long> g_fault[3] = {{-1}, {-1}, {-1}};
+std::once_flag g_fault_env;
+
+void fault_from_env() {
+    const char* e = std::getenv("DEEDS_FAULT_INJECT");
+    if (!e) return;
+    long long n = 0;
+    if (std::sscanf(e, "alloc:%lld", &n) == 1 && n >= 0) g_fault[1] = n;
+    else if (std::sscanf(e, "launch:%lld", &n) == 1 && n >= 0) g_fault[2] = n;
+}
+}  // namespace
+
+bool fault_now(int kind) {
+    std::call_once(g_fault_env, fault_from_env);
+    if (kind < 1 || kind > 2) return false;
+    long long v = g_fault[kind].load(std::memory_order_relaxed);
+    while (v >= 0) {
+        if (g_fault[kind].compare_exchange_weak(v, v - 1, std::memory_order_relaxed)) return v == 0;
+    }
+    return false;
+}
 
 namespace {
 thread_local std::string t_error;
@@ -113,6 +152,13 @@ extern "C" {
 const char* deeds_version(void) { return "deeds_hip 0.1 (gfx950)"; }
 
 const char* deeds_last_error(void) { return t_error.c_str(); }
+
+int deeds_inject_fault(int kind, long long after) {
+    if (kind != DEEDS_FAULT_ALLOC && kind != DEEDS_FAULT_LAUNCH) return DEEDS_ERR_ARG;
+    std::call_once(g_fault_env, fault_from_env);
+    g_fault[kind] = after < 0 ? -1 : after;
+    return DEEDS_OK;
+}
 
 int deeds_set_default_device(int device) {
     return guarded(nullptr, [&] {

@@ -34,7 +34,16 @@ struct Error {
         if (!(cond)) ::deeds::fail(DEEDS_ERR_ARG, std::string(msg)); \
     } while (0)
 
-#define DEEDS_LAUNCH_CHECK() DEEDS_HIP(hipGetLastError())
+// Fault injection (deeds_inject_fault, or DEEDS_FAULT_INJECT=alloc:N / launch:N
+// in the environment): true when an injected failure of `kind` is due.  Kinds:
+// 1 device or pinned allocation, 2 kernel launch.
+bool fault_now(int kind);
+
+#define DEEDS_LAUNCH_CHECK()                                                                 \
+    do {                                                                                     \
+        if (::deeds::fault_now(2)) ::deeds::fail(DEEDS_ERR_HIP, "injected kernel launch failure"); \
+        DEEDS_HIP(hipGetLastError());                                                        \
+    } while (0)
 
 // A device allocation that grows on demand and is kept for reuse.
 struct DevBuf {
@@ -65,6 +74,8 @@ struct deeds_ctx {
         return static_cast<T*>(buf(name, count * sizeof(T)));
     }
     void release_all();
+    // Waits for the context's streams, ignoring errors (after a failed call).
+    void drain();
 };
 
 namespace deeds {
@@ -81,7 +92,10 @@ int guarded(deeds_ctx* ctx, F&& f) {
         f();
         return DEEDS_OK;
     } catch (const Error& e) {
-        if (ctx) ctx->err = e.msg;
+        if (ctx) {
+            ctx->err = e.msg;
+            ctx->drain();  // nothing of the failed call is still running when the caller regains control
+        }
         set_thread_error(e.msg);
         return e.code;
     } catch (const std::bad_alloc&) {

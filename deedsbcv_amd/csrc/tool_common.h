@@ -11,8 +11,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <fstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/deeds_hip.h"
@@ -91,8 +93,29 @@ inline bool read_nifti(const std::string& path, Volume& v, bool as_short) {
     return true;
 }
 
+// One gzip member (RFC 1952: header, deflate stream at zlib's default level
+// -- gzopen's "wb" -- and CRC-32 / size trailer) of `len` bytes.
+inline bool gzip_member(const char* src, size_t len, std::vector<unsigned char>& out) {
+    z_stream zs;
+    memset(&zs, 0, sizeof(zs));
+    if (deflateInit2(&zs, Z_DEFAULT_COMPRESSION, Z_DEFLATED, 15 + 16, 8, Z_DEFAULT_STRATEGY) != Z_OK) return false;
+    out.resize(deflateBound(&zs, (uLong)len) + 32);
+    zs.next_in = (Bytef*)src;
+    zs.avail_in = (uInt)len;
+    zs.next_out = out.data();
+    zs.avail_out = (uInt)out.size();
+    const int rc = deflate(&zs, Z_FINISH);
+    out.resize(zs.total_out);
+    deflateEnd(&zs);
+    return rc == Z_STREAM_END;
+}
+
 // imageIOgzType.h:181-263: input header with dim[0..4] and datatype patched,
-// then the voxels, as one gzip stream.
+// then the voxels.  The reference writes one gzip stream with one thread; here
+// the bytes are cut into chunks compressed on the host cores and written as
+// consecutive gzip members, which gunzip, zlib's gzread (the reference's
+// reader) and Python's gzip all read as the concatenation of the chunks -- the
+// same decompressed bytes.
 inline bool write_nifti(const std::string& path, const std::vector<char>& header_in, const void* data, int m, int n,
                         int o, int16_t datatype, int16_t bitpix, size_t esz) {
     std::vector<char> out(header_in.begin(), header_in.end());
@@ -103,14 +126,34 @@ inline bool write_nifti(const std::string& path, const std::vector<char>& header
     memcpy(out.data() + 70, dt, sizeof(dt));
     size_t bytes = (size_t)m * n * o * esz;
     out.insert(out.end(), (const char*)data, (const char*)data + bytes);
-    gzFile fh = gzopen(path.c_str(), "wb");
+    const size_t total = out.size();
+    // 4 MiB chunks (the file does not depend on the machine), compressed by up to 16 threads
+    const size_t chunk = (size_t)4 << 20;
+    const int parts = (int)((total + chunk - 1) / chunk);
+    const int threads = std::max(1, std::min({parts, 16, (int)std::max(1u, std::thread::hardware_concurrency())}));
+    std::vector<std::vector<unsigned char>> z(parts);
+    std::vector<char> ok(parts, 0);
+    {
+        std::vector<std::thread> th;
+        for (int t = 0; t < threads; t++)
+            th.emplace_back([&, t] {
+                for (int i = t; i < parts; i += threads) {
+                    const size_t b = (size_t)i * chunk, len = std::min(chunk, total - b);
+                    ok[i] = gzip_member(out.data() + b, len, z[i]);
+                }
+            });
+        for (auto& t : th) t.join();
+    }
+    FILE* fh = fopen(path.c_str(), "wb");
     if (!fh) {
         printf("File error. Could not write file.\n");
         return false;
     }
-    int w = gzwrite(fh, out.data(), (unsigned)out.size());
-    gzclose(fh);
-    return w == (int)out.size();
+    bool good = true;
+    for (int i = 0; i < parts; i++) good = good && ok[i] && fwrite(z[i].data(), 1, z[i].size(), fh) == z[i].size();
+    good = (fclose(fh) == 0) && good;
+    if (!good) printf("File error. Could not write file.\n");
+    return good;
 }
 
 // deedsBCV0.cpp:152-174: four text rows, stored column-wise (X[i], X[i+4], ...).

@@ -95,6 +95,17 @@ int deeds_ctx_synchronize(deeds_ctx* ctx);
 /* Release the context's cached workspace (it is re-grown on demand). */
 int deeds_ctx_trim(deeds_ctx* ctx);
 
+/* Fault injection for testing callers' error paths (the reference's failure
+ * points are its I/O exits, imageIOgzType.h:15-17,94-96, deedsBCV0.cpp:92-99):
+ * after `after` further events of `kind` succeed, the next one fails with a
+ * status -- DEEDS_ERR_ALLOC for an allocation (device or pinned workspace),
+ * DEEDS_ERR_HIP for a kernel launch (checked after the launch, whose kernel
+ * runs normally) -- and the kind disarms; after < 0 disarms it.  The same is
+ * armed at load time by DEEDS_FAULT_INJECT=alloc:N or launch:N.  A failed
+ * call leaves no work of its own running (its context streams are drained). */
+enum deeds_fault_kind { DEEDS_FAULT_ALLOC = 1, DEEDS_FAULT_LAUNCH = 2 };
+int deeds_inject_fault(int kind, long long after);
+
 /* ---- host-pointer forms: one per reference function ---------------------- */
 
 /* MINDSSCbox.h:127 descriptor(): 12-channel MIND-SSC, 60-bit codes. */
