@@ -293,15 +293,25 @@ def atlas_bench(a, steps, cpu=True):
                           flow=torch.empty(3 * (mm // g) * (n // g) * (o // g), device="cuda"),
                           warped=torch.empty(f.shape, device="cuda")))
     kw = dict(grid=sched["grid"], radius=sched["radius"], quant=sched["quant"], alpha=sched["alpha"])
-    api.register_batch_device(pairs, concurrency=a.concurrency, **kw)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    times = None
-    for _ in range(steps):
-        times = api.register_batch_device(pairs, concurrency=a.concurrency, **kw)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    batch = [(p["flow"].cpu().numpy(), p["warped"].cpu().numpy()) for p in pairs]
+    from deedsbcv_amd import _lib
+    lib = _lib.load()
+
+    def run_batches(graphs):
+        # the first call runs eagerly, the second captures the level graphs: both untimed
+        lib.deeds_set_batch_graphs(1 if graphs else 0)
+        for _ in range(2):
+            api.register_batch_device(pairs, concurrency=a.concurrency, **kw)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        times = None
+        for _ in range(steps):
+            times = api.register_batch_device(pairs, concurrency=a.concurrency, **kw)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, times, [(p["flow"].cpu().numpy(), p["warped"].cpu().numpy()) for p in pairs]
+
+    wall_eager, _, batch_eager = run_batches(False)
+    wall, times, batch = run_batches(True)
+    log(f"atlas: batch {1e3 * wall / steps:.1f} ms with level graphs, {1e3 * wall_eager / steps:.1f} ms without")
     ctx = api.Context(0)
     ctx.register(pairs[0]["fixed"], pairs[0]["moving"], pairs[0]["flow"], pairs[0]["warped"], **kw)
     torch.cuda.synchronize()
@@ -311,7 +321,7 @@ def atlas_bench(a, steps, cpu=True):
     torch.cuda.synchronize()
     seq = time.perf_counter() - t1
     identical = all(_same(p["flow"].cpu().numpy(), b[0]) and _same(p["warped"].cpu().numpy(), b[1])
-                    for p, b in zip(pairs, batch))
+                    and _same(b[0], e[0]) and _same(b[1], e[1]) for p, b, e in zip(pairs, batch, batch_eager))
     n = len(pairs)
     evals = sum(t["hamming_evals"] for t in times)
     out = {"value": n * steps / wall, "unit": "registrations/s", "ms_per_step": 1e3 * wall / steps, "steps": steps,
@@ -320,8 +330,12 @@ def atlas_bench(a, steps, cpu=True):
                        f"{a.concurrency}", "dims": dims,
            "hamming_evals_per_batch": evals,
            "sequential_registrations_per_s": n / seq, "batch_speedup_vs_sequential": (n * steps / wall) / (n / seq),
+           "batch_without_graphs_registrations_per_s": n * steps / wall_eager,
+           "graph_speedup": wall_eager / wall,
            "identical_to_sequential": identical,
-           "compared": "flows and deformed images of every pair, batch vs one at a time, bit for bit"}
+           "compared": "flows and deformed images of every pair, batch (with and without level graphs) vs one "
+                       "at a time, bit for bit",
+           "autotuner": api.tuning_report()}
     if cpu:
         threads = a.cpu_threads or host_threads()
         log(f"atlas: reference deedsBCV on pairs 0 and 1 with {threads} threads")
@@ -679,6 +693,8 @@ def main():
     line = _line(metric, d, a, "u64 Hamming / f32", "synthetic (seeded CT-like phantom pair, B-spline warp)",
                  a.workload)
     line["host_cpus"] = len(os.sched_getaffinity(0))
+    from deedsbcv_amd import api as _api
+    line["autotuner"] = _api.tuning_report()
     if a.workload == "full" and not a.no_per_config:
         t0 = time.perf_counter()
         line["kernel_rooflines"] = kernel_rooflines(a.steps)

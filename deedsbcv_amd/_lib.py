@@ -65,6 +65,9 @@ SIGNATURES = {
     "deeds_ctx_synchronize": [V],
     "deeds_ctx_trim": [V],
     "deeds_inject_fault": [I, ctypes.c_longlong],
+    "deeds_tuning_report": [V, ctypes.c_size_t],
+    "deeds_set_datacost_tiles": [I, I],
+    "deeds_set_batch_graphs": [I],
     "deeds_descriptor": [V, V, I, I, I, I],
     "deeds_datacost": [V, V, V, I, I, I, I, I, F, F, I],
     "deeds_prims_graph": [V, V, V, V, I, I, I, I],
@@ -110,7 +113,7 @@ SIGNATURES = {
     "deeds_ctx_soft_bp_backward": [V, I, V, V, V, V, V, V, V, V, I, I, I, I, I, F, F],
 }
 _RESTYPES = {"deeds_version": c_char_p, "deeds_last_error": c_char_p, "deeds_ctx_last_error": c_char_p,
-             "deeds_soft_saved_count": ctypes.c_size_t}
+             "deeds_soft_saved_count": ctypes.c_size_t, "deeds_tuning_report": ctypes.c_size_t}
 
 _lib = None
 

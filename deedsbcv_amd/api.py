@@ -69,6 +69,16 @@ def _torch_stream(device):
     return torch.cuda.current_stream(device)
 
 
+def tuning_report():
+    """The autotuner's choices so far in this process (dict)."""
+    import json
+    lib = _lib.load()
+    n = lib.deeds_tuning_report(None, 0)
+    buf = ctypes.create_string_buffer(n + 1)
+    lib.deeds_tuning_report(buf, n + 1)
+    return json.loads(buf.value.decode())
+
+
 def descriptor(im, qs):
     """MIND-SSC codes (uint64, 12 x 5-bit thermometer fields) of image `im`."""
     im = _f32(im)

@@ -95,6 +95,7 @@ void deeds_ctx::release_all() {
         if (kv.second.ptr) (void)hipHostFree(kv.second.ptr);
     pinned_ws.clear();
     soft_cache.reset();  // it points into the released buffers
+    graph_cache.reset();  // so do the captured graphs
 }
 
 namespace deeds {
@@ -152,6 +153,27 @@ extern "C" {
 const char* deeds_version(void) { return "deeds_hip 0.1 (gfx950)"; }
 
 const char* deeds_last_error(void) { return t_error.c_str(); }
+
+int deeds_set_batch_graphs(int enabled) {
+    set_batch_graphs(enabled != 0);
+    return DEEDS_OK;
+}
+
+int deeds_set_datacost_tiles(int planes_per_workgroup, int pipelined) {
+    if (planes_per_workgroup < 0) return DEEDS_ERR_ARG;
+    pin_datacost_tiles(planes_per_workgroup, pipelined ? 1 : 0);
+    return DEEDS_OK;
+}
+
+size_t deeds_tuning_report(char* buf, size_t cap) {
+    const std::string r = "{\"datacost_rows\": " + datacost_tuning_report() + ", \"graphs\": " + graph_report() + "}";
+    if (buf && cap > 0) {
+        const size_t k = std::min(cap - 1, r.size());
+        std::memcpy(buf, r.data(), k);
+        buf[k] = 0;
+    }
+    return r.size();
+}
 
 int deeds_inject_fault(int kind, long long after) {
     if (kind != DEEDS_FAULT_ALLOC && kind != DEEDS_FAULT_LAUNCH) return DEEDS_ERR_ARG;

@@ -63,6 +63,7 @@ struct deeds_ctx {
     std::map<std::string, deeds::DevBuf> ws;
     std::map<std::string, deeds::DevBuf> pinned_ws;
     std::shared_ptr<void> soft_cache;  // last staged soft-variant tree (soft_api.cpp)
+    std::shared_ptr<void> graph_cache;  // a batch's captured level graphs (register.hip)
 
     // Device buffer `name` of at least `bytes` (contents unspecified).
     void* buf(const std::string& name, size_t bytes);
@@ -119,6 +120,10 @@ void register_pair(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, 
 // linearBCV's multi-level affine / rigid estimation on ctx (linear_host.cpp).
 void linear_register(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, int m, int n, int o,
                      const deeds_schedule& S, bool rigid, float* X_out, int* points_used);
+// Numbers of batch level-graph captures and replays so far (JSON object).
+std::string graph_report();
+// Turns the batch level graphs on or off for later batches.
+void set_batch_graphs(bool on);
 // Level-synchronous registration of `count` pairs (device pointers) on ctx;
 // timings is optional (count entries).  Pairs are validated by the caller.
 void register_group(deeds_ctx* ctx, const deeds_pair* pairs, int count, const deeds_schedule& S,

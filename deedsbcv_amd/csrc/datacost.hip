@@ -9,7 +9,14 @@
 // exact integer sums with v_xor + v_bcnt (two 32-bit halves).
 #include "kernels.h"
 
+#include <atomic>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
 
 namespace deeds {
 
@@ -254,6 +261,64 @@ void launch_pad_codes(uint64_t* Mp, const uint64_t* M, int m, int n, int o, int 
     DEEDS_LAUNCH_CHECK();
 }
 
+namespace {
+
+// ---- autotuner of the row-tiled kernel -------------------------------------
+// Per shape (step, sample skips, labels, quantisation, control grid) the
+// launcher picks the label planes per workgroup (the label slab one workgroup
+// evaluates for its row of control points: half a z-slice of planes, one, two
+// or three) and, at 15 and more labels per axis, the pipelined or the plain
+// inner loop, by timing every candidate once on the call's own data with HIP
+// events (all candidates compute the same exact integer sums, so any of them
+// leaves the right cost volume).  The choice is kept for the process.
+// DEEDS_TUNE=0 turns tuning off (one slice of planes, pipelined where allowed).
+struct RowsChoice {
+    int chunk_planes;
+    bool pipe;
+    float ms;
+};
+struct TuneKey {
+    int step, sj, sk, len, q, m1, n1, o1;
+    bool operator<(const TuneKey& b) const {
+        return std::tie(step, sj, sk, len, q, m1, n1, o1) < std::tie(b.step, b.sj, b.sk, b.len, b.q, b.m1, b.n1, b.o1);
+    }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, RowsChoice> g_tuned;
+// a caller-pinned tile shape (deeds_set_datacost_tiles): planes per workgroup (0: tune) and pipelining
+std::atomic<int> g_pin_planes{0}, g_pin_pipe{1};
+
+bool tuning_enabled() {
+    static const bool on = [] {
+        const char* e = getenv("DEEDS_TUNE");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
+}  // namespace
+
+void pin_datacost_tiles(int planes, int pipe) {
+    g_pin_planes = planes;
+    g_pin_pipe = pipe;
+}
+
+std::string datacost_tuning_report() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    std::string out = "[";
+    for (const auto& kv : g_tuned) {
+        const TuneKey& k = kv.first;
+        char buf[256];
+        snprintf(buf, sizeof buf,
+                 "%s{\"step\": %d, \"labels_per_axis\": %d, \"quant\": %d, \"grid\": [%d, %d, %d], "
+                 "\"planes_per_workgroup\": %d, \"pipelined\": %s, \"ms\": %.4f}",
+                 out.size() > 1 ? ", " : "", k.step, k.len, k.q, k.m1, k.n1, k.o1, kv.second.chunk_planes,
+                 kv.second.pipe ? "true" : "false", kv.second.ms);
+        out += buf;
+    }
+    return out + "]";
+}
+
 void launch_datacost(float* cost, const uint64_t* F, const uint64_t* Mp, int m, int n, int o, int step, int hw,
                      int q, const CostPlan& plan, hipStream_t s) {
     int m1 = m / step, n1 = n / step, o1 = o / step;
@@ -270,27 +335,32 @@ void launch_datacost(float* cost, const uint64_t* F, const uint64_t* Mp, int m, 
         const size_t smem = std::max((size_t)RJ * RK * WM * sizeof(uint64_t), (size_t)len * W * sizeof(unsigned));
         const int threads = std::min(1024, ((W + 63) / 64) * 64);
         const int len2 = len * len;
-        const int chunk_planes = len;  // one lz slice of planes per workgroup
-        const int chunks = (len2 + chunk_planes - 1) / chunk_planes;
-        const long long work = (long long)n1 * o1 * chunks;
         // (two staging loads per thread and row cover WM = W + (len-1)q when (len-1)q <= threads)
-        if (plan.skip_i == 1 && W <= 1024 && (len - 1) * q <= threads && smem <= 150 * 1024 &&
-            work < (1LL << 31) - 8) {
+        const bool rows_ok = plan.skip_i == 1 && W <= 1024 && (len - 1) * q <= threads && smem <= 150 * 1024 &&
+                             (long long)n1 * o1 * len2 < (1LL << 31) - 8;
+        // launches the row kernel with `chunk_planes` label planes per workgroup; false when no
+        // instantiation matches the shape
+        auto launch_rows = [&](int chunk_planes, bool pipe) -> bool {
+            const int chunks = (len2 + chunk_planes - 1) / chunk_planes;
+            const long long work = (long long)n1 * o1 * chunks;
             const unsigned blocks = (unsigned)(8 * ((work + 7) / 8));
 #define DEEDS_DCR_LAUNCH(ST, SJ, SK, LEN, QQ, PIPE)                                                            \
     {                                                                                                         \
-        if (smem > 65536)                                                                                     \
+        static bool attr_set = false;                                                                         \
+        if (smem > 65536 && !attr_set) {                                                                      \
             DEEDS_HIP(hipFuncSetAttribute((const void*)k_datacost_rows<ST, SJ, SK, LEN, QQ, PIPE>,            \
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));            \
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));           \
+            attr_set = true;                                                                                  \
+        }                                                                                                     \
         k_datacost_rows<ST, SJ, SK, LEN, QQ, PIPE><<<blocks, threads, smem, s>>>(                             \
             cost, F, Mp, m, n, m1, n1, plan.scale, plan.mp, plan.np, chunk_planes, chunks, (int)work);         \
         DEEDS_LAUNCH_CHECK();                                                                                 \
-        return;                                                                                               \
+        return true;                                                                                          \
     }
 #define DEEDS_DCR(ST, SJ, SK, LEN, QQ)                                                                        \
     if (step == ST && plan.skip_j == SJ && plan.skip_k == SK && len == LEN && q == QQ) {                      \
         if constexpr (LEN >= 15)                                                                              \
-            if (threads <= 256) DEEDS_DCR_LAUNCH(ST, SJ, SK, LEN, QQ, true)                                   \
+            if (pipe && threads <= 256) DEEDS_DCR_LAUNCH(ST, SJ, SK, LEN, QQ, true)                           \
         DEEDS_DCR_LAUNCH(ST, SJ, SK, LEN, QQ, false)                                                          \
     }
             // the five levels of deedsBCV's default schedule (grid 8..4, radius 8..4,
@@ -299,6 +369,49 @@ void launch_datacost(float* cost, const uint64_t* F, const uint64_t* Mp, int m, 
             DEEDS_DCR(5, 2, 2, 11, 2) DEEDS_DCR(4, 1, 1, 9, 1)
 #undef DEEDS_DCR
 #undef DEEDS_DCR_LAUNCH
+            return false;
+        };
+        if (rows_ok) {
+            const TuneKey key{step, plan.skip_j, plan.skip_k, len, q, m1, n1, o1};
+            RowsChoice choice{len, true, 0.0f};
+            bool have = false;
+            {
+                std::lock_guard<std::mutex> lk(g_tune_mu);
+                auto it = g_tuned.find(key);
+                if (it != g_tuned.end()) choice = it->second, have = true;
+            }
+            hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+            DEEDS_HIP(hipStreamIsCapturing(s, &cap));
+            if (!have && g_pin_planes.load() == 0 && tuning_enabled() && cap == hipStreamCaptureStatusNone) {
+                const int cands[4] = {(len + 1) / 2, len, 2 * len, 3 * len};
+                hipEvent_t e0, e1;
+                DEEDS_HIP(hipEventCreate(&e0));
+                DEEDS_HIP(hipEventCreate(&e1));
+                bool any = false;
+                for (int pipe = 1; pipe >= 0; pipe--) {
+                    if (pipe && !(len >= 15 && threads <= 256)) continue;
+                    for (int c : cands) {
+                        if (c > len2) continue;
+                        DEEDS_HIP(hipEventRecord(e0, s));
+                        if (!launch_rows(c, pipe != 0)) break;
+                        DEEDS_HIP(hipEventRecord(e1, s));
+                        DEEDS_HIP(hipEventSynchronize(e1));
+                        float ms = 0.0f;
+                        DEEDS_HIP(hipEventElapsedTime(&ms, e0, e1));
+                        if (!any || ms < choice.ms) choice = RowsChoice{c, pipe != 0, ms};
+                        any = true;
+                    }
+                }
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+                if (any) {
+                    std::lock_guard<std::mutex> lk(g_tune_mu);
+                    g_tuned[key] = choice;
+                }
+            }
+            const int pin = g_pin_planes.load();
+            if (pin > 0 && launch_rows(std::min(pin, len2), g_pin_pipe.load() != 0)) return;
+            if (launch_rows(choice.chunk_planes, choice.pipe)) return;
         }
     }
 #define DEEDS_DC(ST, SI, SJ, SK)                                                                              \

@@ -30,6 +30,10 @@ CostPlan datacost_plan(int m, int n, int o, int step, int hw, float quant, float
 void launch_pad_codes(uint64_t* Mp, const uint64_t* M, int m, int n, int o, int pad, hipStream_t s);
 void launch_datacost(float* cost, const uint64_t* F, const uint64_t* Mp, int m, int n, int o,
                      int step, int hw, int q, const CostPlan& plan, hipStream_t s);
+// The row-kernel tile shapes the autotuner chose so far, as a JSON array.
+std::string datacost_tuning_report();
+// Pins the row kernel's label planes per workgroup (0: autotune) and pipelining.
+void pin_datacost_tiles(int planes, int pipe);
 
 // ---- MST edge costs (mst.hip) ----------------------------------------------
 // ec[node + nb*N] = float SAD of the two step^3 blocks (0 off-grid).

@@ -18,7 +18,9 @@
 // much as one), then every pair's post-BP work.  Every kernel sees the same inputs as in a
 // registration of the pair alone, so the outputs are bit-identical.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <cstdlib>
 #include <cmath>
 #include <condition_variable>
 #include <cstring>
@@ -212,6 +214,7 @@ class PairRun {
             max_cost = std::max(max_cost, (size_t)c.sz1 * c.len3);
             max_sz1 = std::max(max_sz1, (size_t)c.sz1);
         }
+        for (const LevelCfg& c : lv_) hamming_ += 2.0 * (double)c.sz1 * c.len3 * (double)c.plan.samples;
         DEEDS_REQUIRE(m >= 2 * max_qs + 1 && n >= 2 * max_qs + 1 && o >= 2 * max_qs + 1,
                       "volume too small for the MIND patch radius of the schedule");
         DEEDS_REQUIRE(m <= mind_max_dim() && n <= mind_max_dim() && o <= mind_max_dim(),
@@ -274,6 +277,30 @@ class PairRun {
 
     int levels() const { return L_; }
     const LevelCfg& level(int l) const { return lv_[l]; }
+    // Graph mode (a batch level's pre- and post-BP work captured into, or
+    // replayed from, a HIP graph): the per-stage timing events are not recorded
+    // (a captured event record is a dependency, not a timestamp), and the
+    // cross-stream joins are made by the driver around the graph launch.
+    void set_graph_mode(bool on) { graph_ = on; }
+    // While a level is being captured, no other thread may touch the HIP
+    // runtime on this context's behalf (an event query or a copy against a
+    // capturing stream would invalidate the capture).  In that mode the tree
+    // tasks wait for their edge SADs on a host flag, which wait_helpers sets
+    // after this thread has waited for the SADs and the image statistics.
+    void set_host_sad_wait(bool on) { host_sads_ = on; }
+    void wait_helpers() {
+        for (int l = 0; l < L_; l++) DEEDS_HIP(hipEventSynchronize(ev_sad_[l]->e));
+        release_helpers();
+        for (int d = 0; d < 2; d++) sd_fut_[d].wait();
+    }
+    // Sets the flag if it is not set yet (also on an error path, so that no
+    // tree task waits for it forever).
+    void release_helpers() {
+        if (sad_set_) return;
+        for (int l = 0; l < L_; l++) (void)hipEventSynchronize(ev_sad_[l]->e);
+        sad_set_ = true;
+        sad_promise_.set_value();
+    }
 
     // Inputs, affine warp, image statistics and edge SADs (s0).
     void begin() {
@@ -339,12 +366,17 @@ class PairRun {
             std::shared_future<float> sdf = sd_fut_[dir];
             double* spent = &build_time_[2 * l + dir];
             hipEvent_t sad_event = ev_sad_[l]->e;
+            const bool host_wait = host_sads_;
+            std::shared_future<void> sads = sad_ready_;
             // the schedule is packed into this (level, direction)'s pinned slot by the
             // building thread, so the launching thread only enqueues the copy
             char* slot = h_stage_ + stage_off_[l] + dir * ((stage_off_[l + 1] - stage_off_[l]) / 2);
             void* d_stage = d_stage_[dir];
-            trees_[dir][l] = pool.submit([ec, c, sdf, spent, sad_event, slot, d_stage]() {
-                DEEDS_HIP(hipEventSynchronize(sad_event));
+            trees_[dir][l] = pool.submit([ec, c, sdf, spent, sad_event, slot, d_stage, host_wait, sads]() {
+                if (host_wait)
+                    sads.wait();
+                else
+                    DEEDS_HIP(hipEventSynchronize(sad_event));
                 auto tb = std::chrono::steady_clock::now();
                 auto check = std::async(std::launch::async, [ec, c, sdf]() {
                     return weights_distinguish_costs(ec, c.step, c.m1, c.n1, c.o1, sdf.get());
@@ -412,7 +444,7 @@ class PairRun {
     // (the `first` direction's data cost before the other's; -1: concurrent).
     void pre_bp(int l, int first = -1) {
         const LevelCfg& c = lv_[l];
-        DEEDS_HIP(hipEventRecord(EV(l, 0), s0_));
+        if (!graph_) DEEDS_HIP(hipEventRecord(EV(l, 0), s0_));
         if (c.new_mind) {
             // both directions' MIND of warped0 / im1b (deedsBCV0.cpp:230-236)
             launch_descriptor(mind_mov_, warped0_, in_.m, in_.n, in_.o, c.mind_qs, sh_.mw[0][0], sh_.mw[0][1], s0_);
@@ -436,19 +468,18 @@ class PairRun {
             else
                 launch_warp_image_grid(sh_.warped[1], im1b_, u0, u0 + n1, u0 + 2 * n1, in_.m, in_.n, in_.o, c.m1,
                                        c.n1, c.o1, s);
-            if (dir == 0) DEEDS_HIP(hipEventRecord(EV(l, 1), s));
+            if (dir == 0 && !graph_) DEEDS_HIP(hipEventRecord(EV(l, 1), s));
             launch_descriptor(sh_.mind_w[dir], sh_.warped[dir], in_.m, in_.n, in_.o, c.mind_qs, sh_.mw[dir][0],
                               sh_.mw[dir][1], s);
-            if (dir == 0) DEEDS_HIP(hipEventRecord(EV(l, 2), s));
+            if (dir == 0 && !graph_) DEEDS_HIP(hipEventRecord(EV(l, 2), s));
             launch_pad_codes(sh_.mpad[dir], sh_.mind_w[dir], in_.m, in_.n, in_.o, c.plan.pad, s);
             // the second direction's data cost waits for the first's
             if (first >= 0 && dir != first) DEEDS_HIP(hipStreamWaitEvent(s, DC(l, first, 1), 0));
-            DEEDS_HIP(hipEventRecord(DC(l, dir, 0), s));
+            if (!graph_) DEEDS_HIP(hipEventRecord(DC(l, dir, 0), s));
             launch_datacost(cost_[dir], dir ? mind_mov_ : mind_fix_, sh_.mpad[dir], in_.m, in_.n, in_.o, c.step,
                             c.hw, c.quant, c.plan, s);
-            DEEDS_HIP(hipEventRecord(DC(l, dir, 1), s));
-            if (dir == 0) DEEDS_HIP(hipEventRecord(EV(l, 3), s));
-            hamming_ += (double)c.sz1 * c.len3 * (double)c.plan.samples;
+            if (!graph_) DEEDS_HIP(hipEventRecord(DC(l, dir, 1), s));
+            if (dir == 0 && !graph_) DEEDS_HIP(hipEventRecord(EV(l, 3), s));
         }
     }
 
@@ -488,8 +519,16 @@ class PairRun {
         launch_consistent(f, cmtmp_, c.m1, c.n1, c.o1, c.step, s0_);
         launch_jacobian(jac_ + 4 * l, newf_, newf_ + n1, newf_ + 2 * n1, c.m1, c.n1, c.o1, c.step, s0_);
         DEEDS_HIP(hipMemcpyAsync(prevf_, newf_, 6 * n1 * sizeof(float), hipMemcpyDeviceToDevice, s0_));
-        DEEDS_HIP(hipEventRecord(EV(l, 5), s0_));
-        DEEDS_HIP(hipStreamWaitEvent(s1_, EV(l, 5), 0));
+        if (!graph_) {
+            DEEDS_HIP(hipEventRecord(EV(l, 5), s0_));
+            DEEDS_HIP(hipStreamWaitEvent(s1_, EV(l, 5), 0));
+        }
+        post_bp_state(l);
+    }
+    // The host state post_bp leaves for the next level (also when its work
+    // came from a replayed graph): the previous grid is this level's.
+    void post_bp_state(int l) {
+        const LevelCfg& c = lv_[l];
         m2_ = c.m1;
         n2_ = c.n1;
         o2_ = c.o1;
@@ -516,7 +555,7 @@ class PairRun {
         if (!timing) return;
         std::memset(timing, 0, sizeof(*timing));
         timing->levels = L_;
-        for (int l = 0; l < L_; l++) {
+        for (int l = 0; l < L_ && !graph_; l++) {
             timing->transf[l] = elapsed_s(EV(l, 0), EV(l, 1)) + elapsed_s(EV(l, 4), EV(l, 5));
             timing->mind[l] = elapsed_s(EV(l, 1), EV(l, 2));
             timing->data[l] = elapsed_s(EV(l, 2), EV(l, 3));
@@ -528,15 +567,16 @@ class PairRun {
         timing->stats_s = std::max(stats_time_[0], stats_time_[1]);
         for (int l = 0; l < L_; l++) {
             double lvl = 0.0;
-            for (int dir = 0; dir < 2; dir++) lvl += elapsed_s(DC(l, dir, 0), DC(l, dir, 1));
+            if (!graph_)
+                for (int dir = 0; dir < 2; dir++) lvl += elapsed_s(DC(l, dir, 0), DC(l, dir, 1));
             timing->datacost_level_s[l] = lvl;
             timing->datacost_level_evals[l] = 2.0 * lv_[l].sz1 * lv_[l].len3 * (double)lv_[l].plan.samples;
             timing->datacost_kernel_s += lvl;
             timing->tree_build_s += build_time_[2 * l] + build_time_[2 * l + 1];
             timing->tree_build_level_s[l] = std::max(build_time_[2 * l], build_time_[2 * l + 1]);
             timing->tree_wait_level_s[l] = wait_level_[l];
-            timing->datacost_launches += 2;
-            for (int dir = 0; dir < 2; dir++)
+            timing->datacost_launches += graph_ ? 0 : 2;
+            for (int dir = 0; dir < 2 && !graph_; dir++)
                 timing->bp_idle_level_s[l] += std::max(0.0, elapsed_s(DC(l, dir, 1), ev_pre_[2 * l + dir]->e));
             timing->bp_level_s[l] = elapsed_s(ev_pre_[2 * l]->e, EV(l, 4));
         }
@@ -596,6 +636,11 @@ class PairRun {
     int m2_ = 0, n2_ = 0, o2_ = 0;
     size_t sz2_ = 0;
     double hamming_ = 0.0;
+    bool graph_ = false;
+    bool host_sads_ = false;
+    bool sad_set_ = false;
+    std::promise<void> sad_promise_;
+    std::shared_future<void> sad_ready_ = sad_promise_.get_future().share();
     Event ev_start_, ev_stop_, ev_init_, ev_mind_, ev_w0_;
     std::vector<std::unique_ptr<Event>> ev_sad_;  // per level: its edge SADs are on the host
     std::vector<std::unique_ptr<Event>> ev_lvl_, ev_dc_, ev_pre_;
@@ -615,6 +660,101 @@ class PairRun {
     std::vector<char> have_[2];
     std::unique_ptr<Tree> tree_[2];
 };
+
+// HIP graphs of a batch's levels.  A group of pairs runs each level's pre-BP
+// work (MIND, warps, padding, data costs of every pair) and post-BP work
+// (consistent mapping, logging Jacobian, field copy) as fixed launch
+// sequences whose shapes and buffers depend only on the pairs and the
+// schedule, so they are captured into one graph per level and phase and
+// replayed when the same group comes again (the multi-atlas case: one target,
+// the same atlases, call after call).  The BP between them depends on the
+// host-built trees and stays on the streams.  The first call with a given key
+// (pairs' buffers, sizes and affines, the schedule, the context's workspace
+// addresses) runs eagerly, the second captures and launches, later ones
+// replay.  DEEDS_GRAPHS=0 turns graphs off.
+std::atomic<long long> g_graph_captures{0}, g_graph_replays{0};
+
+struct LevelGraphs {
+    std::vector<uint64_t> key, pending;
+    std::vector<hipGraphExec_t> pre, post;
+    void clear() {
+        for (hipGraphExec_t g : pre)
+            if (g) (void)hipGraphExecDestroy(g);
+        for (hipGraphExec_t g : post)
+            if (g) (void)hipGraphExecDestroy(g);
+        pre.clear();
+        post.clear();
+        key.clear();
+    }
+    ~LevelGraphs() { clear(); }
+};
+
+// -1: not set yet (the environment decides), else 0 / 1 from deeds_set_batch_graphs
+std::atomic<int> g_graphs_on{-1};
+
+bool graphs_enabled() {
+    int v = g_graphs_on.load();
+    if (v < 0) {
+        const char* e = getenv("DEEDS_GRAPHS");
+        v = (e && e[0] == '0') ? 0 : 1;
+        int expect = -1;
+        g_graphs_on.compare_exchange_strong(expect, v);
+        v = g_graphs_on.load();
+    }
+    return v != 0;
+}
+
+std::vector<uint64_t> group_key(deeds_ctx* ctx, const std::vector<PairIn>& pairs, const deeds_schedule& S) {
+    std::vector<uint64_t> k;
+    auto put = [&](uint64_t v) { k.push_back(v); };
+    put(pairs.size());
+    put((uint64_t)S.levels);
+    uint32_t ab;
+    std::memcpy(&ab, &S.alpha, 4);
+    put(ab);
+    for (int l = 0; l < S.levels; l++) put(((uint64_t)S.grid[l] << 40) | ((uint64_t)S.radius[l] << 20) | S.quant[l]);
+    for (const PairIn& p : pairs) {
+        put((uintptr_t)p.fixed);
+        put((uintptr_t)p.moving);
+        put((uintptr_t)p.flow);
+        put((uintptr_t)p.warped);
+        put(((uint64_t)p.m << 42) | ((uint64_t)p.n << 21) | (uint64_t)p.o);
+        for (int i = 0; i < 16; i++) {
+            uint32_t b = 0;
+            if (p.X) std::memcpy(&b, &p.X[i], 4);
+            put(p.X ? b : 0xffffffffull);
+        }
+    }
+    for (const auto& kv : ctx->ws) {
+        put((uintptr_t)kv.second.ptr);
+        put(kv.second.bytes);
+    }
+    return k;
+}
+
+// Captures what `enqueue` queues on s0 (and on streams that join it) into a
+// graph and instantiates it; a failed capture is ended before the error
+// propagates, so the stream leaves capture mode.
+template <class F>
+hipGraphExec_t capture_graph(hipStream_t s0, F&& enqueue) {
+    DEEDS_HIP(hipStreamBeginCapture(s0, hipStreamCaptureModeRelaxed));
+    try {
+        enqueue();
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(s0, &g);
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        throw;
+    }
+    hipGraph_t g = nullptr;
+    DEEDS_HIP(hipStreamEndCapture(s0, &g));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) fail(DEEDS_ERR_HIP, std::string("hipGraphInstantiate failed: ") + hipGetErrorString(e));
+    return ex;
+}
 
 // BP staging of one level for `trees` trees with `nodes` nodes in total,
 // 256-byte aligned.
@@ -652,19 +792,71 @@ void run_level_sync(deeds_ctx* ctx, const std::vector<PairIn>& pairs, const deed
     std::vector<std::unique_ptr<PairRun>> runs;
     for (int p = 0; p < P; p++)
         runs.emplace_back(new PairRun(ctx, P == 1 ? std::string() : "b" + std::to_string(p) + ".", pairs[p], S, sh));
+    // graphs for the group's pre- and post-BP work of every level
+    LevelGraphs* G = nullptr;
+    bool replay = false, capture = false;
+    if (P > 1 && graphs_enabled()) {
+        if (!ctx->graph_cache) ctx->graph_cache = std::make_shared<LevelGraphs>();
+        G = static_cast<LevelGraphs*>(ctx->graph_cache.get());
+        const std::vector<uint64_t> key = group_key(ctx, pairs, S);
+        replay = G->key == key && (int)G->pre.size() == L && (int)G->post.size() == L;
+        capture = !replay && G->pending == key;
+        if (!replay) {
+            G->clear();
+            G->pending = key;
+        }
+        if (capture) {
+            G->pre.assign(L, nullptr);
+            G->post.assign(L, nullptr);
+        }
+        for (auto& r : runs) {
+            r->set_graph_mode(replay || capture);
+            r->set_host_sad_wait(capture);
+        }
+    }
+    Event ev_graph;
+    // after a graph launched on s0, s1 continues behind it
+    auto s1_behind_s0 = [&] {
+        DEEDS_HIP(hipEventRecord(ev_graph.e, s0));
+        DEEDS_HIP(hipStreamWaitEvent(s1, ev_graph.e, 0));
+    };
     // declared after the runs: destroyed (drained and joined) before them
     TaskPool pool(tree_threads);
+    // destroyed before the pool: no tree task is left waiting for a host flag
+    struct ReleaseHelpers {
+        std::vector<std::unique_ptr<PairRun>>& runs;
+        bool on;
+        ~ReleaseHelpers() {
+            if (on)
+                for (auto& r : runs) r->release_helpers();
+        }
+    } release_guard{runs, capture};
     for (auto& r : runs) r->begin();
     // trees in the order they are needed: level by level over the pairs
     for (int l = 0; l < L; l++)
         for (auto& r : runs) r->submit_trees(pool, l);
+    if (capture)
+        for (auto& r : runs) r->wait_helpers();
     Event ev_join;
     for (int l = 0; l < L; l++) {
         // one pair: the direction with the longer expected BP computes its data cost first
-        if (P == 1)
+        if (P == 1) {
             runs[0]->pre_bp(l, runs[0]->first_direction(l));
-        else
+        } else if (replay) {
+            DEEDS_HIP(hipGraphLaunch(G->pre[l], s0));
+            s1_behind_s0();
+        } else if (capture) {
+            G->pre[l] = capture_graph(s0, [&] {
+                for (auto& r : runs) r->pre_bp(l);
+                // the backward direction's branch (s1) joins s0 again
+                DEEDS_HIP(hipEventRecord(ev_graph.e, s1));
+                DEEDS_HIP(hipStreamWaitEvent(s0, ev_graph.e, 0));
+            });
+            DEEDS_HIP(hipGraphLaunch(G->pre[l], s0));
+            s1_behind_s0();
+        } else {
             for (auto& r : runs) r->pre_bp(l);
+        }
         // each direction's trees of all pairs in one BP launch sequence on that direction's stream
         const LevelCfg& c = runs[0]->level(l);
         // one pair: the BP whose tree is built first is launched first (the
@@ -682,7 +874,26 @@ void run_level_sync(deeds_ctx* ctx, const std::vector<PairIn>& pairs, const deed
         }
         DEEDS_HIP(hipEventRecord(ev_join.e, s1));
         DEEDS_HIP(hipStreamWaitEvent(s0, ev_join.e, 0));
-        for (auto& r : runs) r->post_bp(l);
+        if (replay) {
+            DEEDS_HIP(hipGraphLaunch(G->post[l], s0));
+            for (auto& r : runs) r->post_bp_state(l);
+            s1_behind_s0();
+        } else if (capture) {
+            G->post[l] = capture_graph(s0, [&] {
+                for (auto& r : runs) r->post_bp(l);
+            });
+            DEEDS_HIP(hipGraphLaunch(G->post[l], s0));
+            s1_behind_s0();
+        } else {
+            for (auto& r : runs) r->post_bp(l);
+        }
+    }
+    if (G && capture) {
+        G->key = G->pending;
+        G->pending.clear();
+        g_graph_captures++;
+    } else if (G && replay) {
+        g_graph_replays++;
     }
     for (auto& r : runs) r->finish();
     DEEDS_HIP(hipStreamSynchronize(s0));
@@ -696,6 +907,13 @@ void register_pair(deeds_ctx* ctx, const float* d_fixed, const float* d_moving, 
                    const deeds_schedule& S, float* d_flow, float* d_warped, deeds_timing* timing) {
     const std::vector<PairIn> one{PairIn{d_fixed, d_moving, X, m, n, o, d_flow, d_warped, timing}};
     run_level_sync(ctx, one, S, 2 * S.levels);
+}
+
+void set_batch_graphs(bool on) { g_graphs_on = on ? 1 : 0; }
+
+std::string graph_report() {
+    return "{\"batch_graph_captures\": " + std::to_string(g_graph_captures.load()) +
+           ", \"batch_graph_replays\": " + std::to_string(g_graph_replays.load()) + "}";
 }
 
 void register_group(deeds_ctx* ctx, const deeds_pair* pairs, int count, const deeds_schedule& S,

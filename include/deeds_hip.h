@@ -95,6 +95,19 @@ int deeds_ctx_synchronize(deeds_ctx* ctx);
 /* Release the context's cached workspace (it is re-grown on demand). */
 int deeds_ctx_trim(deeds_ctx* ctx);
 
+/* The tile shapes the library's autotuner has chosen in this process so far
+ * (per data-cost shape: label planes per workgroup, pipelined inner loop, the
+ * measured kernel time) and the number of batch level graphs captured and
+ * replayed, as a JSON object; writes at most cap-1 bytes plus a
+ * terminating zero into buf (may be NULL) and returns the full length.  No
+ * reference counterpart (the reference has fixed loops). */
+size_t deeds_tuning_report(char* buf, size_t cap);
+/* Pins the data-cost tile shape for later calls in this process instead of
+ * the autotuner's choice: label planes per workgroup (0 restores tuning) and
+ * the pipelined inner loop (used where the shape allows it).  Every shape gives
+ * the same cost volume. */
+int deeds_set_datacost_tiles(int planes_per_workgroup, int pipelined);
+
 /* Fault injection for testing callers' error paths (the reference's failure
  * points are its I/O exits, imageIOgzType.h:15-17,94-96, deedsBCV0.cpp:92-99):
  * after `after` further events of `kind` succeed, the next one fails with a
@@ -189,6 +202,14 @@ int deeds_register_batch(const deeds_pair* pairs, int count, const deeds_schedul
                          deeds_timing* timings);
 int deeds_register_batch_device(int device, const deeds_pair* pairs, int count, const deeds_schedule* sched,
                                 int concurrency, deeds_timing* timings);
+/* Batch level graphs (no reference counterpart).  A group's pre- and post-BP
+ * work of every level is a fixed launch sequence for given pairs and
+ * schedule; the second batch call with the same pairs (same buffers, sizes,
+ * affines) captures it into one HIP graph per level and phase, and later calls
+ * replay the graphs (the BP between them, which depends on the host-built
+ * trees, stays on the streams).  Results are identical either way.  On by
+ * default (DEEDS_GRAPHS=0 in the environment turns it off at load). */
+int deeds_set_batch_graphs(int enabled);
 
 /* applyBCV.cpp:161-198 / applyBCVfloat.cpp:186-198: apply a _displacements.dat
  * field (3*sz1 floats on grid `grid_step`) to a label volume (nearest) or an

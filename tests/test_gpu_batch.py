@@ -61,3 +61,39 @@ def test_batch_rejects_bad_pair():
     f, m = phantom.make_pair((40, 40, 40), 1)
     with pytest.raises(api.DeedsError):
         api.register_batch([(f, m), (f[:2], m[:2])], concurrency=2, **SCHED)
+
+
+def test_batch_graph_replays_equal_sequential():
+    """Device batches of the same pairs call after call: the first runs its
+    level phases on the streams, the second captures them into HIP graphs, the
+    later ones replay the graphs.  Every call's flows and deformed images equal
+    one-at-a-time registrations (eager work and host synchronisations between
+    the calls)."""
+    torch = pytest.importorskip("torch")
+    print("case: batch graph capture and replays", flush=True)
+    pairs = _pairs()
+    seq = [api.register(p[0], p[1], X=p[2] if len(p) > 2 else None, **SCHED)[:2] for p in pairs]
+    dev = []
+    g = SCHED["grid"][-1]
+    for p in pairs:
+        f = torch.from_numpy(p[0].astype(np.float32)).cuda()
+        m = torch.from_numpy(p[1].astype(np.float32)).cuda()
+        o, n, mm = f.shape
+        dev.append(dict(fixed=f, moving=m, X=p[2] if len(p) > 2 else None,
+                        flow=torch.empty(3 * (mm // g) * (n // g) * (o // g), device="cuda"),
+                        warped=torch.empty_like(f)))
+    before = api.tuning_report()["graphs"]
+    for call in range(5):
+        for d in dev:
+            d["flow"].fill_(-1.0)
+            d["warped"].fill_(-1.0)
+        torch.cuda.synchronize()
+        api.register_batch_device(dev, concurrency=5, **SCHED)
+        torch.cuda.synchronize()
+        for i, (d, (flow, warped)) in enumerate(zip(dev, seq)):
+            assert np.array_equal(d["flow"].cpu().numpy(), flow), (call, i)
+            assert np.array_equal(d["warped"].cpu().numpy(), warped), (call, i)
+        _ = api.register(pairs[0][0], pairs[0][1], **SCHED)  # eager work between the calls
+    after = api.tuning_report()["graphs"]
+    assert after["batch_graph_captures"] >= before["batch_graph_captures"] + 1
+    assert after["batch_graph_replays"] >= before["batch_graph_replays"] + 3

@@ -457,3 +457,26 @@ def test_context_rejects_bad_tensors():
         ctx.register(f, f, flow[:10], grid=(4,), radius=(1,), quant=(1,))
     with pytest.raises(ValueError):
         ctx.descriptor(f.cpu(), 1)
+
+
+@pytest.mark.parametrize("case", [(8, 8, 5), (7, 7, 4), (4, 4, 1)])
+def test_cost_every_tile_shape_vs_oracle(oracle, case):
+    """The autotuner's candidates (label planes per workgroup, pipelined or
+    plain inner loop), each pinned in turn, give the oracle's cost volume."""
+    from deedsbcv_amd import _lib
+    step, hw, q = case
+    print(f"case: datacost tile shapes {case}", flush=True)
+    f, mv = phantom.make_pair((96, 40, 36), 33)
+    F = oracle.descriptor(f.astype(np.float32) + 1024, 2)
+    M = oracle.descriptor(mv.astype(np.float32) + 1024, 2)
+    want = oracle.datacost(F, M, step, hw, q, 1.6, 1)
+    len_ = 2 * hw + 1
+    lib = _lib.load()
+    try:
+        for planes in ((len_ + 1) // 2, len_, 2 * len_, 3 * len_, len_ * len_):
+            for pipe in (1, 0):
+                assert lib.deeds_set_datacost_tiles(planes, pipe) == 0
+                _eq(api.dataCostCL(F, M, step, hw, q, 1.6, 1), want, f"cost {case} planes={planes} pipe={pipe}")
+    finally:
+        lib.deeds_set_datacost_tiles(0, 1)
+    assert api.tuning_report()["datacost_rows"] is not None
