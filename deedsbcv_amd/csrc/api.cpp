@@ -353,7 +353,7 @@ BPDevTree pack_tree(const Tree& t, void* pinned, void* d_stage, size_t* bytes) {
     size_t at = 0;
     BPDevTree out{};
     auto put = [&](const std::vector<int>& v) {
-        const int* dst = di + at;
+        int* dst = di + at;
         std::copy(v.begin(), v.end(), hi + at);
         at += v.size();
         return dst;
@@ -363,6 +363,7 @@ BPDevTree pack_tree(const Tree& t, void* pinned, void* d_stage, size_t* bytes) {
     out.depth_start = put(t.depth_start);
     out.prog_start = put(t.prog_start);
     out.prog = put(t.prog);
+    out.n_segments = (int)t.heads.size();
     out.leaves = put(t.leaves);
     out.leaf_w = put(t.leaf_w);
     out.n_leaves = (int)t.leaves.size();
@@ -428,7 +429,7 @@ void run_bp_batch(const std::vector<BPBatchItem>& items, int hw, float quant, vo
         hd[t] = BPBatchTree{it.cost,      it.inds,    d.prog_start, d.prog,  d.leaves, d.leaf_w, d.off3,
                             d.parents,    d.ordered,  d.depth_start, d.sel_info, it.u0,  it.v0,    it.w0,
                             it.u1,        it.v1,      it.w1,        d.sel_labs, d.root, it.tree->N,
-                            it.tree->maxd};
+                            it.tree->maxd, d.n_segments};
     }
     // work list: the leaves of every tree, then round by round every tree's segments of that round
     std::vector<int> round_off(1, 0);

@@ -33,6 +33,7 @@
 // selection workgroup per tree; a workgroup finds its tree and item in a work
 // list, so each tree's arithmetic is exactly its single-tree arithmetic.
 #include "kernels.h"
+#include "mst_host.h"
 
 namespace deeds {
 
@@ -69,11 +70,12 @@ struct BPShape {
 // One pass over all lines: line `ln` has elements base(ln) + j*STRIDE,
 // j = 0..LEN-1; output i of the line goes to base(ln) + i*STRIDE.
 // MODE 0: y pass (I1 = j + line*LEN), 1: x pass (I2 = I1[src of j]),
-// 2: z pass (allinds = I2[src of j]).  Returns the thread's minimum output.
+// 2: z pass (I1 = I2[src of j], the composite argmin table).  Returns the
+// thread's minimum output.
 template <int LEN, int MODE>
 __device__ __forceinline__ float dt_pass(const float* __restrict__ src, float* __restrict__ dst,
                                          const float* __restrict__ zt, const int16_t* __restrict__ iin,
-                                         int16_t* __restrict__ iout, int16_t* __restrict__ gout) {
+                                         int16_t* __restrict__ iout) {
     using B = BPShape<LEN>;
     constexpr int STRIDE = MODE == 0 ? 1 : MODE == 1 ? LEN : B::L2;
     const int t = threadIdx.x;
@@ -116,30 +118,45 @@ __device__ __forceinline__ float dt_pass(const float* __restrict__ src, float* _
             dst[o] = best;
             mn = best < mn ? best : mn;
             if (MODE == 0) iout[o] = (int16_t)(bj + line * LEN);
-            else if (MODE == 1) iout[o] = iin[base + bj * STRIDE];
-            else gout[o] = iin[base + bj * STRIDE];
+            else iout[o] = iin[base + bj * STRIDE];
         }
     }
     return mn;
 }
 
+// One DPP step of a wave minimum: v = min(v, v of the lane CTRL selects);
+// lanes of rows outside ROWS (or without a source lane) keep v.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_min(float v) {
+    const float o = __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, ROWS, 0xf, false));
+    return o < v ? o : v;
+}
+
+// Minimum over the workgroup.  Within a wave by DPP (each lane of a row of 16
+// takes its row's minimum by quad swaps and mirrors, then the row broadcasts
+// carry rows 0-2 into lane 63) instead of a chain of LDS shuffles; across
+// waves through `red` and one barrier.  Every lane of the workgroup must call it.
 __device__ __forceinline__ float block_min(float v, float* red) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        float o = __shfl_xor(v, off, 64);
-        v = o < v ? o : v;
-    }
+    v = dpp_min<0xb1, 0xf>(v);   // quad_perm [1,0,3,2]
+    v = dpp_min<0x4e, 0xf>(v);   // quad_perm [2,3,0,1]
+    v = dpp_min<0x141, 0xf>(v);  // row_half_mirror
+    v = dpp_min<0x140, 0xf>(v);  // row_mirror
+    v = dpp_min<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v = dpp_min<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+    v = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) red[wid] = v;
     __syncthreads();
     float r = red[0];
     for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = red[w] < r ? red[w] : r;
-    __syncthreads();
+    // no second barrier: `red` is written again only after the next message's
+    // table and pass barriers, which every thread reaches after this read
     return r;
 }
 
 // The schedule (programs, leaves) and the parent offsets are written before
-// a BP launch (host copies, k_parent_offsets) and only read inside it.  Read
+// a BP launch (host copies, k_parent_offsets, k_record_offsets) and only read inside it.  Read
 // through the constant address space, their uniform loads become scalar
 // loads: a vector load would wait (vmcnt) behind the node's outstanding
 // argmin and message stores before the next node's addresses are known.
@@ -173,9 +190,51 @@ __global__ void k_parent_offsets(float* __restrict__ off3, const int* __restrict
     off3[3 * c + 2] = (w0[p] - w0[c]) / quant;
 }
 
+// The same offsets written into the records of program block h (fields 4-6),
+// where the BP reads them with the rest of the record.
+__device__ __forceinline__ void record_offsets(int* __restrict__ prog, const int* __restrict__ prog_start, int h,
+                                               const int* __restrict__ parents, const float* __restrict__ u0,
+                                               const float* __restrict__ v0, const float* __restrict__ w0,
+                                               float quant) {
+    int* blk = prog + prog_start[h];
+    const int nodes = blk[0];
+    for (int i = 0; i < nodes; i++) {
+        int* r = blk + kProgRec * (1 + i);
+        const int c = r[0], p = parents[c];
+        if (p < 0) continue;
+        r[4] = __float_as_int((u0[p] - u0[c]) / quant);
+        r[5] = __float_as_int((v0[p] - v0[c]) / quant);
+        r[6] = __float_as_int((w0[p] - w0[c]) / quant);
+    }
+}
+
+__global__ void k_record_offsets(int* __restrict__ prog, const int* __restrict__ prog_start, int segments,
+                                 const int* __restrict__ parents, const float* __restrict__ u0,
+                                 const float* __restrict__ v0, const float* __restrict__ w0, float quant) {
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h < segments) record_offsets(prog, prog_start, h, parents, u0, v0, w0, quant);
+}
+
+// The composite argmin table, built in I1 by the z pass, to the node's row of
+// allinds: contiguous 16-bit stores, none under a branch (lanes past the row
+// store its last entry again, the same value), so a later wait for a load
+// counts them exactly instead of waiting for every outstanding store.
+template <int LEN>
+__device__ __forceinline__ void store_table(int16_t* __restrict__ allinds_c, const int16_t* __restrict__ I1) {
+    using B = BPShape<LEN>;
+    constexpr int ELEMS = (B::L3 + B::BLOCK - 1) / B::BLOCK;
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int e = 0; e < ELEMS; e++) {
+        const int l = (e + 1) * B::BLOCK <= B::L3 ? t + e * B::BLOCK : min(t + e * B::BLOCK, B::L3 - 1);
+        allinds_c[l] = I1[l];
+    }
+}
+
 // One message: A holds the sender's cost scaled by its edge weight.  Runs the
 // three passes (the argmin table goes to allinds_c, the message to Bf) and
-// returns the message's minimum; every thread may read Bf afterwards.
+// returns the message's minimum; every thread may read Bf afterwards, and I1
+// holds the table until the next message's first barrier.
 template <int LEN>
 __device__ __forceinline__ float message(int16_t* allinds_c, float ox, float oy, float oz, float* A, float* Bf,
                                          int16_t* I1, int16_t* I2, float* zy, float* zx, float* zz, float* red) {
@@ -190,12 +249,13 @@ __device__ __forceinline__ float message(int16_t* allinds_c, float ox, float oy,
         zz[q] = dz * dz;
     }
     __syncthreads();
-    dt_pass<LEN, 0>(A, Bf, zy, nullptr, I1, nullptr);
+    dt_pass<LEN, 0>(A, Bf, zy, nullptr, I1);
     __syncthreads();
-    dt_pass<LEN, 1>(Bf, A, zx, I1, I2, nullptr);
+    dt_pass<LEN, 1>(Bf, A, zx, I1, I2);
     __syncthreads();
-    float mn = dt_pass<LEN, 2>(A, Bf, zz, I2, nullptr, allinds_c);
-    return block_min(mn, red);  // its barriers also complete Bf
+    const float mn = block_min(dt_pass<LEN, 2>(A, Bf, zz, I2, I1), red);  // its barrier also completes Bf and I1
+    store_table<LEN>(allinds_c, I1);
+    return mn;
 }
 
 template <int LEN>
@@ -232,54 +292,70 @@ __device__ __forceinline__ void leaf_body(const BPArgs& a, int idx, float* smem,
     for (int l = t; l < B::L3; l += B::BLOCK) row[l] = s.Bf[l] - mn;
 }
 
-// One workgroup runs one segment's program (at a.prog + `off`): the first
-// node's accumulator is its cost plus its children's stored messages in
-// `ordered` order (the reference's fold order, :221-232); then each segment
-// node messages its parent, the next segment node, whose accumulator is its
-// cost plus its children's messages in the same order -- the one just
-// computed taken from LDS, the others (finished in earlier rounds) from their
-// cost rows -- until the segment's last node, which stores its message in its
-// own row -- or, at the root, its accumulator.  The accumulator lives in
-// registers (each thread owns fixed labels); the next node's cost row and up
-// to PF of its stored child messages are loaded while the current message
-// computes.
+// Element e of thread t's labels, clamped into the row: lanes past the row
+// repeat its last entry, so loads, folds and stores need no branch (and the
+// repeated stores write the same value as the entry's own lane).
 template <int LEN>
-__device__ __forceinline__ void prog_body(const BPArgs& a, int off, float* smem, float* red) {
+__device__ __forceinline__ int lab_of(int e) {
+    using B = BPShape<LEN>;
+    const int l = (int)threadIdx.x + e * B::BLOCK;
+    return (e + 1) * B::BLOCK <= B::L3 ? l : min(l, B::L3 - 1);
+}
+
+// One workgroup runs one segment's program block (at a.prog + `off`, layout
+// in mst_host.h): the first node's accumulator is its cost plus its
+// children's stored messages in `ordered` order (the reference's fold order,
+// :221-232); then each segment node messages its parent, the next segment
+// node, whose accumulator is its cost plus its children's messages in the
+// same order -- the one just computed taken from LDS, the others (finished in
+// earlier rounds) from their cost rows -- until the segment's last node,
+// which stores its message in its own row -- or, at the root, its
+// accumulator.  The accumulator lives in registers (each thread owns fixed
+// labels); the next node's cost row and up to PF of its stored child messages
+// are loaded while the current message computes.  The block is staged in LDS
+// (`blk`, kProgStage ints) once, so a node's record costs an LDS read instead
+// of a chain of dependent scalar loads from memory the host wrote.
+template <int LEN>
+__device__ __forceinline__ void prog_body(const BPArgs& a, int off, float* smem, float* red, int* blk) {
     using B = BPShape<LEN>;
     constexpr int ELEMS = (B::L3 + B::BLOCK - 1) / B::BLOCK;
     // stored child messages prefetched per node: what the registers left
     // beside the passes hold without spilling (none from 17 labels on)
     constexpr int PF = ELEMS >= 8 ? 0 : 2;
+    constexpr int SN = (kProgStage + B::BLOCK - 1) / B::BLOCK;
     const BPLds<LEN> s(smem);
     const int t = threadIdx.x;
+    auto row = [&](int v) { return a.cost + (long long)v * B::L3; };
 
-    cint* pg = ro(a.prog) + off;
-    cfloat* off3 = ro(a.off3);
-    const int L = pg[0];
-    cint* rec = pg + 1;  // [node, edgemst bits, k, pp, k children]
-    int node = rec[0];
-    float w = __int_as_float(rec[1]);
+    // stage the block (the padding after the last block keeps these in bounds)
+    int stage[SN];
+#pragma unroll
+    for (int q = 0; q < SN; q++)
+        if (q * B::BLOCK + t < kProgStage) stage[q] = a.prog[off + q * B::BLOCK + t];
+    cint* g = ro(a.prog) + off;
+    // entry q of a stored-children list, which may lie past the staged part
+    auto child_at = [&](int q) {
+        if (q >= kProgStage) return g[q];
+        return __builtin_amdgcn_readfirstlane(blk[q]);
+    };
+    const int L = g[0] - 1;
+    int node = g[kProgRec];
     float C[ELEMS];
     {
-        // the first node's cost and stored child messages: all loads issued
-        // before the first add (a grid node has at most six children)
+        // the first node's cost and stored child messages (all its children):
+        // all loads issued before the first add (a grid node has at most six)
         constexpr int KMAX = 6;
-        const int k0 = rec[2];
+        const int k0 = g[kProgRec + 2];
+        cint* st = g + g[kProgRec + 7];
         float kv[KMAX][ELEMS];
 #pragma unroll
-        for (int e = 0; e < ELEMS; e++) {
-            const int l = t + e * B::BLOCK;
-            C[e] = l < B::L3 ? a.cost[(long long)node * B::L3 + l] : 0.0f;
-        }
+        for (int e = 0; e < ELEMS; e++) C[e] = row(node)[lab_of<LEN>(e)];
 #pragma unroll
         for (int j = 0; j < KMAX; j++) {
             if (j < k0) {
-                const float* kr = a.cost + (long long)rec[4 + j] * B::L3;
+                const float* kr = row(st[j]);
 #pragma unroll
-                for (int e = 0; e < ELEMS; e++) {
-                    const int l = t + e * B::BLOCK;
-                    kv[j][e] = l < B::L3 ? kr[l] : 0.0f;
-                }
+                for (int e = 0; e < ELEMS; e++) kv[j][e] = kr[lab_of<LEN>(e)];
             }
         }
 #pragma unroll
@@ -290,44 +366,53 @@ __device__ __forceinline__ void prog_body(const BPArgs& a, int off, float* smem,
             }
         }
         for (int j = KMAX; j < k0; j++) {
-            const float* kr = a.cost + (long long)rec[4 + j] * B::L3;
+            const float* kr = row(st[j]);
 #pragma unroll
-            for (int e = 0; e < ELEMS; e++) {
-                const int l = t + e * B::BLOCK;
-                if (l < B::L3) C[e] += kr[l];
-            }
+            for (int e = 0; e < ELEMS; e++) C[e] += kr[lab_of<LEN>(e)];
         }
     }
+#pragma unroll
+    for (int q = 0; q < SN; q++)
+        if (q * B::BLOCK + t < kProgStage) blk[q * B::BLOCK + t] = stage[q];
+    __syncthreads();
+    // the current record's fields: node, edgemst, parent offsets
+    auto rd4 = [&](int q) { return *reinterpret_cast<const int4*>(blk + q); };  // q: a multiple of 4
+    int4 h1 = rd4(kProgRec + 4);
+    float w = __int_as_float(__builtin_amdgcn_readfirstlane(rd4(kProgRec).y));
+    int r = kProgRec;
     for (int i = 0; i <= L; i++) {
         if (node == a.root) break;  // the root keeps its accumulator
-        // prefetch the next node's cost row and its first PF stored child messages
-        cint* nrec = rec + 4 + rec[2];
-        const int next = i < L ? nrec[0] : -1;
-        const int nk = next >= 0 ? nrec[2] : 0, npp = next >= 0 ? nrec[3] : -1;
-        cint* nkids = nrec + 4;
-        const int o0 = npp == 0 ? 1 : 0;          // first stored child
-        const int o1 = o0 + 1 + (npp == o0 + 1);  // second stored child
+        const float ox = __int_as_float(__builtin_amdgcn_readfirstlane(h1.x));
+        const float oy = __int_as_float(__builtin_amdgcn_readfirstlane(h1.y));
+        const float oz = __int_as_float(__builtin_amdgcn_readfirstlane(h1.z));
+        // the next record (read whole: past the last record it is unused), then
+        // the next node's cost row and its first PF stored child messages
+        const int nr = r + kProgRec;
+        const int4 n0 = rd4(nr), n1 = rd4(nr + 4);
+        const int next = i < L ? __builtin_amdgcn_readfirstlane(n0.x) : -1;
+        const int nk = __builtin_amdgcn_readfirstlane(n0.z), npp = __builtin_amdgcn_readfirstlane(n0.w);
+        const int ns = __builtin_amdgcn_readfirstlane(n1.w);
+        const int nst = nk - (npp >= 0);  // stored children of the next node
         float nrow[ELEMS], p0[ELEMS], p1[ELEMS];
         if (next >= 0) {
 #pragma unroll
-            for (int e = 0; e < ELEMS; e++) {
-                const int l = t + e * B::BLOCK;
-                nrow[e] = l < B::L3 ? a.cost[(long long)next * B::L3 + l] : 0.0f;
-            }
-            if (PF > 0 && o0 < nk) {
-                const float* kr = a.cost + (long long)nkids[o0] * B::L3;
-#pragma unroll
-                for (int e = 0; e < ELEMS; e++) {
-                    const int l = t + e * B::BLOCK;
-                    p0[e] = l < B::L3 ? kr[l] : 0.0f;
+            for (int e = 0; e < ELEMS; e++) nrow[e] = row(next)[lab_of<LEN>(e)];
+            if (PF > 0 && nst > 0) {
+                int c0, c1 = 0;
+                if (ns + 1 < kProgStage) {
+                    c0 = __builtin_amdgcn_readfirstlane(blk[ns]);
+                    c1 = __builtin_amdgcn_readfirstlane(blk[ns + 1]);
+                } else {
+                    c0 = child_at(ns);
+                    if (nst > 1) c1 = child_at(ns + 1);
                 }
-            }
-            if (PF > 1 && o1 < nk) {
-                const float* kr = a.cost + (long long)nkids[o1] * B::L3;
+                const float* kr = row(c0);
 #pragma unroll
-                for (int e = 0; e < ELEMS; e++) {
-                    const int l = t + e * B::BLOCK;
-                    p1[e] = l < B::L3 ? kr[l] : 0.0f;
+                for (int e = 0; e < ELEMS; e++) p0[e] = kr[lab_of<LEN>(e)];
+                if (PF > 1 && nst > 1) {
+                    const float* kr1 = row(c1);
+#pragma unroll
+                    for (int e = 0; e < ELEMS; e++) p1[e] = kr1[lab_of<LEN>(e)];
                 }
             }
         }
@@ -336,53 +421,50 @@ __device__ __forceinline__ void prog_body(const BPArgs& a, int off, float* smem,
             const int l = t + e * B::BLOCK;
             if (l < B::L3) s.A[l] = C[e] * w;  // costall[child] *= edgemst (:197-202)
         }
-        const float mn = message<LEN>(a.allinds + (long long)node * B::L3, off3[3 * node], off3[3 * node + 1],
-                                      off3[3 * node + 2], s.A, s.Bf, s.I1, s.I2, s.zy, s.zx, s.zz, red);
+        const float mn = message<LEN>(a.allinds + (long long)node * B::L3, ox, oy, oz, s.A, s.Bf, s.I1, s.I2,
+                                      s.zy, s.zx, s.zz, red);
         if (next < 0) {
             // the segment's last node: store its message for the parent's fold
-            float* row = a.cost + (long long)node * B::L3;
 #pragma unroll
             for (int e = 0; e < ELEMS; e++) {
-                const int l = t + e * B::BLOCK;
-                if (l < B::L3) row[l] = s.Bf[l] - mn;
+                const int l = lab_of<LEN>(e);
+                row(node)[l] = s.Bf[l] - mn;
             }
             return;
         }
         // the next node's accumulator: its cost, then its children in order
 #pragma unroll
         for (int e = 0; e < ELEMS; e++) C[e] = nrow[e];
-        for (int j = 0; j < nk; j++) {
+        for (int j = 0, q = 0; j < nk; j++) {
             if (j == npp) {
 #pragma unroll
                 for (int e = 0; e < ELEMS; e++) {
-                    const int l = t + e * B::BLOCK;
-                    if (l < B::L3) C[e] += s.Bf[l] - mn;
+                    const int l = lab_of<LEN>(e);
+                    C[e] += s.Bf[l] - mn;
                 }
-            } else if (PF > 0 && j == o0) {
+                continue;
+            }
+            if (PF > 0 && q == 0) {
 #pragma unroll
                 for (int e = 0; e < ELEMS; e++) C[e] += p0[e];
-            } else if (PF > 1 && j == o1) {
+            } else if (PF > 1 && q == 1) {
 #pragma unroll
                 for (int e = 0; e < ELEMS; e++) C[e] += p1[e];
             } else {
-                const float* kr = a.cost + (long long)nkids[j] * B::L3;
+                const float* kr = row(child_at(ns + q));
 #pragma unroll
-                for (int e = 0; e < ELEMS; e++) {
-                    const int l = t + e * B::BLOCK;
-                    if (l < B::L3) C[e] += kr[l];
-                }
+                for (int e = 0; e < ELEMS; e++) C[e] += kr[lab_of<LEN>(e)];
             }
+            q++;
         }
-        rec = nrec;
+        r = nr;
         node = next;
-        w = __int_as_float(rec[1]);
+        w = __int_as_float(__builtin_amdgcn_readfirstlane(n0.y));
+        h1 = n1;
     }
     // reached the root: its accumulator is what the selection reads
 #pragma unroll
-    for (int e = 0; e < ELEMS; e++) {
-        const int l = t + e * B::BLOCK;
-        if (l < B::L3) a.cost[(long long)node * B::L3 + l] = C[e];
-    }
+    for (int e = 0; e < ELEMS; e++) row(node)[lab_of<LEN>(e)] = C[e];
 }
 
 template <int LEN>
@@ -396,7 +478,8 @@ template <int LEN>
 __global__ __launch_bounds__(BPShape<LEN>::BLOCK) void k_bp_prog(BPArgs a, int head_begin) {
     extern __shared__ float smem[];
     __shared__ float red[16];
-    prog_body<LEN>(a, ro(a.prog_start)[head_begin + blockIdx.x], smem, red);
+    __shared__ __align__(16) int blk[kProgStage];
+    prog_body<LEN>(a, ro(a.prog_start)[head_begin + blockIdx.x], smem, red, blk);
 }
 
 // The spine: heads [head_begin, head_end), one per round, run in order by one
@@ -406,8 +489,9 @@ template <int LEN>
 __global__ __launch_bounds__(BPShape<LEN>::BLOCK) void k_bp_tail(BPArgs a, int head_begin, int head_end) {
     extern __shared__ float smem[];
     __shared__ float red[16];
+    __shared__ __align__(16) int blk[kProgStage];
     for (int h = head_begin; h < head_end; h++) {
-        prog_body<LEN>(a, ro(a.prog_start)[h], smem, red);
+        prog_body<LEN>(a, ro(a.prog_start)[h], smem, red, blk);
         __syncthreads();
     }
 }
@@ -432,8 +516,9 @@ __global__ __launch_bounds__(BPShape<LEN>::BLOCK) void k_bp_prog_b(const BPBatch
                                                                    const int2* __restrict__ work) {
     extern __shared__ float smem[];
     __shared__ float red[16];
+    __shared__ __align__(16) int blk[kProgStage];
     const int2 w = work[blockIdx.x];
-    prog_body<LEN>(args_of(trees[w.x]), w.y, smem, red);
+    prog_body<LEN>(args_of(trees[w.x]), w.y, smem, red, blk);
 }
 
 // Spines of several trees: workgroup b runs tree tails[2b].x's programs at the
@@ -443,10 +528,11 @@ __global__ __launch_bounds__(BPShape<LEN>::BLOCK) void k_bp_tail_b(const BPBatch
                                                                    const int2* __restrict__ tails) {
     extern __shared__ float smem[];
     __shared__ float red[16];
+    __shared__ __align__(16) int blk[kProgStage];
     const int2 a0 = tails[2 * blockIdx.x], a1 = tails[2 * blockIdx.x + 1];
     const BPArgs a = args_of(trees[a0.x]);
     for (int h = a0.y; h < a1.x; h++) {
-        prog_body<LEN>(a, ro(a.prog_start)[h], smem, red);
+        prog_body<LEN>(a, ro(a.prog_start)[h], smem, red, blk);
         __syncthreads();
     }
 }
@@ -542,26 +628,54 @@ __device__ __forceinline__ void select_body(const float* __restrict__ cost, cons
         prev[0] = id;
         emit(root, id);
     }
+    // Thread t owns position t of every depth (wider depths loop below).  Per
+    // depth the chain is one dependent load, allinds[c][label of parent]:
+    // the next depth's (node, parent index) and the depth bound after it are
+    // loaded a depth ahead, the node's u0/v0/w0 beside the label, and the
+    // three field stores wait until the next depth has issued its loads, so
+    // no wait for a load also waits for them.
+    const int t = threadIdx.x;
+    int b = 0, e = 0, E = 0;  // depth_start[d], [d + 1], [d + 2]
     int nc = -1, npp = 0;
-    if (maxd >= 1 && depth_start[1] + (int)threadIdx.x < depth_start[2]) {
-        nc = sel_info[2 * (depth_start[1] + threadIdx.x)];
-        npp = sel_info[2 * (depth_start[1] + threadIdx.x) + 1];
+    if (maxd >= 1) {
+        b = depth_start[1];
+        e = depth_start[2];
+        E = depth_start[maxd >= 2 ? 3 : 2];
+        if (b + t < e) {
+            nc = sel_info[2 * (b + t)];
+            npp = sel_info[2 * (b + t) + 1];
+        }
     }
+    int pc = -1;
+    float pu = 0.0f, pv = 0.0f, pw = 0.0f;  // outputs of the previous depth's node, stored late
     __syncthreads();
     for (int d = 1; d <= maxd; d++) {
-        const int b = depth_start[d], e = depth_start[d + 1];
         const int c = nc, pp = npp;
+        if (pc >= 0) {
+            u1[pc] = pu;
+            v1[pc] = pv;
+            w1[pc] = pw;
+        }
         nc = -1;
-        if (d < maxd && depth_start[d + 1] + (int)threadIdx.x < depth_start[d + 2]) {
-            nc = sel_info[2 * (depth_start[d + 1] + threadIdx.x)];
-            npp = sel_info[2 * (depth_start[d + 1] + threadIdx.x) + 1];
+        if (d < maxd && e + t < E) {
+            nc = sel_info[2 * (e + t)];
+            npp = sel_info[2 * (e + t) + 1];
         }
+        const int E2 = depth_start[d + 3 <= maxd + 1 ? d + 3 : maxd + 1];
         if (c >= 0) {
+            const float a0 = u0[c], a1 = v0[c], a2 = w0[c];
             const int lab = allinds[(long long)c * len3 + prev[pp]];
-            cur[threadIdx.x] = lab;
-            emit(c, lab);
+            cur[t] = lab;
+            // 0 <= lab < len^3: unsigned quotients
+            const unsigned ul = (unsigned)lab, ulen = (unsigned)len;
+            const int lk = (int)(ul / (ulen * ulen)), r2 = (int)(ul - (unsigned)lk * ulen * ulen);
+            const int lj = (int)((unsigned)r2 / ulen), li = r2 - lj * len;
+            pu = (float)(lj - hw) * quant + a0;
+            pv = (float)(li - hw) * quant + a1;
+            pw = (float)(lk - hw) * quant + a2;
         }
-        for (int p = b + (int)blockDim.x + (int)threadIdx.x; p < e; p += blockDim.x) {  // depths wider than the block
+        pc = c;
+        for (int p = b + (int)blockDim.x + t; p < e; p += blockDim.x) {  // depths wider than the block
             const int c2 = sel_info[2 * p], pp2 = sel_info[2 * p + 1];
             const int lab = allinds[(long long)c2 * len3 + prev[pp2]];
             cur[p - b] = lab;
@@ -571,6 +685,14 @@ __device__ __forceinline__ void select_body(const float* __restrict__ cost, cons
         int* tmp = prev;
         prev = cur;
         cur = tmp;
+        b = e;
+        e = E;
+        E = E2;
+    }
+    if (pc >= 0) {
+        u1[pc] = pu;
+        v1[pc] = pv;
+        w1[pc] = pw;
     }
 }
 
@@ -613,10 +735,12 @@ __global__ void k_root_identity(int16_t* __restrict__ allinds, int root, int len
     if (l < len3) allinds[(long long)root * len3 + l] = (int16_t)l;
 }
 
-// Parent offsets of every tree of a batch (blockIdx.y = tree).
+// Parent offsets of every tree of a batch (blockIdx.y = tree), per node and
+// into the program records.
 __global__ void k_parent_offsets_b(const BPBatchTree* __restrict__ trees, float quant) {
     const BPBatchTree& T = trees[blockIdx.y];
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < T.n_segments) record_offsets(T.prog, T.prog_start, c, T.parents, T.u0, T.v0, T.w0, quant);
     if (c >= T.N) return;
     const int p = T.parents[c];
     if (p < 0) return;
@@ -640,6 +764,11 @@ void launch_bp_rounds(float* cost, int16_t* allinds, float* off3, const BPDevTre
                       const float* w0, int hw, float quant, hipStream_t s) {
     if (N > 0) {
         k_parent_offsets<<<ceil_div(N, 256), 256, 0, s>>>(off3, t.parents, u0, v0, w0, quant, N);
+        DEEDS_LAUNCH_CHECK();
+    }
+    if (t.n_segments > 0) {
+        k_record_offsets<<<ceil_div(t.n_segments, 256), 256, 0, s>>>(t.prog, t.prog_start, t.n_segments, t.parents,
+                                                                     u0, v0, w0, quant);
         DEEDS_LAUNCH_CHECK();
     }
     BPArgs a{cost, allinds, t.prog_start, t.prog, t.leaves, t.leaf_w, off3, t.root};

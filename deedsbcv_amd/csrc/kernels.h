@@ -45,7 +45,8 @@ struct BPDevTree {
     const int* parents;      // N
     const int* depth_start;  // maxd+2
     const int* prog_start;   // segments+1 (segments in round order)
-    const int* prog;         // per segment: [L], then L+1 records [node, edgemst bits, k, pp, k children]
+    int* prog;               // per segment a program block (mst_host.h); the offsets are written per pass
+    int n_segments;
     const int* leaves;       // n_leaves leaf nodes (round 0)
     const int* leaf_w;       // their edgemst bits
     int n_leaves;
@@ -75,7 +76,7 @@ struct BPBatchTree {
     float* cost;
     int16_t* allinds;
     const int* prog_start;
-    const int* prog;
+    int* prog;
     const int* leaves;
     const int* leaf_w;
     float* off3;
@@ -86,7 +87,7 @@ struct BPBatchTree {
     const float *u0, *v0, *w0;
     float *u1, *v1, *w1;
     int* sel_labs;  // 2*N ints of scratch (BPDevTree::sel_labs)
-    int root, N, maxd;
+    int root, N, maxd, n_segments;
 };
 // BP of several trees with the same label radius in one launch sequence:
 // the parent offsets of all trees, then one launch per round holding every

@@ -412,7 +412,7 @@ void build_schedule(Tree& t, bool bp_programs, int window) {
     // window among the other children); v takes the smallest, preferring a
     // choice that continues c's segment inside its window, then the child
     // latest in fold order (so fewer stored messages are folded after it).
-    if (window < 1) throw std::runtime_error("tree: BP window must be positive");
+    if (window < 1 || window > kBPWindow) throw std::runtime_error("tree: BP window must be in [1, 16]");
     if ((long long)N * (window + 1) >= (1LL << 31)) throw std::runtime_error("tree: too many nodes for the BP schedule");
     // per node: step, window (-1: leaf) and the child whose segment it
     // continues (-1: none), side by side for the random accesses by child
@@ -485,40 +485,50 @@ void build_schedule(Tree& t, bool bp_programs, int window) {
     t.leaves.assign(rev_leaves.rbegin(), rev_leaves.rend());
     t.leaf_w.resize(t.leaves.size());
     for (size_t i = 0; i < t.leaves.size(); i++) std::memcpy(&t.leaf_w[i], &t.edgemst[t.leaves[i]], 4);
-    // programs: [L] per segment, [node, edgemst bits, k, pp, children] per
-    // node; every non-root node is listed once as a child
+    // programs: per segment a block of kProgRec-int slots -- the header
+    // [nodes], then per node v_i the record [node, edgemst bits, k, pp,
+    // parent offset x, y, z (filled on the device), s] -- followed by each
+    // node's stored children (all but pp, in `ordered` order) at block offset
+    // s.  Every non-root node is a child once: as pp or stored.
     const size_t heads = t.heads.size();
+    const size_t stored = (N > 0 ? (size_t)N - 1 : 0) - ((size_t)internal - heads);
     t.prog_start.assign(heads + 1, 0);
-    t.prog.assign(heads + 4 * (size_t)internal + (N > 0 ? N - 1 : 0), 0);
+    t.prog.assign(kProgRec * (heads + (size_t)internal) + stored + kProgStage, 0);
     int* out = t.prog.data();
     size_t at = 0;
-    std::vector<int> longest(t.rounds, 0);
+    std::vector<int> longest(t.rounds, 0), seg;
     for (size_t h = 0; h < heads; h++) {
         const int x = t.heads[h];
+        seg.clear();
+        for (int cur = x;; cur = t.parents[cur]) {
+            seg.push_back(cur);
+            if (cur == root || sl[t.parents[cur]].pred != cur) break;
+        }
+        const int nodes = (int)seg.size();
         t.prog_start[h] = (int)at;
-        const size_t hdr = at++;
-        int L = -1, prev = -1, cur = x;
-        while (true) {
+        int* blk = out + at;
+        blk[0] = nodes;
+        size_t s_at = (size_t)kProgRec * (1 + nodes);  // stored lists after the records
+        for (int i = 0; i < nodes; i++) {
+            const int cur = seg[i], prev = i > 0 ? seg[i - 1] : -1;
             const int b = t.child_start[cur], k = t.child_start[cur + 1] - b;
             const int* ch = t.child_list.data() + b;
+            int* r = blk + kProgRec * (1 + i);
             int pp = -1;
-            for (int i = 0; i < k; i++)
-                if (ch[i] == prev) pp = i;
-            out[at] = cur;
-            if (cur != root) std::memcpy(&out[at + 1], &t.edgemst[cur], 4);
-            out[at + 2] = k;
-            out[at + 3] = pp;
-            std::copy(ch, ch + k, out + at + 4);
-            at += 4 + (size_t)k;
-            L++;
-            if (cur == root || sl[t.parents[cur]].pred != cur) break;
-            prev = cur;
-            cur = t.parents[cur];
+            r[0] = cur;
+            if (cur != root) std::memcpy(&r[1], &t.edgemst[cur], 4);
+            r[2] = k;
+            r[7] = (int)s_at;
+            for (int j = 0; j < k; j++) {
+                if (ch[j] == prev) pp = j;
+                else blk[s_at++] = ch[j];
+            }
+            r[3] = pp;
         }
-        out[hdr] = L;
-        longest[sl[x].wn] = std::max(longest[sl[x].wn], L + 1);
+        at += s_at;
+        longest[sl[x].wn] = std::max(longest[sl[x].wn], nodes);
     }
-    if (at != t.prog.size()) throw std::runtime_error("tree: BP programs do not cover the tree");
+    if (at + kProgStage != t.prog.size()) throw std::runtime_error("tree: BP programs do not cover the tree");
     t.prog_start[heads] = (int)at;
     t.bp_steps = 0;
     for (int r = 0; r < t.rounds; r++) t.bp_steps += longest[r];
@@ -533,7 +543,7 @@ void build_schedule(Tree& t, bool bp_programs, int window) {
         const int b = t.round_start[r], e = t.round_start[r + 1];
         std::fill(cnt.begin(), cnt.end(), 0);
         for (int h = b; h < e; h++) {
-            const int L = t.prog[t.prog_start[h]];  // L + 1 nodes, inside one window
+            const int L = t.prog[t.prog_start[h]] - 1;  // L + 1 nodes, inside one window
             if (L < 0 || L >= window) throw std::runtime_error("tree: BP segment longer than its window");
             cnt[window - L]++;
         }
@@ -543,13 +553,16 @@ void build_schedule(Tree& t, bool bp_programs, int window) {
         std::vector<int> pos(window + 2, 0);
         for (int i = 1; i < window + 2; i++) pos[i] = cnt[i - 1];
         for (int i = 0; i < e - b; i++) {
-            const int slot = b + pos[window - t.prog[ps[i]]]++;
+            const int slot = b + pos[window - (t.prog[ps[i]] - 1)]++;
             t.prog_start[slot] = ps[i];
             t.heads[slot] = hd[i];
         }
     }
 }
 
-size_t tree_device_ints(int N) { return 18 * (size_t)N + 16; }
+// ordered, parents, depth_start, prog_start: 4N + 2; prog: kProgRec per segment
+// and per internal node plus N - 1 children, and the staging pad; leaves,
+// leaf_w, sel_info: 4N.
+size_t tree_device_ints(int N) { return (2 * kProgRec + 9) * (size_t)N + kProgStage + 16; }
 
 }  // namespace deeds

@@ -47,16 +47,29 @@ struct Tree {
     // Before round 0: every leaf messages its parent (leaf_w = edgemst bits).
     std::vector<int> leaves;
     std::vector<int> leaf_w;
-    // Per segment (in round order) the program one workgroup executes: [L],
-    // then for v_0 .. v_L the record [node, edgemst bits, k, pp, the k
-    // children in `ordered` order]; pp is the position of v_{i-1} among the
-    // children of v_i (-1 for v_0, whose children are all stored).
+    // Per segment (in round order) the program block one workgroup executes,
+    // in slots of kProgRec ints: the header [L + 1, 0...], then for v_0 .. v_L
+    // the record [node, edgemst bits, k, pp, off x, off y, off z, s], then the
+    // nodes' stored children.  pp is the position of v_{i-1} among the k
+    // children of v_i in `ordered` order (-1 for v_0); the other children
+    // ("stored": their messages are in their cost rows) are listed, in that
+    // order, at offset s of the block.  The offsets (the parent's field minus
+    // the node's, in label units) are written on the device before each pass.
+    // prog ends with kProgStage ints of padding, so a block's first
+    // kProgStage ints can always be read.
     std::vector<int> prog_start;  // heads + 1 offsets into prog
     std::vector<int> prog;
 };
 
 // Steps per bottom-up round (one launch each, below the spine).
 constexpr int kBPWindow = 16;
+
+// Ints per program record, and the ints of a program block the device BP
+// stages in LDS (the header and 16 records take 136).
+constexpr int kProgRec = 8;
+constexpr int kProgStage = 256;
+static_assert(kProgRec * (1 + kBPWindow) <= kProgStage, "a block's header and records must fit the staged part");
+
 
 // Upper bound on the ints a device copy of a tree's BP schedule needs
 // (ordered, parents, depth_start, prog_start, prog, leaves, leaf_w, sel_info).

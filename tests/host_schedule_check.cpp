@@ -63,22 +63,27 @@ static int check_programs(Tree& t, int window, const std::vector<float>& data) {
         done[v]++;
     }
     auto run = [&](int h, int rnd, bool tail) {
-        const int* pg = t.prog.data() + t.prog_start[h];
-        const int L = pg[0];
-        const int* rec = pg + 1;
+        const int* blk = t.prog.data() + t.prog_start[h];
+        const int L = blk[0] - 1;
         auto stored = [&](int c) {
             if (wrote[c] == -2 || (!tail && wrote[c] >= rnd) || (tail && wrote[c] > rnd)) bad++;
             return row[c];
         };
+        // a record's children in order: the stored ones from the list at s, pp in between
+        auto child = [&](const int* r, int j) {
+            const int* st = blk + r[7];
+            return r[3] < 0 || j < r[3] ? st[j] : st[j - 1];
+        };
+        const int* rec = blk + kProgRec;
         int node = rec[0];
         float w;
         std::memcpy(&w, &rec[1], 4);
         if (rec[3] != -1) bad++;
         float C = row[node];
-        for (int j = 0; j < rec[2]; j++) C += stored(rec[4 + j]);
+        for (int j = 0; j < rec[2]; j++) C += stored(child(rec, j));
         for (int i = 0; i <= L; i++) {
             if (node == root) break;
-            const int* nrec = rec + 4 + rec[2];
+            const int* nrec = rec + kProgRec;
             const int next = i < L ? nrec[0] : -1;
             const float m = msgf(C * w, node);
             got[node] = m;
@@ -90,9 +95,14 @@ static int check_programs(Tree& t, int window, const std::vector<float>& data) {
             }
             if (t.parents[node] != next) bad++;
             const int nk = nrec[2], npp = nrec[3];
-            if (npp < 0 || npp >= nk || nrec[4 + npp] != node) bad++;
+            if (npp < 0 || npp >= nk) bad++;
+            // the children as listed must be exactly next's children in `ordered` order
+            for (int j = 0; j < nk; j++) {
+                const int c = j == npp ? node : child(nrec, j);
+                if (t.child_list[t.child_start[next] + j] != c) bad++;
+            }
             C = row[next];
-            for (int j = 0; j < nk; j++) C += j == npp ? m : stored(nrec[4 + j]);
+            for (int j = 0; j < nk; j++) C += j == npp ? m : stored(child(nrec, j));
             rec = nrec;
             node = next;
             std::memcpy(&w, &rec[1], 4);
