@@ -240,6 +240,8 @@ def plain_bench(a, name, steps, warmup, cpu=True):
         "host_stats_ms": 1e3 * float(np.mean([t["stats_s"] for t in timings])),
         "gpu_idle_before_bp_level_ms": mean("bp_idle_level_s"),
         "bp_level_ms_fwd_stream": mean("bp_level_s"),
+        "host_tree_ready_level_ms": mean("tree_ready_level_s"),
+        "host_tree_prim_level_ms": mean("tree_shape_level_s"),
         "datacost_level_ms": mean("datacost_level_s"),
         "roofline": {
             "kernel": "k_datacost (Hamming cost volume)", "bound": "valu", "achieved": achieved,

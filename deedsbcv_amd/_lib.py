@@ -30,7 +30,9 @@ class Timing(ctypes.Structure):
                 ("tree_build_s", ctypes.c_double), ("tree_wait_s", ctypes.c_double),
                 ("tree_build_level_s", ctypes.c_double * MAX_LEVELS),
                 ("tree_wait_level_s", ctypes.c_double * MAX_LEVELS), ("stats_s", ctypes.c_double),
-                ("bp_idle_level_s", ctypes.c_double * MAX_LEVELS), ("bp_level_s", ctypes.c_double * MAX_LEVELS)]
+                ("bp_idle_level_s", ctypes.c_double * MAX_LEVELS), ("bp_level_s", ctypes.c_double * MAX_LEVELS),
+                ("tree_ready_level_s", ctypes.c_double * MAX_LEVELS),
+                ("tree_shape_level_s", ctypes.c_double * MAX_LEVELS)]
 
     def as_dict(self):
         L = self.levels
@@ -44,7 +46,9 @@ class Timing(ctypes.Structure):
                 "tree_build_s": self.tree_build_s, "tree_wait_s": self.tree_wait_s,
                 "tree_build_level_s": list(self.tree_build_level_s[:L]),
                 "tree_wait_level_s": list(self.tree_wait_level_s[:L]), "stats_s": self.stats_s,
-                "bp_idle_level_s": list(self.bp_idle_level_s[:L]), "bp_level_s": list(self.bp_level_s[:L])}
+                "bp_idle_level_s": list(self.bp_idle_level_s[:L]), "bp_level_s": list(self.bp_level_s[:L]),
+                "tree_ready_level_s": list(self.tree_ready_level_s[:L]),
+                "tree_shape_level_s": list(self.tree_shape_level_s[:L])}
 
 
 class Pair(ctypes.Structure):

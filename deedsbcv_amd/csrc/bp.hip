@@ -190,29 +190,29 @@ __global__ void k_parent_offsets(float* __restrict__ off3, const int* __restrict
     off3[3 * c + 2] = (w0[p] - w0[c]) / quant;
 }
 
-// The same offsets written into the records of program block h (fields 4-6),
-// where the BP reads them with the rest of the record.
-__device__ __forceinline__ void record_offsets(int* __restrict__ prog, const int* __restrict__ prog_start, int h,
+// The same offsets written into record i of program block h (fields 4-6),
+// where the BP reads them with the rest of the record; one thread per
+// (block, record) slot, kBPWindow slots per block.
+__device__ __forceinline__ void record_offsets(int* __restrict__ prog, const int* __restrict__ prog_start, int slot,
                                                const int* __restrict__ parents, const float* __restrict__ u0,
                                                const float* __restrict__ v0, const float* __restrict__ w0,
                                                float quant) {
-    int* blk = prog + prog_start[h];
-    const int nodes = blk[0];
-    for (int i = 0; i < nodes; i++) {
-        int* r = blk + kProgRec * (1 + i);
-        const int c = r[0], p = parents[c];
-        if (p < 0) continue;
-        r[4] = __float_as_int((u0[p] - u0[c]) / quant);
-        r[5] = __float_as_int((v0[p] - v0[c]) / quant);
-        r[6] = __float_as_int((w0[p] - w0[c]) / quant);
-    }
+    int* blk = prog + prog_start[slot / kBPWindow];
+    const int i = slot % kBPWindow;
+    if (i >= blk[0]) return;
+    int* r = blk + kProgRec * (1 + i);
+    const int c = r[0], p = parents[c];
+    if (p < 0) return;
+    r[4] = __float_as_int((u0[p] - u0[c]) / quant);
+    r[5] = __float_as_int((v0[p] - v0[c]) / quant);
+    r[6] = __float_as_int((w0[p] - w0[c]) / quant);
 }
 
-__global__ void k_record_offsets(int* __restrict__ prog, const int* __restrict__ prog_start, int segments,
+__global__ void k_record_offsets(int* __restrict__ prog, const int* __restrict__ prog_start, int slots,
                                  const int* __restrict__ parents, const float* __restrict__ u0,
                                  const float* __restrict__ v0, const float* __restrict__ w0, float quant) {
-    const int h = blockIdx.x * blockDim.x + threadIdx.x;
-    if (h < segments) record_offsets(prog, prog_start, h, parents, u0, v0, w0, quant);
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot < slots) record_offsets(prog, prog_start, slot, parents, u0, v0, w0, quant);
 }
 
 // The composite argmin table, built in I1 by the z pass, to the node's row of
@@ -740,7 +740,8 @@ __global__ void k_root_identity(int16_t* __restrict__ allinds, int root, int len
 __global__ void k_parent_offsets_b(const BPBatchTree* __restrict__ trees, float quant) {
     const BPBatchTree& T = trees[blockIdx.y];
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < T.n_segments) record_offsets(T.prog, T.prog_start, c, T.parents, T.u0, T.v0, T.w0, quant);
+    for (int slot = c; slot < T.n_segments * kBPWindow; slot += gridDim.x * blockDim.x)
+        record_offsets(T.prog, T.prog_start, slot, T.parents, T.u0, T.v0, T.w0, quant);
     if (c >= T.N) return;
     const int p = T.parents[c];
     if (p < 0) return;
@@ -767,8 +768,9 @@ void launch_bp_rounds(float* cost, int16_t* allinds, float* off3, const BPDevTre
         DEEDS_LAUNCH_CHECK();
     }
     if (t.n_segments > 0) {
-        k_record_offsets<<<ceil_div(t.n_segments, 256), 256, 0, s>>>(t.prog, t.prog_start, t.n_segments, t.parents,
-                                                                     u0, v0, w0, quant);
+        const int slots = t.n_segments * kBPWindow;
+        k_record_offsets<<<ceil_div(slots, 256), 256, 0, s>>>(t.prog, t.prog_start, slots, t.parents, u0, v0, w0,
+                                                              quant);
         DEEDS_LAUNCH_CHECK();
     }
     BPArgs a{cost, allinds, t.prog_start, t.prog, t.leaves, t.leaf_w, off3, t.root};

@@ -13,6 +13,7 @@
 #include <cstdint>
 #include <cstring>
 #include <stdexcept>
+#include <thread>
 
 namespace deeds {
 
@@ -152,15 +153,41 @@ void prim_tree(const Key* key, int m1, int n1, int o1, Tree& t);
 // walking the present SADs in increasing order and advancing the rank when c
 // changes numbers the distinct costs densely.
 bool cost_ranks(const float* ec, float vol, size_t count, std::vector<uint16_t>& rank) {
-    float mx = 0.0f;
-    for (size_t e = 0; e < count; e++) {
-        const float x = ec[e];
-        if (!(x >= 0.0f && x < 4194304.0f) || x != std::floor(x)) return false;
-        mx = std::max(mx, x);
-    }
+    // the scans over all edges split over a few threads on large grids
+    const int T = count >= ((size_t)1 << 18) ? 4 : 1;
+    auto split = [&](auto body) {
+        std::vector<std::thread> th;
+        for (int i = 1; i < T; i++) th.emplace_back(body, i);
+        body(0);
+        for (auto& x : th) x.join();
+    };
+    auto part = [&](int i, size_t& b, size_t& e) {
+        b = count * (size_t)i / (size_t)T;
+        e = count * (size_t)(i + 1) / (size_t)T;
+    };
+    std::vector<float> mxs(T, 0.0f);
+    std::vector<char> okay(T, 1);
+    split([&](int i) {
+        size_t b, e;
+        part(i, b, e);
+        float m = 0.0f;
+        for (size_t x = b; x < e; x++) {
+            const float v = ec[x];
+            if (!(v >= 0.0f && v < 4194304.0f) || v != std::floor(v)) {
+                okay[i] = 0;
+                return;
+            }
+            m = std::max(m, v);
+        }
+        mxs[i] = m;
+    });
+    for (int i = 0; i < T; i++)
+        if (!okay[i]) return false;
+    const float mx = *std::max_element(mxs.begin(), mxs.end());
     const size_t top = (size_t)mx + 1;
+    // (one thread: scattered byte stores from several would share cache lines)
     std::vector<uint8_t> present(top, 0);
-    for (size_t e = 0; e < count; e++) present[(size_t)ec[e]] = 1;
+    for (size_t x = 0; x < count; x++) present[(size_t)ec[x]] = 1;
     std::vector<uint16_t> of_sad(top, 0);
     int r = -1;
     float prev = 0.0f;
@@ -174,7 +201,11 @@ bool cost_ranks(const float* ec, float vol, size_t count, std::vector<uint16_t>&
         of_sad[v] = (uint16_t)r;
     }
     rank.resize(count);
-    for (size_t e = 0; e < count; e++) rank[e] = of_sad[(size_t)ec[e]];
+    split([&](int i) {
+        size_t b, e;
+        part(i, b, e);
+        for (size_t x = b; x < e; x++) rank[x] = of_sad[(size_t)ec[x]];
+    });
     return true;
 }
 
@@ -342,14 +373,16 @@ void prim_tree(const Key* w, int m1, int n1, int o1, Tree& t) {
     std::vector<int> fill(start.begin(), start.end());
     for (int v = 0; v < N; v++) t.ordered[fill[depth[v]]++] = v;
     // slot of the edge to the parent; the reference scans all six slots and
-    // keeps the last whose linear index equals the parent (primsMST.h:198-211)
-    for (int p = 1; p < N; p++) {
-        int c = t.ordered[p], par = t.parents[c];
-        int z = c / (m1 * n1), x = (c - z * m1 * n1) / m1, y = c - z * m1 * n1 - x * m1;
-        for (int nb = 0; nb < 6; nb++) {
-            int idx = y + kNbI[nb] + (x + kNbJ[nb]) * m1 + (z + kNbK[nb]) * m1 * n1;
-            if (idx == par) t.parent_slot[c] = nb;
-        }
+    // keeps the last whose linear index equals the parent (primsMST.h:198-211).
+    // The linear index of slot nb is c + delta[nb] without bounds checks, so
+    // a node walk in index order needs no division.
+    for (int c = 0; c < N; c++) {
+        const int par = t.parents[c];
+        if (par < 0) continue;
+        int slot = -1;
+        for (int nb = 0; nb < 6; nb++)
+            if (c + delta[nb] == par) slot = nb;
+        t.parent_slot[c] = slot;
     }
 }
 
@@ -366,45 +399,52 @@ void build_schedule(Tree& t, bool bp_programs, int window) {
         if (v < 0 || v >= N || pos[v] >= 0) throw std::runtime_error("tree: `ordered` is not a permutation");
         pos[v] = p;
     }
-    std::vector<int> depth(N, 0);
+    // The schedule works in positions of `ordered`: a node's parent and
+    // children lie in the depths next to its own, so the passes below touch
+    // memory near the depth they are at rather than node ids all over the grid.
+    std::vector<int> ppos(N, -1), dpos(N, 0);  // parent position, depth
     for (int p = 1; p < N; p++) {
-        int c = t.ordered[p];
-        int par = t.parents[c];
+        const int par = t.parents[t.ordered[p]];
         if (par < 0 || par >= N) throw std::runtime_error("tree: node without a valid parent");
-        if (pos[par] >= p) throw std::runtime_error("tree: a parent does not precede its child in `ordered`");
-        depth[c] = depth[par] + 1;
+        const int q = pos[par];
+        if (q >= p) throw std::runtime_error("tree: a parent does not precede its child in `ordered`");
+        ppos[p] = q;
+        dpos[p] = dpos[q] + 1;
+        if (dpos[p] < dpos[p - 1]) throw std::runtime_error("tree: `ordered` is not sorted by depth");
     }
-    int maxd = 0;
-    for (int v = 0; v < N; v++) maxd = std::max(maxd, depth[v]);
+    const int maxd = N > 0 ? dpos[N - 1] : 0;
     t.maxd = maxd;
     t.depth_start.assign(maxd + 2, 0);
-    for (int v = 0; v < N; v++) t.depth_start[depth[v] + 1]++;
+    for (int p = 0; p < N; p++) t.depth_start[dpos[p] + 1]++;
     for (int d = 1; d <= maxd + 1; d++) t.depth_start[d] += t.depth_start[d - 1];
-    for (int p = 0; p < N; p++)
-        if (depth[t.ordered[p]] < (p == 0 ? 0 : depth[t.ordered[p - 1]]))
-            throw std::runtime_error("tree: `ordered` is not sorted by depth");
-    // top-down selection: per position, the node and its parent's index within the parent's depth
     t.max_width = 0;
     for (int d = 0; d <= maxd; d++) t.max_width = std::max(t.max_width, t.depth_start[d + 1] - t.depth_start[d]);
-    // children of every node in `ordered` order
-    t.child_start.assign(N + 1, 0);
-    for (int p = 1; p < N; p++) t.child_start[t.parents[t.ordered[p]] + 1]++;
-    for (int v = 0; v < N; v++) t.child_start[v + 1] += t.child_start[v];
-    t.child_list.assign(N > 0 ? N - 1 : 0, 0);
-    std::vector<int> fill(t.child_start.begin(), t.child_start.end() - 1);
-    for (int p = 1; p < N; p++) {
-        int c = t.ordered[p];
-        t.child_list[fill[t.parents[c]]++] = c;
+    if (!bp_programs) {
+        // children of every node in `ordered` order (CSR by node id)
+        t.child_start.assign(N + 1, 0);
+        for (int p = 1; p < N; p++) t.child_start[t.parents[t.ordered[p]] + 1]++;
+        for (int v = 0; v < N; v++) t.child_start[v + 1] += t.child_start[v];
+        t.child_list.assign(N > 0 ? N - 1 : 0, 0);
+        std::vector<int> fill(t.child_start.begin(), t.child_start.end() - 1);
+        for (int p = 1; p < N; p++) {
+            int c = t.ordered[p];
+            t.child_list[fill[t.parents[c]]++] = c;
+        }
+        return;
     }
-    if (!bp_programs) return;
+    // top-down selection: per position, the node and its parent's index within the parent's depth
     t.sel_info.assign(2 * (size_t)N, 0);
     for (int p = 0; p < N; p++) {
-        const int c = t.ordered[p];
-        t.sel_info[2 * p] = c;
-        if (p > 0) {
-            const int par = t.parents[c];
-            t.sel_info[2 * p + 1] = pos[par] - t.depth_start[depth[par]];
-        }
+        t.sel_info[2 * p] = t.ordered[p];
+        if (p > 0) t.sel_info[2 * p + 1] = ppos[p] - t.depth_start[dpos[p] - 1];
+    }
+    // children of every position, as positions in `ordered` order
+    std::vector<int> cs(N + 1, 0), cl(N > 0 ? N - 1 : 0);
+    for (int p = 1; p < N; p++) cs[ppos[p] + 1]++;
+    for (int p = 0; p < N; p++) cs[p + 1] += cs[p];
+    {
+        std::vector<int> fill(cs.begin(), cs.end() - 1);
+        for (int p = 1; p < N; p++) cl[fill[ppos[p]]++] = p;
     }
     // Steps and segments (bottom-up, so every child is placed before its
     // parent).  For each choice of the child c that continues into v, v's
@@ -414,24 +454,23 @@ void build_schedule(Tree& t, bool bp_programs, int window) {
     // latest in fold order (so fewer stored messages are folded after it).
     if (window < 1 || window > kBPWindow) throw std::runtime_error("tree: BP window must be in [1, 16]");
     if ((long long)N * (window + 1) >= (1LL << 31)) throw std::runtime_error("tree: too many nodes for the BP schedule");
-    // per node: step, window (-1: leaf) and the child whose segment it
-    // continues (-1: none), side by side for the random accesses by child
+    // per position: step, window (-1: leaf) and the child (position) whose
+    // segment it continues (-1: none)
     struct Slot {
         int step = -1, wn = -1, pred = -1;
     };
     std::vector<Slot> sl(N);
-    std::vector<int> rev_heads, rev_leaves;  // in descending `ordered` position
+    std::vector<int> rev_heads, rev_leaves;  // positions, descending
     std::vector<int> per_round;              // segment heads per round
     rev_leaves.reserve(N);
     int internal = 0;
     for (int p = N - 1; p >= 0; p--) {
-        const int v = t.ordered[p];
-        const int b = t.child_start[v], k = t.child_start[v + 1] - b;
+        const int b = cs[p], k = cs[p + 1] - b;
         if (k == 0) {
-            if (p > 0) rev_leaves.push_back(v);
+            if (p > 0) rev_leaves.push_back(p);
             continue;
         }
-        const int* ch = t.child_list.data() + b;
+        const int* ch = cl.data() + b;
         // the two latest child windows, for "latest among the others"
         int w1 = -2, w2 = -2, i1 = -1;
         for (int i = 0; i < k; i++) {
@@ -454,35 +493,38 @@ void build_schedule(Tree& t, bool bp_programs, int window) {
                 best = cand, best_same = same, best_c = c;
             }
         }
-        Slot& s = sl[v];
+        Slot& s = sl[p];
         s.step = best;
         s.wn = best / window;
         s.pred = best_same ? best_c : -1;
         internal++;
         if (s.pred < 0) {
-            rev_heads.push_back(v);
+            rev_heads.push_back(p);
             if ((int)per_round.size() <= s.wn) per_round.resize(s.wn + 1, 0);
             per_round[s.wn]++;
         }
     }
     const int root = N > 0 ? t.ordered[0] : -1;
-    t.rounds = N > 0 ? sl[root].wn + 1 : 0;  // the root is the last node placed
+    t.rounds = N > 0 ? sl[0].wn + 1 : 0;  // the root is the last node placed
     per_round.resize(t.rounds, 0);
     t.round_start.assign(t.rounds + 1, 0);
     for (int r = 0; r < t.rounds; r++) t.round_start[r + 1] = t.round_start[r] + per_round[r];
     t.tail_round = t.rounds;
     while (t.tail_round > 0 && t.round_start[t.tail_round] - t.round_start[t.tail_round - 1] == 1) t.tail_round--;
-    // segment heads by round, in `ordered` order within one
-    t.heads.assign(t.round_start[t.rounds], 0);
+    // segment heads by round, in `ordered` order within one (as positions, then as nodes)
+    std::vector<int> head_pos(t.round_start[t.rounds]);
     std::vector<int> fill_r(t.round_start.begin(), t.round_start.end() - 1);
     for (size_t i = rev_heads.size(); i-- > 0;) {
-        const int v = rev_heads[i];
-        t.heads[fill_r[sl[v].wn]++] = v;
+        const int p = rev_heads[i];
+        head_pos[fill_r[sl[p].wn]++] = p;
     }
+    t.heads.resize(head_pos.size());
+    for (size_t h = 0; h < head_pos.size(); h++) t.heads[h] = t.ordered[head_pos[h]];
     // Leaves message their parents before round 0 (one workgroup each).  A
     // segment's last node (unless it is the root) stores its message in its
     // cost row for the parent that folds it in a later round.
-    t.leaves.assign(rev_leaves.rbegin(), rev_leaves.rend());
+    t.leaves.resize(rev_leaves.size());
+    for (size_t i = 0; i < rev_leaves.size(); i++) t.leaves[i] = t.ordered[rev_leaves[rev_leaves.size() - 1 - i]];
     t.leaf_w.resize(t.leaves.size());
     for (size_t i = 0; i < t.leaves.size(); i++) std::memcpy(&t.leaf_w[i], &t.edgemst[t.leaves[i]], 4);
     // programs: per segment a block of kProgRec-int slots -- the header
@@ -498,11 +540,11 @@ void build_schedule(Tree& t, bool bp_programs, int window) {
     size_t at = 0;
     std::vector<int> longest(t.rounds, 0), seg;
     for (size_t h = 0; h < heads; h++) {
-        const int x = t.heads[h];
+        const int x = head_pos[h];
         seg.clear();
-        for (int cur = x;; cur = t.parents[cur]) {
-            seg.push_back(cur);
-            if (cur == root || sl[t.parents[cur]].pred != cur) break;
+        for (int q = x;; q = ppos[q]) {
+            seg.push_back(q);
+            if (q == 0 || sl[ppos[q]].pred != q) break;
         }
         const int nodes = (int)seg.size();
         t.prog_start[h] = (int)at;
@@ -510,9 +552,10 @@ void build_schedule(Tree& t, bool bp_programs, int window) {
         blk[0] = nodes;
         size_t s_at = (size_t)kProgRec * (1 + nodes);  // stored lists after the records
         for (int i = 0; i < nodes; i++) {
-            const int cur = seg[i], prev = i > 0 ? seg[i - 1] : -1;
-            const int b = t.child_start[cur], k = t.child_start[cur + 1] - b;
-            const int* ch = t.child_list.data() + b;
+            const int q = seg[i], prev = i > 0 ? seg[i - 1] : -1;
+            const int cur = t.ordered[q];
+            const int b = cs[q], k = cs[q + 1] - b;
+            const int* ch = cl.data() + b;
             int* r = blk + kProgRec * (1 + i);
             int pp = -1;
             r[0] = cur;
@@ -521,7 +564,7 @@ void build_schedule(Tree& t, bool bp_programs, int window) {
             r[7] = (int)s_at;
             for (int j = 0; j < k; j++) {
                 if (ch[j] == prev) pp = j;
-                else blk[s_at++] = ch[j];
+                else blk[s_at++] = t.ordered[ch[j]];
             }
             r[3] = pp;
         }

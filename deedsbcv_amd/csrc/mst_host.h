@@ -21,7 +21,8 @@ struct Tree {
     // depth) per position of `ordered`, and the largest depth width
     std::vector<int> sel_info;
     int max_width = 0;
-    // children of every node in `ordered` order (CSR by node id)
+    // children of every node in `ordered` order (CSR by node id); built
+    // only without BP programs (the schedule of the programs uses positions)
     std::vector<int> child_start;  // N + 1
     std::vector<int> child_list;   // N - 1
     // Bottom-up rounds.  Every internal node gets a step t(v) >= 0 (leaves:

@@ -266,6 +266,8 @@ class PairRun {
         for (int i = 0; i < 2 * L_; i++) ev_pre_.emplace_back(new Event());
         for (int i = 0; i < L_; i++) ev_sad_.emplace_back(new Event());
         build_time_.assign(2 * L_, 0.0);
+        build_start_.assign(2 * L_, 0.0);
+        build_shape_.assign(2 * L_, 0.0);
         wait_level_.assign(L_, 0.0);
         trees_[0].resize(L_);
         trees_[1].resize(L_);
@@ -304,6 +306,7 @@ class PairRun {
 
     // Inputs, affine warp, image statistics and edge SADs (s0).
     void begin() {
+        t_begin_ = std::chrono::steady_clock::now();
         DEEDS_HIP(hipEventRecord(ev_start_.e, s0_));
         // +1024 HU offset (deedsBCV0.cpp:139-144), affine warp (:210)
         const float identity[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
@@ -365,6 +368,9 @@ class PairRun {
             const LevelCfg c = lv_[l];
             std::shared_future<float> sdf = sd_fut_[dir];
             double* spent = &build_time_[2 * l + dir];
+            double* began = &build_start_[2 * l + dir];
+            double* shaped = &build_shape_[2 * l + dir];
+            const auto t_begin = t_begin_;
             hipEvent_t sad_event = ev_sad_[l]->e;
             const bool host_wait = host_sads_;
             std::shared_future<void> sads = sad_ready_;
@@ -372,17 +378,20 @@ class PairRun {
             // building thread, so the launching thread only enqueues the copy
             char* slot = h_stage_ + stage_off_[l] + dir * ((stage_off_[l + 1] - stage_off_[l]) / 2);
             void* d_stage = d_stage_[dir];
-            trees_[dir][l] = pool.submit([ec, c, sdf, spent, sad_event, slot, d_stage, host_wait, sads]() {
+            trees_[dir][l] = pool.submit([ec, c, sdf, spent, began, shaped, t_begin, sad_event, slot, d_stage,
+                                          host_wait, sads]() {
                 if (host_wait)
                     sads.wait();
                 else
                     DEEDS_HIP(hipEventSynchronize(sad_event));
                 auto tb = std::chrono::steady_clock::now();
+                *began = std::chrono::duration<double>(tb - t_begin).count();
                 auto check = std::async(std::launch::async, [ec, c, sdf]() {
                     return weights_distinguish_costs(ec, c.step, c.m1, c.n1, c.o1, sdf.get());
                 });
                 std::unique_ptr<Tree> t(new Tree());
                 build_tree_shape(ec, c.step, c.m1, c.n1, c.o1, *t);
+                *shaped = std::chrono::duration<double>(std::chrono::steady_clock::now() - tb).count();
                 const bool same_order = check.get();
                 const float sd = sdf.get();
                 if (same_order)
@@ -574,6 +583,9 @@ class PairRun {
             timing->datacost_kernel_s += lvl;
             timing->tree_build_s += build_time_[2 * l] + build_time_[2 * l + 1];
             timing->tree_build_level_s[l] = std::max(build_time_[2 * l], build_time_[2 * l + 1]);
+            timing->tree_ready_level_s[l] = std::max(build_start_[2 * l] + build_time_[2 * l],
+                                                     build_start_[2 * l + 1] + build_time_[2 * l + 1]);
+            timing->tree_shape_level_s[l] = std::max(build_shape_[2 * l], build_shape_[2 * l + 1]);
             timing->tree_wait_level_s[l] = wait_level_[l];
             timing->datacost_launches += graph_ ? 0 : 2;
             for (int dir = 0; dir < 2 && !graph_; dir++)
@@ -646,7 +658,8 @@ class PairRun {
     std::vector<std::unique_ptr<Event>> ev_lvl_, ev_dc_, ev_pre_;
     double stats_time_[2] = {0.0, 0.0};
     std::shared_future<float> sd_fut_[2];
-    std::vector<double> build_time_;
+    std::vector<double> build_time_, build_start_, build_shape_;
+    std::chrono::steady_clock::time_point t_begin_;
     double wait_time_ = 0.0;
     std::vector<double> wait_level_;
     // a level's tree with its schedule already packed into the pinned slot

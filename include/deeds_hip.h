@@ -80,6 +80,10 @@ typedef struct deeds_timing {
     double bp_idle_level_s[DEEDS_MAX_LEVELS];     /* per level: GPU idle between each direction's data cost
                                                      and its BP (waiting for the host tree), summed */
     double bp_level_s[DEEDS_MAX_LEVELS];          /* per level: forward-direction BP time on its stream */
+    double tree_ready_level_s[DEEDS_MAX_LEVELS];  /* per level: host time from the registration's start
+                                                     until both trees were built */
+    double tree_shape_level_s[DEEDS_MAX_LEVELS];  /* per level: the part of the build spent in Prim's
+                                                     (edge ranks, the heap loop, the BFS order), max of both */
 } deeds_timing;
 
 /* ---- library / context ------------------------------------------------- */

@@ -49,6 +49,9 @@ static int check_programs(Tree& t, int window, const std::vector<float>& data) {
             if (depth[c] == d) cost[t.parents[c]] += refmsg[c];
         }
     }
+    // children of every node in `ordered` order
+    std::vector<std::vector<int>> kids(N);
+    for (int p = 1; p < N; p++) kids[t.parents[t.ordered[p]]].push_back(t.ordered[p]);
     // the programs
     std::vector<float> row(data), got(N, NAN);
     std::vector<int> wrote(N, -2), done(N, 0);
@@ -80,7 +83,11 @@ static int check_programs(Tree& t, int window, const std::vector<float>& data) {
         std::memcpy(&w, &rec[1], 4);
         if (rec[3] != -1) bad++;
         float C = row[node];
-        for (int j = 0; j < rec[2]; j++) C += stored(child(rec, j));
+        if (rec[2] != (int)kids[node].size()) bad++;
+        for (int j = 0; j < rec[2]; j++) {
+            if (j < (int)kids[node].size() && child(rec, j) != kids[node][j]) bad++;
+            C += stored(child(rec, j));
+        }
         for (int i = 0; i <= L; i++) {
             if (node == root) break;
             const int* nrec = rec + kProgRec;
@@ -97,9 +104,10 @@ static int check_programs(Tree& t, int window, const std::vector<float>& data) {
             const int nk = nrec[2], npp = nrec[3];
             if (npp < 0 || npp >= nk) bad++;
             // the children as listed must be exactly next's children in `ordered` order
-            for (int j = 0; j < nk; j++) {
+            if (nk != (int)kids[next].size()) bad++;
+            for (int j = 0; j < nk && j < (int)kids[next].size(); j++) {
                 const int c = j == npp ? node : child(nrec, j);
-                if (t.child_list[t.child_start[next] + j] != c) bad++;
+                if (kids[next][j] != c) bad++;
             }
             C = row[next];
             for (int j = 0; j < nk; j++) C += j == npp ? m : stored(child(nrec, j));
