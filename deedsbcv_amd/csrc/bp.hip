@@ -730,6 +730,27 @@ __global__ __launch_bounds__(1024) void k_bp_select_b(const BPBatchTree* __restr
                 T.w1, GLOBAL ? max(1, T.N) : width, len, hw, quant, GLOBAL ? T.sel_labs : labs, rv, ri);
 }
 
+// Reads n 16-byte words of argmin tables just before the label selection
+// walks them, one dependent load per depth: its loads then come from the
+// memory-side cache instead of HBM.  `sink` is written only when `flag` is
+// set (never by the launchers), which keeps the loads.
+__global__ void k_touch(const int4* __restrict__ p, long long n, int* __restrict__ sink, int flag) {
+    int acc = 0;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int4 v = p[i];
+        acc ^= v.x ^ v.y ^ v.z ^ v.w;
+    }
+    if (flag) sink[0] = acc;
+}
+
+void launch_touch(const void* p, size_t bytes, hipStream_t s) {
+    if (((uintptr_t)p & 15) != 0 || bytes < 16) return;
+    const long long n = (long long)(bytes / 16);
+    const int blocks = (int)std::min<long long>(2048, ceil_div(n, 256));
+    k_touch<<<blocks, 256, 0, s>>>(static_cast<const int4*>(p), n, nullptr, 0);
+    DEEDS_LAUNCH_CHECK();
+}
+
 __global__ void k_root_identity(int16_t* __restrict__ allinds, int root, int len3) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l < len3) allinds[(long long)root * len3 + l] = (int16_t)l;
@@ -809,6 +830,7 @@ void launch_bp_select(const float* cost, const int16_t* allinds, const BPDevTree
     int len = 2 * hw + 1;
     const int width = std::max(1, t.max_width);
     const size_t smem = select_smem(width);
+    launch_touch(allinds, (size_t)N * len * len * len * sizeof(int16_t), s);
     if (smem == 0) {
         if (!t.sel_labs || 2LL * width > 2LL * std::max(1, N))
             fail(DEEDS_ERR_INTERNAL, "label selection: no scratch for a depth too wide for LDS");
