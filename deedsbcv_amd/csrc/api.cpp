@@ -160,8 +160,8 @@ int deeds_set_batch_graphs(int enabled) {
 }
 
 int deeds_set_datacost_tiles(int planes_per_workgroup, int pipelined) {
-    if (planes_per_workgroup < 0) return DEEDS_ERR_ARG;
-    pin_datacost_tiles(planes_per_workgroup, pipelined ? 1 : 0);
+    if (planes_per_workgroup < 0 || pipelined < 0 || pipelined > 3) return DEEDS_ERR_ARG;
+    pin_datacost_tiles(planes_per_workgroup, pipelined);
     return DEEDS_OK;
 }
 

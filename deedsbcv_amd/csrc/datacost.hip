@@ -126,11 +126,12 @@ __global__ __launch_bounds__(256) void k_datacost(float* __restrict__ cost, cons
 // over each node's STEP positions through LDS (reusing the moving-row buffer).
 // Sums are exact integers: the result equals the per-node kernels'.
 //
+// Each evaluation accumulates with v_bcnt_u32_b32's own addend: xor, xor,
+// bcnt, bcnt (4 operations; the compiler's popcount adds a fifth).
 // PIPE (rows of at most 256 positions, 15 or more labels per axis): the
 // workgroup is at most 256 threads and the kernel is compiled for three waves
 // per SIMD (what the LDS allows), which leaves registers for the next sample
-// row's LEN moving codes to be read while the current row is evaluated, and
-// each evaluation accumulates with v_bcnt_u32_b32's own addend (4 operations).
+// row's LEN moving codes to be read while the current row is evaluated.
 // Measured alone: 4.4 -> 4.8 T evals/s at 17 labels, 4.6 -> 5.0 at 15; at
 // 13 and 11 labels the smaller LDS footprint allows more waves and the plain
 // loop is faster.
@@ -140,7 +141,7 @@ __device__ __forceinline__ unsigned bcnt_acc(unsigned x, unsigned acc) {
     return r;
 }
 
-template <int STEP, int SJ, int SK, int LEN, int Q, bool PIPE>
+template <int STEP, int SJ, int SK, int LEN, int Q, bool PIPE, bool RING>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, PIPE ? 256 : 1024), amdgpu_waves_per_eu(PIPE ? 3 : 1)))
 void k_datacost_rows(float* __restrict__ cost, const uint64_t* __restrict__ F, const uint64_t* __restrict__ Mp,
                      int m, int n, int m1, int n1, float scale, int mp, int np, int chunk_planes, int chunks,
@@ -156,8 +157,16 @@ void k_datacost_rows(float* __restrict__ cost, const uint64_t* __restrict__ F, c
     const int row = b / chunks, chunk = b - row * chunks;
     const int x = row % n1, z = row / n1;
     const int W = m1 * STEP, WM = W + (len - 1) * q;
-    uint64_t* sM = lds_rows;  // moving rows, then the per-position sums
-    unsigned* sD = reinterpret_cast<unsigned*>(sM);
+    // RING: consecutive planes of one z-slice (lx -> lx + 1) shift the j
+    // sample rows by Q; when Q is a multiple of SJ, D = Q / SJ of the RJ
+    // j-rows are new and the others were staged for the previous plane.  The
+    // j-row of index D*lx + jr then lives in ring slot (D*lx + jr) mod RJ, and
+    // only the new rows are loaded; the sums get their own buffer so the rows
+    // survive (more LDS per workgroup: the autotuner weighs the two).
+    constexpr int D = Q / SJ;
+    constexpr bool REUSE = RING && Q % SJ == 0 && D < RJ;
+    uint64_t* sM = lds_rows;  // moving rows (then, unless REUSE, the per-position sums)
+    unsigned* sD = REUSE ? reinterpret_cast<unsigned*>(sM + R * WM) : reinterpret_cast<unsigned*>(sM);
     const long long mn = (long long)m * n, mpnp = (long long)mp * np;
     const int t = threadIdx.x, T = blockDim.x;  // T >= W, and WM < W + T
     // this thread's fixed codes of the R sample rows stay in registers for every plane
@@ -172,15 +181,28 @@ void k_datacost_rows(float* __restrict__ cost, const uint64_t* __restrict__ F, c
     const int pairs = m1 * len;
     for (int p = p0; p < p1; p++) {
         const int lx = p % len, lz = p / len;
-        __syncthreads();  // the previous plane's sums have been read
+        __syncthreads();  // the previous plane's sums (and rows) have been read
         const uint64_t* mbase = Mp + (long long)(x * STEP + lx * q) * mp + (long long)(z * STEP + lz * q) * mpnp;
+        // LDS row of sample row r = (jr, kr): ring slot of jr within kr's RJ rows
+        int rb[R];
 #pragma unroll
         for (int r = 0; r < R; r++) {
+            if constexpr (REUSE) {
+                const int jr = r % RJ, kr = r / RJ;
+                rb[r] = (kr * RJ + (D * lx + jr) % RJ) * WM;
+            } else {
+                rb[r] = r * WM;
+            }
+        }
+        const bool all = !REUSE || p == p0 || lx == 0;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            if (!all && r % RJ < RJ - D) continue;  // staged for the previous plane
             const uint64_t* src = mbase + (long long)((r % RJ) * SJ) * mp + (long long)((r / RJ) * SK) * mpnp;
             const uint64_t a = t < WM ? src[t] : 0;
             const uint64_t b2 = t + T < WM ? src[t + T] : 0;
-            if (t < WM) sM[r * WM + t] = a;
-            if (t + T < WM) sM[r * WM + t + T] = b2;
+            if (t < WM) sM[rb[r] + t] = a;
+            if (t + T < WM) sM[rb[r] + t + T] = b2;
         }
         __syncthreads();
         unsigned acc[LEN];
@@ -189,13 +211,13 @@ void k_datacost_rows(float* __restrict__ cost, const uint64_t* __restrict__ F, c
         if (t < W && PIPE) {
             uint64_t mv[2][LEN];  // this row's and the next row's moving codes
 #pragma unroll
-            for (int ys = 0; ys < LEN; ys++) mv[0][ys] = sM[t + ys * Q];
+            for (int ys = 0; ys < LEN; ys++) mv[0][ys] = sM[rb[0] + t + ys * Q];
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 const int cur = r & 1;
                 if (r + 1 < R) {
 #pragma unroll
-                    for (int ys = 0; ys < LEN; ys++) mv[cur ^ 1][ys] = sM[(r + 1) * WM + t + ys * Q];
+                    for (int ys = 0; ys < LEN; ys++) mv[cur ^ 1][ys] = sM[rb[r + 1] + t + ys * Q];
                 }
                 const uint64_t f = fr[r];
 #pragma unroll
@@ -208,12 +230,15 @@ void k_datacost_rows(float* __restrict__ cost, const uint64_t* __restrict__ F, c
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 const uint64_t f = fr[r];
-                const uint64_t* mr = sM + r * WM + t;
+                const uint64_t* mr = sM + rb[r] + t;
 #pragma unroll
-                for (int ys = 0; ys < LEN; ys++) acc[ys] += __builtin_popcountll(f ^ mr[ys * Q]);
+                for (int ys = 0; ys < LEN; ys++) {
+                    const uint64_t d = f ^ mr[ys * Q];
+                    acc[ys] = bcnt_acc((unsigned)d, bcnt_acc((unsigned)(d >> 32), acc[ys]));
+                }
             }
         }
-        __syncthreads();  // every moving row has been read: the buffer takes the sums
+        __syncthreads();  // every moving row has been read (unless REUSE, the buffer takes the sums)
         if (t < W) {
 #pragma unroll
             for (int ys = 0; ys < LEN; ys++) sD[t * LEN + ys] = acc[ys];
@@ -267,14 +292,16 @@ namespace {
 // Per shape (step, sample skips, labels, quantisation, control grid) the
 // launcher picks the label planes per workgroup (the label slab one workgroup
 // evaluates for its row of control points: half a z-slice of planes, one, two
-// or three) and, at 15 and more labels per axis, the pipelined or the plain
-// inner loop, by timing every candidate once on the call's own data with HIP
-// events (all candidates compute the same exact integer sums, so any of them
-// leaves the right cost volume).  The choice is kept for the process.
+// or three), at 15 and more labels per axis the pipelined or the plain inner
+// loop, and, where the quantisation is a multiple of the j skip, whether
+// sample rows are kept from plane to plane (the ring), by timing every
+// candidate once on the call's own data with HIP events (all candidates
+// compute the same exact integer sums, so any of them leaves the right cost
+// volume).  The choice is kept for the process.
 // DEEDS_TUNE=0 turns tuning off (one slice of planes, pipelined where allowed).
 struct RowsChoice {
     int chunk_planes;
-    bool pipe;
+    int mode;  // bit 0: pipelined inner loop, bit 1: sample-row ring
     float ms;
 };
 struct TuneKey {
@@ -311,9 +338,9 @@ std::string datacost_tuning_report() {
         char buf[256];
         snprintf(buf, sizeof buf,
                  "%s{\"step\": %d, \"labels_per_axis\": %d, \"quant\": %d, \"grid\": [%d, %d, %d], "
-                 "\"planes_per_workgroup\": %d, \"pipelined\": %s, \"ms\": %.4f}",
+                 "\"planes_per_workgroup\": %d, \"pipelined\": %s, \"row_ring\": %s, \"ms\": %.4f}",
                  out.size() > 1 ? ", " : "", k.step, k.len, k.q, k.m1, k.n1, k.o1, kv.second.chunk_planes,
-                 kv.second.pipe ? "true" : "false", kv.second.ms);
+                 (kv.second.mode & 1) ? "true" : "false", (kv.second.mode & 2) ? "true" : "false", kv.second.ms);
         out += buf;
     }
     return out + "]";
@@ -332,36 +359,47 @@ void launch_datacost(float* cost, const uint64_t* F, const uint64_t* Mp, int m, 
     {
         const int W = m1 * step, WM = W + (len - 1) * q;
         const int RJ = (step + plan.skip_j - 1) / plan.skip_j, RK = (step + plan.skip_k - 1) / plan.skip_k;
-        const size_t smem = std::max((size_t)RJ * RK * WM * sizeof(uint64_t), (size_t)len * W * sizeof(unsigned));
+        // with the sample-row ring the sums have their own buffer
+        const size_t rows_b = (size_t)RJ * RK * WM * sizeof(uint64_t), sums_b = (size_t)len * W * sizeof(unsigned);
+        const size_t smem = std::max(rows_b, sums_b), smem_ring = rows_b + sums_b;
+        const bool ring_ok = q % plan.skip_j == 0 && q / plan.skip_j < RJ && smem_ring <= 150 * 1024;
         const int threads = std::min(1024, ((W + 63) / 64) * 64);
         const int len2 = len * len;
         // (two staging loads per thread and row cover WM = W + (len-1)q when (len-1)q <= threads)
         const bool rows_ok = plan.skip_i == 1 && W <= 1024 && (len - 1) * q <= threads && smem <= 150 * 1024 &&
                              (long long)n1 * o1 * len2 < (1LL << 31) - 8;
-        // launches the row kernel with `chunk_planes` label planes per workgroup; false when no
-        // instantiation matches the shape
-        auto launch_rows = [&](int chunk_planes, bool pipe) -> bool {
+        // launches the row kernel with `chunk_planes` label planes per workgroup in form `mode`
+        // (bit 0: pipelined, bit 1: sample-row ring, each where the shape allows it); false when
+        // no instantiation matches the shape
+        auto launch_rows = [&](int chunk_planes, int mode) -> bool {
             const int chunks = (len2 + chunk_planes - 1) / chunk_planes;
             const long long work = (long long)n1 * o1 * chunks;
             const unsigned blocks = (unsigned)(8 * ((work + 7) / 8));
-#define DEEDS_DCR_LAUNCH(ST, SJ, SK, LEN, QQ, PIPE)                                                            \
+#define DEEDS_DCR_LAUNCH(ST, SJ, SK, LEN, QQ, PIPE, RING)                                                      \
     {                                                                                                         \
         static bool attr_set = false;                                                                         \
-        if (smem > 65536 && !attr_set) {                                                                      \
-            DEEDS_HIP(hipFuncSetAttribute((const void*)k_datacost_rows<ST, SJ, SK, LEN, QQ, PIPE>,            \
+        const size_t sm = RING ? smem_ring : smem;                                                            \
+        if (sm > 65536 && !attr_set) {                                                                        \
+            DEEDS_HIP(hipFuncSetAttribute((const void*)k_datacost_rows<ST, SJ, SK, LEN, QQ, PIPE, RING>,      \
                                           hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));           \
             attr_set = true;                                                                                  \
         }                                                                                                     \
-        k_datacost_rows<ST, SJ, SK, LEN, QQ, PIPE><<<blocks, threads, smem, s>>>(                             \
+        k_datacost_rows<ST, SJ, SK, LEN, QQ, PIPE, RING><<<blocks, threads, sm, s>>>(                         \
             cost, F, Mp, m, n, m1, n1, plan.scale, plan.mp, plan.np, chunk_planes, chunks, (int)work);         \
         DEEDS_LAUNCH_CHECK();                                                                                 \
         return true;                                                                                          \
     }
 #define DEEDS_DCR(ST, SJ, SK, LEN, QQ)                                                                        \
     if (step == ST && plan.skip_j == SJ && plan.skip_k == SK && len == LEN && q == QQ) {                      \
-        if constexpr (LEN >= 15)                                                                              \
-            if (pipe && threads <= 256) DEEDS_DCR_LAUNCH(ST, SJ, SK, LEN, QQ, true)                           \
-        DEEDS_DCR_LAUNCH(ST, SJ, SK, LEN, QQ, false)                                                          \
+        const bool ring = mode >= 2 && ring_ok;                                                               \
+        if constexpr (LEN >= 15) {                                                                            \
+            if ((mode & 1) && threads <= 256) {                                                               \
+                if (ring) DEEDS_DCR_LAUNCH(ST, SJ, SK, LEN, QQ, true, true)                                   \
+                DEEDS_DCR_LAUNCH(ST, SJ, SK, LEN, QQ, true, false)                                            \
+            }                                                                                                 \
+        }                                                                                                     \
+        if (ring) DEEDS_DCR_LAUNCH(ST, SJ, SK, LEN, QQ, false, true)                                          \
+        DEEDS_DCR_LAUNCH(ST, SJ, SK, LEN, QQ, false, false)                                                   \
     }
             // the five levels of deedsBCV's default schedule (grid 8..4, radius 8..4,
             // quantisation 5..1); other schedules take the per-node kernels below
@@ -373,7 +411,7 @@ void launch_datacost(float* cost, const uint64_t* F, const uint64_t* Mp, int m, 
         };
         if (rows_ok) {
             const TuneKey key{step, plan.skip_j, plan.skip_k, len, q, m1, n1, o1};
-            RowsChoice choice{len, true, 0.0f};
+            RowsChoice choice{len, 1, 0.0f};
             bool have = false;
             {
                 std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -388,17 +426,18 @@ void launch_datacost(float* cost, const uint64_t* F, const uint64_t* Mp, int m, 
                 DEEDS_HIP(hipEventCreate(&e0));
                 DEEDS_HIP(hipEventCreate(&e1));
                 bool any = false;
-                for (int pipe = 1; pipe >= 0; pipe--) {
-                    if (pipe && !(len >= 15 && threads <= 256)) continue;
+                for (int mode = 3; mode >= 0; mode--) {
+                    if ((mode & 1) && !(len >= 15 && threads <= 256)) continue;
+                    if ((mode & 2) && !ring_ok) continue;
                     for (int c : cands) {
                         if (c > len2) continue;
                         DEEDS_HIP(hipEventRecord(e0, s));
-                        if (!launch_rows(c, pipe != 0)) break;
+                        if (!launch_rows(c, mode)) break;
                         DEEDS_HIP(hipEventRecord(e1, s));
                         DEEDS_HIP(hipEventSynchronize(e1));
                         float ms = 0.0f;
                         DEEDS_HIP(hipEventElapsedTime(&ms, e0, e1));
-                        if (!any || ms < choice.ms) choice = RowsChoice{c, pipe != 0, ms};
+                        if (!any || ms < choice.ms) choice = RowsChoice{c, mode, ms};
                         any = true;
                     }
                 }
@@ -410,8 +449,8 @@ void launch_datacost(float* cost, const uint64_t* F, const uint64_t* Mp, int m, 
                 }
             }
             const int pin = g_pin_planes.load();
-            if (pin > 0 && launch_rows(std::min(pin, len2), g_pin_pipe.load() != 0)) return;
-            if (launch_rows(choice.chunk_planes, choice.pipe)) return;
+            if (pin > 0 && launch_rows(std::min(pin, len2), g_pin_pipe.load())) return;
+            if (launch_rows(choice.chunk_planes, choice.mode)) return;
         }
     }
 #define DEEDS_DC(ST, SI, SJ, SK)                                                                              \

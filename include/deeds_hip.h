@@ -108,8 +108,9 @@ int deeds_ctx_trim(deeds_ctx* ctx);
 size_t deeds_tuning_report(char* buf, size_t cap);
 /* Pins the data-cost tile shape for later calls in this process instead of
  * the autotuner's choice: label planes per workgroup (0 restores tuning) and
- * the pipelined inner loop (used where the shape allows it).  Every shape gives
- * the same cost volume. */
+ * the form, bit 0 the pipelined inner loop, bit 1 sample rows kept from plane
+ * to plane (each used where the shape allows it).  Every shape gives the same
+ * cost volume. */
 int deeds_set_datacost_tiles(int planes_per_workgroup, int pipelined);
 
 /* Fault injection for testing callers' error paths (the reference's failure

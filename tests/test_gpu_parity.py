@@ -459,10 +459,11 @@ def test_context_rejects_bad_tensors():
         ctx.descriptor(f.cpu(), 1)
 
 
-@pytest.mark.parametrize("case", [(8, 8, 5), (7, 7, 4), (4, 4, 1)])
+@pytest.mark.parametrize("case", [(8, 8, 5), (7, 7, 4), (6, 6, 3), (5, 5, 2), (4, 4, 1)])
 def test_cost_every_tile_shape_vs_oracle(oracle, case):
-    """The autotuner's candidates (label planes per workgroup, pipelined or
-    plain inner loop), each pinned in turn, give the oracle's cost volume."""
+    """The autotuner's candidates (label planes per workgroup; pipelined or
+    plain inner loop; sample rows kept from plane to plane or restaged), each
+    pinned in turn, give the oracle's cost volume."""
     from deedsbcv_amd import _lib
     step, hw, q = case
     print(f"case: datacost tile shapes {case}", flush=True)
@@ -474,7 +475,7 @@ def test_cost_every_tile_shape_vs_oracle(oracle, case):
     lib = _lib.load()
     try:
         for planes in ((len_ + 1) // 2, len_, 2 * len_, 3 * len_, len_ * len_):
-            for pipe in (1, 0):
+            for pipe in (1, 0, 3, 2):
                 assert lib.deeds_set_datacost_tiles(planes, pipe) == 0
                 _eq(api.dataCostCL(F, M, step, hw, q, 1.6, 1), want, f"cost {case} planes={planes} pipe={pipe}")
     finally:
