@@ -345,7 +345,7 @@ size_t tree_stage_bytes(int N) {
 // must stay untouched until the copy has executed on `s`.
 BPDevTree pack_tree(const Tree& t, void* pinned, void* d_stage, size_t* bytes) {
     size_t ints = t.ordered.size() + t.parents.size() + t.depth_start.size() + t.prog_start.size() + t.prog.size() +
-                  t.leaves.size() + t.leaf_w.size() + t.sel_info.size();
+                  t.leaves.size() + t.leaf_w.size() + t.sel_info.size() + t.sel_gp.size();
     size_t bound = tree_ints_bound(t.N);
     if (ints > bound) fail(DEEDS_ERR_INTERNAL, "tree schedule larger than its staging bound");
     int* hi = static_cast<int*>(pinned);
@@ -368,6 +368,7 @@ BPDevTree pack_tree(const Tree& t, void* pinned, void* d_stage, size_t* bytes) {
     out.leaf_w = put(t.leaf_w);
     out.n_leaves = (int)t.leaves.size();
     out.sel_info = put(t.sel_info);
+    out.sel_gp = put(t.sel_gp);
     out.max_width = t.max_width;
     out.root = t.N > 0 ? t.ordered[0] : 0;
     out.off3 = reinterpret_cast<float*>(di + bound);
@@ -384,11 +385,12 @@ BPDevTree stage_tree(const Tree& t, void* pinned, void* d_stage, hipStream_t s) 
 }
 
 // Bottom-up rounds, then the top-down selection.
-void run_bp(const Tree& t, const BPDevTree& dt, float* d_cost, int16_t* d_inds, const float* u0, const float* v0,
-            const float* w0, float* u1, float* v1, float* w1, int hw, float quant, hipStream_t s) {
+void run_bp(const Tree& t, const BPDevTree& dt, float* d_cost, int16_t* d_inds, int16_t* d_comp, const float* u0,
+            const float* v0, const float* w0, float* u1, float* v1, float* w1, int hw, float quant, hipStream_t s) {
     launch_bp_rounds(d_cost, d_inds, dt.off3, dt, t.N, t.round_start.data(), t.rounds, t.tail_round, u0, v0, w0, hw,
                      quant, s);
-    launch_bp_select(d_cost, d_inds, dt, t.N, t.maxd, u0, v0, w0, u1, v1, w1, hw, quant, s);
+    const int first_even = t.maxd >= 2 ? t.depth_start[2] : t.N;
+    launch_bp_select(d_cost, d_inds, d_comp, first_even, dt, t.N, t.maxd, u0, v0, w0, u1, v1, w1, hw, quant, s);
 }
 
 namespace {
@@ -408,7 +410,7 @@ void run_bp_batch(const std::vector<BPBatchItem>& items, int hw, float quant, vo
     if (T == 1) {
         // one tree: the single-tree kernels take their arguments by value (no descriptor loads)
         const BPBatchItem& it = items[0];
-        run_bp(*it.tree, it.dt, it.cost, it.inds, it.u0, it.v0, it.w0, it.u1, it.v1, it.w1, hw, quant, s);
+        run_bp(*it.tree, it.dt, it.cost, it.inds, it.comp, it.u0, it.v0, it.w0, it.u1, it.v1, it.w1, hw, quant, s);
         return;
     }
     long long nodes = 0;
@@ -495,7 +497,8 @@ int deeds_ctx_regularisation(deeds_ctx* ctx, float* d_cost, const float* d_u0, c
         void* d_stage = ctx->buf("bp.tree", stage_bytes);
         BPDevTree dt = stage_tree(t, pinned, d_stage, ctx->stream);
         int16_t* inds = d_allinds ? d_allinds : ctx->get<int16_t>("bp.inds", (size_t)t.N * len3);
-        run_bp(t, dt, d_cost, inds, d_u0, d_v0, d_w0, d_u1, d_v1, d_w1, hw, quant, ctx->stream);
+        int16_t* comp = ctx->get<int16_t>("bp.comp", (size_t)t.N * len3);
+        run_bp(t, dt, d_cost, inds, comp, d_u0, d_v0, d_w0, d_u1, d_v1, d_w1, hw, quant, ctx->stream);
         // the caller's table gets the root's row as well (regularisation.h:273-274)
         if (d_allinds) launch_root_identity(d_allinds, t.ordered[0], (int)len3, ctx->stream);
         // the staging copy must complete before the pinned block is released

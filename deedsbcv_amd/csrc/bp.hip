@@ -730,6 +730,150 @@ __global__ __launch_bounds__(1024) void k_bp_select_b(const BPBatchTree* __restr
                 T.w1, GLOBAL ? max(1, T.N) : width, len, hw, quant, GLOBAL ? T.sel_labs : labs, rv, ri);
 }
 
+// ---- two-depth label selection ------------------------------------------
+// The selection's chain is one dependent load per BFS depth.  Composing the
+// tables of a node c at an even depth d >= 2 and of its parent p,
+// comp[c][x] = allinds[c][allinds[p][x]] (the label of c given its
+// grandparent's label x: regularisation.h:273-293 applied twice), lets one
+// workgroup walk only the even depths; the odd depths then follow in one
+// parallel pass from their parents' labels.  Exact: the same table entries,
+// looked up in two steps ahead of time.
+
+// label -> displacement field entries u1 = (lj-hw)q + u0, v1 = (li-hw)q + v0,
+// w1 = (lk-hw)q + w0 (regularisation.h:281-288), 0 <= lab < len^3
+__device__ __forceinline__ void emit_label(int c, int lab, int len, int hw, float quant, const float* __restrict__ u0,
+                                           const float* __restrict__ v0, const float* __restrict__ w0,
+                                           float* __restrict__ u1, float* __restrict__ v1, float* __restrict__ w1) {
+    const unsigned ul = (unsigned)lab, ulen = (unsigned)len;
+    const int lk = (int)(ul / (ulen * ulen)), r2 = (int)(ul - (unsigned)lk * ulen * ulen);
+    const int lj = (int)((unsigned)r2 / ulen), li = r2 - lj * len;
+    u1[c] = (float)(lj - hw) * quant + u0[c];
+    v1[c] = (float)(li - hw) * quant + v0[c];
+    w1[c] = (float)(lk - hw) * quant + w0[c];
+}
+
+// One workgroup per position p >= `first` (the first position at depth 2).
+__global__ __launch_bounds__(256) void k_compose(int16_t* __restrict__ comp, const int16_t* __restrict__ allinds,
+                                                 const int* __restrict__ ordered, const int* __restrict__ parents,
+                                                 const int* __restrict__ sel_gp, int first, int len3) {
+    const int p = first + blockIdx.x;
+    if (sel_gp[p] < 0) return;  // odd depth
+    const int c = ordered[p];
+    const int16_t* ac = allinds + (long long)c * len3;
+    const int16_t* ap = allinds + (long long)parents[c] * len3;
+    int16_t* b = comp + (long long)c * len3;
+    for (int x = threadIdx.x; x < len3; x += blockDim.x) b[x] = ac[ap[x]];
+}
+
+// The even depths (one workgroup; the structure of select_body with the
+// composite tables and the grandparent indices).  Each even node's label also
+// goes to lab[node] for the odd pass.  labs: two LDS rows of `width` labels.
+__global__ __launch_bounds__(1024) void k_bp_select2(const float* __restrict__ cost, const int16_t* __restrict__ comp,
+                                                     const int* __restrict__ ordered,
+                                                     const int* __restrict__ depth_start,
+                                                     const int* __restrict__ sel_gp, int maxd,
+                                                     const float* __restrict__ u0, const float* __restrict__ v0,
+                                                     const float* __restrict__ w0, float* __restrict__ u1,
+                                                     float* __restrict__ v1, float* __restrict__ w1, int width,
+                                                     int len, int hw, float quant, int* __restrict__ lab) {
+    extern __shared__ int labs[];
+    __shared__ float rv[16];
+    __shared__ int ri[16];
+    const int len3 = len * len * len;
+    const int t = threadIdx.x;
+    const int root = ordered[0];
+    const float* rc = cost + (long long)root * len3;
+    float bv = 3.402823466e38f;
+    int bi = 0x7fffffff;
+    for (int l = t; l < len3; l += blockDim.x) {
+        const float v = rc[l];
+        if (v < bv || (v == bv && l < bi)) { bv = v; bi = l; }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const float ov = __shfl_xor(bv, off, 64);
+        const int oi = __shfl_xor(bi, off, 64);
+        if (ov < bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+    }
+    if ((t & 63) == 0) { rv[t >> 6] = bv; ri[t >> 6] = bi; }
+    __syncthreads();
+    int* prev = labs;
+    int* cur = labs + width;
+    if (t == 0) {
+        float v = rv[0];
+        int id = ri[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++)
+            if (rv[w] < v || (rv[w] == v && ri[w] < id)) { v = rv[w]; id = ri[w]; }
+        prev[0] = id;
+        lab[root] = id;
+        emit_label(root, id, len, hw, quant, u0, v0, w0, u1, v1, w1);
+    }
+    // d = 2, 4, ...: positions [b, e) of depth d; E = depth_start[d + 3] (the next even depth's end)
+    int b = 0, e = 0, E = 0;
+    int nc = -1, ngp = 0;
+    if (maxd >= 2) {
+        b = depth_start[2];
+        e = depth_start[3];
+        E = depth_start[maxd >= 4 ? 5 : maxd + 1];
+        if (b + t < e) {
+            nc = ordered[b + t];
+            ngp = sel_gp[b + t];
+        }
+    }
+    int pc = -1, plab = 0;
+    __syncthreads();
+    for (int d = 2; d <= maxd; d += 2) {
+        const int c = nc, gp = ngp;
+        if (pc >= 0) emit_label(pc, plab, len, hw, quant, u0, v0, w0, u1, v1, w1);
+        // the next even depth d + 2: positions [depth_start[d + 2], E)
+        const int nb = d + 2 <= maxd ? depth_start[d + 2] : 0;
+        nc = -1;
+        if (d + 2 <= maxd && nb + t < E) {
+            nc = ordered[nb + t];
+            ngp = sel_gp[nb + t];
+        }
+        const int E2 = depth_start[d + 5 <= maxd + 1 ? d + 5 : maxd + 1];
+        if (c >= 0) {
+            const int l = comp[(long long)c * len3 + prev[gp]];
+            cur[t] = l;
+            lab[c] = l;
+            plab = l;
+        }
+        pc = c;
+        for (int q = b + (int)blockDim.x + t; q < e; q += blockDim.x) {  // depths wider than the block
+            const int c2 = ordered[q];
+            const int l = comp[(long long)c2 * len3 + prev[sel_gp[q]]];
+            cur[q - b] = l;
+            lab[c2] = l;
+            emit_label(c2, l, len, hw, quant, u0, v0, w0, u1, v1, w1);
+        }
+        __syncthreads();
+        int* tmp = prev;
+        prev = cur;
+        cur = tmp;
+        b = nb;
+        e = d + 3 <= maxd + 1 ? depth_start[d + 3] : 0;
+        E = E2;
+    }
+    if (pc >= 0) emit_label(pc, plab, len, hw, quant, u0, v0, w0, u1, v1, w1);
+}
+
+// The odd depths: every node there takes its own table at its parent's label.
+__global__ __launch_bounds__(256) void k_select_odd(const int16_t* __restrict__ allinds,
+                                                    const int* __restrict__ ordered,
+                                                    const int* __restrict__ parents, const int* __restrict__ sel_gp,
+                                                    const int* __restrict__ lab, int N, int len, int hw, float quant,
+                                                    const float* __restrict__ u0, const float* __restrict__ v0,
+                                                    const float* __restrict__ w0, float* __restrict__ u1,
+                                                    float* __restrict__ v1, float* __restrict__ w1) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0 || p >= N || sel_gp[p] >= 0) return;  // the root and the even depths are done
+    const int len3 = len * len * len;
+    const int c = ordered[p];
+    const int l = allinds[(long long)c * len3 + lab[parents[c]]];
+    emit_label(c, l, len, hw, quant, u0, v0, w0, u1, v1, w1);
+}
+
 // Reads n 16-byte words of argmin tables just before the label selection
 // walks them, one dependent load per depth: its loads then come from the
 // memory-side cache instead of HBM.  `sink` is written only when `flag` is
@@ -824,13 +968,29 @@ void launch_bp_rounds(float* cost, int16_t* allinds, float* off3, const BPDevTre
     }
 }
 
-void launch_bp_select(const float* cost, const int16_t* allinds, const BPDevTree& t, int N, int maxd,
-                      const float* u0, const float* v0, const float* w0, float* u1, float* v1, float* w1, int hw,
-                      float quant, hipStream_t s) {
+void launch_bp_select(const float* cost, const int16_t* allinds, int16_t* comp, int first_even, const BPDevTree& t,
+                      int N, int maxd, const float* u0, const float* v0, const float* w0, float* u1, float* v1,
+                      float* w1, int hw, float quant, hipStream_t s) {
     int len = 2 * hw + 1;
     const int width = std::max(1, t.max_width);
     const size_t smem = select_smem(width);
-    launch_touch(allinds, (size_t)N * len * len * len * sizeof(int16_t), s);
+    const int len3 = len * len * len;
+    if (comp && smem > 0 && maxd >= 2 && t.sel_labs && first_even < N) {
+        // two-depth walk: compose the even depths' tables, walk them, then the odd depths in parallel
+        k_compose<<<N - first_even, 256, 0, s>>>(comp, allinds, t.ordered, t.parents, t.sel_gp, first_even, len3);
+        DEEDS_LAUNCH_CHECK();
+        if (smem > 65536)
+            DEEDS_HIP(hipFuncSetAttribute((const void*)k_bp_select2, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)smem));
+        k_bp_select2<<<1, 1024, smem, s>>>(cost, comp, t.ordered, t.depth_start, t.sel_gp, maxd, u0, v0, w0, u1, v1,
+                                           w1, width, len, hw, quant, t.sel_labs);
+        DEEDS_LAUNCH_CHECK();
+        k_select_odd<<<ceil_div(N, 256), 256, 0, s>>>(allinds, t.ordered, t.parents, t.sel_gp, t.sel_labs, N, len, hw,
+                                                      quant, u0, v0, w0, u1, v1, w1);
+        DEEDS_LAUNCH_CHECK();
+        return;
+    }
+    launch_touch(allinds, (size_t)N * len3 * sizeof(int16_t), s);
     if (smem == 0) {
         if (!t.sel_labs || 2LL * width > 2LL * std::max(1, N))
             fail(DEEDS_ERR_INTERNAL, "label selection: no scratch for a depth too wide for LDS");

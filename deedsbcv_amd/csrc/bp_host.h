@@ -17,9 +17,10 @@ BPDevTree pack_tree(const Tree& t, void* pinned, void* d_stage, size_t* bytes);
 // pack_tree, then the copy into d_stage enqueued on s; `pinned` must stay
 // untouched until the copy has executed.
 BPDevTree stage_tree(const Tree& t, void* pinned, void* d_stage, hipStream_t s);
-// Bottom-up rounds, then the top-down selection, of one tree.
-void run_bp(const Tree& t, const BPDevTree& dt, float* d_cost, int16_t* d_inds, const float* u0, const float* v0,
-            const float* w0, float* u1, float* v1, float* w1, int hw, float quant, hipStream_t s);
+// Bottom-up rounds, then the top-down selection, of one tree; d_comp: null
+// or N*len^3 int16 of scratch for the selection's composed tables.
+void run_bp(const Tree& t, const BPDevTree& dt, float* d_cost, int16_t* d_inds, int16_t* d_comp, const float* u0,
+            const float* v0, const float* w0, float* u1, float* v1, float* w1, int hw, float quant, hipStream_t s);
 
 // One tree of a batched BP: its host schedule, its staged device copy, its
 // cost volume / argmin tables and its fields.
@@ -28,6 +29,7 @@ struct BPBatchItem {
     BPDevTree dt;
     float* cost;
     int16_t* inds;
+    int16_t* comp;  // null or scratch for composed tables (used when the item runs alone)
     const float *u0, *v0, *w0;
     float *u1, *v1, *w1;
 };

@@ -20,6 +20,9 @@ struct Tree {
     // top-down selection: (node, index of its parent within the parent's
     // depth) per position of `ordered`, and the largest depth width
     std::vector<int> sel_info;
+    // two-depth selection: per position at an even depth >= 2 the index of
+    // its grandparent within the grandparent's depth, else -1
+    std::vector<int> sel_gp;
     int max_width = 0;
     // children of every node in `ordered` order (CSR by node id); built
     // only without BP programs (the schedule of the programs uses positions)
@@ -73,7 +76,7 @@ static_assert(kProgRec * (1 + kBPWindow) <= kProgStage, "a block's header and re
 
 
 // Upper bound on the ints a device copy of a tree's BP schedule needs
-// (ordered, parents, depth_start, prog_start, prog, leaves, leaf_w, sel_info).
+// (ordered, parents, depth_start, prog_start, prog, leaves, leaf_w, sel_info, sel_gp).
 size_t tree_device_ints(int N);
 
 // primsMST.h:101-110: float mean and std of the image, bit-identical to the

@@ -228,6 +228,9 @@ class PairRun {
         for (int d = 0; d < 2; d++) {
             cost_[d] = get<float>("cost." + std::to_string(d), max_cost);
             inds_[d] = get<int16_t>("inds." + std::to_string(d), max_cost);
+            // a pair registered alone (no tag) runs the single-tree BP, whose
+            // selection walks composed tables; a batch's trees use the batched one
+            comp_[d] = tag.empty() ? get<int16_t>("comp." + std::to_string(d), max_cost) : nullptr;
         }
         // control-grid fields: previous level (fwd u,v,w, bwd u,v,w), this level's u0 per direction, new fields
         const size_t sz2_0 = (size_t)(m / S.grid[0]) * (n / S.grid[0]) * (o / S.grid[0]);
@@ -512,8 +515,8 @@ class PairRun {
         float* u0 = g0_[dir];
         const size_t n1 = c.sz1;
         float* nf = newf_ + dir * 3 * n1;
-        items.push_back(BPBatchItem{tree_[dir].get(), dt, cost_[dir], inds_[dir], u0, u0 + n1, u0 + 2 * n1, nf,
-                                    nf + n1, nf + 2 * n1});
+        items.push_back(BPBatchItem{tree_[dir].get(), dt, cost_[dir], inds_[dir], comp_[dir], u0, u0 + n1,
+                                    u0 + 2 * n1, nf, nf + n1, nf + 2 * n1});
     }
 
     // Marks the end of the forward direction's BP of level l on s0.
@@ -631,6 +634,7 @@ class PairRun {
     uint64_t *mind_mov_, *mind_fix_;
     float* cost_[2];
     int16_t* inds_[2];
+    int16_t* comp_[2];
     float *prevf_, *newf_, *cmtmp_;
     float* g0_[2];
     double *jac_, *ssd_;
