@@ -1,12 +1,14 @@
 // MIND-SSC descriptor on gfx950 (reference: MINDSSCbox.h:2-232).
 //
 // Pipeline per call (6 float channels in HBM):
-//   k_ssd_box_i   squared difference to the shifted image (:79-90, :113-117)
-//                 fused with the box pass along i (:2-29)
-//   k_box_lines   box pass along j, then along k (:31-73), box radius 1..3
-//                 (k_box_cols for larger radii)
+//   k_ssd_box_ij  squared difference to the shifted image (:79-90, :113-117)
+//                 fused with the box passes along i and j (:2-50), one
+//                 workgroup per slice and channel (box radius 1..3, m <= 1024)
+//   k_box_lines   box pass along k (:52-73)
 //   k_mind_code   12 SSC values -> min/noise normalisation -> 6-level
 //                 thermometer, packed into a 64-bit code (:172-222)
+// Other radii and sizes take k_ssd_box_i (the SSD and the i pass) and
+// k_box_cols for j and k.
 // The box filter must reproduce the reference's float rounding: every line is
 // one sequential float prefix sum (the sums exceed 2^24, so a tree scan would
 // change bits) followed by the clamped window difference.  Along i (lines
@@ -192,17 +194,127 @@ __global__ __launch_bounds__(256) void k_box_lines(float* __restrict__ dst, cons
     }
 }
 
+// Logical tile of workgroup b in a 1-D launch of 8 * per workgroups: the
+// GPU's eight XCDs take workgroups round robin (b mod 8), so XCD x gets the
+// contiguous tiles [x * per, (x + 1) * per) and the neighbouring lines a tile
+// rereads sit in that XCD's L2.  Tiles >= the real count are skipped by the caller.
+__device__ __forceinline__ int xcd_tile(int b, int per) { return (b & 7) * per + (b >> 3); }
+
+// i and j passes fused, with the SSD to the shifted image (MINDSSCbox.h:
+// 79-90, 113-117, then the box passes :2-50 along i and j).  One workgroup
+// per (slice k, channel), one thread per column i (blockDim >= m).  The
+// slice goes through LDS in chunks of R rows: the threads fill the chunk
+// with squared differences (coalesced), one thread per row runs the row's
+// sequential i prefix in place, then every thread walks its column down the
+// chunk, taking the i window from LDS and carrying the j prefix (a running
+// float sum and the last 2*HW + 2 prefix values) in registers from chunk to
+// chunk, and writes the j window HW rows behind.  The arithmetic of every
+// value is that of the separate passes; only the i-filtered planes no longer
+// go through HBM.  Tiles: (k, ch) in xcd_tile order, k-major with the six
+// channels of a slice adjacent, so the slice and its shifted copy are read
+// from HBM about once.
+template <int HW>
+__global__ __launch_bounds__(1024) void k_ssd_box_ij(float* __restrict__ out, const float* __restrict__ im, int m,
+                                                      int n, int o, int R, int per) {
+    extern __shared__ float tile[];
+    const int tid = xcd_tile((int)blockIdx.x, per);
+    if (tid >= 6 * o) return;
+    const int k = tid / 6, ch = tid - 6 * (tid / 6);
+    const int ld = m + 1;
+    const long long mn = (long long)m * n;
+    const int di = c_ssd_shift[ch][0] * HW, dj = c_ssd_shift[ch][1] * HW, dk = c_ssd_shift[ch][2] * HW;
+    const int kk = k + dk;
+    const bool k_in = kk >= 0 && kk < o;
+    const int i = threadIdx.x;
+    const bool col = i < m;
+    const int ic = col ? i : m - 1;  // clamped column: loads need no branch
+    const int ii = ic + di;
+    const bool ii_in = ii >= 0 && ii < m;
+    const float* cur = im + (long long)k * mn;
+    const float* sh = im + (long long)(k_in ? kk : k) * mn;
+    float* dst = out + (long long)ch * mn * o + (long long)k * mn;
+    constexpr int H = 2 * HW + 2;
+    float hist[H];  // after row t: hist[q] = P[t - 2*HW - 1 + q] of the column's j prefix
+#pragma unroll
+    for (int q = 0; q < H; q++) hist[q] = 0.0f;
+    float acc = 0.0f;
+    constexpr int U = 8;
+    for (int j0 = 0; j0 < n; j0 += R) {
+        const int rows = min(R, n - j0);
+        // fill the chunk: every load of a group of U rows in flight before its stores
+        for (int rb = 0; rb < rows; rb += U) {
+            float cv[U], sv[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int j = min(j0 + rb + u, n - 1), jj = j + dj;
+                const bool in = ii_in && k_in && jj >= 0 && jj < n;
+                cv[u] = cur[ic + (long long)j * m];
+                sv[u] = sh[(in ? ii : ic) + (long long)(in ? jj : j) * m];
+                if (!in) sv[u] = cv[u];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (col && rb + u < rows) {
+                    const float d = sv[u] - cv[u];
+                    tile[(rb + u) * ld + i] = d * d;
+                }
+            }
+        }
+        __syncthreads();
+        if (i < rows) prefix_in_place(tile + i * ld, m, 1);
+        __syncthreads();
+        if (col) {
+            for (int r = 0; r < rows; r++) {
+                const float x = window(tile + r * ld, i, m, HW, 1);
+                const int t = j0 + r;
+                acc = t == 0 ? x : acc + x;
+#pragma unroll
+                for (int q = 0; q + 1 < H; q++) hist[q] = hist[q + 1];
+                hist[H - 1] = acc;
+                // output t - HW: P[min(t, n-1)] - P[t - 2HW - 1] (MINDSSCbox.h:17-27)
+                const int so = t - HW;
+                if (so >= 0) {
+                    float v = acc;
+                    if (so > HW) v = v - hist[0];
+                    dst[i + (long long)so * m] = v;
+                }
+            }
+        }
+        __syncthreads();  // the next chunk overwrites the tile
+    }
+    if (!col) return;
+    // the last HW rows clamp the window's upper end at n - 1
+#pragma unroll
+    for (int e = 0; e < HW; e++) {
+        const int so = n - HW + e;
+        if (so < 0) continue;
+        float v = hist[H - 1];
+        if (so > HW) v = v - hist[e + 1];
+        dst[i + (long long)so * m] = v;
+    }
+}
+
 struct Thr {
     float t[5];
 };
 
-// Grid (ceil(m / 256), n, o): one thread per voxel, coordinates from the
-// launch shape (no divisions).
+// One thread per voxel; a workgroup takes `nbi` consecutive chunks of one
+// row (j, k).  The rows are ordered in blocks of kCodeRows j-rows, every k
+// of a block before the next block, and spread over the XCDs in contiguous
+// ranges (xcd_tile): the neighbours a voxel reads at j +- qs and k - qs are
+// then rows its XCD read shortly before, found in its L2.
+constexpr int kCodeRows = 32;
+
 __global__ __launch_bounds__(256) void k_mind_code(uint64_t* __restrict__ out, const float* __restrict__ d, int m,
-                                                   int n, int o, int qs, Thr thr) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+                                                   int n, int o, int qs, Thr thr, int nbi, int per) {
+    const int tile = xcd_tile((int)blockIdx.x, per);
+    if (tile >= nbi * n * o) return;
+    const int r = tile / nbi;
+    const int i = (tile - r * nbi) * blockDim.x + threadIdx.x;
     if (i >= m) return;
-    const int j = blockIdx.y, k = blockIdx.z;
+    const int jb = r / (kCodeRows * o), rem = r - jb * kCodeRows * o;
+    const int rows_b = min(kCodeRows, n - jb * kCodeRows);
+    const int k = rem / rows_b, j = jb * kCodeRows + (rem - k * rows_b);
     const long long sz = (long long)m * n * o;
     const long long p = i + (long long)j * m + (long long)k * m * n;
     float v[12];
@@ -237,6 +349,11 @@ __global__ __launch_bounds__(256) void k_mind_code(uint64_t* __restrict__ out, c
 }
 
 size_t rows_smem(int m) { return (size_t)TILE_I * (m + 1) * sizeof(float); }
+
+// the fused i/j pass: one thread per column (at most 1024), and rows per LDS
+// chunk so that the chunk stays near 40 KiB (several workgroups per CU)
+constexpr int kIJMaxM = 1024;
+int ij_rows(int m) { return std::max(8, std::min(32, 10240 / (m + 1))); }
 size_t cols_smem(int L) { return (size_t)L * (TILE + 1) * sizeof(float); }
 
 }  // namespace
@@ -258,24 +375,28 @@ void launch_descriptor(uint64_t* out, const float* im, int m, int n, int o, int 
         DEEDS_HIP(hipFuncSetAttribute((const void*)k_box_cols, hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds));
         attr = true;
     }
-    // i pass (with the SSD): im -> w0; j pass: w0 -> w1; k pass: w1 -> w0
-    k_ssd_box_i<<<dim3(ceil_div(n, TILE_I), o, 6), BOX_THREADS, rows_smem(m), s>>>(w0, im, m, n, o, qs);
-    DEEDS_LAUNCH_CHECK();
-    if (qs >= 1 && qs <= 3) {
-        const long long lines_j = (long long)m * o * 6, lines_k = (long long)m * n * 6;
-        const unsigned gj = (unsigned)ceil_div(lines_j, 256), gk = (unsigned)ceil_div(lines_k, 256);
+    if (qs >= 1 && qs <= 3 && m <= kIJMaxM) {
+        // i and j passes fused: im -> w1; k pass: w1 -> w0
+        const int R = ij_rows(m);
+        const int per = (int)ceil_div(6LL * o, 8);
+        const int threads = 64 * (int)ceil_div(m, 64);
+        const size_t smem = (size_t)R * (m + 1) * sizeof(float);
+        const unsigned gk = (unsigned)ceil_div((long long)m * n * 6, 256);
         if (qs == 1) {
-            k_box_lines<1><<<gj, 256, 0, s>>>(w1, w0, m, n, o, 1);
+            k_ssd_box_ij<1><<<8 * per, threads, smem, s>>>(w1, im, m, n, o, R, per);
             k_box_lines<1><<<gk, 256, 0, s>>>(w0, w1, m, n, o, 2);
         } else if (qs == 2) {
-            k_box_lines<2><<<gj, 256, 0, s>>>(w1, w0, m, n, o, 1);
+            k_ssd_box_ij<2><<<8 * per, threads, smem, s>>>(w1, im, m, n, o, R, per);
             k_box_lines<2><<<gk, 256, 0, s>>>(w0, w1, m, n, o, 2);
         } else {
-            k_box_lines<3><<<gj, 256, 0, s>>>(w1, w0, m, n, o, 1);
+            k_ssd_box_ij<3><<<8 * per, threads, smem, s>>>(w1, im, m, n, o, R, per);
             k_box_lines<3><<<gk, 256, 0, s>>>(w0, w1, m, n, o, 2);
         }
         DEEDS_LAUNCH_CHECK();
     } else {
+        // i pass (with the SSD): im -> w0; j pass: w0 -> w1; k pass: w1 -> w0
+        k_ssd_box_i<<<dim3(ceil_div(n, TILE_I), o, 6), BOX_THREADS, rows_smem(m), s>>>(w0, im, m, n, o, qs);
+        DEEDS_LAUNCH_CHECK();
         k_box_cols<<<dim3(ceil_div(m, TILE), o, 6), BOX_THREADS, cols_smem(n), s>>>(w1, w0, m, n, o, 1, qs);
         DEEDS_LAUNCH_CHECK();
         k_box_cols<<<dim3(ceil_div(m, TILE), n, 6), BOX_THREADS, cols_smem(o), s>>>(w0, w1, m, n, o, 2, qs);
@@ -284,7 +405,9 @@ void launch_descriptor(uint64_t* out, const float* im, int m, int n, int o, int 
     Thr thr;
     mind_thresholds(thr.t);
     const int ct = std::min(256, 64 * (int)ceil_div(m, 64));
-    k_mind_code<<<dim3(ceil_div(m, ct), n, o), ct, 0, s>>>(out, w0, m, n, o, qs, thr);
+    const int nbi = (int)ceil_div(m, ct);
+    const int per = (int)ceil_div((long long)nbi * n * o, 8);
+    k_mind_code<<<8 * per, ct, 0, s>>>(out, w0, m, n, o, qs, thr, nbi, per);
     DEEDS_LAUNCH_CHECK();
 }
 
