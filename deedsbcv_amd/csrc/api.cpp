@@ -345,7 +345,7 @@ size_t tree_stage_bytes(int N) {
 // must stay untouched until the copy has executed on `s`.
 BPDevTree pack_tree(const Tree& t, void* pinned, void* d_stage, size_t* bytes) {
     size_t ints = t.ordered.size() + t.parents.size() + t.depth_start.size() + t.prog_start.size() + t.prog.size() +
-                  t.leaves.size() + t.leaf_w.size() + t.sel_info.size() + t.sel_gp.size();
+                  t.leaves.size() + t.leaf_w.size() + t.sel_info.size() + t.sel_jump.size();
     size_t bound = tree_ints_bound(t.N);
     if (ints > bound) fail(DEEDS_ERR_INTERNAL, "tree schedule larger than its staging bound");
     int* hi = static_cast<int*>(pinned);
@@ -368,7 +368,7 @@ BPDevTree pack_tree(const Tree& t, void* pinned, void* d_stage, size_t* bytes) {
     out.leaf_w = put(t.leaf_w);
     out.n_leaves = (int)t.leaves.size();
     out.sel_info = put(t.sel_info);
-    out.sel_gp = put(t.sel_gp);
+    out.sel_jump = put(t.sel_jump);
     out.max_width = t.max_width;
     out.root = t.N > 0 ? t.ordered[0] : 0;
     out.off3 = reinterpret_cast<float*>(di + bound);
@@ -389,8 +389,8 @@ void run_bp(const Tree& t, const BPDevTree& dt, float* d_cost, int16_t* d_inds, 
             const float* v0, const float* w0, float* u1, float* v1, float* w1, int hw, float quant, hipStream_t s) {
     launch_bp_rounds(d_cost, d_inds, dt.off3, dt, t.N, t.round_start.data(), t.rounds, t.tail_round, u0, v0, w0, hw,
                      quant, s);
-    const int first_even = t.maxd >= 2 ? t.depth_start[2] : t.N;
-    launch_bp_select(d_cost, d_inds, d_comp, first_even, dt, t.N, t.maxd, u0, v0, w0, u1, v1, w1, hw, quant, s);
+    const int first_anchor = t.maxd >= kSelJump ? t.depth_start[kSelJump] : t.N;
+    launch_bp_select(d_cost, d_inds, d_comp, first_anchor, dt, t.N, t.maxd, u0, v0, w0, u1, v1, w1, hw, quant, s);
 }
 
 namespace {

@@ -730,14 +730,16 @@ __global__ __launch_bounds__(1024) void k_bp_select_b(const BPBatchTree* __restr
                 T.w1, GLOBAL ? max(1, T.N) : width, len, hw, quant, GLOBAL ? T.sel_labs : labs, rv, ri);
 }
 
-// ---- two-depth label selection ------------------------------------------
-// The selection's chain is one dependent load per BFS depth.  Composing the
-// tables of a node c at an even depth d >= 2 and of its parent p,
-// comp[c][x] = allinds[c][allinds[p][x]] (the label of c given its
-// grandparent's label x: regularisation.h:273-293 applied twice), lets one
-// workgroup walk only the even depths; the odd depths then follow in one
-// parallel pass from their parents' labels.  Exact: the same table entries,
-// looked up in two steps ahead of time.
+// ---- label selection over jumps of kSelJump depths -----------------------
+// The selection's chain is one dependent load per BFS depth.  For a node c at
+// an anchor depth d (a multiple of J = kSelJump, d >= J) with ancestors
+// a_1 .. a_J, the composed table
+//   comp[c][x] = allinds[c][allinds[a_1][ ... allinds[a_{J-1}][x] ... ]]
+// is the label of c given the label x of a_J (regularisation.h:273-293
+// applied J times), so one workgroup walks only the anchor depths; every
+// other node then follows in one parallel pass from its anchor's label
+// through the tables of the nodes between.  Exact: the same table entries,
+// looked up ahead of time.
 
 // label -> displacement field entries u1 = (lj-hw)q + u0, v1 = (li-hw)q + v0,
 // w1 = (lk-hw)q + w0 (regularisation.h:281-288), 0 <= lab < len^3
@@ -752,33 +754,51 @@ __device__ __forceinline__ void emit_label(int c, int lab, int len, int hw, floa
     w1[c] = (float)(lk - hw) * quant + w0[c];
 }
 
-// One workgroup per position p >= `first` (the first position at depth 2).
+// One workgroup per position p >= `first` (the first position at depth J);
+// the tables of a_1 .. a_{J-1} are staged in LDS ((J-1) * len3 int16).
 __global__ __launch_bounds__(256) void k_compose(int16_t* __restrict__ comp, const int16_t* __restrict__ allinds,
                                                  const int* __restrict__ ordered, const int* __restrict__ parents,
-                                                 const int* __restrict__ sel_gp, int first, int len3) {
+                                                 const int* __restrict__ sel_jump, int first, int len3) {
+    extern __shared__ int16_t tabs[];
     const int p = first + blockIdx.x;
-    if (sel_gp[p] < 0) return;  // odd depth
+    if (sel_jump[p] < 0) return;  // between anchors
     const int c = ordered[p];
+    int anc[kSelJump - 1];
+    int a = c;
+#pragma unroll
+    for (int j = 0; j < kSelJump - 1; j++) anc[j] = a = parents[a];
+#pragma unroll
+    for (int j = 0; j < kSelJump - 1; j++) {
+        const int16_t* src = allinds + (long long)anc[j] * len3;
+        for (int x = threadIdx.x; x < len3; x += blockDim.x) tabs[j * len3 + x] = src[x];
+    }
+    __syncthreads();
     const int16_t* ac = allinds + (long long)c * len3;
-    const int16_t* ap = allinds + (long long)parents[c] * len3;
-    int16_t* b = comp + (long long)c * len3;
-    for (int x = threadIdx.x; x < len3; x += blockDim.x) b[x] = ac[ap[x]];
+    int16_t* dst = comp + (long long)c * len3;
+    for (int x = threadIdx.x; x < len3; x += blockDim.x) {
+        int y = x;
+#pragma unroll
+        for (int j = kSelJump - 2; j >= 0; j--) y = tabs[j * len3 + y];
+        dst[x] = ac[y];
+    }
 }
 
-// The even depths (one workgroup; the structure of select_body with the
-// composite tables and the grandparent indices).  Each even node's label also
-// goes to lab[node] for the odd pass.  labs: two LDS rows of `width` labels.
-__global__ __launch_bounds__(1024) void k_bp_select2(const float* __restrict__ cost, const int16_t* __restrict__ comp,
-                                                     const int* __restrict__ ordered,
-                                                     const int* __restrict__ depth_start,
-                                                     const int* __restrict__ sel_gp, int maxd,
-                                                     const float* __restrict__ u0, const float* __restrict__ v0,
-                                                     const float* __restrict__ w0, float* __restrict__ u1,
-                                                     float* __restrict__ v1, float* __restrict__ w1, int width,
-                                                     int len, int hw, float quant, int* __restrict__ lab) {
+// The anchor depths (one workgroup; the structure of select_body with the
+// composed tables and the J-th ancestors' indices).  Each anchor's label also
+// goes to lab[node] for the fill pass.  labs: two LDS rows of `width` labels.
+__global__ __launch_bounds__(1024) void k_bp_select_jump(const float* __restrict__ cost,
+                                                         const int16_t* __restrict__ comp,
+                                                         const int* __restrict__ ordered,
+                                                         const int* __restrict__ depth_start,
+                                                         const int* __restrict__ sel_jump, int maxd,
+                                                         const float* __restrict__ u0, const float* __restrict__ v0,
+                                                         const float* __restrict__ w0, float* __restrict__ u1,
+                                                         float* __restrict__ v1, float* __restrict__ w1, int width,
+                                                         int len, int hw, float quant, int* __restrict__ lab) {
     extern __shared__ int labs[];
     __shared__ float rv[16];
     __shared__ int ri[16];
+    constexpr int J = kSelJump;
     const int len3 = len * len * len;
     const int t = threadIdx.x;
     const int root = ordered[0];
@@ -808,33 +828,35 @@ __global__ __launch_bounds__(1024) void k_bp_select2(const float* __restrict__ c
         lab[root] = id;
         emit_label(root, id, len, hw, quant, u0, v0, w0, u1, v1, w1);
     }
-    // d = 2, 4, ...: positions [b, e) of depth d; E = depth_start[d + 3] (the next even depth's end)
-    int b = 0, e = 0, E = 0;
-    int nc = -1, ngp = 0;
-    if (maxd >= 2) {
-        b = depth_start[2];
-        e = depth_start[3];
-        E = depth_start[maxd >= 4 ? 5 : maxd + 1];
+    // anchor depth d: positions [b, e); its (node, ancestor index) pairs are loaded a depth ahead
+    int b = 0, e = 0;
+    int nc = -1, nga = 0;
+    if (maxd >= J) {
+        b = depth_start[J];
+        e = depth_start[J + 1];
         if (b + t < e) {
             nc = ordered[b + t];
-            ngp = sel_gp[b + t];
+            nga = sel_jump[b + t];
         }
     }
     int pc = -1, plab = 0;
     __syncthreads();
-    for (int d = 2; d <= maxd; d += 2) {
-        const int c = nc, gp = ngp;
+    for (int d = J; d <= maxd; d += J) {
+        const int c = nc, ga = nga;
         if (pc >= 0) emit_label(pc, plab, len, hw, quant, u0, v0, w0, u1, v1, w1);
-        // the next even depth d + 2: positions [depth_start[d + 2], E)
-        const int nb = d + 2 <= maxd ? depth_start[d + 2] : 0;
-        nc = -1;
-        if (d + 2 <= maxd && nb + t < E) {
-            nc = ordered[nb + t];
-            ngp = sel_gp[nb + t];
+        // the next anchor depth d + J: positions [nb, ne)
+        int nb = 0, ne = 0;
+        if (d + J <= maxd) {
+            nb = depth_start[d + J];
+            ne = depth_start[d + J + 1];
         }
-        const int E2 = depth_start[d + 5 <= maxd + 1 ? d + 5 : maxd + 1];
+        nc = -1;
+        if (nb + t < ne) {
+            nc = ordered[nb + t];
+            nga = sel_jump[nb + t];
+        }
         if (c >= 0) {
-            const int l = comp[(long long)c * len3 + prev[gp]];
+            const int l = comp[(long long)c * len3 + prev[ga]];
             cur[t] = l;
             lab[c] = l;
             plab = l;
@@ -842,7 +864,7 @@ __global__ __launch_bounds__(1024) void k_bp_select2(const float* __restrict__ c
         pc = c;
         for (int q = b + (int)blockDim.x + t; q < e; q += blockDim.x) {  // depths wider than the block
             const int c2 = ordered[q];
-            const int l = comp[(long long)c2 * len3 + prev[sel_gp[q]]];
+            const int l = comp[(long long)c2 * len3 + prev[sel_jump[q]]];
             cur[q - b] = l;
             lab[c2] = l;
             emit_label(c2, l, len, hw, quant, u0, v0, w0, u1, v1, w1);
@@ -852,25 +874,40 @@ __global__ __launch_bounds__(1024) void k_bp_select2(const float* __restrict__ c
         prev = cur;
         cur = tmp;
         b = nb;
-        e = d + 3 <= maxd + 1 ? depth_start[d + 3] : 0;
-        E = E2;
+        e = ne;
     }
     if (pc >= 0) emit_label(pc, plab, len, hw, quant, u0, v0, w0, u1, v1, w1);
 }
 
-// The odd depths: every node there takes its own table at its parent's label.
-__global__ __launch_bounds__(256) void k_select_odd(const int16_t* __restrict__ allinds,
-                                                    const int* __restrict__ ordered,
-                                                    const int* __restrict__ parents, const int* __restrict__ sel_gp,
-                                                    const int* __restrict__ lab, int N, int len, int hw, float quant,
-                                                    const float* __restrict__ u0, const float* __restrict__ v0,
-                                                    const float* __restrict__ w0, float* __restrict__ u1,
-                                                    float* __restrict__ v1, float* __restrict__ w1) {
+// Every node between anchors: its label from its anchor's through the
+// tables of the r = -sel_jump nodes from it up to the anchor.
+__global__ __launch_bounds__(256) void k_select_fill(const int16_t* __restrict__ allinds,
+                                                     const int* __restrict__ ordered,
+                                                     const int* __restrict__ parents,
+                                                     const int* __restrict__ sel_jump, const int* __restrict__ lab,
+                                                     int N, int len, int hw, float quant,
+                                                     const float* __restrict__ u0, const float* __restrict__ v0,
+                                                     const float* __restrict__ w0, float* __restrict__ u1,
+                                                     float* __restrict__ v1, float* __restrict__ w1) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p == 0 || p >= N || sel_gp[p] >= 0) return;  // the root and the even depths are done
+    if (p >= N) return;
+    const int r = -sel_jump[p];
+    if (r <= 0) return;  // anchors (the root included) are labelled by the walk
     const int len3 = len * len * len;
     const int c = ordered[p];
-    const int l = allinds[(long long)c * len3 + lab[parents[c]]];
+    int chain[kSelJump - 1];
+    int a = c;
+#pragma unroll
+    for (int i = 0; i < kSelJump - 1; i++) {
+        if (i < r) {
+            chain[i] = a;
+            a = parents[a];
+        }
+    }
+    int l = lab[a];
+#pragma unroll
+    for (int i = kSelJump - 2; i >= 0; i--)
+        if (i < r) l = allinds[(long long)chain[i] * len3 + l];
     emit_label(c, l, len, hw, quant, u0, v0, w0, u1, v1, w1);
 }
 
@@ -968,25 +1005,32 @@ void launch_bp_rounds(float* cost, int16_t* allinds, float* off3, const BPDevTre
     }
 }
 
-void launch_bp_select(const float* cost, const int16_t* allinds, int16_t* comp, int first_even, const BPDevTree& t,
-                      int N, int maxd, const float* u0, const float* v0, const float* w0, float* u1, float* v1,
-                      float* w1, int hw, float quant, hipStream_t s) {
+void launch_bp_select(const float* cost, const int16_t* allinds, int16_t* comp, int first_anchor,
+                      const BPDevTree& t, int N, int maxd, const float* u0, const float* v0, const float* w0,
+                      float* u1, float* v1, float* w1, int hw, float quant, hipStream_t s) {
     int len = 2 * hw + 1;
     const int width = std::max(1, t.max_width);
     const size_t smem = select_smem(width);
     const int len3 = len * len * len;
-    if (comp && smem > 0 && maxd >= 2 && t.sel_labs && first_even < N) {
-        // two-depth walk: compose the even depths' tables, walk them, then the odd depths in parallel
-        k_compose<<<N - first_even, 256, 0, s>>>(comp, allinds, t.ordered, t.parents, t.sel_gp, first_even, len3);
-        DEEDS_LAUNCH_CHECK();
+    if (comp && smem > 0 && t.sel_labs && N > 0) {
+        // walk the anchor depths on composed tables, then every other depth in one parallel pass
+        if (first_anchor < N) {
+            const size_t tsmem = (size_t)(kSelJump - 1) * len3 * sizeof(int16_t);
+            if (tsmem > 65536)
+                DEEDS_HIP(hipFuncSetAttribute((const void*)k_compose, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              (int)tsmem));
+            k_compose<<<N - first_anchor, 256, tsmem, s>>>(comp, allinds, t.ordered, t.parents, t.sel_jump,
+                                                           first_anchor, len3);
+            DEEDS_LAUNCH_CHECK();
+        }
         if (smem > 65536)
-            DEEDS_HIP(hipFuncSetAttribute((const void*)k_bp_select2, hipFuncAttributeMaxDynamicSharedMemorySize,
+            DEEDS_HIP(hipFuncSetAttribute((const void*)k_bp_select_jump, hipFuncAttributeMaxDynamicSharedMemorySize,
                                           (int)smem));
-        k_bp_select2<<<1, 1024, smem, s>>>(cost, comp, t.ordered, t.depth_start, t.sel_gp, maxd, u0, v0, w0, u1, v1,
-                                           w1, width, len, hw, quant, t.sel_labs);
+        k_bp_select_jump<<<1, 1024, smem, s>>>(cost, comp, t.ordered, t.depth_start, t.sel_jump, maxd, u0, v0, w0,
+                                               u1, v1, w1, width, len, hw, quant, t.sel_labs);
         DEEDS_LAUNCH_CHECK();
-        k_select_odd<<<ceil_div(N, 256), 256, 0, s>>>(allinds, t.ordered, t.parents, t.sel_gp, t.sel_labs, N, len, hw,
-                                                      quant, u0, v0, w0, u1, v1, w1);
+        k_select_fill<<<ceil_div(N, 256), 256, 0, s>>>(allinds, t.ordered, t.parents, t.sel_jump, t.sel_labs, N,
+                                                       len, hw, quant, u0, v0, w0, u1, v1, w1);
         DEEDS_LAUNCH_CHECK();
         return;
     }

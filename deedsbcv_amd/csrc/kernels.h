@@ -51,7 +51,7 @@ struct BPDevTree {
     const int* leaf_w;       // their edgemst bits
     int n_leaves;
     const int* sel_info;     // 2*N: per position of ordered, (node, parent's index within its depth)
-    const int* sel_gp;       // N: per position at an even depth >= 2, the grandparent's index within its depth
+    const int* sel_jump;     // N: per position, its kSelJump-th ancestor's index within its depth (anchors) or -(depth mod kSelJump)
     int max_width;           // largest number of nodes at one depth
     float* off3;             // 3*N floats of scratch (parent offsets)
     int* sel_labs;           // 2*N ints of scratch: the selection's label rows when a depth is too wide for LDS
@@ -67,10 +67,11 @@ void launch_bp_rounds(float* cost, int16_t* allinds, float* off3, const BPDevTre
                       const int* round_start, int rounds, int tail_round, const float* u0, const float* v0,
                       const float* w0, int hw, float quant, hipStream_t s);
 // Top-down selection.  With `comp` (N*len^3 int16 of scratch) and a tree
-// of depth >= 2 whose depths fit LDS it walks only the even depths on
-// composed tables (first_even: the first position at depth 2), else every
-// depth.
-void launch_bp_select(const float* cost, const int16_t* allinds, int16_t* comp, int first_even, const BPDevTree& t,
+// whose depths fit LDS it walks only the anchor depths (multiples of
+// kSelJump) on composed tables (first_anchor: the first position at depth
+// kSelJump, N when there is none) and fills the depths between in one
+// parallel pass, else it walks every depth.
+void launch_bp_select(const float* cost, const int16_t* allinds, int16_t* comp, int first_anchor, const BPDevTree& t,
                       int N, int maxd, const float* u0, const float* v0, const float* w0, float* u1, float* v1,
                       float* w1, int hw, float quant, hipStream_t s);
 // regularisation.h:273-274: the root's argmin row is the identity (allinds[root][l] = l).

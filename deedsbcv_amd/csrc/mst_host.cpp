@@ -434,11 +434,18 @@ void build_schedule(Tree& t, bool bp_programs, int window) {
     }
     // top-down selection: per position, the node and its parent's index within the parent's depth
     t.sel_info.assign(2 * (size_t)N, 0);
-    t.sel_gp.assign(N, -1);
+    t.sel_jump.assign(N, 0);
     for (int p = 0; p < N; p++) {
         t.sel_info[2 * p] = t.ordered[p];
         if (p > 0) t.sel_info[2 * p + 1] = ppos[p] - t.depth_start[dpos[p] - 1];
-        if (dpos[p] >= 2 && dpos[p] % 2 == 0) t.sel_gp[p] = ppos[ppos[p]] - t.depth_start[dpos[p] - 2];
+        const int d = dpos[p];
+        if (d % kSelJump != 0) {
+            t.sel_jump[p] = -(d % kSelJump);
+        } else if (d > 0) {
+            int q = p;
+            for (int j = 0; j < kSelJump; j++) q = ppos[q];
+            t.sel_jump[p] = q - t.depth_start[d - kSelJump];
+        }
     }
     // children of every position, as positions in `ordered` order
     std::vector<int> cs(N + 1, 0), cl(N > 0 ? N - 1 : 0);
@@ -607,7 +614,7 @@ void build_schedule(Tree& t, bool bp_programs, int window) {
 
 // ordered, parents, depth_start, prog_start: 4N + 2; prog: kProgRec per segment
 // and per internal node plus N - 1 children, and the staging pad; leaves,
-// leaf_w, sel_info, sel_gp: 5N.
+// leaf_w, sel_info, sel_jump: 5N.
 size_t tree_device_ints(int N) { return (2 * kProgRec + 10) * (size_t)N + kProgStage + 16; }
 
 }  // namespace deeds

@@ -20,9 +20,11 @@ struct Tree {
     // top-down selection: (node, index of its parent within the parent's
     // depth) per position of `ordered`, and the largest depth width
     std::vector<int> sel_info;
-    // two-depth selection: per position at an even depth >= 2 the index of
-    // its grandparent within the grandparent's depth, else -1
-    std::vector<int> sel_gp;
+    // selection over jumps of kSelJump depths: per position at a depth d
+    // that is a multiple of kSelJump (an anchor), the index of its
+    // kSelJump-th ancestor within that ancestor's depth (0 at the root);
+    // elsewhere -(d mod kSelJump), the steps up to its anchor
+    std::vector<int> sel_jump;
     int max_width = 0;
     // children of every node in `ordered` order (CSR by node id); built
     // only without BP programs (the schedule of the programs uses positions)
@@ -68,6 +70,10 @@ struct Tree {
 // Steps per bottom-up round (one launch each, below the spine).
 constexpr int kBPWindow = 16;
 
+// Depths per step of the label selection's walk (the tables of the depths
+// between two anchors are composed beforehand).
+constexpr int kSelJump = 8;
+
 // Ints per program record, and the ints of a program block the device BP
 // stages in LDS (the header and 16 records take 136).
 constexpr int kProgRec = 8;
@@ -76,7 +82,7 @@ static_assert(kProgRec * (1 + kBPWindow) <= kProgStage, "a block's header and re
 
 
 // Upper bound on the ints a device copy of a tree's BP schedule needs
-// (ordered, parents, depth_start, prog_start, prog, leaves, leaf_w, sel_info, sel_gp).
+// (ordered, parents, depth_start, prog_start, prog, leaves, leaf_w, sel_info, sel_jump).
 size_t tree_device_ints(int N);
 
 // primsMST.h:101-110: float mean and std of the image, bit-identical to the

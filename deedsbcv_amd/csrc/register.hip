@@ -341,15 +341,24 @@ class PairRun {
         };
         sd_fut_[0] = std::async(std::launch::async, stats_task, 0).share();
         sd_fut_[1] = std::async(std::launch::async, stats_task, 1).share();
-        // finest level first: its trees take longest to build, so they start first
-        for (int l = L_ - 1; l >= 0; l--) {
+        // Edge SADs: the finest level first (its trees take longest to build),
+        // then the first level (its trees are needed first), both on s0; the
+        // levels between on s1, which has nothing else to do until the first
+        // level's descriptors are done, so s0 goes on to them right away.
+        auto sads = [&](int l, hipStream_t s) {
             const size_t cnt = 6 * (size_t)lv_[l].sz1;
             for (int dir = 0; dir < 2; dir++) {
                 float* d = d_ec_ + dir * ec_total_ + ec_off_[l];
-                launch_edge_sad(d, dir ? warped0_ : im1b_, lv_[l].step, in_.m, in_.n, in_.o, s0_);
-                d2h(h_ec_ + dir * ec_total_ + ec_off_[l], d, cnt, s0_);
+                launch_edge_sad(d, dir ? warped0_ : im1b_, lv_[l].step, in_.m, in_.n, in_.o, s);
+                d2h(h_ec_ + dir * ec_total_ + ec_off_[l], d, cnt, s);
             }
-            DEEDS_HIP(hipEventRecord(ev_sad_[l]->e, s0_));
+            DEEDS_HIP(hipEventRecord(ev_sad_[l]->e, s));
+        };
+        sads(L_ - 1, s0_);
+        if (L_ > 1) sads(0, s0_);
+        if (L_ > 2) {
+            DEEDS_HIP(hipStreamWaitEvent(s1_, w0_event, 0));
+            for (int l = 1; l < L_ - 1; l++) sads(l, s1_);
         }
         // fields start at zero on the level-0 grid
         m2_ = in_.m / S_.grid[0];
@@ -363,9 +372,14 @@ class PairRun {
 
     // Queues the two trees of level l on the pool.  Prim's on the edge costs
     // runs before the image statistics are known; the exact weights then
-    // confirm the order (else the tree is rebuilt from the weights).  The
-    // confirmation runs beside Prim's once the statistics arrive.
+    // confirm the order (else the tree is rebuilt from the weights).  For the
+    // first level (needed first) and the finest (the longest build, which
+    // the last BP waits for) the confirmation runs on a thread of its own
+    // beside Prim's; the other levels confirm after Prim's on the building
+    // thread, so fewer threads compete for the cores while the finest
+    // level's Prim's runs (measured alone on the host: its tree 3-4 ms sooner).
     void submit_trees(TaskPool& pool, int l) {
+        const bool check_beside = l == 0 || l == L_ - 1;
         for (int dir = 0; dir < 2; dir++) {
             const float* ec = h_ec_ + dir * ec_total_ + ec_off_[l];
             const LevelCfg c = lv_[l];
@@ -382,20 +396,22 @@ class PairRun {
             char* slot = h_stage_ + stage_off_[l] + dir * ((stage_off_[l + 1] - stage_off_[l]) / 2);
             void* d_stage = d_stage_[dir];
             trees_[dir][l] = pool.submit([ec, c, sdf, spent, began, shaped, t_begin, sad_event, slot, d_stage,
-                                          host_wait, sads]() {
+                                          host_wait, sads, check_beside]() {
                 if (host_wait)
                     sads.wait();
                 else
                     DEEDS_HIP(hipEventSynchronize(sad_event));
                 auto tb = std::chrono::steady_clock::now();
                 *began = std::chrono::duration<double>(tb - t_begin).count();
-                auto check = std::async(std::launch::async, [ec, c, sdf]() {
+                auto confirm = [ec, c, sdf]() {
                     return weights_distinguish_costs(ec, c.step, c.m1, c.n1, c.o1, sdf.get());
-                });
+                };
+                std::future<bool> check;
+                if (check_beside) check = std::async(std::launch::async, confirm);
                 std::unique_ptr<Tree> t(new Tree());
                 build_tree_shape(ec, c.step, c.m1, c.n1, c.o1, *t);
                 *shaped = std::chrono::duration<double>(std::chrono::steady_clock::now() - tb).count();
-                const bool same_order = check.get();
+                const bool same_order = check_beside ? check.get() : confirm();
                 const float sd = sdf.get();
                 if (same_order)
                     finish_tree(ec, sd, c.step, *t);
