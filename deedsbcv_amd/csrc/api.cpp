@@ -686,8 +686,8 @@ const float kIdentityX[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
 void warp_ssd(deeds_ctx* ctx, const float* d_im1b, const float* d_warped, const float* d_before, size_t sz,
               float* ssd01) {
     double* acc = ctx->get<double>("warp.ssd", 2);
-    DEEDS_HIP(hipMemsetAsync(acc, 0, 2 * sizeof(double), ctx->stream));
-    launch_ssd(acc, d_im1b, d_warped, d_before, (long long)sz, ctx->stream);
+    launch_ssd(acc, ctx->get<double>("red.part", kRedPartDoubles), d_im1b, d_warped, d_before, (long long)sz,
+               ctx->stream);
     double h[2];
     d2h(h, acc, 2, ctx->stream);
     DEEDS_HIP(hipStreamSynchronize(ctx->stream));
@@ -730,7 +730,8 @@ int deeds_ctx_jacobian(deeds_ctx* ctx, const float* d_u, const float* d_v, const
         DEEDS_REQUIRE(factor != 0, "factor must be nonzero");
         const size_t sz = (size_t)m * n * o;
         double* acc = ctx->get<double>("jac.acc", 4);
-        launch_jacobian(acc, d_u, d_v, d_w, m, n, o, factor, ctx->stream);
+        launch_jacobian(acc, ctx->get<double>("red.part", kRedPartDoubles), d_u, d_v, d_w, m, n, o, factor,
+                        ctx->stream);
         double h[3];
         d2h(h, acc, 3, ctx->stream);
         DEEDS_HIP(hipStreamSynchronize(ctx->stream));

@@ -128,11 +128,17 @@ void launch_warp_affine_s(int16_t* out, const int16_t* in, const Affine& A, cons
                           const float* w, int m, int n, int o, hipStream_t s);
 // In place on the six fields; tmp holds 6*m*n*o floats.
 void launch_consistent(float* f[6], float* tmp, int m, int n, int o, int factor, hipStream_t s);
-// acc[0] += sum (ref-before)^2, acc[1] += sum (ref-warped)^2 in double (logging only).
-void launch_ssd(double* acc, const float* ref, const float* warped, const float* before, long long sz,
+// Partial sums of the logging reductions below: per workgroup, then added
+// in a fixed order, so the figures do not vary from run to run.
+constexpr int kRedBlocks = 1024;
+constexpr int kRedPartDoubles = 3 * kRedBlocks;
+// acc[0] = sum (ref-before)^2, acc[1] = sum (ref-warped)^2 in double (logging
+// only); part: kRedPartDoubles of scratch.
+void launch_ssd(double* acc, double* part, const float* ref, const float* warped, const float* before, long long sz,
                 hipStream_t s);
-// out[0] = std(det J), out[1] = fraction det J < 0 (logging only); scratch >= 3 doubles.
-void launch_jacobian(double* scratch, const float* u, const float* v, const float* w, int m, int n, int o,
+// acc[0] = sum det J, acc[1] = sum det J^2, acc[2] = #(det J < 0) in double
+// (logging only); part: kRedPartDoubles of scratch.
+void launch_jacobian(double* acc, double* part, const float* u, const float* v, const float* w, int m, int n, int o,
                      int factor, hipStream_t s);
 
 // ---- affine pre-alignment (linear.hip) ------------------------------------

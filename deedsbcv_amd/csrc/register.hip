@@ -241,6 +241,7 @@ class PairRun {
         cmtmp_ = get<float>("cm", 6 * fmax);
         jac_ = get<double>("jac", 4 * (size_t)L_);
         ssd_ = get<double>("ssd", 2);
+        red_part_ = get<double>("redpart", kRedPartDoubles);
         // MST inputs: edge SADs of every level on both images
         ec_off_.resize(L_);
         for (int l = 0; l < L_; l++) {
@@ -545,7 +546,7 @@ class PairRun {
         const size_t n1 = c.sz1;
         float* f[6] = {newf_, newf_ + n1, newf_ + 2 * n1, newf_ + 3 * n1, newf_ + 4 * n1, newf_ + 5 * n1};
         launch_consistent(f, cmtmp_, c.m1, c.n1, c.o1, c.step, s0_);
-        launch_jacobian(jac_ + 4 * l, newf_, newf_ + n1, newf_ + 2 * n1, c.m1, c.n1, c.o1, c.step, s0_);
+        launch_jacobian(jac_ + 4 * l, red_part_, newf_, newf_ + n1, newf_ + 2 * n1, c.m1, c.n1, c.o1, c.step, s0_);
         DEEDS_HIP(hipMemcpyAsync(prevf_, newf_, 6 * n1 * sizeof(float), hipMemcpyDeviceToDevice, s0_));
         if (!graph_) {
             DEEDS_HIP(hipEventRecord(EV(l, 5), s0_));
@@ -571,8 +572,7 @@ class PairRun {
             float* wtmp = sh_.warped[0];
             launch_warp_affine_grid(wtmp, im1_, A_, prevf_, prevf_ + sz2_, prevf_ + 2 * sz2_, in_.m, in_.n, in_.o,
                                     m2_, n2_, o2_, s0_);
-            launch_zero((float*)ssd_, 4, s0_);
-            launch_ssd(ssd_, im1b_, wtmp, im1_, sz_, s0_);
+            launch_ssd(ssd_, red_part_, im1b_, wtmp, im1_, sz_, s0_);
             launch_offset(in_.warped, wtmp, 1024.0f, sz_, s0_);
         }
     }
@@ -653,7 +653,7 @@ class PairRun {
     int16_t* comp_[2];
     float *prevf_, *newf_, *cmtmp_;
     float* g0_[2];
-    double *jac_, *ssd_;
+    double *jac_, *ssd_, *red_part_;
     size_t ec_total_ = 0;
     std::vector<size_t> ec_off_;
     float* d_ec_;

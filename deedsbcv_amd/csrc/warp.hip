@@ -213,7 +213,7 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
     return r;
 }
 
-__global__ void k_ssd(double* acc, const float* __restrict__ ref, const float* __restrict__ warped,
+__global__ void k_ssd(double* __restrict__ part, const float* __restrict__ ref, const float* __restrict__ warped,
                       const float* __restrict__ before, long long sz) {
     __shared__ double red[4];
     double s0 = 0.0, s1 = 0.0;
@@ -226,8 +226,27 @@ __global__ void k_ssd(double* acc, const float* __restrict__ ref, const float* _
     s0 = block_sum(s0, red);
     s1 = block_sum(s1, red);
     if (threadIdx.x == 0) {
-        atomicAdd(acc, s0);
-        atomicAdd(acc + 1, s1);
+        part[2 * blockIdx.x] = s0;
+        part[2 * blockIdx.x + 1] = s1;
+    }
+}
+
+// acc[c] = sum over the `blocks` partials part[b * stride + c], c < stride:
+// each thread adds its blocks in order, then a fixed tree over the threads.
+__global__ __launch_bounds__(256) void k_sum_parts(double* __restrict__ acc, const double* __restrict__ part,
+                                                   int blocks, int stride) {
+    __shared__ double red[256];
+    for (int c = 0; c < stride; c++) {
+        double v = 0.0;
+        for (int b = threadIdx.x; b < blocks; b += blockDim.x) v += part[(long long)b * stride + c];
+        red[threadIdx.x] = v;
+        __syncthreads();
+        for (int h = 128; h > 0; h >>= 1) {
+            if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) acc[c] = red[0];
+        __syncthreads();
     }
 }
 
@@ -257,7 +276,7 @@ __device__ __forceinline__ float cdiff(const float* f, int i, int j, int k, int 
     return acc;
 }
 
-__global__ void k_jacobian(double* acc, const float* __restrict__ u, const float* __restrict__ v,
+__global__ void k_jacobian(double* __restrict__ part, const float* __restrict__ u, const float* __restrict__ v,
                            const float* __restrict__ w, int m, int n, int o, float f1) {
     __shared__ double red[4];
     long long sz = (long long)m * n * o;
@@ -283,9 +302,9 @@ __global__ void k_jacobian(double* acc, const float* __restrict__ u, const float
     s2 = block_sum(s2, red);
     neg = block_sum(neg, red);
     if (threadIdx.x == 0) {
-        atomicAdd(acc, s);
-        atomicAdd(acc + 1, s2);
-        atomicAdd(acc + 2, neg);
+        part[3 * blockIdx.x] = s;
+        part[3 * blockIdx.x + 1] = s2;
+        part[3 * blockIdx.x + 2] = neg;
     }
 }
 
@@ -382,22 +401,33 @@ void launch_consistent(float* f[6], float* tmp, int m, int n, int o, int factor,
     DEEDS_LAUNCH_CHECK();
 }
 
-void launch_ssd(double* acc, const float* ref, const float* warped, const float* before, long long sz, hipStream_t s) {
-    unsigned g = (unsigned)std::min<long long>(ceil_div(sz, TB), 1024);
-    if (g == 0) return;
-    k_ssd<<<g, TB, 0, s>>>(acc, ref, warped, before, sz);
+void launch_ssd(double* acc, double* part, const float* ref, const float* warped, const float* before, long long sz,
+                hipStream_t s) {
+    const unsigned g = (unsigned)std::min<long long>(ceil_div(sz, TB), kRedBlocks);
+    if (g == 0) {
+        k_zero_d<<<1, 64, 0, s>>>(acc, 2);
+        DEEDS_LAUNCH_CHECK();
+        return;
+    }
+    k_ssd<<<g, TB, 0, s>>>(part, ref, warped, before, sz);
+    DEEDS_LAUNCH_CHECK();
+    k_sum_parts<<<1, 256, 0, s>>>(acc, part, (int)g, 2);
     DEEDS_LAUNCH_CHECK();
 }
 
-void launch_jacobian(double* scratch, const float* u, const float* v, const float* w, int m, int n, int o, int factor,
-                     hipStream_t s) {
+void launch_jacobian(double* acc, double* part, const float* u, const float* v, const float* w, int m, int n, int o,
+                     int factor, hipStream_t s) {
     long long sz = (long long)m * n * o;
-    k_zero_d<<<1, 64, 0, s>>>(scratch, 3);
-    DEEDS_LAUNCH_CHECK();
-    if (sz == 0) return;
+    if (sz == 0) {
+        k_zero_d<<<1, 64, 0, s>>>(acc, 3);
+        DEEDS_LAUNCH_CHECK();
+        return;
+    }
     float f1 = (float)(1.0 / (float)factor);
-    unsigned g = (unsigned)std::min<long long>(ceil_div(sz, TB), 1024);
-    k_jacobian<<<g, TB, 0, s>>>(scratch, u, v, w, m, n, o, f1);
+    const unsigned g = (unsigned)std::min<long long>(ceil_div(sz, TB), kRedBlocks);
+    k_jacobian<<<g, TB, 0, s>>>(part, u, v, w, m, n, o, f1);
+    DEEDS_LAUNCH_CHECK();
+    k_sum_parts<<<1, 256, 0, s>>>(acc, part, (int)g, 3);
     DEEDS_LAUNCH_CHECK();
 }
 

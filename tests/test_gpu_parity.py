@@ -237,6 +237,18 @@ def test_jacobian_logging(oracle):
     s0, neg0 = oracle.jacobian(g[0], g[1], g[2], 4)
     assert abs(s - float(s0)) <= 1e-4 * max(1.0, float(s0))
     assert abs(neg - float(neg0)) < 1e-6
+    # the partial sums are added in a fixed order: the figures repeat exactly
+    assert api.jacobian(g[0], g[1], g[2], 4) == (s, neg)
+
+
+def test_logging_figures_repeat_exactly():
+    print("case: logging figures of two identical registrations", flush=True)
+    f, mv = phantom.make_pair((48, 44, 40), 77)
+    sched = dict(grid=(6, 5, 4), radius=(4, 3, 2), quant=(3, 2, 1))
+    _, _, t1 = api.register(f, mv, sched["grid"], sched["radius"], sched["quant"], 1.6)
+    _, _, t2 = api.register(f, mv, sched["grid"], sched["radius"], sched["quant"], 1.6)
+    for k in ("ssd_before", "ssd_after", "jac_std", "jac_neg"):
+        assert t1[k] == t2[k], (k, t1[k], t2[k])
 
 
 def test_apply_matches_reference_applyBCV(golden):
