@@ -284,9 +284,12 @@ def atlas_bench(a, steps, cpu=True):
     sched = ph.SCHEDULE_FULL
     dims = ph.atlas_dims(100, 16)
     log("generating 16 atlas pairs")
-    # workers forked from a fresh server process, not from this one (its GPU runtime has threads)
-    with cf.ProcessPoolExecutor(max_workers=host_threads(), mp_context=mp.get_context("forkserver")) as ex:
-        vols = list(ex.map(ph.workload_crossmodal, [100 + i for i in range(16)], dims))
+    if a.serial_data:
+        vols = [ph.workload_crossmodal(100 + i, d) for i, d in enumerate(dims)]
+    else:
+        # workers forked from a fresh server process, not from this one (its GPU runtime has threads)
+        with cf.ProcessPoolExecutor(max_workers=host_threads(), mp_context=mp.get_context("forkserver")) as ex:
+            vols = list(ex.map(ph.workload_crossmodal, [100 + i for i in range(16)], dims))
     g = sched["grid"][-1]
     pairs = []
     for (f, m), (mm, n, o) in zip(vols, dims):
@@ -657,6 +660,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
     ap.add_argument("--cpu-budget", type=float, default=240.0)
+    ap.add_argument("--serial-data", action="store_true",
+                    help="generate the atlas pairs in this process (under a profiler, whose signal handlers "
+                         "the worker processes cannot inherit)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed steps, copy a 1 GiB device buffer once (a known-size kernel that "
                          "calibrates HBM byte counters in a profiler run)")

@@ -333,10 +333,14 @@ void prim_tree(const Key* w, int m1, int n1, int o1, Tree& t) {
     if (N == 0) return;
     // Prim's from the grid centre (primsMST.h:123-164)
     const int root = m1 / 2 + (n1 / 2) * m1 + (o1 / 2) * m1 * n1;
-    std::vector<uint8_t> in_tree(N, 0);
+    // membership as a bit set: the check after every pop then reads a 16 KB
+    // array (at 131072 nodes) that stays in L1
+    std::vector<uint64_t> in_tree(((size_t)N + 63) / 64, 0);
+    auto member = [&](int v) { return (bool)((in_tree[(size_t)v >> 6] >> (v & 63)) & 1u); };
+    auto add = [&](int v) { in_tree[(size_t)v >> 6] |= (uint64_t)1 << (v & 63); };
     std::vector<int> depth(N, 0);
     EdgeHeap<Key> heap((size_t)4 * N + 16);
-    in_tree[root] = 1;
+    add(root);
     int last = root;
     // linear offset of each neighbour slot (valid wherever the neighbour exists)
     const int delta[6] = {-m1, m1, -1, 1, -m1 * n1, m1 * n1};
@@ -354,12 +358,12 @@ void prim_tree(const Key* w, int m1, int n1, int o1, Tree& t) {
             if (heap.empty()) throw std::runtime_error("prims: heap exhausted");
             const uint32_t code = heap.pop();
             const int ea = (int)(code >> 3), eb = ea + delta[code & 7u];
-            bool a = in_tree[ea], b = in_tree[eb];
+            const bool a = member(ea), b = member(eb);
             if (a != b) {
                 if (a) { from = ea; to = eb; } else { from = eb; to = ea; }
             }
         }
-        in_tree[to] = 1;
+        add(to);
         depth[to] = depth[from] + 1;
         t.parents[to] = from;
         last = to;

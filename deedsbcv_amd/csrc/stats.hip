@@ -114,7 +114,20 @@ __global__ __launch_bounds__(kThreads) void k_stats_plan(const float* __restrict
     }
 }
 
+__global__ __launch_bounds__(256) void k_gather_chunks(float* __restrict__ dst, const float* __restrict__ x,
+                                                       long long n, const int* __restrict__ list) {
+    const long long k0 = (long long)list[blockIdx.x] * kStatsChunk;
+    float* d = dst + (long long)blockIdx.x * kStatsChunk;
+    for (int t = threadIdx.x; t < kStatsChunk && k0 + t < n; t += blockDim.x) d[t] = x[k0 + t];
+}
+
 }  // namespace
+
+void launch_gather_chunks(float* dst, const float* x, long long n, const int* list, int count, hipStream_t s) {
+    if (count <= 0) return;
+    k_gather_chunks<<<count, 256, 0, s>>>(dst, x, n, list);
+    DEEDS_LAUNCH_CHECK();
+}
 
 void launch_stats_sums(int mode, const float* x, long long n, float mu, double* sums, hipStream_t s) {
     const unsigned chunks = (unsigned)((n + kStatsChunk - 1) / kStatsChunk);

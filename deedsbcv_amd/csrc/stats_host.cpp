@@ -207,28 +207,71 @@ float accumulate_device(const Step& st, int mode, const float* d_im, long long c
     launch_stats_sums(mode, d_im, count, mu, d_sums, s);
     DEEDS_HIP(hipMemcpyAsync(h_sums, d_sums, nch * sizeof(double), hipMemcpyDeviceToHost, s));
     DEEDS_HIP(hipStreamSynchronize(s));
+    std::vector<double> at(nch + 1);  // the double prefix of the addends at each chunk boundary
     double run = 0;
     for (size_t c = 0; c < nch; c++) {
+        at[c] = run;
         const float g = (float)run;
         h_E[c] = std::isnormal(g) ? ulp_exp(g) : kStatsNoPlan;
         run += h_sums[c];
     }
+    at[nch] = run;
     DEEDS_HIP(hipMemcpyAsync(d_E, h_E, nch * sizeof(int), hipMemcpyHostToDevice, s));
     launch_stats_plan(mode, d_im, count, mu, d_E, d_out, d_halves, s);
     DEEDS_HIP(hipMemcpyAsync(h_out, d_out, nch * sizeof(StatsChunk), hipMemcpyDeviceToHost, s));
     DEEDS_HIP(hipMemcpyAsync(h_halves, d_halves, nch * kStatsMaxHalves * sizeof(long long), hipMemcpyDeviceToHost,
                              s));
     DEEDS_HIP(hipStreamSynchronize(s));
+    // Chunks likely to run as the sequential loop: no plan, or a running
+    // value near a power of two (within 1/128 of the double prefix, which the
+    // float sum tracks far closer than that): the binade the plan assumed may
+    // not hold.  They are gathered on the device and copied in one go.
+    int* h_list = reinterpret_cast<int*>(hb + StatsScratch::off_list(count));
+    int* d_list = reinterpret_cast<int*>(db + StatsScratch::off_list(count));
+    float* h_stage = reinterpret_cast<float*>(hb + StatsScratch::off_stage(count));
+    float* d_stage = reinterpret_cast<float*>(db + StatsScratch::off_stage(count));
+    std::vector<int> slot(nch, -1);
+    int ns = 0;
+    for (size_t c = 0; c < nch && ns < kStatsPrefetch; c++) {
+        bool suspect = !h_out[c].planned;
+        if (!suspect) {
+            const double a = at[c], b = at[c + 1];
+            if (!(a > 0 && b > 0) && !(a < 0 && b < 0)) {
+                suspect = true;  // a sign change or zero
+            } else {
+                const double lo = std::min(std::fabs(a), std::fabs(b)) * (1.0 - 1.0 / 128);
+                const double hi = std::max(std::fabs(a), std::fabs(b)) * (1.0 + 1.0 / 128);
+                int el, eh;
+                std::frexp(lo, &el);
+                std::frexp(hi, &eh);
+                suspect = el != eh;
+            }
+        }
+        if (suspect) {
+            slot[c] = ns;
+            h_list[ns++] = (int)c;
+        }
+    }
+    if (ns > 0) {
+        DEEDS_HIP(hipMemcpyAsync(d_list, h_list, ns * sizeof(int), hipMemcpyHostToDevice, s));
+        launch_gather_chunks(d_stage, d_im, count, d_list, ns, s);
+        DEEDS_HIP(hipMemcpyAsync(h_stage, d_stage, (size_t)ns * kStatsChunk * sizeof(float), hipMemcpyDeviceToHost, s));
+        DEEDS_HIP(hipStreamSynchronize(s));
+    }
     float* h_chunk = reinterpret_cast<float*>(hb + StatsScratch::off_chunk(count));
     Step local = st;
-    local.x = h_chunk;
     float v = 0.0f;
     for (size_t c = 0; c < nch; c++) {
         const StatsChunk& p = h_out[c];
         if (p.planned && apply_plan(h_E[c], p.sum, p.lo, p.hi, h_halves + c * kStatsMaxHalves, p.nh, v)) continue;
         const long long k0 = (long long)c * kStatsChunk, k1 = std::min(count, k0 + kStatsChunk);
-        DEEDS_HIP(hipMemcpyAsync(h_chunk, d_im + k0, (k1 - k0) * sizeof(float), hipMemcpyDeviceToHost, s));
-        DEEDS_HIP(hipStreamSynchronize(s));
+        if (slot[c] >= 0) {
+            local.x = h_stage + (size_t)slot[c] * kStatsChunk;
+        } else {
+            DEEDS_HIP(hipMemcpyAsync(h_chunk, d_im + k0, (k1 - k0) * sizeof(float), hipMemcpyDeviceToHost, s));
+            DEEDS_HIP(hipStreamSynchronize(s));
+            local.x = h_chunk;
+        }
         for (long long t = 0; t < k1 - k0; t++) v = local.step(v, t);
     }
     return v;
