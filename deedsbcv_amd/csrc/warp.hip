@@ -22,10 +22,11 @@ __device__ __forceinline__ float trilinear(const float* __restrict__ in, float x
     int x = (int)floorf(x1), y = (int)floorf(y1), z = (int)floorf(z1);
     float dx = x1 - (float)x, dy = y1 - (float)y, dz = z1 - (float)z;
     x += addj; y += addi; z += addk;
-    const long long s1 = m2, s2 = (long long)m2 * n2;
-    long long y0 = clampi(y, 0, m2 - 1), y1c = clampi(y + 1, 0, m2 - 1);
-    long long x0 = clampi(x, 0, n2 - 1) * s1, x1c = clampi(x + 1, 0, n2 - 1) * s1;
-    long long z0 = clampi(z, 0, o2 - 1) * s2, z1c = clampi(z + 1, 0, o2 - 1) * s2;
+    // 32-bit offsets: every volume holds fewer than 2^31 voxels (checked by the callers)
+    const int s1 = m2, s2 = m2 * n2;
+    const int y0 = clampi(y, 0, m2 - 1), y1c = clampi(y + 1, 0, m2 - 1);
+    const int x0 = clampi(x, 0, n2 - 1) * s1, x1c = clampi(x + 1, 0, n2 - 1) * s1;
+    const int z0 = clampi(z, 0, o2 - 1) * s2, z1c = clampi(z + 1, 0, o2 - 1) * s2;
     float i000 = in[y0 + x0 + z0], i100 = in[y1c + x0 + z0], i010 = in[y0 + x1c + z0], i001 = in[y0 + x0 + z1c];
     float i110 = in[y1c + x1c + z0], i101 = in[y1c + x0 + z1c], i011 = in[y0 + x1c + z1c];
     float i111 = in[y1c + x1c + z1c];
@@ -41,10 +42,14 @@ __device__ __forceinline__ float trilinear(const float* __restrict__ in, float x
     return (float)s;
 }
 
+// (i, j, k) of voxel p in 32-bit unsigned arithmetic (fewer than 2^31 voxels)
 __device__ __forceinline__ void voxel_ijk(long long p, int m, int n, int& i, int& j, int& k) {
-    i = (int)(p % m);
-    j = (int)((p / m) % n);
-    k = (int)(p / ((long long)m * n));
+    const unsigned q = (unsigned)p, um = (unsigned)m, un = (unsigned)n;
+    const unsigned r = q / um;
+    i = (int)(q - r * um);
+    const unsigned kk = r / un;
+    j = (int)(r - kk * un);
+    k = (int)kk;
 }
 
 // float affine coordinate, summed left to right (dataCostD.h:285-287)
