@@ -16,6 +16,19 @@ namespace {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// A 1-D launch over voxels in blocks of 256, in 8 * per workgroups: the GPU's
+// eight XCDs take workgroups round robin, so XCD x gets the contiguous voxel
+// blocks [x * per, (x + 1) * per) -- a slab of rows whose neighbouring rows,
+// which the trilinear gathers also read, sit in the same L2.
+struct Span {
+    int per, blocks;
+    __device__ __forceinline__ long long voxel() const {
+        const int b = ((int)blockIdx.x & 7) * per + ((int)blockIdx.x >> 3);
+        return b < blocks ? (long long)b * blockDim.x + threadIdx.x : -1;
+    }
+    unsigned grid() const { return 8u * (unsigned)per; }
+};
+
 // transformations.h:5-26 / dataCostD.h:280-295 for one sample point.
 __device__ __forceinline__ float trilinear(const float* __restrict__ in, float x1, float y1, float z1, int addj,
                                            int addi, int addk, int m2, int n2, int o2) {
@@ -72,10 +85,10 @@ __device__ __forceinline__ void upsampled(const float* u0, const float* v0, cons
 
 __global__ void k_upsample(float* __restrict__ u1, float* __restrict__ v1, float* __restrict__ w1,
                            const float* __restrict__ u0, const float* __restrict__ v0, const float* __restrict__ w0,
-                           int m, int n, int o, int m2, int n2, int o2, UpScale sc) {
+                           int m, int n, int o, int m2, int n2, int o2, UpScale sc, Span sp) {
     long long sz = (long long)m * n * o;
-    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= sz) return;
+    const long long p = sp.voxel();
+    if (p < 0 || p >= sz) return;
     int i, j, k;
     voxel_ijk(p, m, n, i, j, k);
     float u, v, w;
@@ -87,10 +100,10 @@ __global__ void k_upsample(float* __restrict__ u1, float* __restrict__ v1, float
 
 __global__ void k_interp3(float* __restrict__ out, const float* __restrict__ in, const float* __restrict__ x1,
                           const float* __restrict__ y1, const float* __restrict__ z1, int m, int n, int o, int m2,
-                          int n2, int o2, int add_identity) {
+                          int n2, int o2, int add_identity, Span sp) {
     long long sz = (long long)m * n * o;
-    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= sz) return;
+    const long long p = sp.voxel();
+    if (p < 0 || p >= sz) return;
     int i, j, k;
     voxel_ijk(p, m, n, i, j, k);
     out[p] = add_identity ? trilinear(in, x1[p], y1[p], z1[p], j, i, k, m2, n2, o2)
@@ -99,10 +112,10 @@ __global__ void k_interp3(float* __restrict__ out, const float* __restrict__ in,
 
 __global__ void k_warp_affine(float* __restrict__ out, const float* __restrict__ in, Affine A,
                               const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
-                              int m, int n, int o) {
+                              int m, int n, int o, Span sp) {
     long long sz = (long long)m * n * o;
-    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= sz) return;
+    const long long p = sp.voxel();
+    if (p < 0 || p >= sz) return;
     int i, j, k;
     voxel_ijk(p, m, n, i, j, k);
     float y1 = affine_coord(A.r, i, j, k, v[p]);
@@ -114,10 +127,10 @@ __global__ void k_warp_affine(float* __restrict__ out, const float* __restrict__
 __global__ void k_warp_affine_grid(float* __restrict__ out, const float* __restrict__ in, Affine A,
                                    const float* __restrict__ u0, const float* __restrict__ v0,
                                    const float* __restrict__ w0, int m, int n, int o, int m2, int n2, int o2,
-                                   UpScale sc) {
+                                   UpScale sc, Span sp) {
     long long sz = (long long)m * n * o;
-    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= sz) return;
+    const long long p = sp.voxel();
+    if (p < 0 || p >= sz) return;
     int i, j, k;
     voxel_ijk(p, m, n, i, j, k);
     float u, v, w;
@@ -131,10 +144,10 @@ __global__ void k_warp_affine_grid(float* __restrict__ out, const float* __restr
 __global__ void k_warp_image_grid(float* __restrict__ out, const float* __restrict__ in,
                                   const float* __restrict__ u0, const float* __restrict__ v0,
                                   const float* __restrict__ w0, int m, int n, int o, int m2, int n2, int o2,
-                                  UpScale sc) {
+                                  UpScale sc, Span sp) {
     long long sz = (long long)m * n * o;
-    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= sz) return;
+    const long long p = sp.voxel();
+    if (p < 0 || p >= sz) return;
     int i, j, k;
     voxel_ijk(p, m, n, i, j, k);
     float u, v, w;
@@ -144,10 +157,10 @@ __global__ void k_warp_image_grid(float* __restrict__ out, const float* __restri
 
 __global__ void k_warp_affine_s(int16_t* __restrict__ out, const int16_t* __restrict__ in, Affine A,
                                 const float* __restrict__ u, const float* __restrict__ v,
-                                const float* __restrict__ w, int m, int n, int o) {
+                                const float* __restrict__ w, int m, int n, int o, Span sp) {
     long long sz = (long long)m * n * o;
-    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= sz) return;
+    const long long p = sp.voxel();
+    if (p < 0 || p >= sz) return;
     int i, j, k;
     voxel_ijk(p, m, n, i, j, k);
     int y = (int)roundf(affine_coord(A.r, i, j, k, v[p]));
@@ -175,10 +188,10 @@ struct Six {
 };
 
 // One Jacobi iteration from the previous iterate `s` into `d`.
-__global__ void k_consistent_iter(Six d, Six s, int m, int n, int o) {
+__global__ void k_consistent_iter(Six d, Six s, int m, int n, int o, Span sp) {
     long long sz = (long long)m * n * o;
-    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= sz) return;
+    const long long p = sp.voxel();
+    if (p < 0 || p >= sz) return;
     int i, j, k;
     voxel_ijk(p, m, n, i, j, k);
     const float *us = s.f[0], *vs = s.f[1], *ws = s.f[2], *us2 = s.f[3], *vs2 = s.f[4], *ws2 = s.f[5];
@@ -323,6 +336,11 @@ UpScale make_scale(int m, int n, int o, int m2, int n2, int o2) {
 
 constexpr int TB = 256;
 
+Span span_of(long long sz) {
+    const int blocks = (int)ceil_div(sz, TB);
+    return Span{(int)ceil_div(blocks, 8), blocks};
+}
+
 }  // namespace
 
 Affine affine_from_X(const float* X) {
@@ -337,8 +355,8 @@ void launch_upsample(float* u1, float* v1, float* w1, const float* u0, const flo
                      int o, int m2, int n2, int o2, hipStream_t s) {
     long long sz = (long long)m * n * o;
     if (sz == 0) return;
-    k_upsample<<<ceil_div(sz, TB), TB, 0, s>>>(u1, v1, w1, u0, v0, w0, m, n, o, m2, n2, o2,
-                                               make_scale(m, n, o, m2, n2, o2));
+    k_upsample<<<span_of(sz).grid(), TB, 0, s>>>(u1, v1, w1, u0, v0, w0, m, n, o, m2, n2, o2,
+                                               make_scale(m, n, o, m2, n2, o2), span_of(sz));
     DEEDS_LAUNCH_CHECK();
 }
 
@@ -346,7 +364,7 @@ void launch_interp3(float* out, const float* in, const float* x1, const float* y
                     int o, int m2, int n2, int o2, int add_identity, hipStream_t s) {
     long long sz = (long long)m * n * o;
     if (sz == 0) return;
-    k_interp3<<<ceil_div(sz, TB), TB, 0, s>>>(out, in, x1, y1, z1, m, n, o, m2, n2, o2, add_identity);
+    k_interp3<<<span_of(sz).grid(), TB, 0, s>>>(out, in, x1, y1, z1, m, n, o, m2, n2, o2, add_identity, span_of(sz));
     DEEDS_LAUNCH_CHECK();
 }
 
@@ -354,7 +372,7 @@ void launch_warp_affine(float* out, const float* in, const Affine& A, const floa
                         int m, int n, int o, hipStream_t s) {
     long long sz = (long long)m * n * o;
     if (sz == 0) return;
-    k_warp_affine<<<ceil_div(sz, TB), TB, 0, s>>>(out, in, A, u, v, w, m, n, o);
+    k_warp_affine<<<span_of(sz).grid(), TB, 0, s>>>(out, in, A, u, v, w, m, n, o, span_of(sz));
     DEEDS_LAUNCH_CHECK();
 }
 
@@ -362,8 +380,8 @@ void launch_warp_affine_grid(float* out, const float* in, const Affine& A, const
                              const float* w0, int m, int n, int o, int m2, int n2, int o2, hipStream_t s) {
     long long sz = (long long)m * n * o;
     if (sz == 0) return;
-    k_warp_affine_grid<<<ceil_div(sz, TB), TB, 0, s>>>(out, in, A, u0, v0, w0, m, n, o, m2, n2, o2,
-                                                       make_scale(m, n, o, m2, n2, o2));
+    k_warp_affine_grid<<<span_of(sz).grid(), TB, 0, s>>>(out, in, A, u0, v0, w0, m, n, o, m2, n2, o2,
+                                                       make_scale(m, n, o, m2, n2, o2), span_of(sz));
     DEEDS_LAUNCH_CHECK();
 }
 
@@ -371,8 +389,8 @@ void launch_warp_image_grid(float* out, const float* in, const float* u0, const 
                             int n, int o, int m2, int n2, int o2, hipStream_t s) {
     long long sz = (long long)m * n * o;
     if (sz == 0) return;
-    k_warp_image_grid<<<ceil_div(sz, TB), TB, 0, s>>>(out, in, u0, v0, w0, m, n, o, m2, n2, o2,
-                                                      make_scale(m, n, o, m2, n2, o2));
+    k_warp_image_grid<<<span_of(sz).grid(), TB, 0, s>>>(out, in, u0, v0, w0, m, n, o, m2, n2, o2,
+                                                      make_scale(m, n, o, m2, n2, o2), span_of(sz));
     DEEDS_LAUNCH_CHECK();
 }
 
@@ -380,7 +398,7 @@ void launch_warp_affine_s(int16_t* out, const int16_t* in, const Affine& A, cons
                           const float* w, int m, int n, int o, hipStream_t s) {
     long long sz = (long long)m * n * o;
     if (sz == 0) return;
-    k_warp_affine_s<<<ceil_div(sz, TB), TB, 0, s>>>(out, in, A, u, v, w, m, n, o);
+    k_warp_affine_s<<<span_of(sz).grid(), TB, 0, s>>>(out, in, A, u, v, w, m, n, o, span_of(sz));
     DEEDS_LAUNCH_CHECK();
 }
 
@@ -389,6 +407,7 @@ void launch_consistent(float* f[6], float* tmp, int m, int n, int o, int factor,
     if (sz == 0) return;
     float f1 = (float)(1.0 / (float)factor);
     unsigned g = ceil_div(sz, TB);
+    const Span sp = span_of(sz);
     k_scale6<<<g, TB, 0, s>>>(tmp, f[0], f[1], f[2], f[3], f[4], f[5], sz, f1);
     DEEDS_LAUNCH_CHECK();
     Six a, b;
@@ -398,8 +417,8 @@ void launch_consistent(float* f[6], float* tmp, int m, int n, int o, int factor,
     }
     // ten iterations ping-pong tmp -> f -> tmp ...; the tenth lands in tmp
     for (int it = 0; it < 10; it++) {
-        if (it % 2 == 0) k_consistent_iter<<<g, TB, 0, s>>>(b, a, m, n, o);
-        else k_consistent_iter<<<g, TB, 0, s>>>(a, b, m, n, o);
+        if (it % 2 == 0) k_consistent_iter<<<sp.grid(), TB, 0, s>>>(b, a, m, n, o, sp);
+        else k_consistent_iter<<<sp.grid(), TB, 0, s>>>(a, b, m, n, o, sp);
         DEEDS_LAUNCH_CHECK();
     }
     k_unscale6<<<g, TB, 0, s>>>(b, tmp, sz, (float)factor);
