@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <mutex>
 #include <cstring>
+#include <chrono>
 #include <future>
 #include <memory>
 #include <type_traits>
@@ -593,31 +594,37 @@ int deeds_ctx_prims_graph(deeds_ctx* ctx, const float* d_im, int* ordered, int* 
         DEEDS_HIP(hipEventCreate(&ev));
         std::unique_ptr<std::remove_pointer<hipEvent_t>::type, hipError_t (*)(hipEvent_t)> hold(ev, hipEventDestroy);
         DEEDS_HIP(hipEventRecord(ev, ctx->stream));
-        // Prim's on the edge costs while the GPU computes the statistics (as the registration driver does)
+        // Prim's on the edge costs (integer SADs only) while the GPU computes the statistics, as the
+        // registration driver does; the caller gets no BP schedule
         Tree t;
         auto shape = std::async(std::launch::async, [&] {
             DEEDS_HIP(hipEventSynchronize(ev));
-            build_tree_shape(h_ec, step, m1, n1, o1, t);
+            return build_tree_shape(h_ec, step, m1, n1, o1, t);
         });
         float mean = 0, sd = 0;
         StatsScratch sc;
         const size_t bytes = StatsScratch::bytes(sz);
         sc.dev = ctx->buf("prims.stats", bytes);
         sc.host = ctx->pinned("prims.stats", bytes);
-        bool same_order = false;
+        bool shaped = false, same_order = false;
         try {
             image_stats_device(d_im, sz, sc, ctx->stream, &mean, &sd);
-            // the order check runs beside Prim's
-            same_order = weights_distinguish_costs(h_ec, step, m1, n1, o1, sd);
+            if (shape.wait_for(std::chrono::seconds(0)) == std::future_status::ready) {
+                shaped = shape.get();  // done, or declined (non-integer SADs)
+                if (shaped) same_order = weights_distinguish_costs(h_ec, step, m1, n1, o1, sd);
+            } else {
+                // Prim's on the costs is running: the order check runs beside it
+                same_order = weights_distinguish_costs(h_ec, step, m1, n1, o1, sd);
+                shaped = shape.get();
+            }
         } catch (...) {
-            shape.wait();
+            if (shape.valid()) shape.wait();
             throw;
         }
-        shape.get();
-        if (same_order)
-            finish_tree(h_ec, sd, step, t);
+        if (shaped && same_order)
+            finish_tree(h_ec, sd, step, t, false);
         else
-            build_tree(h_ec, sd, step, m1, n1, o1, t);
+            build_tree(h_ec, sd, step, m1, n1, o1, t, false);
         std::copy(t.ordered.begin(), t.ordered.end(), ordered);
         std::copy(t.parents.begin(), t.parents.end(), parents);
         std::copy(t.edgemst.begin(), t.edgemst.end(), edgemst);

@@ -288,18 +288,15 @@ bool weights_distinguish_costs(const float* ec, int step, int m1, int n1, int o1
     return true;
 }
 
-void build_tree_shape(const float* ec, int step, int m1, int n1, int o1, Tree& t) {
+bool build_tree_shape(const float* ec, int step, int m1, int n1, int o1, Tree& t) {
     const size_t count = (size_t)6 * m1 * n1 * o1;
     std::vector<uint16_t> rank;
-    if (cost_ranks(ec, (float)((double)step * step * step), count, rank)) {
-        prim_tree(rank.data(), m1, n1, o1, t);
-        return;
-    }
-    std::vector<float> c = edge_costs(ec, step, count);
-    prim_tree(c.data(), m1, n1, o1, t);
+    if (!cost_ranks(ec, (float)((double)step * step * step), count, rank)) return false;
+    prim_tree(rank.data(), m1, n1, o1, t);
+    return true;
 }
 
-void finish_tree(const float* ec, float sd, int step, Tree& t) {
+void finish_tree(const float* ec, float sd, int step, Tree& t, bool schedule) {
     const int N = t.N;
     const float vol = (float)((double)step * step * step);
     const float two_sd = 2.0f * sd;
@@ -308,17 +305,17 @@ void finish_tree(const float* ec, float sd, int step, Tree& t) {
         const int nb = t.parent_slot[v];
         if (nb >= 0) t.edgemst[v] = -edge_weight(ec[v + (size_t)nb * N] / vol, two_sd);
     }
-    build_schedule(t);
+    if (schedule) build_schedule(t);
 }
 
-void build_tree(const float* ec, float sd, int step, int m1, int n1, int o1, Tree& t) {
+void build_tree(const float* ec, float sd, int step, int m1, int n1, int o1, Tree& t, bool schedule) {
     // weights -exp(-(c / step^3) / (2 sd)) as the keys (primsMST.h:112-117)
     const int N = m1 * n1 * o1;
     std::vector<float> w = edge_costs(ec, step, (size_t)6 * N);
     const float two_sd = 2.0f * sd;
     for (float& x : w) x = edge_weight(x, two_sd);
     prim_tree(w.data(), m1, n1, o1, t);
-    finish_tree(ec, sd, step, t);
+    finish_tree(ec, sd, step, t, schedule);
 }
 
 namespace {

@@ -91,17 +91,23 @@ size_t tree_device_ints(int N);
 void image_stats(const float* im, long long count, float* mean, float* sd);
 
 // primsMST.h:112-223 given the SAD edge costs ec[node + nb*N] of the (m1,n1,o1)
-// control grid, the image std `sd` and the grid step.
-void build_tree(const float* ec, float sd, int step, int m1, int n1, int o1, Tree& t);
+// control grid, the image std `sd` and the grid step.  Without `schedule`
+// only ordered, parents, parent_slot and edgemst are filled.
+void build_tree(const float* ec, float sd, int step, int m1, int n1, int o1, Tree& t, bool schedule = true);
 
 // The same tree in two steps, so the image statistics need not be known when
 // Prim's runs: build_tree_shape orders edges by their cost c = SAD/step^3,
 // which is the reference's order by weight whenever
 // weights_distinguish_costs(...) holds for the final sd; finish_tree then
-// computes edgemst and the schedule.  When the check fails, use build_tree.
-void build_tree_shape(const float* ec, int step, int m1, int n1, int o1, Tree& t);
+// computes edgemst (and the schedule).  When the check fails, use build_tree.
+// build_tree_shape runs Prim's only when the SADs are integers below 2^22
+// (16-bit cost ranks, see cost_ranks): distinct non-integer costs are nearly
+// all distinct floats, which the weight map, flat to a few ulps over a grid
+// of 10^5 edges, then merges in places, so the check would fail and Prim's
+// would run twice.  It returns false, leaving t as it was, otherwise.
+bool build_tree_shape(const float* ec, int step, int m1, int n1, int o1, Tree& t);
 bool weights_distinguish_costs(const float* ec, int step, int m1, int n1, int o1, float sd);
-void finish_tree(const float* ec, float sd, int step, Tree& t);
+void finish_tree(const float* ec, float sd, int step, Tree& t, bool schedule = true);
 
 // Fills depth_start, the child lists and (with bp_programs) the selection
 // info and the bottom-up rounds and programs (rounds of `window` steps) from

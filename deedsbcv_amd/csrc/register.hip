@@ -410,9 +410,10 @@ class PairRun {
                 std::future<bool> check;
                 if (check_beside) check = std::async(std::launch::async, confirm);
                 std::unique_ptr<Tree> t(new Tree());
-                build_tree_shape(ec, c.step, c.m1, c.n1, c.o1, *t);
+                // (declined for non-integer SADs: then Prim's runs once, on the weights)
+                const bool by_cost = build_tree_shape(ec, c.step, c.m1, c.n1, c.o1, *t);
                 *shaped = std::chrono::duration<double>(std::chrono::steady_clock::now() - tb).count();
-                const bool same_order = check_beside ? check.get() : confirm();
+                const bool same_order = by_cost && (check_beside ? check.get() : confirm());
                 const float sd = sdf.get();
                 if (same_order)
                     finish_tree(ec, sd, c.step, *t);

@@ -58,9 +58,9 @@ constexpr int soft_bp_out(int L) {
 }
 constexpr int soft_bp_s(int L) { return (L + soft_bp_out(L) - 1) / soft_bp_out(L); }
 // LDS bytes of one soft BP workgroup: six label cubes, three padded quadratic
-// tables and a reduction area.
+// tables and their exponential forms, and a reduction area.
 constexpr size_t soft_bp_smem(int L, size_t rsize) {
-    return (6 * (size_t)L * L * L + 3 * (2 * (size_t)L + 1 + 2 * (size_t)soft_bp_s(L) * soft_bp_out(L)) + 32) *
+    return (6 * (size_t)L * L * L + 6 * (2 * (size_t)L + 1 + 2 * (size_t)soft_bp_s(L) * soft_bp_out(L)) + 32) *
            rsize;
 }
 // Largest label radius with a compiled soft BP kernel.

@@ -39,9 +39,12 @@ struct SS {
     static constexpr size_t SMEM = soft_bp_smem(LEN, sizeof(R));
 };
 
+// Label cubes, the quadratic tables z of an edge per axis, their exponential
+// forms k = exp(-(z - zmin) / T) and a reduction area whose slots 16..18
+// hold zmin per axis.
 template <typename R, int LEN>
 struct Cubes {
-    R *X0, *X1, *X2, *X3, *Ga, *Gb, *zy, *zx, *zz, *red;
+    R *X0, *X1, *X2, *X3, *Ga, *Gb, *zy, *zx, *zz, *ky, *kx, *kz, *red;
     __device__ explicit Cubes(R* b) {
         using H = SS<R, LEN>;
         X0 = b;
@@ -53,8 +56,29 @@ struct Cubes {
         zy = Gb + H::L3 + H::PAD;  // zy[q] for q = -PAD .. 2*LEN+PAD: offset q - LEN
         zx = zy + H::ZT;
         zz = zx + H::ZT;
-        red = zz - H::PAD + H::ZT;
+        ky = zz + H::ZT;
+        kx = ky + H::ZT;
+        kz = kx + H::ZT;
+        red = kz - H::PAD + H::ZT;
     }
+};
+
+// A log-sum-exp pass runs as a convolution in the exponential domain,
+// sum_b exp(-(in(b) - min in) / T) exp(-(z(a - b) - zmin) / T), when that sum
+// stays above 2^-ConvLimit (every term of it is then a normal number and the
+// largest one is at least the sum / LEN); below it, the output is computed
+// term by term, shifted by its own minimum, as the exact form.
+template <typename R>
+struct Conv;
+template <>
+struct Conv<float> {
+    static constexpr float limit = 100.0f;       // log2 units
+    static constexpr float floor = 0x1p-100f;
+};
+template <>
+struct Conv<double> {
+    static constexpr double limit = 900.0;
+    static constexpr double floor = 0x1p-900;
 };
 
 // Lines of the cube along axis AX (0: i / stride 1, 1: j / stride LEN, 2: k / stride LEN^2).
@@ -66,38 +90,53 @@ struct Line {
     }
 };
 
-// out(a) = mn_a - T ln sum_b exp(-(in(b) + z[a - b]) / T) along axis AX, mn_a = min_b(in(b) + z[a - b]);
-// returns the smallest output this thread wrote.
+// out(a) = -T ln sum_b exp(-(in(b) + z[a - b]) / T) along axis AX (z the
+// quadratic table, k its exponential form, zmin its minimum over the label
+// differences); returns the smallest output this thread wrote.
 template <typename R, int LEN, int AX>
-__device__ __forceinline__ R lse_pass(const R* __restrict__ in, R* __restrict__ out, const R* __restrict__ zt, R iT2,
-                                      R Tln2) {
+__device__ __forceinline__ R lse_pass(const R* __restrict__ in, R* __restrict__ out, const R* __restrict__ zt,
+                                      const R* __restrict__ kt, R zmin, R iT2, R Tln2) {
     using H = SS<R, LEN>;
     using G = Line<LEN, AX>;
     const int line = threadIdx.x / H::S, sub = threadIdx.x - line * H::S;
     R lo = big<R>();
     if (line >= H::L2) return lo;
     const int base = G::base(line);
-    R v[LEN];
+    R v[LEN], mi = big<R>();
 #pragma unroll
-    for (int j = 0; j < LEN; j++) v[j] = in[base + j * G::STRIDE];
+    for (int j = 0; j < LEN; j++) {
+        v[j] = in[base + j * G::STRIDE];
+        mi = v[j] < mi ? v[j] : mi;
+    }
+    R e[LEN];
+#pragma unroll
+    for (int j = 0; j < LEN; j++) e[j] = FastMath<R>::exp2((mi - v[j]) * iT2);
     const int i0 = sub * H::OUT;
-    R zw[H::OUT + LEN];  // zw[r - j + LEN] = zt[(i0 + r) - j + LEN]
+    R kw[H::OUT + LEN];  // kw[r - j + LEN] = kt[(i0 + r) - j + LEN]
 #pragma unroll
-    for (int q = 0; q < H::OUT + LEN; q++) zw[q] = zt[i0 + q];
+    for (int q = 0; q < H::OUT + LEN; q++) kw[q] = kt[i0 + q];
+    const R shift = mi + zmin;
 #pragma unroll
     for (int r = 0; r < H::OUT; r++) {
         if (i0 + r < LEN) {
-            R c[LEN];
-            R mn = big<R>();
+            R sum = 0;
 #pragma unroll
-            for (int j = 0; j < LEN; j++) {
-                c[j] = v[j] + zw[r - j + LEN];
-                mn = c[j] < mn ? c[j] : mn;
+            for (int j = 0; j < LEN; j++) sum += e[j] * kw[r - j + LEN];
+            R o;
+            if (sum >= Conv<R>::floor) {
+                o = shift - Tln2 * FastMath<R>::log2(sum);
+            } else {
+                R c[LEN], mn = big<R>();
+#pragma unroll
+                for (int j = 0; j < LEN; j++) {
+                    c[j] = v[j] + zt[i0 + r - j + LEN];
+                    mn = c[j] < mn ? c[j] : mn;
+                }
+                R s2 = 0;
+#pragma unroll
+                for (int j = 0; j < LEN; j++) s2 += FastMath<R>::exp2((mn - c[j]) * iT2);
+                o = mn - Tln2 * FastMath<R>::log2(s2);
             }
-            R s = 0;
-#pragma unroll
-            for (int j = 0; j < LEN; j++) s += FastMath<R>::exp2((mn - c[j]) * iT2);
-            const R o = mn - Tln2 * FastMath<R>::log2(s);
             out[base + (i0 + r) * G::STRIDE] = o;
             lo = o < lo ? o : lo;
         }
@@ -105,34 +144,61 @@ __device__ __forceinline__ R lse_pass(const R* __restrict__ in, R* __restrict__ 
     return lo;
 }
 
-// Adjoint of lse_pass: gin(b) = sum_a gout(a) exp(-(in(b) + z[a - b] - out(a)) / T).
+// Adjoint of lse_pass: gin(b) = sum_a gout(a) exp(-(in(b) + z[a - b] - out(a)) / T)
+//   = exp(-(in(b) - min in) / T) sum_a w(a) k[a - b],  w(a) = gout(a) exp((out(a) - min in - zmin) / T),
+// with the exact term for every a whose weight factor exceeds 2^ConvLimit and
+// for every b whose factor is below 2^-ConvLimit.
 template <typename R, int LEN, int AX>
 __device__ __forceinline__ void lse_adj(const R* __restrict__ in, const R* __restrict__ out,
                                         const R* __restrict__ gout, R* __restrict__ gin, const R* __restrict__ zt,
-                                        R iT2) {
+                                        const R* __restrict__ kt, R zmin, R iT2) {
     using H = SS<R, LEN>;
     using G = Line<LEN, AX>;
     const int line = threadIdx.x / H::S, sub = threadIdx.x - line * H::S;
     if (line >= H::L2) return;
     const int base = G::base(line);
-    R go[LEN], ou[LEN];
+    R go[LEN], ou[LEN], mi = big<R>();
 #pragma unroll
     for (int a = 0; a < LEN; a++) {
         go[a] = gout[base + a * G::STRIDE];
         ou[a] = out[base + a * G::STRIDE];
+        const R x = in[base + a * G::STRIDE];
+        mi = x < mi ? x : mi;
+    }
+    R w[LEN];
+    bool any_big = false;
+#pragma unroll
+    for (int a = 0; a < LEN; a++) {
+        const R ex = (ou[a] - mi - zmin) * iT2;
+        const bool bg = ex > Conv<R>::limit;
+        any_big |= bg;
+        w[a] = bg ? (R)0 : go[a] * FastMath<R>::exp2(ex);
     }
     const int i0 = sub * H::OUT;
-    R zw[H::OUT + LEN - 1];  // zw[a - r + OUT - 1] = zt[a - (i0 + r) + LEN]
+    R kw[H::OUT + LEN - 1];  // kw[a - r + OUT - 1] = kt[a - (i0 + r) + LEN]
 #pragma unroll
-    for (int q = 0; q < H::OUT + LEN - 1; q++) zw[q] = zt[LEN - i0 - (H::OUT - 1) + q];
+    for (int q = 0; q < H::OUT + LEN - 1; q++) kw[q] = kt[LEN - i0 - (H::OUT - 1) + q];
 #pragma unroll
     for (int r = 0; r < H::OUT; r++) {
         const int b = i0 + r;
         if (b < LEN) {
             const R x = in[base + b * G::STRIDE];
+            const R eb = (mi - x) * iT2;
             R acc = 0;
+            if (eb >= -Conv<R>::limit) {
 #pragma unroll
-            for (int a = 0; a < LEN; a++) acc += go[a] * FastMath<R>::exp2((ou[a] - x - zw[a - r + H::OUT - 1]) * iT2);
+                for (int a = 0; a < LEN; a++) acc += w[a] * kw[a - r + H::OUT - 1];
+                acc *= FastMath<R>::exp2(eb);
+                if (any_big) {
+#pragma unroll
+                    for (int a = 0; a < LEN; a++)
+                        if ((ou[a] - mi - zmin) * iT2 > Conv<R>::limit)
+                            acc += go[a] * FastMath<R>::exp2((ou[a] - x - zt[a - b + LEN]) * iT2);
+                }
+            } else {
+#pragma unroll
+                for (int a = 0; a < LEN; a++) acc += go[a] * FastMath<R>::exp2((ou[a] - x - zt[a - b + LEN]) * iT2);
+            }
             gin[base + b * G::STRIDE] = acc;
         }
     }
@@ -178,11 +244,34 @@ __device__ void edge_tables(const BPState<R>& a, int c, int p, R sign, const Cub
     const R ou = sign * (a.u0[p] - a.u0[c]) / a.quant;
     const R ov = sign * (a.u0[a.N + p] - a.u0[a.N + c]) / a.quant;
     const R ow = sign * (a.u0[2 * a.N + p] - a.u0[2 * a.N + c]) / a.quant;
+    // zmin: the table's minimum over the label differences -(LEN-1) .. LEN-1,
+    // at the integer next to -off on either side
+    R zm[3];
+    const R offs[3] = {ov, ou, ow};
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++) {
+        const R o = offs[ax];
+        R t0 = floor(-o), t1 = t0 + 1;
+        t0 = t0 < (R)(1 - LEN) ? (R)(1 - LEN) : (t0 > (R)(LEN - 1) ? (R)(LEN - 1) : t0);
+        t1 = t1 < (R)(1 - LEN) ? (R)(1 - LEN) : (t1 > (R)(LEN - 1) ? (R)(LEN - 1) : t1);
+        const R z0 = (t0 + o) * (t0 + o) / w, z1 = (t1 + o) * (t1 + o) / w;
+        zm[ax] = z0 < z1 ? z0 : z1;
+    }
+    const R iT2 = (R)kLog2e / a.T;
     for (int q = threadIdx.x; q < H::ZT; q += blockDim.x) {
         const R t = (R)(q - H::PAD - LEN);
-        s.zy[q - H::PAD] = (t + ov) * (t + ov) / w;
-        s.zx[q - H::PAD] = (t + ou) * (t + ou) / w;
-        s.zz[q - H::PAD] = (t + ow) * (t + ow) / w;
+        const R y = (t + ov) * (t + ov) / w, x = (t + ou) * (t + ou) / w, z = (t + ow) * (t + ow) / w;
+        s.zy[q - H::PAD] = y;
+        s.zx[q - H::PAD] = x;
+        s.zz[q - H::PAD] = z;
+        s.ky[q - H::PAD] = FastMath<R>::exp2(-(y - zm[0]) * iT2);
+        s.kx[q - H::PAD] = FastMath<R>::exp2(-(x - zm[1]) * iT2);
+        s.kz[q - H::PAD] = FastMath<R>::exp2(-(z - zm[2]) * iT2);
+    }
+    if (threadIdx.x == 0) {
+        s.red[16] = zm[0];
+        s.red[17] = zm[1];
+        s.red[18] = zm[2];
     }
 }
 
@@ -191,11 +280,11 @@ template <typename R, int LEN>
 __device__ R transform(const Cubes<R, LEN>& s, R T) {
     const R iT2 = (R)kLog2e / T, Tln2 = T * (R)kLn2;
     __syncthreads();
-    lse_pass<R, LEN, 0>(s.X0, s.X1, s.zy, iT2, Tln2);
+    lse_pass<R, LEN, 0>(s.X0, s.X1, s.zy, s.ky, s.red[16], iT2, Tln2);
     __syncthreads();
-    lse_pass<R, LEN, 1>(s.X1, s.X2, s.zx, iT2, Tln2);
+    lse_pass<R, LEN, 1>(s.X1, s.X2, s.zx, s.kx, s.red[17], iT2, Tln2);
     __syncthreads();
-    const R lo = lse_pass<R, LEN, 2>(s.X2, s.X3, s.zz, iT2, Tln2);
+    const R lo = lse_pass<R, LEN, 2>(s.X2, s.X3, s.zz, s.kz, s.red[18], iT2, Tln2);
     return block_reduce<R>(lo, s.red, true);
 }
 
@@ -204,11 +293,11 @@ template <typename R, int LEN>
 __device__ void transform_adj(const Cubes<R, LEN>& s, R T) {
     const R iT2 = (R)kLog2e / T;
     __syncthreads();
-    lse_adj<R, LEN, 2>(s.X2, s.X3, s.Ga, s.Gb, s.zz, iT2);
+    lse_adj<R, LEN, 2>(s.X2, s.X3, s.Ga, s.Gb, s.zz, s.kz, s.red[18], iT2);
     __syncthreads();
-    lse_adj<R, LEN, 1>(s.X1, s.X2, s.Gb, s.Ga, s.zx, iT2);
+    lse_adj<R, LEN, 1>(s.X1, s.X2, s.Gb, s.Ga, s.zx, s.kx, s.red[17], iT2);
     __syncthreads();
-    lse_adj<R, LEN, 0>(s.X0, s.X1, s.Ga, s.Gb, s.zy, iT2);
+    lse_adj<R, LEN, 0>(s.X0, s.X1, s.Ga, s.Gb, s.zy, s.ky, s.red[16], iT2);
     __syncthreads();
 }
 

@@ -313,10 +313,376 @@ __global__ __launch_bounds__(CB) void k_sc_dM_fold(R* __restrict__ dM, const R* 
     acc.store(dM, M, ti + (long long)tj * g.m + (long long)tk * g.m * g.n, g.scale);
 }
 
+// ---- row form of dM ---------------------------------------------------------------
+// With every voxel of a block sampled along i (skip_i 1), the fixed voxel
+// u = T + du of a node and label ly pair with the moving voxel u + (ly - HW) Q
+// of a row, so a thread that owns STEP consecutive moving voxels loads each
+// fixed voxel of its window once and uses it for every label on it.  Threads
+// run i-blocks fastest: the lanes of a wavefront share the row (jp, kp).
+
+constexpr int floor_div(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+
+// dM: one thread per i-block of STEP voxels [T, T + STEP), T = Y * STEP, of a
+// row (jp, kp) of the zero-padded box.  For each (lz, lx) whose sample row
+// s = t - d lies on the sample lattice, the fixed voxels u = T + du of the
+// nodes Y + dy along i pair with the outputs T + du + (ly - HW) Q of the block;
+// those pairs are known at compile time.  Outputs are written as k_sc_dM_pull
+// writes them: interior voxels to dM, the rest as sums to `part`.
+template <typename R, int STEP, int LEN, int Q>
+__global__ __launch_bounds__(CB) void k_sc_dM_rows(R* __restrict__ dM, R* __restrict__ part,
+                                                   const R* __restrict__ gc, const R* __restrict__ F,
+                                                   const R* __restrict__ M, CostGeom g, int nb, int y0) {
+    constexpr int HW = (LEN - 1) / 2, PI = HW * Q, L3 = LEN * LEN * LEN;
+    constexpr int DY0 = floor_div(-PI, STEP), DY1 = floor_div(STEP - 1 + PI, STEP);
+    const int mp = g.m + 2 * PI, np = g.n + 2 * PI, op = g.o + 2 * PI;
+    const long long e = (long long)blockIdx.x * CB + threadIdx.x;
+    if (e >= (long long)nb * np * op) return;
+    const int Y = (int)(e % nb) - y0;
+    const long long rr = e / nb;
+    const int jp = (int)(rr % np), kp = (int)(rr / np);
+    const int T = Y * STEP, tj = jp - PI, tk = kp - PI;
+    const long long mn = (long long)g.m * g.n;
+    R acc[STEP][12], G[STEP];
+#pragma unroll
+    for (int t = 0; t < STEP; t++) {
+        G[t] = 0;
+#pragma unroll
+        for (int ch = 0; ch < 12; ch++) acc[t][ch] = 0;
+    }
+    for (int lz = 0; lz < LEN; lz++) {
+        const int s_k = tk - (lz - HW) * Q;
+        if (s_k < 0 || s_k >= g.o1 * STEP) continue;
+        const int z = s_k / STEP;
+        if ((s_k - z * STEP) % g.sk) continue;
+        for (int lx = 0; lx < LEN; lx++) {
+            const int s_j = tj - (lx - HW) * Q;
+            if (s_j < 0 || s_j >= g.n1 * STEP) continue;
+            const int x = s_j / STEP;
+            if ((s_j - x * STEP) % g.sj) continue;
+            const long long nrow = (long long)(z * g.n1 + x) * g.m1;
+            const R* frow = F + 12 * ((long long)s_k * mn + (long long)s_j * g.m + T);
+            const int lb = (lz * LEN + lx) * LEN;
+#pragma unroll
+            for (int dy = DY0; dy <= DY1; dy++) {
+                const int y = Y + dy;
+                if (y < 0 || y >= g.m1) continue;
+                const R* gl = gc + (nrow + y) * L3 + lb;
+                R gv[LEN];
+#pragma unroll
+                for (int ly = 0; ly < LEN; ly++) gv[ly] = gl[ly];
+#pragma unroll
+                for (int si = 0; si < STEP; si++) {
+                    const int du = dy * STEP + si;
+                    if (du + (LEN - 1 - HW) * Q < 0 || du - HW * Q >= STEP) continue;  // no output in the block
+                    R fv[12];
+                    load12(fv, frow + 12 * du);
+#pragma unroll
+                    for (int ly = 0; ly < LEN; ly++) {
+                        const int to = du + (ly - HW) * Q;
+                        if (to >= 0 && to < STEP) {
+                            G[to] += gv[ly];
+#pragma unroll
+                            for (int ch = 0; ch < 12; ch++) acc[to][ch] += gv[ly] * fv[ch];
+                        }
+                    }
+                }
+            }
+        }
+    }
+    const bool inner_jk = tj >= 1 && tj <= g.n - 2 && tk >= 1 && tk <= g.o - 2;
+#pragma unroll
+    for (int t = 0; t < STEP; t++) {
+        const int ti = T + t;
+        if (ti < -PI || ti >= g.m + PI) continue;
+        if (inner_jk && ti >= 1 && ti <= g.m - 2) {
+            R mv[12];
+            const long long v = ti + (long long)tj * g.m + tk * mn;
+            load12(mv, M + 12 * v);
+#pragma unroll
+            for (int ch = 0; ch < 12; ch++) dM[12 * v + ch] = (R)(2.0 * g.scale) * (mv[ch] * G[t] - acc[t][ch]);
+        } else {
+            R* pp = part + 13 * ((ti + PI) + (long long)jp * mp + (long long)kp * mp * np);
+#pragma unroll
+            for (int ch = 0; ch < 12; ch++) pp[ch] = acc[t][ch];
+            pp[12] = G[t];
+        }
+    }
+}
+
+// ---- per-node forms ---------------------------------------------------------------
+// A node's moving box (the voxels its samples and labels reach, on the lattice
+// the sample skips and the quantisation step span) is staged in LDS one
+// k-plane at a time with coalesced 16-byte loads; a workgroup covers NY
+// consecutive nodes along i, whose boxes overlap in all but NY*STEP voxels of
+// each row.  Workgroups run in XCD-contiguous ranges, so neighbouring boxes
+// meet in one L2.
+
+constexpr int kNodeMaxNk = 4;  // sample planes per node (k) the per-node kernels hold
+
+struct BoxGeom {
+    int dk, dj;       // plane / row step of the used lattice, gcd(skip, quantisation)
+    int nw, nv, np;   // planes, rows per plane, voxels per row of the staged box
+    int ny, nyb;      // nodes per workgroup along i, workgroups along i
+    int per;          // workgroups per XCD
+};
+
+__device__ __forceinline__ int xcd_tile(int b, int per) { return (b & 7) * per + (b >> 3); }
+
+// Stages plane koff of the box of nodes y0.. into Ms ([nv][np][12]).
+template <typename R>
+__device__ __forceinline__ void stage_plane(R* __restrict__ Ms, const R* __restrict__ M, const CostGeom& g,
+                                            const BoxGeom& b, int kb, int jb, int ib, int koff) {
+    constexpr int V = 12 * (int)sizeof(R) / 16;  // 16-byte chunks per voxel
+    const long long mn = (long long)g.m * g.n;
+    const long long ko = (long long)clampi(kb + koff, 0, g.o - 1) * mn;
+    const int n = b.nv * b.np * V;
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+        const int v = e / (b.np * V), r = e - v * (b.np * V), p = r / V, c = r - p * V;
+        const long long vox = ko + (long long)clampi(jb + v * b.dj, 0, g.n - 1) * g.m + clampi(ib + p, 0, g.m - 1);
+        reinterpret_cast<uint4*>(Ms)[(v * b.np + p) * V + c] = reinterpret_cast<const uint4*>(M + 12 * vox)[c];
+    }
+}
+
+// The label lz with sk * skip_k + lz * Q == koff for each sample plane sk, or -1.
+__device__ __forceinline__ bool plane_labels(const CostGeom& g, int koff, int (&lz)[kNodeMaxNk]) {
+    bool any = false;
+#pragma unroll
+    for (int sk = 0; sk < kNodeMaxNk; sk++) {
+        const int r = koff - sk * g.sk;
+        lz[sk] = sk < g.nk && r >= 0 && r % g.q == 0 && r / g.q < g.len ? r / g.q : -1;
+        any |= lz[sk] >= 0;
+    }
+    return any;
+}
+
+// Cost: the waves of node slot ny hold its labels (lx, ly); each plane adds
+// the partial sums of its (sample plane, lz) pairs to the node's label sums
+// in LDS (a slot per (lz, thread): no two threads share one).  The node's
+// fixed samples are the same for the whole wave (scalar loads).
+template <typename R>
+__global__ __launch_bounds__(1024) void k_sc_cost_node(R* __restrict__ cost, const R* __restrict__ F,
+                                                       const R* __restrict__ M, CostGeom g, BoxGeom b, int tpn) {
+    extern __shared__ __align__(16) unsigned char sc_smem[];
+    const int tile = xcd_tile((int)blockIdx.x, b.per);
+    if (tile >= b.nyb * g.n1 * g.o1) return;
+    const int L = g.len, L2 = L * L, L3 = L2 * L;
+    R* Ms = reinterpret_cast<R*>(sc_smem);
+    R* acc = Ms + (size_t)b.nv * b.np * 12;  // [ny][L3]
+    const int yb = tile % b.nyb, x = (tile / b.nyb) % g.n1, z = tile / (b.nyb * g.n1);
+    // tpn is a multiple of the wavefront size: the slot is the same over a wave
+    const int slot = __builtin_amdgcn_readfirstlane(threadIdx.x / tpn), t = threadIdx.x - slot * tpn;
+    const int y0 = yb * b.ny, y = y0 + slot;
+    const bool active = t < L2 && y < g.m1;
+    const int lx = t / L, ly = t - lx * L;
+    for (int l = threadIdx.x; l < b.ny * L3; l += blockDim.x) acc[l] = 0;
+    const long long mn = (long long)g.m * g.n;
+    const int P = g.hw * g.q;
+    const int kb = z * g.step - P, jb = x * g.step - P, ib = y0 * g.step - P;
+    // the node's samples, uniform over the wave: read through the constant
+    // address space, so the loads are scalar (F is not written in the kernel)
+    using CR = const R __attribute__((address_space(4)));
+    CR* Fn = (CR*)(F + 12 * ((long long)z * g.step * mn + (long long)x * g.step * g.m +
+                              (long long)min(y, g.m1 - 1) * g.step));
+    R* an = acc + (size_t)slot * L3;
+    for (int w = 0; w < b.nw; w++) {
+        int lz[kNodeMaxNk];
+        if (!plane_labels(g, w * b.dk, lz)) continue;
+        __syncthreads();
+        stage_plane(Ms, M, g, b, kb, jb, ib, w * b.dk);
+        __syncthreads();
+        if (!active) continue;
+        R part[kNodeMaxNk];
+#pragma unroll
+        for (int sk = 0; sk < kNodeMaxNk; sk++) part[sk] = 0;
+        for (int sj = 0; sj < g.nj; sj++) {
+            const R* mrow = Ms + (size_t)(((sj * g.sj + lx * g.q) / b.dj) * b.np + slot * g.step + ly * g.q) * 12;
+            CR* frow = Fn + 12LL * sj * g.sj * g.m;
+            for (int si = 0; si < g.step; si++) {
+                R mv[12];
+                load12(mv, mrow + 12 * si);
+#pragma unroll
+                for (int sk = 0; sk < kNodeMaxNk; sk++) {
+                    if (lz[sk] < 0) continue;
+                    CR* fs = frow + 12 * ((long long)sk * g.sk * mn + si);
+#pragma unroll
+                    for (int ch = 0; ch < 12; ch++) {
+                        const R d = fs[ch] - mv[ch];
+                        part[sk] += d * d;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int sk = 0; sk < kNodeMaxNk; sk++)
+            if (lz[sk] >= 0) an[lz[sk] * L2 + t] += part[sk];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < b.ny * L3; e += blockDim.x) {
+        const int s2 = e / L3, l = e - s2 * L3;
+        if (y0 + s2 < g.m1) {
+            const long long node = y0 + s2 + (long long)g.m1 * (x + (long long)g.n1 * z);
+            cost[node * L3 + l] = acc[e] * (R)g.scale;
+        }
+    }
+}
+
+// dF: the waves of node slot ny hold its samples (si, sj, sk); each thread
+// sums g[l] M[s + d_l] and g[l] over the labels, plane by plane, with the
+// node's cost-gradient slices of the plane's lz values staged beside it.
+template <typename R>
+__global__ __launch_bounds__(1024) void k_sc_dF_node(R* __restrict__ dF, const R* __restrict__ gc,
+                                                     const R* __restrict__ F, const R* __restrict__ M, CostGeom g,
+                                                     BoxGeom b, int tpn) {
+    extern __shared__ __align__(16) unsigned char sc_smem[];
+    const int tile = xcd_tile((int)blockIdx.x, b.per);
+    if (tile >= b.nyb * g.n1 * g.o1) return;
+    const int L = g.len, L2 = L * L, L3 = L2 * L;
+    R* Ms = reinterpret_cast<R*>(sc_smem);
+    R* gs = Ms + (size_t)b.nv * b.np * 12;  // [ny][kNodeMaxNk][L2]
+    const int yb = tile % b.nyb, x = (tile / b.nyb) % g.n1, z = tile / (b.nyb * g.n1);
+    const int slot = threadIdx.x / tpn, t = threadIdx.x - slot * tpn;
+    const int y0 = yb * b.ny, y = y0 + slot;
+    const bool active = t < g.ns && y < g.m1;
+    const int si = t % g.step, sj = (t / g.step) % g.nj, sk = t / (g.step * g.nj);
+    const long long mn = (long long)g.m * g.n;
+    const int P = g.hw * g.q;
+    const int kb = z * g.step - P, jb = x * g.step - P, ib = y0 * g.step - P;
+    const long long nrow = (long long)g.m1 * (x + (long long)g.n1 * z);
+    R A[12], G = 0;
+#pragma unroll
+    for (int ch = 0; ch < 12; ch++) A[ch] = 0;
+    for (int w = 0; w < b.nw; w++) {
+        int lz[kNodeMaxNk];
+        if (!plane_labels(g, w * b.dk, lz)) continue;
+        __syncthreads();
+        stage_plane(Ms, M, g, b, kb, jb, ib, w * b.dk);
+        for (int e = threadIdx.x; e < b.ny * kNodeMaxNk * L2; e += blockDim.x) {
+            const int s2 = e / (kNodeMaxNk * L2), r = e - s2 * (kNodeMaxNk * L2), k2 = r / L2, l2 = r - k2 * L2;
+            int lzk = -1;
+#pragma unroll
+            for (int q = 0; q < kNodeMaxNk; q++)
+                if (q == k2) lzk = lz[q];
+            if (lzk >= 0 && y0 + s2 < g.m1) gs[e] = gc[(nrow + y0 + s2) * L3 + lzk * L2 + l2];
+        }
+        __syncthreads();
+        int mylz = -1;
+#pragma unroll
+        for (int q = 0; q < kNodeMaxNk; q++)
+            if (q == sk) mylz = lz[q];
+        if (!active || mylz < 0) continue;
+        const R* gsl = gs + (size_t)(slot * kNodeMaxNk + sk) * L2;
+        for (int lx = 0; lx < L; lx++) {
+            const R* mrow = Ms + (size_t)(((sj * g.sj + lx * g.q) / b.dj) * b.np + slot * g.step + si) * 12;
+            for (int ly = 0; ly < L; ly++) {
+                const R gv = gsl[lx * L + ly];
+                R mv[12];
+                load12(mv, mrow + 12 * ly * g.q);
+                G += gv;
+#pragma unroll
+                for (int ch = 0; ch < 12; ch++) A[ch] += gv * mv[ch];
+            }
+        }
+    }
+    if (!active) return;
+    const long long fidx = 12 * ((long long)(z * g.step + sk * g.sk) * mn + (long long)(x * g.step + sj * g.sj) * g.m +
+                                 (long long)y * g.step + si);
+    R fv[12];
+    load12(fv, F + fidx);
+#pragma unroll
+    for (int ch = 0; ch < 12; ch++) dF[fidx + ch] = (R)(2.0 * g.scale) * (fv[ch] * G - A[ch]);
+}
+
 template <typename R>
 __global__ void k_fill(R* p, R v, long long n) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// ---- per-node dispatch ----------------------------------------------------------
+
+inline int gcd_int(int a, int b) { return b == 0 ? a : gcd_int(b, a % b); }
+
+constexpr int kNodeY = 2;  // nodes per workgroup along i
+
+inline BoxGeom box_geom(const CostGeom& g) {
+    BoxGeom b{};
+    b.dk = gcd_int(g.sk, g.q);
+    b.dj = gcd_int(g.sj, g.q);
+    b.nw = ((g.nk - 1) * g.sk + (g.len - 1) * g.q) / b.dk + 1;
+    b.nv = ((g.nj - 1) * g.sj + (g.len - 1) * g.q) / b.dj + 1;
+    b.ny = g.m1 < kNodeY ? (g.m1 > 0 ? g.m1 : 1) : kNodeY;
+    b.np = b.ny * g.step + (g.len - 1) * g.q;
+    b.nyb = (g.m1 + b.ny - 1) / b.ny;
+    b.per = (b.nyb * g.n1 * g.o1 + 7) / 8;
+    return b;
+}
+
+// LDS bytes of the per-node forms, or 0 when the shape has none
+// (samples skipped along i, more than kNodeMaxNk sample planes, more threads
+// than a workgroup holds, or more LDS than 64 KB, which would leave a CU
+// fewer than two workgroups).
+template <typename R>
+size_t node_smem_cost(const CostGeom& g, const BoxGeom& b) {
+    if (g.si != 1 || g.nk > kNodeMaxNk) return 0;
+    const int tpn = (g.len * g.len + 63) / 64 * 64;
+    if (tpn * b.ny > 1024) return 0;
+    const size_t bytes = ((size_t)b.nv * b.np * 12 + (size_t)b.ny * g.len * g.len * g.len) * sizeof(R);
+    return bytes <= 64 * 1024 ? bytes : 0;
+}
+template <typename R>
+size_t node_smem_dF(const CostGeom& g, const BoxGeom& b) {
+    if (g.si != 1 || g.nk > kNodeMaxNk) return 0;
+    const int tpn = (g.ns + 63) / 64 * 64;
+    if (tpn * b.ny > 1024) return 0;
+    const size_t bytes = ((size_t)b.nv * b.np * 12 + (size_t)b.ny * kNodeMaxNk * g.len * g.len) * sizeof(R);
+    return bytes <= 64 * 1024 ? bytes : 0;
+}
+
+// ---- row-form dispatch ---------------------------------------------------------
+
+// (STEP, LEN, Q) shapes with compiled row kernels; other shapes run the
+// per-label kernels above.
+#define DEEDS_SOFT_ROW_SHAPES(X) X(6, 13, 2) X(6, 5, 2) X(4, 5, 2) X(5, 3, 2)
+
+// i-blocks of the dM row kernel: blocks Y = ymin .. ymax cover [-P, m + P)
+struct DMBlocks {
+    int nb, y0;
+};
+inline DMBlocks dM_blocks(const CostGeom& g) {
+    const int P = g.q * g.hw;
+    const int ymin = floor_div(-P, g.step), ymax = floor_div(g.m - 1 + P, g.step);
+    return DMBlocks{ymax - ymin + 1, -ymin};
+}
+
+inline unsigned grid_of(long long threads) {
+    const long long b = (threads + CB - 1) / CB;
+    if (b > 0x7fffffffLL) fail(DEEDS_ERR_ARG, "soft cost volume too large");
+    return (unsigned)b;
+}
+
+template <typename R, int STEP, int LEN, int Q>
+struct RowForms {
+    static void dM(R* dM, R* part, const R* gcost, const R* F, const R* M, const CostGeom& g, hipStream_t s) {
+        const DMBlocks b = dM_blocks(g);
+        const int P = g.q * g.hw;
+        const long long threads = (long long)b.nb * (g.n + 2 * P) * (g.o + 2 * P);
+        k_sc_dM_rows<R, STEP, LEN, Q><<<grid_of(threads), CB, 0, s>>>(dM, part, gcost, F, M, g, b.nb, b.y0);
+        DEEDS_LAUNCH_CHECK();
+    }
+};
+
+// Calls Op<R, STEP, LEN, Q>::f(args...) for a shape with row kernels; false otherwise.
+template <typename R, typename Fn>
+bool with_rows(const CostGeom& g, Fn&& fn) {
+    if (g.si != 1) return false;
+#define DEEDS_ROW_CASE(S, L, QQ)                          \
+    if (g.step == S && g.len == L && g.q == QQ) {          \
+        fn(RowForms<R, S, L, QQ>{});                       \
+        return true;                                       \
+    }
+    DEEDS_SOFT_ROW_SHAPES(DEEDS_ROW_CASE)
+#undef DEEDS_ROW_CASE
+    return false;
 }
 
 }  // namespace
@@ -330,6 +696,14 @@ void soft_cost(R* cost, const R* F, const R* M, int m, int n, int o, int step, i
     const long long nodes = (long long)g.m1 * g.n1 * g.o1;
     const int L3 = g.len * g.len * g.len;
     if (nodes == 0) return;
+    const BoxGeom b = box_geom(g);
+    if (const size_t smem = node_smem_cost<R>(g, b)) {
+        const int tpn = (g.len * g.len + 63) / 64 * 64;
+        set_smem((const void*)k_sc_cost_node<R>, smem);
+        k_sc_cost_node<R><<<8 * b.per, tpn * b.ny, smem, s>>>(cost, F, M, g, b, tpn);
+        DEEDS_LAUNCH_CHECK();
+        return;
+    }
     const long long blocks = nodes * ((L3 + CB - 1) / CB);
     if (blocks > 0x7fffffffLL) fail(DEEDS_ERR_ARG, "soft cost volume too large");
     const size_t smem = (size_t)g.ns * 12 * sizeof(R);
@@ -347,18 +721,31 @@ void soft_cost_backward(R* dF, R* dM, const R* gcost, const R* F, const R* M, in
     k_fill<R><<<ceil_div(12 * sz, 256), 256, 0, s>>>(dF, (R)0, 12 * sz);
     DEEDS_LAUNCH_CHECK();
     const int nodes = g.m1 * g.n1 * g.o1;
-    if (nodes > 0) {
+    const BoxGeom b = box_geom(g);
+    const size_t smem_dF = nodes > 0 ? node_smem_dF<R>(g, b) : 0;
+    bool dF_done = false;
+    if (smem_dF) {
+        const int tpn = (g.ns + 63) / 64 * 64;
+        set_smem((const void*)k_sc_dF_node<R>, smem_dF);
+        k_sc_dF_node<R><<<8 * b.per, tpn * b.ny, smem_dF, s>>>(dF, gcost, F, M, g, b, tpn);
+        DEEDS_LAUNCH_CHECK();
+        dF_done = true;
+    }
+    if (nodes > 0 && !dF_done) {
         const size_t L3 = (size_t)g.len * g.len * g.len;
         const size_t smem = (((L3 + 1) & ~(size_t)1) + (size_t)(CB / g.ns) * g.ns * 13) * sizeof(R);
         set_smem((const void*)k_sc_dF<R>, smem);
         k_sc_dF<R><<<nodes, CB, smem, s>>>(dF, gcost, F, M, g);
         DEEDS_LAUNCH_CHECK();
     }
-    // pull over the padded box (interior voxels finish there), then fold the pad region onto the faces
     const long long P = (long long)g.q * g.hw;
-    const long long padded = (m + 2 * P) * (n + 2 * P) * (o + 2 * P);
-    k_sc_dM_pull<R><<<ceil_div(padded, CB), CB, 0, s>>>(dM, part, gcost, F, M, g);
-    DEEDS_LAUNCH_CHECK();
+    if (!(nodes > 0 && with_rows<R>(g, [&](auto rf) { rf.dM(dM, part, gcost, F, M, g, s); }))) {
+        // pull over the padded box (interior voxels finish there)
+        const long long padded = (m + 2 * P) * (n + 2 * P) * (o + 2 * P);
+        k_sc_dM_pull<R><<<ceil_div(padded, CB), CB, 0, s>>>(dM, part, gcost, F, M, g);
+        DEEDS_LAUNCH_CHECK();
+    }
+    // fold the pad region onto the faces
     const long long mi = m > 2 ? m - 2 : 0, ni = n > 2 ? n - 2 : 0;
     const long long faces = 2LL * n * o + 2 * mi * o + 2 * mi * ni;
     k_sc_dM_fold<R><<<ceil_div(faces, CB), CB, 0, s>>>(dM, part, M, g);
