@@ -37,7 +37,22 @@ namespace deeds {
 // Edge weights below this are raised to it before dividing the pairwise term.
 constexpr float kSoftMinEdgeWeight = 1e-6f;
 
-// A tree in BFS order with its depth schedule on the host and its arrays on the device.
+// Rounds of one sweep direction: round r runs segments round_start[r] ..
+// round_start[r+1]-1, one workgroup each, in one launch; segment s is the
+// node list d_nodes[d_seg_start[s] .. d_seg_start[s+1]), each node the parent
+// (upward) or a child (downward) of the one before it, so a workgroup keeps
+// the message it just computed on chip for the next node.
+struct SoftRounds {
+    std::vector<int> round_start;
+    const int* d_seg_start = nullptr;
+    const int* d_nodes = nullptr;
+    int rounds() const { return round_start.empty() ? 0 : (int)round_start.size() - 1; }
+};
+
+// Steps per round of the soft sweeps (one launch each).
+constexpr int kSoftWindow = 16;
+
+// A tree in BFS order with its sweep schedules on the host and its arrays on the device.
 struct SoftTree {
     int N = 0, maxd = 0;
     std::vector<int> depth_start;  // host: maxd + 2 offsets into ordered
@@ -46,6 +61,8 @@ struct SoftTree {
     const int* d_child_start = nullptr;  // N + 1
     const int* d_child_list = nullptr;   // N - 1
     const float* d_edgemst = nullptr;
+    SoftRounds up;    // children before parents, every node (the root last)
+    SoftRounds down;  // parents before children, every node but the root
 };
 
 // Workgroup geometry of the soft BP for L labels per axis: each line of the

@@ -1,10 +1,13 @@
 // Soft variant, stage 3: sum-product belief propagation on the tree and its
 // adjoint (the model is stated in soft.h).
 //
-// Forward sweeps, one launch per BFS depth, one workgroup per node:
-//   upward   (deepest depth first)  h_c = C_c + sum of the children's messages,
+// Each sweep runs in windowed segment rounds (SoftRounds): one launch per
+// round, one workgroup per segment, the segment's nodes in order with the
+// message of the node before kept on chip for the next one.
+// Forward sweeps:
+//   upward   (children first)       h_c = C_c + sum of the children's messages,
 //            m_c = DTup_c(h_c) - min
-//   downward (root side first)      g_c = h_c + DTdown_c(g_p - m_c) - min
+//   downward (parents first)        g_c = h_c + DTdown_c(g_p - m_c) - min
 //   marginals P_c = softmax(-g_c / T), u1_c = u0_c + E_{P_c}[displacement]
 // DT is three log-sum-exp passes, one per label axis, against the quadratic
 // pairwise table of the edge (the soft counterpart of messageDT,
@@ -14,19 +17,31 @@
 // index arithmetic or LDS traffic beyond one read of the line.
 //
 // Backward sweeps: the marginals' own terms of A_c = d/dg_c for all nodes in
-// one launch; then deepest first, A_c += the adjoints Dt_k of the children's
-// downward transforms and Dt_c = DTdown_c^T(A_c); then root side first,
+// one launch; then children first, A_c += the adjoints Dt_k of the children's
+// downward transforms and Dt_c = DTdown_c^T(A_c); then parents first,
 // dH_c = A_c + DTup_c^T(dH_p - Dt_c), which is d/dC_c.  Each transform is recomputed from the saved beliefs in LDS
 // (all four pass buffers kept) and differentiated pass by pass in reverse.
 // The minimum subtracted from each message only shifts beliefs by constants
 // the marginals ignore, so it has no adjoint term.
 #include "soft_common.h"
 
+// The soft BP reproduces no reference rounding: let a*b+c contract to FMA here.
+#pragma clang fp contract(fast)
+
 namespace deeds {
 
 namespace {
 
 using namespace soft_dev;
+
+// The tree arrays and schedules are written before a sweep and only read in
+// it; read through the constant address space, their uniform loads are scalar.
+using cint = const int __attribute__((address_space(4)));
+__device__ __forceinline__ cint* ro(const int* p) { return (cint*)p; }
+template <typename R>
+__device__ __forceinline__ const R __attribute__((address_space(4)))* ro(const R* p) {
+    return (const R __attribute__((address_space(4)))*)p;
+}
 
 template <typename R, int LEN>
 struct SS {
@@ -37,6 +52,7 @@ struct SS {
     static constexpr int PAD = S * OUT;  // table entries beyond both ends read by the last windows
     static constexpr int ZT = 2 * LEN + 1 + 2 * PAD;
     static constexpr size_t SMEM = soft_bp_smem(LEN, sizeof(R));
+    static constexpr int KL = (L3 + BLOCK - 1) / BLOCK;  // label-cube entries per thread
 };
 
 // Label cubes, the quadratic tables z of an edge per axis, their exponential
@@ -106,7 +122,7 @@ __device__ __forceinline__ R lse_pass(const R* __restrict__ in, R* __restrict__ 
 #pragma unroll
     for (int j = 0; j < LEN; j++) {
         v[j] = in[base + j * G::STRIDE];
-        mi = v[j] < mi ? v[j] : mi;
+        mi = rmin(v[j], mi);
     }
     R e[LEN];
 #pragma unroll
@@ -130,7 +146,7 @@ __device__ __forceinline__ R lse_pass(const R* __restrict__ in, R* __restrict__ 
 #pragma unroll
                 for (int j = 0; j < LEN; j++) {
                     c[j] = v[j] + zt[i0 + r - j + LEN];
-                    mn = c[j] < mn ? c[j] : mn;
+                    mn = rmin(c[j], mn);
                 }
                 R s2 = 0;
 #pragma unroll
@@ -138,7 +154,7 @@ __device__ __forceinline__ R lse_pass(const R* __restrict__ in, R* __restrict__ 
                 o = mn - Tln2 * FastMath<R>::log2(s2);
             }
             out[base + (i0 + r) * G::STRIDE] = o;
-            lo = o < lo ? o : lo;
+            lo = rmin(o, lo);
         }
     }
     return lo;
@@ -163,7 +179,7 @@ __device__ __forceinline__ void lse_adj(const R* __restrict__ in, const R* __res
         go[a] = gout[base + a * G::STRIDE];
         ou[a] = out[base + a * G::STRIDE];
         const R x = in[base + a * G::STRIDE];
-        mi = x < mi ? x : mi;
+        mi = rmin(x, mi);
     }
     R w[LEN];
     bool any_big = false;
@@ -239,11 +255,12 @@ struct BPGrad {
 template <typename R, int LEN>
 __device__ void edge_tables(const BPState<R>& a, int c, int p, R sign, const Cubes<R, LEN>& s) {
     using H = SS<R, LEN>;
-    const float wf = a.t.edgemst[c];
+    const float wf = ro(a.t.edgemst)[c];
     const R w = (R)(wf > kSoftMinEdgeWeight ? wf : kSoftMinEdgeWeight);
-    const R ou = sign * (a.u0[p] - a.u0[c]) / a.quant;
-    const R ov = sign * (a.u0[a.N + p] - a.u0[a.N + c]) / a.quant;
-    const R ow = sign * (a.u0[2 * a.N + p] - a.u0[2 * a.N + c]) / a.quant;
+    const auto u0 = ro(a.u0);
+    const R ou = sign * (u0[p] - u0[c]) / a.quant;
+    const R ov = sign * (u0[a.N + p] - u0[a.N + c]) / a.quant;
+    const R ow = sign * (u0[2 * a.N + p] - u0[2 * a.N + c]) / a.quant;
     // zmin: the table's minimum over the label differences -(LEN-1) .. LEN-1,
     // at the integer next to -off on either side
     R zm[3];
@@ -301,42 +318,83 @@ __device__ void transform_adj(const Cubes<R, LEN>& s, R T) {
     __syncthreads();
 }
 
+// Schedule of one launch: segment blockIdx.x of the round that starts at seg0.
+struct SegArgs {
+    const int* seg_start;
+    const int* nodes;
+    int seg0;
+};
+
+// Upward: h_c = C_c + the children's messages (in child order; the child
+// just before in the segment from X3 - its minimum, as stored), m_c = DTup(h_c) - min.
 template <typename R, int LEN>
-__global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_up(BPState<R> a, int pos0) {
+__global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_up(BPState<R> a, SegArgs g) {
     using H = SS<R, LEN>;
     extern __shared__ __align__(16) unsigned char sp_smem[];
     const Cubes<R, LEN> s(reinterpret_cast<R*>(sp_smem));
-    const int c = a.t.ordered[pos0 + blockIdx.x];
-    const long long cb = (long long)c * H::L3;
-    const int c0 = a.t.child_start[c], c1 = a.t.child_start[c + 1];
-    for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) {
-        R h = a.cost[cb + l];
-        for (int ci = c0; ci < c1; ci++) h += a.Msg[(long long)a.t.child_list[ci] * H::L3 + l];
-        a.H[cb + l] = h;
-        s.X0[l] = h;
+    cint* nd = ro(g.nodes);
+    cint* cs = ro(a.t.child_start);
+    cint* cl = ro(a.t.child_list);
+    const int i0 = ro(g.seg_start)[g.seg0 + blockIdx.x], i1 = ro(g.seg_start)[g.seg0 + blockIdx.x + 1];
+    int prev = -1;
+    R pmn = 0;
+    for (int i = i0; i < i1; i++) {
+        const int c = nd[i];
+        const long long cb = (long long)c * H::L3;
+        const int c0 = cs[c], c1 = cs[c + 1];
+        for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) {
+            R h = a.cost[cb + l];
+            for (int ci = c0; ci < c1; ci++) {
+                const int k = cl[ci];
+                h += k == prev ? s.X3[l] - pmn : a.Msg[(long long)k * H::L3 + l];
+            }
+            a.H[cb + l] = h;
+            s.X0[l] = h;
+        }
+        const int p = ro(a.t.parents)[c];
+        if (p < 0) {  // the root's full belief is its upward belief
+            for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) a.G[cb + l] = s.X0[l];
+            return;
+        }
+        edge_tables<R, LEN>(a, c, p, (R)1, s);
+        pmn = transform<R, LEN>(s, a.T);
+        for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) a.Msg[cb + l] = s.X3[l] - pmn;
+        prev = c;
     }
-    const int p = a.t.parents[c];
-    if (p < 0) {  // the root's full belief is its upward belief
-        for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) a.G[cb + l] = s.X0[l];
-        return;
-    }
-    edge_tables<R, LEN>(a, c, p, (R)1, s);
-    const R mn = transform<R, LEN>(s, a.T);
-    for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) a.Msg[cb + l] = s.X3[l] - mn;
 }
 
+// Downward: g_c = h_c + DTdown(g_p - m_c) - min, g_p of the node just before
+// in the segment from registers.
 template <typename R, int LEN>
-__global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_down(BPState<R> a, int pos0) {
+__global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_down(BPState<R> a, SegArgs g) {
     using H = SS<R, LEN>;
     extern __shared__ __align__(16) unsigned char sp_smem[];
     const Cubes<R, LEN> s(reinterpret_cast<R*>(sp_smem));
-    const int c = a.t.ordered[pos0 + blockIdx.x];
-    const int p = a.t.parents[c];
-    const long long cb = (long long)c * H::L3, pb = (long long)p * H::L3;
-    for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) s.X0[l] = a.G[pb + l] - a.Msg[cb + l];
-    edge_tables<R, LEN>(a, c, p, (R)-1, s);
-    const R mn = transform<R, LEN>(s, a.T);
-    for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) a.G[cb + l] = a.H[cb + l] + (s.X3[l] - mn);
+    cint* nd = ro(g.nodes);
+    const int i0 = ro(g.seg_start)[g.seg0 + blockIdx.x], i1 = ro(g.seg_start)[g.seg0 + blockIdx.x + 1];
+    int prev = -1;
+    R gk[H::KL];
+    for (int i = i0; i < i1; i++) {
+        const int c = nd[i];
+        const int p = ro(a.t.parents)[c];
+        const long long cb = (long long)c * H::L3, pb = (long long)p * H::L3;
+#pragma unroll
+        for (int k = 0; k < H::KL; k++) {
+            const int l = threadIdx.x + k * H::BLOCK;
+            if (l < H::L3) s.X0[l] = (p == prev ? gk[k] : a.G[pb + l]) - a.Msg[cb + l];
+        }
+        edge_tables<R, LEN>(a, c, p, (R)-1, s);
+        const R mn = transform<R, LEN>(s, a.T);
+#pragma unroll
+        for (int k = 0; k < H::KL; k++) {
+            const int l = threadIdx.x + k * H::BLOCK;
+            if (l < H::L3) {
+                gk[k] = a.H[cb + l] + (s.X3[l] - mn);
+                a.G[cb + l] = gk[k];
+            }
+        }
+        prev = c;
+    }
 }
 
 // Marginal of node c into P (softmax of -g_c / T); e = expected displacement (u, v, w).
@@ -348,7 +406,7 @@ __device__ void marginal(const BPState<R>& a, int c, R* P, R* red, R e[3]) {
     for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) {
         const R g = a.G[cb + l];
         P[l] = g;
-        lo = g < lo ? g : lo;
+        lo = rmin(g, lo);
     }
     const R mn = block_reduce<R>(lo, red, true);
     const R iT2 = (R)kLog2e / a.T;
@@ -410,53 +468,83 @@ __global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_bwd_marg(BPState<R> 
     }
 }
 
-// Deepest depth first: A_c = own term + the children's Dt (in child order),
-// then Dt_c = DTdown_c^T(A_c).
+// Children first: A_c = own term + the children's Dt (in child order; the
+// child just before in the segment from Gb), then Dt_c = DTdown_c^T(A_c).  At
+// the root, dH = A.
 template <typename R, int LEN>
-__global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_bwd_down(BPState<R> a, BPGrad<R> b, int pos0) {
+__global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_bwd_down(BPState<R> a, BPGrad<R> b, SegArgs g) {
     using H = SS<R, LEN>;
     extern __shared__ __align__(16) unsigned char sp_smem[];
     const Cubes<R, LEN> s(reinterpret_cast<R*>(sp_smem));
-    const int c = a.t.ordered[pos0 + blockIdx.x];
-    const long long cb = (long long)c * H::L3;
-    const int c0 = a.t.child_start[c], c1 = a.t.child_start[c + 1];
-    for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) {
-        R g = b.A[cb + l];
-        for (int ci = c0; ci < c1; ci++) g += b.Dt[(long long)a.t.child_list[ci] * H::L3 + l];
-        b.A[cb + l] = g;
-        s.Ga[l] = g;
+    cint* nd = ro(g.nodes);
+    cint* cs = ro(a.t.child_start);
+    cint* cl = ro(a.t.child_list);
+    const int i0 = ro(g.seg_start)[g.seg0 + blockIdx.x], i1 = ro(g.seg_start)[g.seg0 + blockIdx.x + 1];
+    int prev = -1;
+    for (int i = i0; i < i1; i++) {
+        const int c = nd[i];
+        const long long cb = (long long)c * H::L3;
+        const int c0 = cs[c], c1 = cs[c + 1];
+        for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) {
+            R v = b.A[cb + l];
+            for (int ci = c0; ci < c1; ci++) {
+                const int k = cl[ci];
+                v += k == prev ? s.Gb[l] : b.Dt[(long long)k * H::L3 + l];
+            }
+            b.A[cb + l] = v;
+            s.Ga[l] = v;
+        }
+        const int p = ro(a.t.parents)[c];
+        if (p < 0) {
+            for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) b.dH[cb + l] = s.Ga[l];
+            return;
+        }
+        const long long pb = (long long)p * H::L3;
+        for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) s.X0[l] = a.G[pb + l] - a.Msg[cb + l];
+        edge_tables<R, LEN>(a, c, p, (R)-1, s);
+        transform<R, LEN>(s, a.T);
+        transform_adj<R, LEN>(s, a.T);
+        for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) b.Dt[cb + l] = s.Gb[l];
+        prev = c;
     }
-    const int p = a.t.parents[c];
-    if (p < 0) return;
-    const long long pb = (long long)p * H::L3;
-    for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) s.X0[l] = a.G[pb + l] - a.Msg[cb + l];
-    edge_tables<R, LEN>(a, c, p, (R)-1, s);
-    transform<R, LEN>(s, a.T);
-    transform_adj<R, LEN>(s, a.T);
-    for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) b.Dt[cb + l] = s.Gb[l];
 }
 
+// Parents first: dH_c = A_c + DTup_c^T(dH_p - Dt_c), dH_p of the node just
+// before in the segment from registers.
 template <typename R, int LEN>
-__global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_bwd_up(BPState<R> a, BPGrad<R> b, int pos0) {
+__global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_bwd_up(BPState<R> a, BPGrad<R> b, SegArgs g) {
     using H = SS<R, LEN>;
     extern __shared__ __align__(16) unsigned char sp_smem[];
     const Cubes<R, LEN> s(reinterpret_cast<R*>(sp_smem));
-    const int c = a.t.ordered[pos0 + blockIdx.x];
-    const long long cb = (long long)c * H::L3;
-    const int p = a.t.parents[c];
-    if (p < 0) {
-        for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) b.dH[cb + l] = b.A[cb + l];
-        return;
+    cint* nd = ro(g.nodes);
+    const int i0 = ro(g.seg_start)[g.seg0 + blockIdx.x], i1 = ro(g.seg_start)[g.seg0 + blockIdx.x + 1];
+    int prev = -1;
+    R hk[H::KL];
+    for (int i = i0; i < i1; i++) {
+        const int c = nd[i];
+        const int p = ro(a.t.parents)[c];
+        const long long cb = (long long)c * H::L3, pb = (long long)p * H::L3;
+#pragma unroll
+        for (int k = 0; k < H::KL; k++) {
+            const int l = threadIdx.x + k * H::BLOCK;
+            if (l < H::L3) {
+                s.X0[l] = a.H[cb + l];
+                s.Ga[l] = (p == prev ? hk[k] : b.dH[pb + l]) - b.Dt[cb + l];
+            }
+        }
+        edge_tables<R, LEN>(a, c, p, (R)1, s);
+        transform<R, LEN>(s, a.T);
+        transform_adj<R, LEN>(s, a.T);
+#pragma unroll
+        for (int k = 0; k < H::KL; k++) {
+            const int l = threadIdx.x + k * H::BLOCK;
+            if (l < H::L3) {
+                hk[k] = b.A[cb + l] + s.Gb[l];
+                b.dH[cb + l] = hk[k];
+            }
+        }
+        prev = c;
     }
-    const long long pb = (long long)p * H::L3;
-    for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) {
-        s.X0[l] = a.H[cb + l];
-        s.Ga[l] = b.dH[pb + l] - b.Dt[cb + l];
-    }
-    edge_tables<R, LEN>(a, c, p, (R)1, s);
-    transform<R, LEN>(s, a.T);
-    transform_adj<R, LEN>(s, a.T);
-    for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) b.dH[cb + l] = b.A[cb + l] + s.Gb[l];
 }
 
 template <typename R>
@@ -479,20 +567,22 @@ BPState<R> bp_state(const SoftTree& t, const R* cost, const R* u0, R* u1, R* sav
 template <typename R, int LEN>
 struct Sweeps {
     using H = SS<R, LEN>;
+    // one launch per round of r, the kernel's arguments followed by the round's SegArgs
+    template <typename... Args>
+    static void rounds(const SoftRounds& r, void (*kernel)(Args..., SegArgs), hipStream_t s, const Args&... args) {
+        for (int k = 0; k < r.rounds(); k++) {
+            const SegArgs g{r.d_seg_start, r.d_nodes, r.round_start[k]};
+            const unsigned n = (unsigned)(r.round_start[k + 1] - r.round_start[k]);
+            kernel<<<n, H::BLOCK, H::SMEM, s>>>(args..., g);
+            DEEDS_LAUNCH_CHECK();
+        }
+    }
     static void forward(const SoftTree& t, const BPState<R>& a, hipStream_t s) {
         set_smem((const void*)k_sp_up<R, LEN>, H::SMEM);
         set_smem((const void*)k_sp_down<R, LEN>, H::SMEM);
         set_smem((const void*)k_sp_marginals<R, LEN>, H::SMEM);
-        for (int d = t.maxd; d >= 0; d--) {
-            const int b = t.depth_start[d], n = t.depth_start[d + 1] - b;
-            k_sp_up<R, LEN><<<n, H::BLOCK, H::SMEM, s>>>(a, b);
-            DEEDS_LAUNCH_CHECK();
-        }
-        for (int d = 1; d <= t.maxd; d++) {
-            const int b = t.depth_start[d], n = t.depth_start[d + 1] - b;
-            k_sp_down<R, LEN><<<n, H::BLOCK, H::SMEM, s>>>(a, b);
-            DEEDS_LAUNCH_CHECK();
-        }
+        rounds<BPState<R>>(t.up, k_sp_up<R, LEN>, s, a);
+        rounds<BPState<R>>(t.down, k_sp_down<R, LEN>, s, a);
         k_sp_marginals<R, LEN><<<t.N, H::BLOCK, H::SMEM, s>>>(a);
         DEEDS_LAUNCH_CHECK();
     }
@@ -502,16 +592,8 @@ struct Sweeps {
         set_smem((const void*)k_sp_bwd_up<R, LEN>, H::SMEM);
         k_sp_bwd_marg<R, LEN><<<t.N, H::BLOCK, H::SMEM, s>>>(a, g);
         DEEDS_LAUNCH_CHECK();
-        for (int d = t.maxd; d >= 0; d--) {
-            const int b = t.depth_start[d], n = t.depth_start[d + 1] - b;
-            k_sp_bwd_down<R, LEN><<<n, H::BLOCK, H::SMEM, s>>>(a, g, b);
-            DEEDS_LAUNCH_CHECK();
-        }
-        for (int d = 0; d <= t.maxd; d++) {
-            const int b = t.depth_start[d], n = t.depth_start[d + 1] - b;
-            k_sp_bwd_up<R, LEN><<<n, H::BLOCK, H::SMEM, s>>>(a, g, b);
-            DEEDS_LAUNCH_CHECK();
-        }
+        rounds<BPState<R>, BPGrad<R>>(t.up, k_sp_bwd_down<R, LEN>, s, a, g);
+        rounds<BPState<R>, BPGrad<R>>(t.down, k_sp_bwd_up<R, LEN>, s, a, g);
     }
 };
 

@@ -39,6 +39,10 @@ __device__ __forceinline__ double big() { return 1.0e300; }
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// minimum as one instruction (v_min_f32 / v_min_f64)
+__device__ __forceinline__ float rmin(float a, float b) { return __builtin_fminf(a, b); }
+__device__ __forceinline__ double rmin(double a, double b) { return __builtin_fmin(a, b); }
+
 // Minimum (is_min) or sum over the workgroup; every thread gets the result.
 // red: one slot per wavefront (blockDim.x / 64 <= 16).
 template <typename R>
@@ -46,14 +50,14 @@ __device__ R block_reduce(R v, R* red, bool is_min) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         const R o = __shfl_xor(v, off, 64);
-        v = is_min ? (o < v ? o : v) : v + o;
+        v = is_min ? rmin(o, v) : v + o;
     }
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     __syncthreads();
     if (lane == 0) red[wid] = v;
     __syncthreads();
     R r = red[0];
-    for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = is_min ? (red[w] < r ? red[w] : r) : r + red[w];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = is_min ? rmin(red[w], r) : r + red[w];
     __syncthreads();
     return r;
 }
