@@ -94,20 +94,25 @@ void soft_features_backward(R* gim, const R* gphi, const R* im, int m, int n, in
 template <typename R>
 void soft_cost(R* cost, const R* F, const R* M, int m, int n, int o, int step, int hw, float quant, float alpha,
                hipStream_t s);
-// part: soft_cost_part_count(...) reals of workspace (sums of the padded positions).
+// part: soft_cost_part_count(...) reals of workspace (sums of the padded positions);
+// gT: soft_cost_grad_t_count(...) reals (the cost gradient, label-major).
 template <typename R>
 void soft_cost_backward(R* dF, R* dM, const R* gcost, const R* F, const R* M, int m, int n, int o, int step, int hw,
-                        float quant, float alpha, R* part, hipStream_t s);
+                        float quant, float alpha, R* part, R* gT, hipStream_t s);
 inline size_t soft_cost_part_count(int m, int n, int o, int hw, float quant) {
     const size_t P = (size_t)quant * hw;
     return 13 * (m + 2 * P) * (n + 2 * P) * (o + 2 * P);
+}
+inline size_t soft_cost_grad_t_count(int m, int n, int o, int step, int hw) {
+    const size_t L = 2 * (size_t)hw + 1, N = (size_t)(m / step) * (n / step) * (o / step);
+    return L * L * L * N;
 }
 
 // saved: 3*N*L^3 reals (upward beliefs, upward messages, full beliefs).
 template <typename R>
 void soft_bp_forward(const SoftTree& t, const R* cost, const R* u0, R* u1, R* saved, int hw, float quant, float T,
                      hipStream_t s);
-// scratch: 2*N*L^3 reals.
+// scratch: 9*N*L^3 reals (A, Dt and the transforms recomputed for the backward steps).
 template <typename R>
 void soft_bp_backward(const SoftTree& t, const R* cost, const R* u0, const R* saved, const R* gu1, R* gcost,
                       R* scratch, int hw, float quant, float T, hipStream_t s);

@@ -396,16 +396,19 @@ int deeds_ctx_soft_cost_backward(deeds_ctx* ctx, int real, void* d_gfixed, void*
         const size_t L = 2 * (size_t)hw + 1;
         DEEDS_REQUIRE(L * L * L * rsize(real) <= (size_t)kMaxLds, "label radius too large for the soft cost backward");
         const size_t part = soft_cost_part_count(m, n, o, hw, quant);
+        const size_t gt = soft_cost_grad_t_count(m, n, o, step, hw);
         if (real == DEEDS_REAL_F32)
             soft_cost_backward<float>(static_cast<float*>(d_gfixed), static_cast<float*>(d_gmoving),
                                       static_cast<const float*>(d_gcost), static_cast<const float*>(d_fixed_phi),
                                       static_cast<const float*>(d_moving_phi), m, n, o, step, hw, quant, alpha,
-                                      ctx->get<float>("soft.cost.part", part), ctx->stream);
+                                      ctx->get<float>("soft.cost.part", part), ctx->get<float>("soft.cost.gt", gt),
+                                      ctx->stream);
         else
             soft_cost_backward<double>(static_cast<double*>(d_gfixed), static_cast<double*>(d_gmoving),
                                        static_cast<const double*>(d_gcost), static_cast<const double*>(d_fixed_phi),
                                        static_cast<const double*>(d_moving_phi), m, n, o, step, hw, quant, alpha,
-                                       ctx->get<double>("soft.cost.part", part), ctx->stream);
+                                       ctx->get<double>("soft.cost.part", part), ctx->get<double>("soft.cost.gt", gt),
+                                       ctx->stream);
     });
 }
 
@@ -441,7 +444,7 @@ int deeds_ctx_soft_bp_backward(deeds_ctx* ctx, int real, void* d_gcost, const vo
         const int N = (m / step) * (n / step) * (o / step);
         const size_t L = 2 * (size_t)hw + 1;
         const SoftTree& st = stage_soft_tree(ctx, ordered, parents, edgemst, N);
-        const size_t scratch = 2 * (size_t)N * L * L * L;
+        const size_t scratch = 9 * (size_t)N * L * L * L;
         if (real == DEEDS_REAL_F32)
             soft_bp_backward<float>(st, static_cast<const float*>(d_cost), static_cast<const float*>(d_u0),
                                     static_cast<const float*>(d_saved), static_cast<const float*>(d_gu1),

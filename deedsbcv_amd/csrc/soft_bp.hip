@@ -17,10 +17,13 @@
 // index arithmetic or LDS traffic beyond one read of the line.
 //
 // Backward sweeps: the marginals' own terms of A_c = d/dg_c for all nodes in
-// one launch; then children first, A_c += the adjoints Dt_k of the children's
-// downward transforms and Dt_c = DTdown_c^T(A_c); then parents first,
-// dH_c = A_c + DTup_c^T(dH_p - Dt_c), which is d/dC_c.  Each transform is recomputed from the saved beliefs in LDS
-// (all four pass buffers kept) and differentiated pass by pass in reverse.
+// one launch, and in one more launch every node's upward and downward
+// transforms recomputed from the saved beliefs and messages (all four pass
+// buffers kept); then children first, A_c += the adjoints Dt_k of the
+// children's downward transforms and Dt_c = DTdown_c^T(A_c); then parents
+// first, dH_c = A_c + DTup_c^T(dH_p - Dt_c), which is d/dC_c.  A step of the
+// sweeps reads its node's recomputed passes and differentiates them pass by
+// pass in reverse.
 // The minimum subtracted from each message only shifts beliefs by constants
 // the marginals ignore, so it has no adjoint term.
 #include "soft_common.h"
@@ -247,6 +250,13 @@ struct BPGrad {
     R* A;          // total d/dg_c
     R* Dt;         // d/d(input of the downward transform of c)
     R* dH;         // d/dh_c == d/dC_c
+    R* RX;         // N x 7 x L3: recomputed passes X1..X3 of the upward and X0..X3 of the downward transform
+};
+
+// Offsets in RX of a node's recomputed passes.
+template <int L3>
+struct RXAt {
+    static constexpr int UP = 0, DOWN = 3 * L3, STRIDE = 7 * L3;
 };
 
 // Quadratic tables of the edge c -> p: z[a - b] = (a - b + off)^2 / w per axis
@@ -303,6 +313,19 @@ __device__ R transform(const Cubes<R, LEN>& s, R T) {
     __syncthreads();
     const R lo = lse_pass<R, LEN, 2>(s.X2, s.X3, s.zz, s.kz, s.red[18], iT2, Tln2);
     return block_reduce<R>(lo, s.red, true);
+}
+
+// X0 -> X1 -> X2 -> X3 with the current tables, without the minimum.
+template <typename R, int LEN>
+__device__ void passes(const Cubes<R, LEN>& s, R T) {
+    const R iT2 = (R)kLog2e / T, Tln2 = T * (R)kLn2;
+    __syncthreads();
+    lse_pass<R, LEN, 0>(s.X0, s.X1, s.zy, s.ky, s.red[16], iT2, Tln2);
+    __syncthreads();
+    lse_pass<R, LEN, 1>(s.X1, s.X2, s.zx, s.kx, s.red[17], iT2, Tln2);
+    __syncthreads();
+    lse_pass<R, LEN, 2>(s.X2, s.X3, s.zz, s.kz, s.red[18], iT2, Tln2);
+    __syncthreads();
 }
 
 // d/dX3 in Ga -> d/dX0 in Gb (uses Ga as scratch).
@@ -468,12 +491,48 @@ __global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_bwd_marg(BPState<R> 
     }
 }
 
+// The forward transforms the backward steps differentiate, recomputed for
+// every node at once (their inputs are the saved beliefs and messages, so
+// they do not wait on the sequential sweeps): workgroup 2c + 1 recomputes
+// node c's downward transform (input g_p - m_c, stored with its passes),
+// workgroup 2c its upward one (input h_c, passes stored).
+template <typename R, int LEN>
+__global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_bwd_passes(BPState<R> a, BPGrad<R> b) {
+    using H = SS<R, LEN>;
+    using X = RXAt<H::L3>;
+    extern __shared__ __align__(16) unsigned char sp_smem[];
+    const Cubes<R, LEN> s(reinterpret_cast<R*>(sp_smem));
+    const int c = blockIdx.x >> 1, down = blockIdx.x & 1;
+    const int p = ro(a.t.parents)[c];
+    if (p < 0) return;
+    const long long cb = (long long)c * H::L3, pb = (long long)p * H::L3;
+    R* rx = b.RX + (long long)c * X::STRIDE + (down ? X::DOWN : X::UP);
+    for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) {
+        if (down) {
+            const R x = a.G[pb + l] - a.Msg[cb + l];
+            s.X0[l] = x;
+            rx[l] = x;
+        } else {
+            s.X0[l] = a.H[cb + l];
+        }
+    }
+    edge_tables<R, LEN>(a, c, p, down ? (R)-1 : (R)1, s);
+    passes<R, LEN>(s, a.T);
+    R* o = rx + (down ? H::L3 : 0);
+    for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) {
+        o[l] = s.X1[l];
+        o[H::L3 + l] = s.X2[l];
+        o[2 * H::L3 + l] = s.X3[l];
+    }
+}
+
 // Children first: A_c = own term + the children's Dt (in child order; the
 // child just before in the segment from Gb), then Dt_c = DTdown_c^T(A_c).  At
 // the root, dH = A.
 template <typename R, int LEN>
 __global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_bwd_down(BPState<R> a, BPGrad<R> b, SegArgs g) {
     using H = SS<R, LEN>;
+    using X = RXAt<H::L3>;
     extern __shared__ __align__(16) unsigned char sp_smem[];
     const Cubes<R, LEN> s(reinterpret_cast<R*>(sp_smem));
     cint* nd = ro(g.nodes);
@@ -485,24 +544,28 @@ __global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_bwd_down(BPState<R> 
         const int c = nd[i];
         const long long cb = (long long)c * H::L3;
         const int c0 = cs[c], c1 = cs[c + 1];
+        const int p = ro(a.t.parents)[c];
+        const R* rx = b.RX + (long long)c * X::STRIDE + X::DOWN;
         for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) {
             R v = b.A[cb + l];
             for (int ci = c0; ci < c1; ci++) {
                 const int k = cl[ci];
                 v += k == prev ? s.Gb[l] : b.Dt[(long long)k * H::L3 + l];
             }
+            if (p >= 0) {  // the recomputed downward transform of c
+                s.X0[l] = rx[l];
+                s.X1[l] = rx[H::L3 + l];
+                s.X2[l] = rx[2 * H::L3 + l];
+                s.X3[l] = rx[3 * H::L3 + l];
+            }
             b.A[cb + l] = v;
             s.Ga[l] = v;
         }
-        const int p = ro(a.t.parents)[c];
         if (p < 0) {
             for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) b.dH[cb + l] = s.Ga[l];
             return;
         }
-        const long long pb = (long long)p * H::L3;
-        for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) s.X0[l] = a.G[pb + l] - a.Msg[cb + l];
         edge_tables<R, LEN>(a, c, p, (R)-1, s);
-        transform<R, LEN>(s, a.T);
         transform_adj<R, LEN>(s, a.T);
         for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) b.Dt[cb + l] = s.Gb[l];
         prev = c;
@@ -514,6 +577,7 @@ __global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_bwd_down(BPState<R> 
 template <typename R, int LEN>
 __global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_bwd_up(BPState<R> a, BPGrad<R> b, SegArgs g) {
     using H = SS<R, LEN>;
+    using X = RXAt<H::L3>;
     extern __shared__ __align__(16) unsigned char sp_smem[];
     const Cubes<R, LEN> s(reinterpret_cast<R*>(sp_smem));
     cint* nd = ro(g.nodes);
@@ -524,16 +588,19 @@ __global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_bwd_up(BPState<R> a,
         const int c = nd[i];
         const int p = ro(a.t.parents)[c];
         const long long cb = (long long)c * H::L3, pb = (long long)p * H::L3;
+        const R* rx = b.RX + (long long)c * X::STRIDE + X::UP;  // the recomputed upward transform of c
 #pragma unroll
         for (int k = 0; k < H::KL; k++) {
             const int l = threadIdx.x + k * H::BLOCK;
             if (l < H::L3) {
                 s.X0[l] = a.H[cb + l];
+                s.X1[l] = rx[l];
+                s.X2[l] = rx[H::L3 + l];
+                s.X3[l] = rx[2 * H::L3 + l];
                 s.Ga[l] = (p == prev ? hk[k] : b.dH[pb + l]) - b.Dt[cb + l];
             }
         }
         edge_tables<R, LEN>(a, c, p, (R)1, s);
-        transform<R, LEN>(s, a.T);
         transform_adj<R, LEN>(s, a.T);
 #pragma unroll
         for (int k = 0; k < H::KL; k++) {
@@ -590,7 +657,10 @@ struct Sweeps {
         set_smem((const void*)k_sp_bwd_marg<R, LEN>, H::SMEM);
         set_smem((const void*)k_sp_bwd_down<R, LEN>, H::SMEM);
         set_smem((const void*)k_sp_bwd_up<R, LEN>, H::SMEM);
+        set_smem((const void*)k_sp_bwd_passes<R, LEN>, H::SMEM);
         k_sp_bwd_marg<R, LEN><<<t.N, H::BLOCK, H::SMEM, s>>>(a, g);
+        DEEDS_LAUNCH_CHECK();
+        k_sp_bwd_passes<R, LEN><<<2 * t.N, H::BLOCK, H::SMEM, s>>>(a, g);
         DEEDS_LAUNCH_CHECK();
         rounds<BPState<R>, BPGrad<R>>(t.up, k_sp_bwd_down<R, LEN>, s, a, g);
         rounds<BPState<R>, BPGrad<R>>(t.down, k_sp_bwd_up<R, LEN>, s, a, g);
@@ -638,7 +708,8 @@ void soft_bp_backward(const SoftTree& t, const R* cost, const R* u0, const R* sa
                       R* scratch, int hw, float quant, float T, hipStream_t s) {
     const BPState<R> a = bp_state<R>(t, cost, u0, nullptr, const_cast<R*>(saved), hw, quant, T);
     const size_t L = 2 * (size_t)hw + 1;
-    const BPGrad<R> g{gu1, scratch, scratch + (size_t)t.N * L * L * L, gcost};
+    const size_t NL3 = (size_t)t.N * L * L * L;
+    const BPGrad<R> g{gu1, scratch, scratch + NL3, gcost, scratch + 2 * NL3};
     dispatch_len<R, RunBackward>(2 * hw + 1, t, a, g, s);
 }
 

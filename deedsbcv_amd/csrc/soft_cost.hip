@@ -206,11 +206,12 @@ struct DMAcc {
 #pragma unroll
         for (int ch = 0; ch < 12; ch++) f[ch] += w * fv[ch];
     }
-    __device__ __forceinline__ void store(R* __restrict__ dM, const R* __restrict__ M, long long t, double scale) {
+    // dM[t] = k (M_t g - f)
+    __device__ __forceinline__ void store(R* __restrict__ dM, const R* __restrict__ M, long long t, R k) {
         R mv[12];
         load12(mv, M + 12 * t);
 #pragma unroll
-        for (int ch = 0; ch < 12; ch++) dM[12 * t + ch] = (R)(2.0 * scale) * (mv[ch] * g - f[ch]);
+        for (int ch = 0; ch < 12; ch++) dM[12 * t + ch] = k * (mv[ch] * g - f[ch]);
     }
 };
 
@@ -219,8 +220,8 @@ struct DMAcc {
 // position P each label has at most one sample per axis, s = P - d_l, walked
 // without divisions.  Voxels strictly inside the volume receive exactly the
 // pairs at their own position and write dM; positions on or beyond a face
-// write their sums (g, g*F) to `part` for k_sc_dM_fold, since the clamping
-// sends all of them to the face voxel they clamp to.
+// write their sums (g, g*F), times 2 a1, to `part` for the folds, since
+// the clamping sends all of them to the face voxel they clamp to.
 template <typename R>
 __global__ __launch_bounds__(CB) void k_sc_dM_pull(R* __restrict__ dM, R* __restrict__ part, const R* __restrict__ gc,
                                                    const R* __restrict__ F, const R* __restrict__ M, CostGeom g) {
@@ -250,22 +251,67 @@ __global__ __launch_bounds__(CB) void k_sc_dM_pull(R* __restrict__ dM, R* __rest
     }
     const int ti = ip - P, tj = jp - P, tk = kp - P;
     if (ti >= 1 && ti <= g.m - 2 && tj >= 1 && tj <= g.n - 2 && tk >= 1 && tk <= g.o - 2) {
-        acc.store(dM, M, ti + (long long)tj * g.m + tk * mn, g.scale);
+        acc.store(dM, M, ti + (long long)tj * g.m + tk * mn, (R)(2.0 * g.scale));
     } else {
         R* pp = part + 13 * e;
+        const R k = (R)(2.0 * g.scale);
 #pragma unroll
-        for (int ch = 0; ch < 12; ch++) pp[ch] = acc.f[ch];
-        pp[12] = acc.g;
+        for (int ch = 0; ch < 12; ch++) pp[ch] = k * acc.f[ch];
+        pp[12] = k * acc.g;
     }
 }
 
-// dM of the face voxels: the sums of every padded position that clamps to
-// the voxel (a line, square or cube of the pad region beyond the faces).
-// Faces are enumerated without overlap: both i-faces whole, then the
-// j-faces and k-faces without the voxels already covered.
+// dM of the face voxels: the sums (already times 2 a1) of every padded
+// position that clamps to the voxel (a line, square or cube of the pad
+// region beyond the faces).  The clamp acts on each axis alone, so the pad
+// is folded one axis at a time, in place: along i for every row, then along
+// j for the positions inside [0, m) along i, then along k inside [0, m) x
+// [0, n); each fold adds a pad's sums onto the face slot next to it (one
+// thread per channel and line: both sides in order, so a one-voxel axis
+// folds both pads into its single slot).  Face voxels then read their slot.
+template <typename R, int AX>
+__global__ __launch_bounds__(CB) void k_sc_dM_fold(R* __restrict__ part, CostGeom g) {
+    const int P = g.q * g.hw;
+    const int mp = g.m + 2 * P, np = g.n + 2 * P, op = g.o + 2 * P;
+    // lines: AX 0 over (jp, kp) of the padded box, AX 1 over (i, kp), AX 2 over (i, j) inside the volume
+    const int w0 = AX == 0 ? np : g.m;
+    const int w1 = AX == 0 ? op : (AX == 1 ? op : g.n);
+    const long long e = (long long)blockIdx.x * CB + threadIdx.x;
+    if (e >= 13LL * w0 * w1) return;
+    const int ch = (int)(e % 13);
+    const long long r = e / 13;
+    const int a = (int)(r % w0), b = (int)(r / w0);
+    long long base, stride;
+    int len;
+    if (AX == 0) {
+        base = (long long)a * mp + (long long)b * mp * np;
+        stride = 1;
+        len = g.m;
+    } else if (AX == 1) {
+        base = (a + P) + (long long)b * mp * np;
+        stride = mp;
+        len = g.n;
+    } else {
+        base = (a + P) + (long long)(b + P) * mp;
+        stride = (long long)mp * np;
+        len = g.o;
+    }
+    R* q = part + 13 * base + ch;
+    R lo = 0, hi = 0;
+    for (int t = 0; t < P; t++) {
+        lo += q[13 * stride * t];
+        hi += q[13 * stride * (len + P + t)];
+    }
+    q[13 * stride * P] += lo;
+    q[13 * stride * (len - 1 + P)] += hi;
+}
+
+// Face voxels enumerated without overlap (both i-faces whole, then the
+// j-faces and k-faces without the voxels already covered) write dM from
+// their folded slot.
 template <typename R>
-__global__ __launch_bounds__(CB) void k_sc_dM_fold(R* __restrict__ dM, const R* __restrict__ part,
-                                                   const R* __restrict__ M, CostGeom g) {
+__global__ __launch_bounds__(CB) void k_sc_dM_faces(R* __restrict__ dM, const R* __restrict__ part,
+                                                    const R* __restrict__ M, CostGeom g) {
     const long long e0 = (long long)blockIdx.x * CB + threadIdx.x;
     const long long mi = g.m > 2 ? g.m - 2 : 0, ni = g.n > 2 ? g.n - 2 : 0;
     const long long ci = 2LL * g.n * g.o, cj = 2 * mi * g.o, ck = 2 * mi * ni;
@@ -297,115 +343,300 @@ __global__ __launch_bounds__(CB) void k_sc_dM_fold(R* __restrict__ dM, const R* 
     }
     const int P = g.q * g.hw;
     const int mp = g.m + 2 * P, np = g.n + 2 * P;
-    // padded index ranges that clamp to (ti, tj, tk)
-    const int i0 = ti == 0 ? 0 : ti + P, i1 = ti == g.m - 1 ? g.m - 1 + 2 * P : ti + P;
-    const int j0 = tj == 0 ? 0 : tj + P, j1 = tj == g.n - 1 ? g.n - 1 + 2 * P : tj + P;
-    const int k0 = tk == 0 ? 0 : tk + P, k1 = tk == g.o - 1 ? g.o - 1 + 2 * P : tk + P;
+    const R* pp = part + 13 * ((ti + P) + (long long)(tj + P) * mp + (long long)(tk + P) * mp * np);
     DMAcc<R> acc;
-    for (int kp = k0; kp <= k1; kp++)
-        for (int jp = j0; jp <= j1; jp++)
-            for (int ip = i0; ip <= i1; ip++) {
-                const R* pp = part + 13 * (ip + (long long)jp * mp + (long long)kp * mp * np);
 #pragma unroll
-                for (int ch = 0; ch < 12; ch++) acc.f[ch] += pp[ch];
-                acc.g += pp[12];
-            }
-    acc.store(dM, M, ti + (long long)tj * g.m + (long long)tk * g.m * g.n, g.scale);
+    for (int ch = 0; ch < 12; ch++) acc.f[ch] = pp[ch];
+    acc.g = pp[12];
+    acc.store(dM, M, ti + (long long)tj * g.m + (long long)tk * g.m * g.n, (R)1);
 }
 
-// ---- row form of dM ---------------------------------------------------------------
-// With every voxel of a block sampled along i (skip_i 1), the fixed voxel
-// u = T + du of a node and label ly pair with the moving voxel u + (ly - HW) Q
-// of a row, so a thread that owns STEP consecutive moving voxels loads each
-// fixed voxel of its window once and uses it for every label on it.  Threads
-// run i-blocks fastest: the lanes of a wavefront share the row (jp, kp).
+// ---- row forms of the adjoint ----------------------------------------------------
+// With every voxel of a block sampled along i (skip_i 1), both adjoints are
+// correlations along i of a source row with the nodes' cost-gradient slices:
+//   dM: output t (padded box) gathers g[n(s), l] F[s] over s = t - d_l;
+//   dF: output s (a sample) gathers g[n(s), l] M[clamp(s + d_l)].
+// A workgroup is one wavefront that owns 64 consecutive positions along i,
+// one per lane, and J output rows along j that lie Q apart, so that one
+// source row pairs with every output row of the group, each through its own
+// label lx.  With output row jj and source row sr (both Q apart, the source
+// rows starting HW*Q before the first output row) the label is lx = jj - sr
+// + 2 HW for dM and lx = sr - jj for dF, valid for jj in [sr - 2 HW, sr].
+// Per step (lz, source row) the wavefront holds in LDS the source row (zero
+// beyond the volume for dM, clamped for dF) and the gradient slab
+// g[jj][ly][node slot] of the nodes under its window (zero where a row has
+// no label for the source row or a slot no node); each lane then adds, per
+// label ly, one source voxel (lanes 48 bytes apart: no bank conflicts) into
+// its J accumulators.  The next step's row and slab are loaded into
+// registers while the current step computes (the gradient is read from its
+// label-major copy gT[l][node]: the slots of one (jj, ly) are consecutive).
 
 constexpr int floor_div(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
 
-// dM: one thread per i-block of STEP voxels [T, T + STEP), T = Y * STEP, of a
-// row (jp, kp) of the zero-padded box.  For each (lz, lx) whose sample row
-// s = t - d lies on the sample lattice, the fixed voxels u = T + du of the
-// nodes Y + dy along i pair with the outputs T + du + (ly - HW) Q of the block;
-// those pairs are known at compile time.  Outputs are written as k_sc_dM_pull
-// writes them: interior voxels to dM, the rest as sums to `part`.
-template <typename R, int STEP, int LEN, int Q>
-__global__ __launch_bounds__(CB) void k_sc_dM_rows(R* __restrict__ dM, R* __restrict__ part,
-                                                   const R* __restrict__ gc, const R* __restrict__ F,
-                                                   const R* __restrict__ M, CostGeom g, int nb, int y0) {
-    constexpr int HW = (LEN - 1) / 2, PI = HW * Q, L3 = LEN * LEN * LEN;
-    constexpr int DY0 = floor_div(-PI, STEP), DY1 = floor_div(STEP - 1 + PI, STEP);
-    const int mp = g.m + 2 * PI, np = g.n + 2 * PI, op = g.o + 2 * PI;
-    const long long e = (long long)blockIdx.x * CB + threadIdx.x;
-    if (e >= (long long)nb * np * op) return;
-    const int Y = (int)(e % nb) - y0;
-    const long long rr = e / nb;
-    const int jp = (int)(rr % np), kp = (int)(rr / np);
-    const int T = Y * STEP, tj = jp - PI, tk = kp - PI;
-    const long long mn = (long long)g.m * g.n;
-    R acc[STEP][12], G[STEP];
+constexpr int kRowJ = 2;    // output rows per workgroup of the row forms
+constexpr int kRowNT = 64;  // lanes per workgroup of the row forms
+
+// gT: the cost gradient gc [N][L3] as [L3][N]: row l, column node.
+// 64 x 64 tiles through LDS.
+template <typename R>
+__global__ __launch_bounds__(256) void k_sc_grad_t(R* __restrict__ gT, const R* __restrict__ gc, int N, int L3) {
+    __shared__ R tile[64][65];
+    const int l0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
 #pragma unroll
-    for (int t = 0; t < STEP; t++) {
-        G[t] = 0;
-#pragma unroll
-        for (int ch = 0; ch < 12; ch++) acc[t][ch] = 0;
+    for (int k = 0; k < 16; k++) {
+        const int r = ty + 4 * k, node = n0 + r, l = l0 + tx;
+        tile[r][tx] = node < N && l < L3 ? gc[(long long)node * L3 + l] : (R)0;
     }
-    for (int lz = 0; lz < LEN; lz++) {
-        const int s_k = tk - (lz - HW) * Q;
-        if (s_k < 0 || s_k >= g.o1 * STEP) continue;
-        const int z = s_k / STEP;
-        if ((s_k - z * STEP) % g.sk) continue;
-        for (int lx = 0; lx < LEN; lx++) {
-            const int s_j = tj - (lx - HW) * Q;
-            if (s_j < 0 || s_j >= g.n1 * STEP) continue;
-            const int x = s_j / STEP;
-            if ((s_j - x * STEP) % g.sj) continue;
-            const long long nrow = (long long)(z * g.n1 + x) * g.m1;
-            const R* frow = F + 12 * ((long long)s_k * mn + (long long)s_j * g.m + T);
-            const int lb = (lz * LEN + lx) * LEN;
+    __syncthreads();
 #pragma unroll
-            for (int dy = DY0; dy <= DY1; dy++) {
-                const int y = Y + dy;
-                if (y < 0 || y >= g.m1) continue;
-                const R* gl = gc + (nrow + y) * L3 + lb;
-                R gv[LEN];
+    for (int k = 0; k < 16; k++) {
+        const int r = ty + 4 * k, l = l0 + r, node = n0 + tx;
+        if (l < L3 && node < N) gT[(long long)l * N + node] = tile[tx][r];
+    }
+}
+
+// The adjoint sums of one lane: J rows of twelve channels and the gradient sum.
+template <typename R, int J>
+struct RowSums {
+    R acc[J][12], G[J];
+    __device__ RowSums() {
 #pragma unroll
-                for (int ly = 0; ly < LEN; ly++) gv[ly] = gl[ly];
+        for (int jj = 0; jj < J; jj++) {
+            G[jj] = 0;
 #pragma unroll
-                for (int si = 0; si < STEP; si++) {
-                    const int du = dy * STEP + si;
-                    if (du + (LEN - 1 - HW) * Q < 0 || du - HW * Q >= STEP) continue;  // no output in the block
-                    R fv[12];
-                    load12(fv, frow + 12 * du);
-#pragma unroll
-                    for (int ly = 0; ly < LEN; ly++) {
-                        const int to = du + (ly - HW) * Q;
-                        if (to >= 0 && to < STEP) {
-                            G[to] += gv[ly];
-#pragma unroll
-                            for (int ch = 0; ch < 12; ch++) acc[to][ch] += gv[ly] * fv[ch];
-                        }
-                    }
-                }
-            }
+            for (int ch = 0; ch < 12; ch++) acc[jj][ch] = 0;
         }
     }
-    const bool inner_jk = tj >= 1 && tj <= g.n - 2 && tk >= 1 && tk <= g.o - 2;
+    __device__ __forceinline__ void add(int jj, R w, const R (&fv)[12]) {
+        G[jj] += w;
 #pragma unroll
-    for (int t = 0; t < STEP; t++) {
-        const int ti = T + t;
-        if (ti < -PI || ti >= g.m + PI) continue;
-        if (inner_jk && ti >= 1 && ti <= g.m - 2) {
+        for (int ch = 0; ch < 12; ch++) acc[jj][ch] += w * fv[ch];
+    }
+};
+
+// One step's data of a row-form wavefront: the source row ((kRowNT + 2P)
+// voxels of twelve channels) and the gradient slab [J][LEN][NS], loaded
+// into registers (load) and then written to LDS (store).
+template <typename R, int P, int J, int LEN, int NS>
+struct RowStage {
+    static constexpr int V = 12 * (int)sizeof(R) / 16;  // 16-byte chunks per voxel
+    static constexpr int NF = (kRowNT + 2 * P) * V, PF = (NF + kRowNT - 1) / kRowNT;
+    static constexpr int NG = J * LEN * NS, PG = (NG + kRowNT - 1) / kRowNT;
+    static constexpr size_t ROW_BYTES = (size_t)NF * 16;
+    uint4 f[PF];
+    R w[PG];
+    // row: source row index of the [o][n][m][12] volume; p0: its first position;
+    // zero positions outside [0, m) (ZERO) or clamp them; slab(jj, ly, slot)
+    // returns the gradient (a read of gT or zero)
+    template <bool ZERO, typename Slab>
+    __device__ __forceinline__ void load(const R* __restrict__ src, long long row, int m, int p0, Slab slab) {
+        const int lane = threadIdx.x;
+#pragma unroll
+        for (int k = 0; k < PF; k++) {
+            const int e = lane + k * kRowNT, v = e / V, c = e - v * V;
+            int p = p0 + v;
+            f[k] = make_uint4(0u, 0u, 0u, 0u);
+            if (e < NF && (!ZERO || (p >= 0 && p < m))) {
+                p = clampi(p, 0, m - 1);
+                f[k] = reinterpret_cast<const uint4*>(src + 12 * (row * m + p))[c];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < PG; k++) {
+            const int e = lane + k * kRowNT, slot = e % NS, r = e / NS, ly = r % LEN, jj = r / LEN;
+            w[k] = e < NG ? slab(jj, ly, slot) : (R)0;
+        }
+    }
+    __device__ __forceinline__ void store(unsigned char* lds) const {
+        const int lane = threadIdx.x;
+#pragma unroll
+        for (int k = 0; k < PF; k++) {
+            const int e = lane + k * kRowNT;
+            if (e < NF) reinterpret_cast<uint4*>(lds)[e] = f[k];
+        }
+        R* gs = reinterpret_cast<R*>(lds + ROW_BYTES);
+#pragma unroll
+        for (int k = 0; k < PG; k++) {
+            const int e = lane + k * kRowNT;
+            if (e < NG) gs[e] = w[k];
+        }
+    }
+};
+
+// dM: workgroup (i tile, row group, kp); output rows tj = tjb + Q jj of the
+// padded box [-P, n + P), tk = kp - P.  Interior voxels write dM, positions
+// on or beyond a face their sums, times 2 a1, to `part` for the folds
+// (every padded position is written exactly once).
+template <typename R, int STEP, int LEN, int Q, int J>
+__global__ __launch_bounds__(kRowNT) void k_sc_dM_rows(R* __restrict__ dM, R* __restrict__ part,
+                                                       const R* __restrict__ gT, const R* __restrict__ F,
+                                                       const R* __restrict__ M, CostGeom g, int itiles, int groups) {
+    constexpr int HW = (LEN - 1) / 2, P = HW * Q, L2 = LEN * LEN, SR = J + 2 * HW, STEPS = LEN * SR;
+    constexpr int NS = (kRowNT + 2 * P) / STEP + 2;  // node slots under the source window
+    using Stage = RowStage<R, P, J, LEN, NS>;
+    __shared__ __align__(16) unsigned char lds[Stage::ROW_BYTES + Stage::NG * sizeof(R)];
+    const R* Fs = reinterpret_cast<const R*>(lds);
+    const R* gs = reinterpret_cast<const R*>(lds + Stage::ROW_BYTES);
+    int b = blockIdx.x;
+    const int it = b % itiles;
+    b /= itiles;
+    const int grp = b % groups, kp = b / groups;
+    const int tk = kp - P, tjb = -P + grp % Q + Q * J * (grp / Q);
+    const int t_lo = -P + it * kRowNT, lane = threadIdx.x, ti = t_lo + lane;
+    const int w0 = t_lo - P, nlo = floor_div(w0, STEP), off0 = w0 - nlo * STEP;  // window [w0, w0 + 64 + 2P)
+    const int N = g.m1 * g.n1 * g.o1;
+    // steps (lz, source row) with a sample plane and a sample row
+    auto valid = [&](int pr) {
+        const int lz = pr / SR, sr = pr - lz * SR;
+        const int s_k = tk - Q * (lz - HW), s_j = tjb - P + Q * sr;
+        return s_k >= 0 && s_k < g.o1 * STEP && (s_k % STEP) % g.sk == 0 && s_j >= 0 && s_j < g.n1 * STEP &&
+               (s_j % STEP) % g.sj == 0;
+    };
+    auto next = [&](int pr) {
+        while (pr < STEPS && !valid(pr)) pr++;
+        return pr;
+    };
+    auto fetch = [&](Stage& st, int pr) {
+        const int lz = pr / SR, sr = pr - lz * SR;
+        const int s_k = tk - Q * (lz - HW), s_j = tjb - P + Q * sr;
+        const int nrow = ((s_k / STEP) * g.n1 + s_j / STEP) * g.m1;
+        st.template load<true>(F, (long long)s_k * g.n + s_j, g.m, w0, [&](int jj, int ly, int slot) {
+            const int lx = jj - sr + 2 * HW, node_i = nlo + slot;
+            return lx >= 0 && lx < LEN && node_i >= 0 && node_i < g.m1
+                       ? gT[(long long)(lz * L2 + lx * LEN + ly) * N + nrow + node_i] : (R)0;
+        });
+    };
+    RowSums<R, J> S;
+    Stage st;
+    int pr = next(0);
+    if (pr < STEPS) fetch(st, pr);
+    while (pr < STEPS) {
+        __syncthreads();
+        st.store(lds);
+        __syncthreads();
+        const int nx = next(pr + 1);
+        if (nx < STEPS) fetch(st, nx);  // in flight while this step computes
+#pragma unroll
+        for (int ly = 0; ly < LEN; ly++) {
+            // source position ti + P - Q ly: window entry lane + 2P - Q ly, node slot below
+            const int e = lane + 2 * P - Q * ly;
+            R fv[12];
+            load12(fv, Fs + 12 * e);
+            const R* gl = gs + ly * NS + (e + off0) / STEP;
+#pragma unroll
+            for (int jj = 0; jj < J; jj++) S.add(jj, gl[jj * LEN * NS], fv);
+            __builtin_amdgcn_sched_barrier(0);  // one label's reads at a time (registers)
+        }
+        pr = nx;
+    }
+    if (ti >= g.m + P) return;
+    const int mp = g.m + 2 * P, np = g.n + 2 * P;
+    const long long mn = (long long)g.m * g.n;
+    const bool inner_ik = ti >= 1 && ti <= g.m - 2 && tk >= 1 && tk <= g.o - 2;
+    const R k2 = (R)(2.0 * g.scale);
+#pragma unroll
+    for (int jj = 0; jj < J; jj++) {
+        const int tj = tjb + Q * jj;
+        if (tj >= g.n + P) continue;
+        if (inner_ik && tj >= 1 && tj <= g.n - 2) {
             R mv[12];
             const long long v = ti + (long long)tj * g.m + tk * mn;
             load12(mv, M + 12 * v);
 #pragma unroll
-            for (int ch = 0; ch < 12; ch++) dM[12 * v + ch] = (R)(2.0 * g.scale) * (mv[ch] * G[t] - acc[t][ch]);
+            for (int ch = 0; ch < 12; ch++) dM[12 * v + ch] = k2 * (mv[ch] * S.G[jj] - S.acc[jj][ch]);
         } else {
-            R* pp = part + 13 * ((ti + PI) + (long long)jp * mp + (long long)kp * mp * np);
+            R* pp = part + 13 * ((ti + P) + (long long)(tj + P) * mp + (long long)kp * mp * np);
 #pragma unroll
-            for (int ch = 0; ch < 12; ch++) pp[ch] = acc[t][ch];
-            pp[12] = G[t];
+            for (int ch = 0; ch < 12; ch++) pp[ch] = k2 * S.acc[jj][ch];
+            pp[12] = k2 * S.G[jj];
         }
+    }
+}
+
+// dF: workgroup (i tile, row group, s_k) over the sampled positions
+// [0, m1 STEP) x [0, n1 STEP) x [0, o1 STEP); output rows s_j = sjb + Q jj
+// that are sample rows write dF = 2 a1 (F G - sum g M) (the rest of dF was
+// cleared before).  Lanes past the last sample write nothing.
+template <typename R, int STEP, int LEN, int Q, int J>
+__global__ __launch_bounds__(kRowNT) void k_sc_dF_rows(R* __restrict__ dF, const R* __restrict__ gT,
+                                                       const R* __restrict__ F, const R* __restrict__ M, CostGeom g,
+                                                       int itiles, int groups) {
+    constexpr int HW = (LEN - 1) / 2, P = HW * Q, L2 = LEN * LEN, SR = J + 2 * HW, STEPS = LEN * SR;
+    constexpr int NS = kRowNT / STEP + 2;  // node slots under the lanes
+    using Stage = RowStage<R, P, J, LEN, NS>;
+    __shared__ __align__(16) unsigned char lds[Stage::ROW_BYTES + Stage::NG * sizeof(R)];
+    const R* Ms = reinterpret_cast<const R*>(lds);
+    const R* gs = reinterpret_cast<const R*>(lds + Stage::ROW_BYTES);
+    int b = blockIdx.x;
+    const int it = b % itiles;
+    b /= itiles;
+    const int grp = b % groups, s_k = b / groups;
+    if ((s_k % STEP) % g.sk) return;  // not a sample plane
+    const int sjb = grp % Q + Q * J * (grp / Q);
+    unsigned rows = 0;  // bit jj: output row jj is a sample row
+#pragma unroll
+    for (int jj = 0; jj < J; jj++) {
+        const int s_j = sjb + Q * jj;
+        if (s_j < g.n1 * STEP && (s_j % STEP) % g.sj == 0) rows |= 1u << jj;
+    }
+    if (!rows) return;
+    const int s_lo = it * kRowNT, lane = threadIdx.x, s_i = s_lo + lane;
+    const int nlo = s_lo / STEP, slot = (lane + s_lo - nlo * STEP) / STEP;
+    const int N = g.m1 * g.n1 * g.o1;
+    const int zrow = (s_k / STEP) * g.n1;
+    // steps (lz, source row) that pair with a sample row
+    auto valid = [&](int pr) {
+        const int sr = pr % SR, jlo = max(0, sr - 2 * HW), jhi = min(J - 1, sr);
+        return ((rows >> jlo) & ((2u << (jhi - jlo)) - 1u)) != 0;
+    };
+    auto next = [&](int pr) {
+        while (pr < STEPS && !valid(pr)) pr++;
+        return pr;
+    };
+    auto fetch = [&](Stage& st, int pr) {
+        const int lz = pr / SR, sr = pr - lz * SR;
+        const int m_k = clampi(s_k + Q * (lz - HW), 0, g.o - 1), m_j = clampi(sjb - P + Q * sr, 0, g.n - 1);
+        st.template load<false>(M, (long long)m_k * g.n + m_j, g.m, s_lo - P, [&](int jj, int ly, int sl) {
+            const int lx = sr - jj, node_i = nlo + sl;
+            const int nrow = (zrow + (sjb + Q * jj) / STEP) * g.m1;
+            return lx >= 0 && lx < LEN && ((rows >> jj) & 1u) && node_i < g.m1
+                       ? gT[(long long)(lz * L2 + lx * LEN + ly) * N + nrow + node_i] : (R)0;
+        });
+    };
+    RowSums<R, J> S;
+    Stage st;
+    int pr = next(0);
+    if (pr < STEPS) fetch(st, pr);
+    while (pr < STEPS) {
+        __syncthreads();
+        st.store(lds);
+        __syncthreads();
+        const int nx = next(pr + 1);
+        if (nx < STEPS) fetch(st, nx);  // in flight while this step computes
+#pragma unroll
+        for (int ly = 0; ly < LEN; ly++) {
+            // moving position s_i + Q (ly - HW): window entry lane + Q ly
+            R mv[12];
+            load12(mv, Ms + 12 * (lane + Q * ly));
+            const R* gl = gs + ly * NS + slot;
+#pragma unroll
+            for (int jj = 0; jj < J; jj++) S.add(jj, gl[jj * LEN * NS], mv);
+            __builtin_amdgcn_sched_barrier(0);  // one label's reads at a time (registers)
+        }
+        pr = nx;
+    }
+    if (s_i >= g.m1 * STEP) return;
+    const long long mn = (long long)g.m * g.n;
+    const R k2 = (R)(2.0 * g.scale);
+#pragma unroll
+    for (int jj = 0; jj < J; jj++) {
+        if (!((rows >> jj) & 1u)) continue;
+        const long long fidx = 12 * ((long long)s_k * mn + (long long)(sjb + Q * jj) * g.m + s_i);
+        R fv[12];
+        load12(fv, F + fidx);
+#pragma unroll
+        for (int ch = 0; ch < 12; ch++) dF[fidx + ch] = k2 * (fv[ch] * S.G[jj] - S.acc[jj][ch]);
     }
 }
 
@@ -644,29 +875,27 @@ size_t node_smem_dF(const CostGeom& g, const BoxGeom& b) {
 // per-label kernels above.
 #define DEEDS_SOFT_ROW_SHAPES(X) X(6, 13, 2) X(6, 5, 2) X(4, 5, 2) X(5, 3, 2)
 
-// i-blocks of the dM row kernel: blocks Y = ymin .. ymax cover [-P, m + P)
-struct DMBlocks {
-    int nb, y0;
-};
-inline DMBlocks dM_blocks(const CostGeom& g) {
-    const int P = g.q * g.hw;
-    const int ymin = floor_div(-P, g.step), ymax = floor_div(g.m - 1 + P, g.step);
-    return DMBlocks{ymax - ymin + 1, -ymin};
-}
-
-inline unsigned grid_of(long long threads) {
-    const long long b = (threads + CB - 1) / CB;
-    if (b > 0x7fffffffLL) fail(DEEDS_ERR_ARG, "soft cost volume too large");
-    return (unsigned)b;
+inline unsigned grid_of(long long blocks) {
+    if (blocks > 0x7fffffffLL) fail(DEEDS_ERR_ARG, "soft cost volume too large");
+    return (unsigned)blocks;
 }
 
 template <typename R, int STEP, int LEN, int Q>
 struct RowForms {
-    static void dM(R* dM, R* part, const R* gcost, const R* F, const R* M, const CostGeom& g, hipStream_t s) {
-        const DMBlocks b = dM_blocks(g);
-        const int P = g.q * g.hw;
-        const long long threads = (long long)b.nb * (g.n + 2 * P) * (g.o + 2 * P);
-        k_sc_dM_rows<R, STEP, LEN, Q><<<grid_of(threads), CB, 0, s>>>(dM, part, gcost, F, M, g, b.nb, b.y0);
+    static constexpr int P = (LEN - 1) / 2 * Q, J = kRowJ;
+    // row groups: per residue of the row index modulo Q, J rows Q apart
+    static int groups(int rows) { return Q * (((rows + Q - 1) / Q + J - 1) / J); }
+    static int tiles(int W) { return (W + kRowNT - 1) / kRowNT; }
+    static void dM(R* dM, R* part, const R* gT, const R* F, const R* M, const CostGeom& g, hipStream_t s) {
+        const int it = tiles(g.m + 2 * P), gr = groups(g.n + 2 * P);
+        k_sc_dM_rows<R, STEP, LEN, Q, J><<<grid_of((long long)it * gr * (g.o + 2 * P)), kRowNT, 0, s>>>(
+            dM, part, gT, F, M, g, it, gr);
+        DEEDS_LAUNCH_CHECK();
+    }
+    static void dF(R* dF, const R* gT, const R* F, const R* M, const CostGeom& g, hipStream_t s) {
+        const int it = tiles(g.m1 * STEP), gr = groups(g.n1 * STEP);
+        k_sc_dF_rows<R, STEP, LEN, Q, J><<<grid_of((long long)it * gr * (g.o1 * STEP)), kRowNT, 0, s>>>(
+            dF, gT, F, M, g, it, gr);
         DEEDS_LAUNCH_CHECK();
     }
 };
@@ -714,48 +943,62 @@ void soft_cost(R* cost, const R* F, const R* M, int m, int n, int o, int step, i
 
 template <typename R>
 void soft_cost_backward(R* dF, R* dM, const R* gcost, const R* F, const R* M, int m, int n, int o, int step, int hw,
-                        float quant, float alpha, R* part, hipStream_t s) {
+                        float quant, float alpha, R* part, R* gT, hipStream_t s) {
     const CostGeom g = cost_geom(m, n, o, step, hw, quant, alpha);
     const long long sz = (long long)m * n * o;
     // voxels outside every sampled block get no gradient from the cost
     k_fill<R><<<ceil_div(12 * sz, 256), 256, 0, s>>>(dF, (R)0, 12 * sz);
     DEEDS_LAUNCH_CHECK();
     const int nodes = g.m1 * g.n1 * g.o1;
-    const BoxGeom b = box_geom(g);
-    const size_t smem_dF = nodes > 0 ? node_smem_dF<R>(g, b) : 0;
-    bool dF_done = false;
-    if (smem_dF) {
-        const int tpn = (g.ns + 63) / 64 * 64;
-        set_smem((const void*)k_sc_dF_node<R>, smem_dF);
-        k_sc_dF_node<R><<<8 * b.per, tpn * b.ny, smem_dF, s>>>(dF, gcost, F, M, g, b, tpn);
+    // shapes with row kernels: both adjoints as row correlations over the
+    // label-major gradient
+    const bool rows = nodes > 0 && with_rows<R>(g, [&](auto rf) {
+        const int L3 = g.len * g.len * g.len;
+        k_sc_grad_t<R><<<dim3((L3 + 63) / 64, (nodes + 63) / 64), 256, 0, s>>>(gT, gcost, nodes, L3);
         DEEDS_LAUNCH_CHECK();
-        dF_done = true;
-    }
-    if (nodes > 0 && !dF_done) {
-        const size_t L3 = (size_t)g.len * g.len * g.len;
-        const size_t smem = (((L3 + 1) & ~(size_t)1) + (size_t)(CB / g.ns) * g.ns * 13) * sizeof(R);
-        set_smem((const void*)k_sc_dF<R>, smem);
-        k_sc_dF<R><<<nodes, CB, smem, s>>>(dF, gcost, F, M, g);
-        DEEDS_LAUNCH_CHECK();
-    }
-    const long long P = (long long)g.q * g.hw;
-    if (!(nodes > 0 && with_rows<R>(g, [&](auto rf) { rf.dM(dM, part, gcost, F, M, g, s); }))) {
+        rf.dF(dF, gT, F, M, g, s);
+        rf.dM(dM, part, gT, F, M, g, s);
+    });
+    if (!rows) {
+        const BoxGeom b = box_geom(g);
+        if (const size_t smem = nodes > 0 ? node_smem_dF<R>(g, b) : 0) {
+            const int tpn = (g.ns + 63) / 64 * 64;
+            set_smem((const void*)k_sc_dF_node<R>, smem);
+            k_sc_dF_node<R><<<8 * b.per, tpn * b.ny, smem, s>>>(dF, gcost, F, M, g, b, tpn);
+            DEEDS_LAUNCH_CHECK();
+        } else if (nodes > 0) {
+            const size_t L3 = (size_t)g.len * g.len * g.len;
+            const size_t smem = (((L3 + 1) & ~(size_t)1) + (size_t)(CB / g.ns) * g.ns * 13) * sizeof(R);
+            set_smem((const void*)k_sc_dF<R>, smem);
+            k_sc_dF<R><<<nodes, CB, smem, s>>>(dF, gcost, F, M, g);
+            DEEDS_LAUNCH_CHECK();
+        }
         // pull over the padded box (interior voxels finish there)
+        const long long P = (long long)g.q * g.hw;
         const long long padded = (m + 2 * P) * (n + 2 * P) * (o + 2 * P);
         k_sc_dM_pull<R><<<ceil_div(padded, CB), CB, 0, s>>>(dM, part, gcost, F, M, g);
         DEEDS_LAUNCH_CHECK();
     }
-    // fold the pad region onto the faces
+    // fold the pad region onto the faces, one axis at a time, then the face voxels
+    {
+        const long long P = (long long)g.q * g.hw;
+        k_sc_dM_fold<R, 0><<<ceil_div(13 * (n + 2 * P) * (o + 2 * P), CB), CB, 0, s>>>(part, g);
+        DEEDS_LAUNCH_CHECK();
+        k_sc_dM_fold<R, 1><<<ceil_div(13LL * m * (o + 2 * P), CB), CB, 0, s>>>(part, g);
+        DEEDS_LAUNCH_CHECK();
+        k_sc_dM_fold<R, 2><<<ceil_div(13LL * m * n, CB), CB, 0, s>>>(part, g);
+        DEEDS_LAUNCH_CHECK();
+    }
     const long long mi = m > 2 ? m - 2 : 0, ni = n > 2 ? n - 2 : 0;
     const long long faces = 2LL * n * o + 2 * mi * o + 2 * mi * ni;
-    k_sc_dM_fold<R><<<ceil_div(faces, CB), CB, 0, s>>>(dM, part, M, g);
+    k_sc_dM_faces<R><<<ceil_div(faces, CB), CB, 0, s>>>(dM, part, M, g);
     DEEDS_LAUNCH_CHECK();
 }
 
 #define DEEDS_SOFT_COST(R)                                                                                     \
     template void soft_cost<R>(R*, const R*, const R*, int, int, int, int, int, float, float, hipStream_t);         \
     template void soft_cost_backward<R>(R*, R*, const R*, const R*, const R*, int, int, int, int, int, float, float, \
-                                        R*, hipStream_t);
+                                        R*, R*, hipStream_t);
 
 DEEDS_SOFT_COST(float)
 DEEDS_SOFT_COST(double)

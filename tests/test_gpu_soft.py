@@ -72,6 +72,31 @@ def test_cost_matches_dense():
         assert _rel(got32, want) < 1e-5
 
 
+# (o, n, m), step, hw, quant: every shape with row kernels for the adjoint
+# (step 6 / radius 6 / quantisation 2 is the soft level's), one with two
+# i-tiles (m + 2 hw q above 256 lanes), and one that takes the per-node and
+# pull kernels instead.
+@pytest.mark.parametrize("dims,step,hw,q", [((26, 27, 28), 6, 6, 2), ((20, 19, 21), 6, 2, 2), ((13, 14, 17), 4, 2, 2),
+                                            ((16, 17, 18), 5, 1, 2), ((10, 9, 262), 4, 2, 2), ((15, 16, 17), 4, 2, 1)])
+def test_cost_adjoint_matches_dense_autograd(dims, step, hw, q):
+    print(f"case: soft cost adjoint {dims} step {step} hw {hw} q {q}", flush=True)
+    rng = np.random.default_rng(11)
+    o, n, m = dims
+    F = rng.random((o, n, m, 12))
+    M = rng.random((o, n, m, 12))
+    N = (m // step) * (n // step) * (o // step)
+    gC = rng.normal(size=(N, (2 * hw + 1) ** 3))
+    Fd = torch.from_numpy(F).requires_grad_(True)
+    Md = torch.from_numpy(M).requires_grad_(True)
+    sd.cost(Fd, Md, step, hw, q).backward(torch.from_numpy(gC))
+    for dtype, tol in [(F64, 1e-11), (torch.float32, 2e-5)]:
+        Fg = _cuda(F, dtype).requires_grad_(True)
+        Mg = _cuda(M, dtype).requires_grad_(True)
+        soft.soft_cost(Fg, Mg, step, hw, q).backward(_cuda(gC, dtype))
+        assert _rel(Fg.grad.cpu(), Fd.grad) < tol, dtype
+        assert _rel(Mg.grad.cpu(), Md.grad) < tol, dtype
+
+
 @pytest.mark.parametrize("tree", [([-1, 0, 1], [0, 1, 2]), ([1, -1, 1, 1], [1, 0, 2, 3])])
 def test_bp_matches_enumeration(tree):
     print(f"case: soft BP enumeration {tree}", flush=True)
