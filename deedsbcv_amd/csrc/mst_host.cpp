@@ -618,4 +618,189 @@ void build_schedule(Tree& t, bool bp_programs, int window) {
 // leaf_w, sel_info, sel_jump: 5N.
 size_t tree_device_ints(int N) { return (2 * kProgRec + 10) * (size_t)N + kProgStage + 16; }
 
+// The soft sweeps' schedule (declared in mst_host.h with its rules).
+void soft_schedule(const int* ordered, const int* parents, const float* edgemst, int N, int W, SoftSchedule& out) {
+    // ---- validate the given order (positions p of `ordered`) ----
+    std::vector<int> pos(N, -1);
+    for (int p = 0; p < N; p++) {
+        const int v = ordered[p];
+        if (v < 0 || v >= N) throw std::runtime_error("ordered holds an out-of-range node");
+        if (pos[v] >= 0) throw std::runtime_error("tree: `ordered` is not a permutation");
+        pos[v] = p;
+    }
+    std::vector<int> pp(N, -1), dp(N, 0);  // parent position, depth
+    for (int p = 1; p < N; p++) {
+        const int par = parents[ordered[p]];
+        if (par < 0 || par >= N) throw std::runtime_error("tree: node without a valid parent");
+        const int q = pos[par];
+        if (q >= p) throw std::runtime_error("tree: a parent does not precede its child in `ordered`");
+        pp[p] = q;
+        dp[p] = dp[q] + 1;
+        if (dp[p] < dp[p - 1]) throw std::runtime_error("tree: `ordered` is not sorted by depth");
+    }
+    // ---- centre: heights (best and second-best child height + 1) and upward paths ----
+    std::vector<int> b1(N, 0), b2(N, 0), a1(N, -1), upd(N, 0);
+    for (int p = N - 1; p >= 1; p--) {
+        const int q = pp[p], v = b1[p] + 1;
+        if (v > b1[q]) {
+            b2[q] = b1[q];
+            b1[q] = v;
+            a1[q] = p;
+        } else if (v > b2[q]) {
+            b2[q] = v;
+        }
+    }
+    int pc = 0;  // the centre's position
+    if (N > 2) {
+        for (int p = 1; p < N; p++) {
+            const int q = pp[p];
+            upd[p] = 1 + std::max(upd[q], a1[q] == p ? b2[q] : b1[q]);
+        }
+        int best = std::max(b1[0], upd[0]);
+        for (int p = 1; p < N; p++) {
+            const int e = std::max(b1[p], upd[p]);
+            if (e < best || (e == best && ordered[p] < ordered[pc])) {
+                best = e;
+                pc = p;
+            }
+        }
+    }
+    // ---- the tree rooted at the centre, still in the given positions ----
+    // np_[p]: new parent position (-1 at the centre); w_[p]: weight of that edge
+    std::vector<int> np_(pp), nd(N, 0);
+    std::vector<float> w_(N, 0.0f);
+    for (int p = 1; p < N; p++) w_[p] = edgemst[ordered[p]];
+    if (pc != 0) {
+        // the path centre -> old root reverses: q's new parent is the path node below it
+        std::vector<char> onpath(N, 0);
+        for (int q = pc; q != 0; q = pp[q]) onpath[q] = 1;
+        onpath[0] = 1;
+        for (int below = pc, q = pp[pc];; below = q, q = pp[q]) {
+            np_[q] = below;
+            w_[q] = edgemst[ordered[below]];
+            if (q == 0) break;
+        }
+        np_[pc] = -1;
+        w_[pc] = 0.0f;
+        for (int p = 0; p < N; p++) nd[p] = onpath[p] ? dp[pc] - dp[p] : nd[pp[p]] + 1;
+    } else {
+        nd = dp;
+    }
+    // ---- BFS order from the centre, sorted by (depth, id): counting sort of the ids ----
+    int maxd = 0;
+    for (int p = 0; p < N; p++) maxd = std::max(maxd, nd[p]);
+    std::vector<int> ndv(N);  // new depth by id
+    for (int p = 0; p < N; p++) ndv[ordered[p]] = nd[p];
+    out.maxd = maxd;
+    out.depth_start.assign(maxd + 2, 0);
+    for (int v = 0; v < N; v++) out.depth_start[ndv[v] + 1]++;
+    for (int d = 0; d <= maxd; d++) out.depth_start[d + 1] += out.depth_start[d];
+    out.ordered.resize(N);
+    {
+        std::vector<int> fill(out.depth_start.begin(), out.depth_start.end() - 1);
+        for (int v = 0; v < N; v++) out.ordered[fill[ndv[v]]++] = v;
+    }
+    // r: positions of the new order; rp[r] = new parent position; old position of r = pos[ordered_new[r]]
+    std::vector<int> npos(N);  // new position by old position
+    for (int r = 0; r < N; r++) npos[pos[out.ordered[r]]] = r;
+    std::vector<int> rp(N, -1);
+    out.parents.assign(N, -1);
+    out.edgemst.assign(N, 0.0f);
+    for (int r = 0; r < N; r++) {
+        const int p = pos[out.ordered[r]], q = np_[p];
+        rp[r] = q < 0 ? -1 : npos[q];
+        out.parents[out.ordered[r]] = q < 0 ? -1 : ordered[q];
+        out.edgemst[out.ordered[r]] = w_[p];
+    }
+    // ---- children: CSR by position (children in BFS order) and by id for the device ----
+    std::vector<int> cs(N + 1, 0), cl(N > 0 ? N - 1 : 0);
+    for (int r = 1; r < N; r++) cs[rp[r] + 1]++;
+    for (int r = 0; r < N; r++) cs[r + 1] += cs[r];
+    {
+        std::vector<int> fill(cs.begin(), cs.end() - 1);
+        for (int r = 1; r < N; r++) cl[fill[rp[r]]++] = r;
+    }
+    out.child_start.assign(N + 1, 0);
+    for (int r = 0; r < N; r++) out.child_start[out.ordered[r] + 1] = cs[r + 1] - cs[r];
+    for (int v = 0; v < N; v++) out.child_start[v + 1] += out.child_start[v];
+    out.child_list.assign(cl.size(), 0);
+    for (int r = 0; r < N; r++) {
+        const int at = out.child_start[out.ordered[r]];
+        for (int e = cs[r]; e < cs[r + 1]; e++) out.child_list[at + e - cs[r]] = out.ordered[cl[e]];
+    }
+    // ---- rounds (positions r; heads in BFS order) ----
+    auto next_window = [W](int x) { return (x / W + 1) * W; };
+    auto pack = [&](const std::vector<int>& heads, const std::vector<int>& step, const std::vector<int>& next,
+                    RoundsHost& o) {
+        int rounds = 0;
+        std::vector<int> len(heads.size());
+        for (size_t i = 0; i < heads.size(); i++) {
+            int n = 0;
+            for (int r = heads[i]; r >= 0; r = next[r]) n++;
+            len[i] = n;  // at most W: a segment stays in one window
+            rounds = std::max(rounds, step[heads[i]] / W + 1);
+        }
+        std::vector<int> bstart((size_t)rounds * W + 1, 0);
+        auto bucket = [&](size_t i) { return (size_t)(step[heads[i]] / W) * W + (W - len[i]); };
+        for (size_t i = 0; i < heads.size(); i++) bstart[bucket(i) + 1]++;
+        for (size_t k = 0; k + 1 < bstart.size(); k++) bstart[k + 1] += bstart[k];
+        std::vector<int> sorted(heads.size());
+        for (size_t i = 0; i < heads.size(); i++) sorted[bstart[bucket(i)]++] = (int)i;
+        o.round_start.assign(rounds + 1, 0);
+        o.seg_start.assign(1, 0);
+        o.seg_start.reserve(heads.size() + 1);
+        o.nodes.clear();
+        o.nodes.reserve(N);
+        for (int i : sorted) {
+            o.round_start[step[heads[i]] / W + 1]++;
+            for (int r = heads[i]; r >= 0; r = next[r]) o.nodes.push_back(out.ordered[r]);
+            o.seg_start.push_back((int)o.nodes.size());
+        }
+        for (int k = 0; k < rounds; k++) o.round_start[k + 1] += o.round_start[k];
+    };
+    // upward
+    std::vector<int> tu(N, 0), up_next(N, -1), heads;
+    std::vector<char> continued(N, 0);
+    for (int r = N - 1; r >= 0; r--) {
+        const int c0 = cs[r], c1 = cs[r + 1];
+        if (c0 == c1) continue;
+        int best = cl[c0];
+        for (int e = c0 + 1; e < c1; e++)
+            if (tu[cl[e]] > tu[best]) best = cl[e];
+        int tv = tu[best] + 1;
+        for (int e = c0; e < c1; e++)
+            if (cl[e] != best) tv = std::max(tv, next_window(tu[cl[e]]));
+        tu[r] = tv;
+        if (tv == tu[best] + 1 && tv / W == tu[best] / W) {
+            up_next[best] = r;
+            continued[r] = 1;
+        }
+    }
+    for (int r = 0; r < N; r++)
+        if (!continued[r]) heads.push_back(r);
+    pack(heads, tu, up_next, out.up);
+    // downward
+    std::vector<int> height(N, 0), heavy(N, -1);
+    for (int r = N - 1; r >= 1; r--) height[rp[r]] = std::max(height[rp[r]], height[r] + 1);
+    for (int r = 0; r < N; r++)
+        for (int e = cs[r]; e < cs[r + 1]; e++)
+            if (heavy[r] < 0 || height[cl[e]] > height[heavy[r]]) heavy[r] = cl[e];
+    std::vector<int> td(N, 0), down_next(N, -1);
+    heads.clear();
+    for (int r = 1; r < N; r++) {
+        const int p = rp[r];
+        if (p == 0) {
+            td[r] = 0;
+            heads.push_back(r);
+        } else if (r == heavy[p] && (td[p] + 1) / W == td[p] / W) {
+            td[r] = td[p] + 1;
+            down_next[p] = r;
+        } else {
+            td[r] = next_window(td[p]);
+            heads.push_back(r);
+        }
+    }
+    pack(heads, td, down_next, out.down);
+}
+
 }  // namespace deeds

@@ -116,4 +116,49 @@ void finish_tree(const float* ec, float sd, int step, Tree& t, bool schedule = t
 // ordered/parents do not describe one.
 void build_schedule(Tree& t, bool bp_programs = true, int window = kBPWindow);
 
+// Windowed segment rounds of one sweep direction: one launch per round, one
+// workgroup per segment (round_start: rounds + 1 offsets into the segments;
+// seg_start: segments + 1 offsets into nodes).
+struct RoundsHost {
+    std::vector<int> round_start, seg_start, nodes;
+};
+
+// The soft sweeps' tree: the MST re-rooted at its centre, in BFS order
+// sorted by (depth, id), with its child lists and the rounds of both sweeps.
+struct SoftSchedule {
+    int maxd = 0;
+    std::vector<int> ordered, parents, depth_start, child_start, child_list;
+    std::vector<float> edgemst;
+    RoundsHost up, down;
+};
+
+// Builds the schedule from the caller's tree (ordered / parents / edgemst
+// as primsGraph returns them, primsMST.h:166-211), working in positions of
+// BFS orders: a node's parent and children sit in the depths next to its
+// own, so every pass walks nearby memory.
+//
+// Re-rooting.  The sum-product marginals do not depend on the root -- the
+// pairwise term |(u0_p + q l_p) - (u0_c + q l_c)|^2 / w is symmetric in the
+// two ends of an edge -- while the sequential depth of the sweeps drops
+// from the height to half the diameter.  The new root is the centre: the
+// node of least eccentricity (the smallest id among ties), with
+// eccentricity = max(height below the node, longest path leaving it
+// upwards), both from one pass each way over the given order.  Distances
+// from the centre follow from the path between it and the old root (nodes
+// on it reverse their edge, each edge keeping its weight).
+//
+// Rounds, W steps each.  Upward, every node gets a step: leaves 0; a node
+// the later of t(c*) + 1, c* its child of latest step (the first among
+// ties), and the start of the window after every other child's step.  It
+// continues c*'s segment when that is t(c*) + 1 in the same window.
+// Downward (every node but the root): the root's children start at step 0;
+// a node continues its parent's segment at t(p) + 1 when it is the
+// parent's child of greatest height (the first among ties) and that step
+// stays in the parent's window, else it starts at the next window.  A
+// segment runs in the round of its first node's window; within a round the
+// longest segments come first (then BFS order).
+void soft_schedule(const int* ordered, const int* parents, const float* edgemst, int N, int W, SoftSchedule& out);
+
+
+
 }  // namespace deeds

@@ -52,186 +52,6 @@ struct SoftTreeCache {
     SoftTree t;
 };
 
-// Breadth-first distances from s over the undirected tree (CSR adjacency);
-// par / wpar (when given) receive each node's BFS parent and the weight bits
-// of the edge to it.
-std::vector<int> bfs_dist(const std::vector<int>& adj_start, const std::vector<int>& adj,
-                          const std::vector<int>& wbits, int s, std::vector<int>* par = nullptr,
-                          std::vector<int>* wpar = nullptr) {
-    std::vector<int> d(adj_start.size() - 1, -1), q;
-    q.reserve(d.size());
-    d[s] = 0;
-    q.push_back(s);
-    for (size_t h = 0; h < q.size(); h++) {
-        const int x = q[h];
-        for (int e = adj_start[x]; e < adj_start[x + 1]; e++)
-            if (d[adj[e]] < 0) {
-                d[adj[e]] = d[x] + 1;
-                if (par) {
-                    (*par)[adj[e]] = x;
-                    (*wpar)[adj[e]] = wbits[e];
-                }
-                q.push_back(adj[e]);
-            }
-    }
-    return d;
-}
-
-// The same tree (same edges, each edge keeping its weight) rooted at its
-// centre, the node of least eccentricity (the smallest id among ties), in
-// BFS order sorted by (depth, id).  The sum-product marginals do not depend
-// on the root -- the pairwise term |(u0_p + q l_p) - (u0_c + q l_c)|^2 / w is
-// symmetric in the two ends of an edge -- while the number of sequential
-// depths of the sweeps drops from the height to half the diameter.  The last
-// node of `ordered` is a farthest node from the root, so one end of a
-// diameter; two more searches give the eccentricities.
-void reroot_at_centre(Tree& t) {
-    const int N = t.N;
-    if (N <= 2) return;
-    std::vector<int> deg(N + 1, 0);
-    for (int c = 0; c < N; c++)
-        if (t.parents[c] >= 0) {
-            deg[c + 1]++;
-            deg[t.parents[c] + 1]++;
-        }
-    std::vector<int> adj_start(N + 1, 0);
-    for (int v = 0; v < N; v++) adj_start[v + 1] = adj_start[v] + deg[v + 1];
-    std::vector<int> adj(adj_start[N]), wbits(adj_start[N]), fill(adj_start.begin(), adj_start.end() - 1);
-    for (int c = 0; c < N; c++) {
-        const int p = t.parents[c];
-        if (p < 0) continue;
-        int wb;
-        std::memcpy(&wb, &t.edgemst[c], sizeof(int));
-        adj[fill[c]] = p;
-        wbits[fill[c]++] = wb;
-        adj[fill[p]] = c;
-        wbits[fill[p]++] = wb;
-    }
-    // diameter ends a, b: eccentricity(v) = max(d(a, v), d(b, v))
-    const int a = t.ordered[N - 1];
-    const std::vector<int> da = bfs_dist(adj_start, adj, wbits, a);
-    const int b = (int)(std::max_element(da.begin(), da.end()) - da.begin());
-    const std::vector<int> db = bfs_dist(adj_start, adj, wbits, b);
-    int centre = 0;
-    for (int v = 1; v < N; v++)
-        if (std::max(da[v], db[v]) < std::max(da[centre], db[centre])) centre = v;
-    std::vector<int> parents(N, -1), wpar(N, 0);
-    const std::vector<int> dc = bfs_dist(adj_start, adj, wbits, centre, &parents, &wpar);
-    // (depth, id) order: a counting sort of the ids by depth
-    const int maxd = *std::max_element(dc.begin(), dc.end());
-    std::vector<int> start(maxd + 2, 0);
-    for (int v = 0; v < N; v++) start[dc[v] + 1]++;
-    for (int d = 0; d <= maxd; d++) start[d + 1] += start[d];
-    std::vector<int> order(N);
-    for (int v = 0; v < N; v++) order[start[dc[v]]++] = v;
-    std::vector<float> w(N, 0.0f);
-    for (int v = 0; v < N; v++)
-        if (parents[v] >= 0) std::memcpy(&w[v], &wpar[v], sizeof(float));
-    t.ordered = order;
-    t.parents = parents;
-    t.edgemst = w;
-}
-
-// Windowed segment rounds of the two sweep directions (built from ordered,
-// parents and the child lists of t), W steps per round.  Upward, every node
-// gets a step: leaves 0; a node the later of t(c*) + 1, c* its child of
-// latest step (the first among ties), and the start of the window after
-// every other child's step.  It continues c*'s segment when that is
-// t(c*) + 1 in the same window.  Downward (every node but the root): the
-// root's children start at step 0; a node continues its parent's segment at
-// t(p) + 1 when it is the parent's child of greatest height (the first
-// among ties) and that step stays in the parent's window, else it starts at
-// the next window.  A segment runs in the round of its first node's window;
-// within a round the longest segments come first.
-struct RoundsHost {
-    std::vector<int> round_start, seg_start, nodes;
-};
-
-void soft_rounds(const Tree& t, int W, RoundsHost& up, RoundsHost& down) {
-    const int N = t.N;
-    auto next_window = [W](int x) { return (x / W + 1) * W; };
-    auto pack = [&](const std::vector<int>& heads, const std::vector<int>& step, const std::vector<int>& next,
-                    RoundsHost& out) {
-        // heads in `ordered` order; next[v] = the node after v in its segment or -1.
-        // Buckets by (round, W - length), each keeping the heads' order.
-        int rounds = 0;
-        std::vector<int> len(heads.size());
-        for (size_t i = 0; i < heads.size(); i++) {
-            int n = 0;
-            for (int v = heads[i]; v >= 0; v = next[v]) n++;
-            len[i] = n;  // at most W: a segment stays in one window
-            rounds = std::max(rounds, step[heads[i]] / W + 1);
-        }
-        std::vector<int> bstart((size_t)rounds * W + 1, 0);
-        auto bucket = [&](size_t i) { return (size_t)(step[heads[i]] / W) * W + (W - len[i]); };
-        for (size_t i = 0; i < heads.size(); i++) bstart[bucket(i) + 1]++;
-        for (size_t k = 0; k + 1 < bstart.size(); k++) bstart[k + 1] += bstart[k];
-        std::vector<int> sorted(heads.size());
-        for (size_t i = 0; i < heads.size(); i++) sorted[bstart[bucket(i)]++] = (int)i;
-        out.round_start.assign(rounds + 1, 0);
-        out.seg_start.assign(1, 0);
-        out.seg_start.reserve(heads.size() + 1);
-        out.nodes.clear();
-        out.nodes.reserve(N);
-        for (int i : sorted) {
-            out.round_start[step[heads[i]] / W + 1]++;
-            for (int v = heads[i]; v >= 0; v = next[v]) out.nodes.push_back(v);
-            out.seg_start.push_back((int)out.nodes.size());
-        }
-        for (int r = 0; r < rounds; r++) out.round_start[r + 1] += out.round_start[r];
-    };
-    // upward
-    std::vector<int> tu(N, 0), up_next(N, -1), heads;
-    std::vector<char> continued(N, 0);
-    for (int i = N - 1; i >= 0; i--) {
-        const int v = t.ordered[i];
-        const int c0 = t.child_start[v], c1 = t.child_start[v + 1];
-        if (c0 == c1) continue;
-        int best = t.child_list[c0];
-        for (int e = c0 + 1; e < c1; e++)
-            if (tu[t.child_list[e]] > tu[best]) best = t.child_list[e];
-        int tv = tu[best] + 1;
-        for (int e = c0; e < c1; e++)
-            if (t.child_list[e] != best) tv = std::max(tv, next_window(tu[t.child_list[e]]));
-        tu[v] = tv;
-        if (tv == tu[best] + 1 && tv / W == tu[best] / W) {
-            up_next[best] = v;
-            continued[v] = 1;
-        }
-    }
-    for (int i = 0; i < N; i++)
-        if (!continued[t.ordered[i]]) heads.push_back(t.ordered[i]);
-    pack(heads, tu, up_next, up);
-    // downward
-    std::vector<int> height(N, 0);
-    for (int i = N - 1; i >= 0; i--) {
-        const int v = t.ordered[i];
-        for (int e = t.child_start[v]; e < t.child_start[v + 1]; e++)
-            height[v] = std::max(height[v], height[t.child_list[e]] + 1);
-    }
-    std::vector<int> td(N, 0), down_next(N, -1);
-    heads.clear();
-    for (int i = 1; i < N; i++) {
-        const int c = t.ordered[i], p = t.parents[c];
-        if (p == t.ordered[0]) {
-            td[c] = 0;
-            heads.push_back(c);
-            continue;
-        }
-        int heavy = t.child_list[t.child_start[p]];
-        for (int e = t.child_start[p] + 1; e < t.child_start[p + 1]; e++)
-            if (height[t.child_list[e]] > height[heavy]) heavy = t.child_list[e];
-        if (c == heavy && (td[p] + 1) / W == td[p] / W) {
-            td[c] = td[p] + 1;
-            down_next[p] = c;
-        } else {
-            td[c] = next_window(td[p]);
-            heads.push_back(c);
-        }
-    }
-    pack(heads, td, down_next, down);
-}
-
 const SoftTree& stage_soft_tree(deeds_ctx* ctx, const int* ordered, const int* parents, const float* edgemst,
                                 int N) {
     DEEDS_REQUIRE(ordered && parents && edgemst, "null tree pointer");
@@ -241,21 +61,14 @@ const SoftTree& stage_soft_tree(deeds_ctx* ctx, const int* ordered, const int* p
         std::memcmp(edgemst, cache->edgemst.data(), (size_t)N * sizeof(float)) == 0)
         return cache->t;
     ctx->soft_cache.reset();
-    Tree t;
-    t.N = N;
-    t.ordered.assign(ordered, ordered + N);
-    t.parents.assign(parents, parents + N);
-    t.edgemst.assign(edgemst, edgemst + N);
-    for (int v : t.ordered) DEEDS_REQUIRE(v >= 0 && v < N, "ordered holds an out-of-range node");
+    SoftSchedule t;
     try {
-        build_schedule(t, false);  // validates the tree
-        reroot_at_centre(t);
-        build_schedule(t, false);
+        soft_schedule(ordered, parents, edgemst, N, kSoftWindow, t);  // validates the tree
     } catch (const std::runtime_error& e) {
         fail(DEEDS_ERR_ARG, e.what());
     }
-    RoundsHost up, down;
-    soft_rounds(t, kSoftWindow, up, down);
+    const RoundsHost& up = t.up;
+    const RoundsHost& down = t.down;
     const size_t ints = (size_t)N + N + (N + 1) + (N > 0 ? N - 1 : 0) + up.seg_start.size() + up.nodes.size() +
                         down.seg_start.size() + down.nodes.size();
     const size_t bytes = ints * sizeof(int) + (size_t)N * sizeof(float);

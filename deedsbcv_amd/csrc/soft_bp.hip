@@ -259,18 +259,32 @@ struct RXAt {
     static constexpr int UP = 0, DOWN = 3 * L3, STRIDE = 7 * L3;
 };
 
-// Quadratic tables of the edge c -> p: z[a - b] = (a - b + off)^2 / w per axis
-// (label axis 0 is i / v, 1 is j / u, 2 is k / w); sign +1 for the upward
-// transform (a = parent label, b = child label), -1 for the downward one.
-template <typename R, int LEN>
-__device__ void edge_tables(const BPState<R>& a, int c, int p, R sign, const Cubes<R, LEN>& s) {
-    using H = SS<R, LEN>;
+// The scalars behind the quadratic tables of the edge c -> p: its weight
+// and the offsets (u0_p - u0_c) / quant per axis, times sign (+1 for the
+// upward transform, a = parent label, b = child label; -1 for the downward
+// one).  Loaded on their own so that a sweep can fetch them a step ahead.
+template <typename R>
+struct EdgeIn {
+    R w, ou, ov, ow;
+};
+template <typename R>
+__device__ __forceinline__ EdgeIn<R> edge_in(const BPState<R>& a, int c, int p, R sign) {
     const float wf = ro(a.t.edgemst)[c];
-    const R w = (R)(wf > kSoftMinEdgeWeight ? wf : kSoftMinEdgeWeight);
     const auto u0 = ro(a.u0);
-    const R ou = sign * (u0[p] - u0[c]) / a.quant;
-    const R ov = sign * (u0[a.N + p] - u0[a.N + c]) / a.quant;
-    const R ow = sign * (u0[2 * a.N + p] - u0[2 * a.N + c]) / a.quant;
+    EdgeIn<R> e;
+    e.w = (R)(wf > kSoftMinEdgeWeight ? wf : kSoftMinEdgeWeight);
+    e.ou = sign * (u0[p] - u0[c]) / a.quant;
+    e.ov = sign * (u0[a.N + p] - u0[a.N + c]) / a.quant;
+    e.ow = sign * (u0[2 * a.N + p] - u0[2 * a.N + c]) / a.quant;
+    return e;
+}
+
+// Quadratic tables of the edge: z[a - b] = (a - b + off)^2 / w per axis
+// (label axis 0 is i / v, 1 is j / u, 2 is k / w).
+template <typename R, int LEN>
+__device__ void edge_tables(const BPState<R>& a, const EdgeIn<R>& e, const Cubes<R, LEN>& s) {
+    using H = SS<R, LEN>;
+    const R w = e.w, ou = e.ou, ov = e.ov, ow = e.ow;
     // zmin: the table's minimum over the label differences -(LEN-1) .. LEN-1,
     // at the integer next to -off on either side
     R zm[3];
@@ -348,8 +362,41 @@ struct SegArgs {
     int seg0;
 };
 
-// Upward: h_c = C_c + the children's messages (in child order; the child
-// just before in the segment from X3 - its minimum, as stored), m_c = DTup(h_c) - min.
+// Every sweep step loads its node's inputs one step ahead, while the step
+// before computes (the loads stay in flight across the workgroup barriers,
+// which wait for LDS only): rows of the node itself, the stored rows of up
+// to two children other than the one before it in the segment (further
+// ones, rare in an MST, are read when the step starts), and its edge's
+// table scalars.  The chained neighbour's row comes from LDS or registers.
+
+// Whether a sweep loads a step ahead: where the inputs of a step fit in
+// registers beside the transform's (f32 with at most three label entries
+// per thread); elsewhere a step loads its inputs when it starts.
+template <typename R, int LEN>
+constexpr bool sweep_prefetch() {
+    return sizeof(R) == 4 && SS<R, LEN>::KL <= 3;
+}
+
+// The first two children of c other than `chained` (or -1), and whether there are more.
+struct OtherKids {
+    int k0 = -1, k1 = -1;
+    bool more = false;
+};
+__device__ __forceinline__ OtherKids other_kids(cint* cs, cint* cl, int c, int chained) {
+    OtherKids o;
+    for (int ci = cs[c]; ci < cs[c + 1]; ci++) {
+        const int k = cl[ci];
+        if (k == chained) continue;
+        if (o.k0 < 0) o.k0 = k;
+        else if (o.k1 < 0) o.k1 = k;
+        else o.more = true;
+    }
+    return o;
+}
+
+// Upward: h_c = C_c + the children's messages (the other children's stored
+// messages, then the message of the child just before in the segment from
+// X3 - its minimum), m_c = DTup(h_c) - min.
 template <typename R, int LEN>
 __global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_up(BPState<R> a, SegArgs g) {
     using H = SS<R, LEN>;
@@ -359,30 +406,65 @@ __global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_up(BPState<R> a, Seg
     cint* cs = ro(a.t.child_start);
     cint* cl = ro(a.t.child_list);
     const int i0 = ro(g.seg_start)[g.seg0 + blockIdx.x], i1 = ro(g.seg_start)[g.seg0 + blockIdx.x + 1];
+    struct In {
+        int c, p;
+        OtherKids kid;
+        R cost[H::KL], m0[H::KL], m1[H::KL];
+        EdgeIn<R> e;
+    };
+    auto fetch = [&](In& in, int c, int chained) {
+        in.c = c;
+        in.p = ro(a.t.parents)[c];
+        in.kid = other_kids(cs, cl, c, chained);
+        const long long cb = (long long)c * H::L3;
+#pragma unroll
+        for (int k = 0; k < H::KL; k++) {
+            const int l = threadIdx.x + k * H::BLOCK;
+            if (l < H::L3) {
+                in.cost[k] = a.cost[cb + l];
+                in.m0[k] = in.kid.k0 >= 0 ? a.Msg[(long long)in.kid.k0 * H::L3 + l] : (R)0;
+                in.m1[k] = in.kid.k1 >= 0 ? a.Msg[(long long)in.kid.k1 * H::L3 + l] : (R)0;
+            }
+        }
+        if (in.p >= 0) in.e = edge_in(a, c, in.p, (R)1);
+    };
+    constexpr bool PF = sweep_prefetch<R, LEN>();
+    In cur, nxt;
+    if (PF) fetch(cur, nd[i0], -1);
     int prev = -1;
     R pmn = 0;
     for (int i = i0; i < i1; i++) {
-        const int c = nd[i];
+        if (!PF) fetch(cur, nd[i], prev);
+        const int c = cur.c, p = cur.p;
         const long long cb = (long long)c * H::L3;
-        const int c0 = cs[c], c1 = cs[c + 1];
-        for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) {
-            R h = a.cost[cb + l];
-            for (int ci = c0; ci < c1; ci++) {
-                const int k = cl[ci];
-                h += k == prev ? s.X3[l] - pmn : a.Msg[(long long)k * H::L3 + l];
+        const bool chained = prev >= 0;  // the node before in the segment is a child of c
+#pragma unroll
+        for (int k = 0; k < H::KL; k++) {
+            const int l = threadIdx.x + k * H::BLOCK;
+            if (l < H::L3) {
+                R h = cur.cost[k] + cur.m0[k] + cur.m1[k];
+                if (cur.kid.more) {
+                    int seen = 0;
+                    for (int ci = cs[c]; ci < cs[c + 1]; ci++) {
+                        const int kk = cl[ci];
+                        if (kk != prev && seen++ >= 2) h += a.Msg[(long long)kk * H::L3 + l];
+                    }
+                }
+                if (chained) h += s.X3[l] - pmn;
+                a.H[cb + l] = h;
+                s.X0[l] = h;
             }
-            a.H[cb + l] = h;
-            s.X0[l] = h;
         }
-        const int p = ro(a.t.parents)[c];
         if (p < 0) {  // the root's full belief is its upward belief
             for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) a.G[cb + l] = s.X0[l];
             return;
         }
-        edge_tables<R, LEN>(a, c, p, (R)1, s);
+        if (PF && i + 1 < i1) fetch(nxt, nd[i + 1], c);
+        edge_tables<R, LEN>(a, cur.e, s);
         pmn = transform<R, LEN>(s, a.T);
         for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) a.Msg[cb + l] = s.X3[l] - pmn;
         prev = c;
+        if (PF) cur = nxt;
     }
 }
 
@@ -395,28 +477,54 @@ __global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_down(BPState<R> a, S
     const Cubes<R, LEN> s(reinterpret_cast<R*>(sp_smem));
     cint* nd = ro(g.nodes);
     const int i0 = ro(g.seg_start)[g.seg0 + blockIdx.x], i1 = ro(g.seg_start)[g.seg0 + blockIdx.x + 1];
-    int prev = -1;
-    R gk[H::KL];
-    for (int i = i0; i < i1; i++) {
-        const int c = nd[i];
-        const int p = ro(a.t.parents)[c];
-        const long long cb = (long long)c * H::L3, pb = (long long)p * H::L3;
+    struct In {
+        int c, p;
+        R msg[H::KL], h[H::KL], gp[H::KL];
+        EdgeIn<R> e;
+    };
+    auto fetch = [&](In& in, int c, int chained) {
+        in.c = c;
+        in.p = ro(a.t.parents)[c];
+        const long long cb = (long long)c * H::L3, pb = (long long)in.p * H::L3;
 #pragma unroll
         for (int k = 0; k < H::KL; k++) {
             const int l = threadIdx.x + k * H::BLOCK;
-            if (l < H::L3) s.X0[l] = (p == prev ? gk[k] : a.G[pb + l]) - a.Msg[cb + l];
+            if (l < H::L3) {
+                in.msg[k] = a.Msg[cb + l];
+                in.h[k] = a.H[cb + l];
+                in.gp[k] = in.p != chained ? a.G[pb + l] : (R)0;
+            }
         }
-        edge_tables<R, LEN>(a, c, p, (R)-1, s);
+        in.e = edge_in(a, c, in.p, (R)-1);
+    };
+    constexpr bool PF = sweep_prefetch<R, LEN>();
+    In cur, nxt;
+    if (PF) fetch(cur, nd[i0], -1);
+    int prev = -1;
+    R gk[H::KL];
+    for (int i = i0; i < i1; i++) {
+        if (!PF) fetch(cur, nd[i], prev);
+        const int c = cur.c;
+        const bool chained = cur.p == prev;
+        const long long cb = (long long)c * H::L3;
+#pragma unroll
+        for (int k = 0; k < H::KL; k++) {
+            const int l = threadIdx.x + k * H::BLOCK;
+            if (l < H::L3) s.X0[l] = (chained ? gk[k] : cur.gp[k]) - cur.msg[k];
+        }
+        if (PF && i + 1 < i1) fetch(nxt, nd[i + 1], c);
+        edge_tables<R, LEN>(a, cur.e, s);
         const R mn = transform<R, LEN>(s, a.T);
 #pragma unroll
         for (int k = 0; k < H::KL; k++) {
             const int l = threadIdx.x + k * H::BLOCK;
             if (l < H::L3) {
-                gk[k] = a.H[cb + l] + (s.X3[l] - mn);
+                gk[k] = cur.h[k] + (s.X3[l] - mn);
                 a.G[cb + l] = gk[k];
             }
         }
         prev = c;
+        if (PF) cur = nxt;
     }
 }
 
@@ -516,7 +624,7 @@ __global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_bwd_passes(BPState<R
             s.X0[l] = a.H[cb + l];
         }
     }
-    edge_tables<R, LEN>(a, c, p, down ? (R)-1 : (R)1, s);
+    edge_tables<R, LEN>(a, edge_in(a, c, p, down ? (R)-1 : (R)1), s);
     passes<R, LEN>(s, a.T);
     R* o = rx + (down ? H::L3 : 0);
     for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) {
@@ -526,9 +634,10 @@ __global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_bwd_passes(BPState<R
     }
 }
 
-// Children first: A_c = own term + the children's Dt (in child order; the
-// child just before in the segment from Gb), then Dt_c = DTdown_c^T(A_c).  At
-// the root, dH = A.
+// Children first: A_c = own term + the children's Dt (the other children's
+// stored ones, then the one of the child just before in the segment from
+// Gb), then Dt_c = DTdown_c^T(A_c) over the recomputed downward transform.
+// At the root, dH = A.
 template <typename R, int LEN>
 __global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_bwd_down(BPState<R> a, BPGrad<R> b, SegArgs g) {
     using H = SS<R, LEN>;
@@ -539,41 +648,79 @@ __global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_bwd_down(BPState<R> 
     cint* cs = ro(a.t.child_start);
     cint* cl = ro(a.t.child_list);
     const int i0 = ro(g.seg_start)[g.seg0 + blockIdx.x], i1 = ro(g.seg_start)[g.seg0 + blockIdx.x + 1];
+    struct In {
+        int c, p;
+        OtherKids kid;
+        R A[H::KL], d0[H::KL], d1[H::KL], x[4][H::KL];
+        EdgeIn<R> e;
+    };
+    auto fetch = [&](In& in, int c, int chained) {
+        in.c = c;
+        in.p = ro(a.t.parents)[c];
+        in.kid = other_kids(cs, cl, c, chained);
+        const long long cb = (long long)c * H::L3;
+        const R* rx = b.RX + (long long)c * X::STRIDE + X::DOWN;
+#pragma unroll
+        for (int k = 0; k < H::KL; k++) {
+            const int l = threadIdx.x + k * H::BLOCK;
+            if (l < H::L3) {
+                in.A[k] = b.A[cb + l];
+                in.d0[k] = in.kid.k0 >= 0 ? b.Dt[(long long)in.kid.k0 * H::L3 + l] : (R)0;
+                in.d1[k] = in.kid.k1 >= 0 ? b.Dt[(long long)in.kid.k1 * H::L3 + l] : (R)0;
+                if (in.p >= 0) {  // the recomputed downward transform of c
+#pragma unroll
+                    for (int q = 0; q < 4; q++) in.x[q][k] = rx[q * H::L3 + l];
+                }
+            }
+        }
+        if (in.p >= 0) in.e = edge_in(a, c, in.p, (R)-1);
+    };
+    constexpr bool PF = sweep_prefetch<R, LEN>();
+    In cur, nxt;
+    if (PF) fetch(cur, nd[i0], -1);
     int prev = -1;
     for (int i = i0; i < i1; i++) {
-        const int c = nd[i];
+        if (!PF) fetch(cur, nd[i], prev);
+        const int c = cur.c, p = cur.p;
         const long long cb = (long long)c * H::L3;
-        const int c0 = cs[c], c1 = cs[c + 1];
-        const int p = ro(a.t.parents)[c];
-        const R* rx = b.RX + (long long)c * X::STRIDE + X::DOWN;
-        for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) {
-            R v = b.A[cb + l];
-            for (int ci = c0; ci < c1; ci++) {
-                const int k = cl[ci];
-                v += k == prev ? s.Gb[l] : b.Dt[(long long)k * H::L3 + l];
+        const bool chained = prev >= 0;
+#pragma unroll
+        for (int k = 0; k < H::KL; k++) {
+            const int l = threadIdx.x + k * H::BLOCK;
+            if (l < H::L3) {
+                R v = cur.A[k] + cur.d0[k] + cur.d1[k];
+                if (cur.kid.more) {
+                    int seen = 0;
+                    for (int ci = cs[c]; ci < cs[c + 1]; ci++) {
+                        const int kk = cl[ci];
+                        if (kk != prev && seen++ >= 2) v += b.Dt[(long long)kk * H::L3 + l];
+                    }
+                }
+                if (chained) v += s.Gb[l];
+                s.X0[l] = cur.x[0][k];
+                s.X1[l] = cur.x[1][k];
+                s.X2[l] = cur.x[2][k];
+                s.X3[l] = cur.x[3][k];
+                b.A[cb + l] = v;
+                s.Ga[l] = v;
             }
-            if (p >= 0) {  // the recomputed downward transform of c
-                s.X0[l] = rx[l];
-                s.X1[l] = rx[H::L3 + l];
-                s.X2[l] = rx[2 * H::L3 + l];
-                s.X3[l] = rx[3 * H::L3 + l];
-            }
-            b.A[cb + l] = v;
-            s.Ga[l] = v;
         }
         if (p < 0) {
             for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) b.dH[cb + l] = s.Ga[l];
             return;
         }
-        edge_tables<R, LEN>(a, c, p, (R)-1, s);
+        if (PF && i + 1 < i1) fetch(nxt, nd[i + 1], c);
+        edge_tables<R, LEN>(a, cur.e, s);
         transform_adj<R, LEN>(s, a.T);
         for (int l = threadIdx.x; l < H::L3; l += H::BLOCK) b.Dt[cb + l] = s.Gb[l];
         prev = c;
+        if (PF) cur = nxt;
     }
 }
 
-// Parents first: dH_c = A_c + DTup_c^T(dH_p - Dt_c), dH_p of the node just
-// before in the segment from registers.
+// Parents first: dH_c = A_c + DTup_c^T(dH_p - Dt_c) over the recomputed
+// upward transform, dH_p of the node just before in the segment from
+// registers.
 template <typename R, int LEN>
 __global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_bwd_up(BPState<R> a, BPGrad<R> b, SegArgs g) {
     using H = SS<R, LEN>;
@@ -582,35 +729,64 @@ __global__ __launch_bounds__((SS<R, LEN>::BLOCK)) void k_sp_bwd_up(BPState<R> a,
     const Cubes<R, LEN> s(reinterpret_cast<R*>(sp_smem));
     cint* nd = ro(g.nodes);
     const int i0 = ro(g.seg_start)[g.seg0 + blockIdx.x], i1 = ro(g.seg_start)[g.seg0 + blockIdx.x + 1];
-    int prev = -1;
-    R hk[H::KL];
-    for (int i = i0; i < i1; i++) {
-        const int c = nd[i];
-        const int p = ro(a.t.parents)[c];
-        const long long cb = (long long)c * H::L3, pb = (long long)p * H::L3;
-        const R* rx = b.RX + (long long)c * X::STRIDE + X::UP;  // the recomputed upward transform of c
+    struct In {
+        int c, p;
+        R h[H::KL], x[3][H::KL], dt[H::KL], A[H::KL], dhp[H::KL];
+        EdgeIn<R> e;
+    };
+    auto fetch = [&](In& in, int c, int chained) {
+        in.c = c;
+        in.p = ro(a.t.parents)[c];
+        const long long cb = (long long)c * H::L3, pb = (long long)in.p * H::L3;
+        const R* rx = b.RX + (long long)c * X::STRIDE + X::UP;
 #pragma unroll
         for (int k = 0; k < H::KL; k++) {
             const int l = threadIdx.x + k * H::BLOCK;
             if (l < H::L3) {
-                s.X0[l] = a.H[cb + l];
-                s.X1[l] = rx[l];
-                s.X2[l] = rx[H::L3 + l];
-                s.X3[l] = rx[2 * H::L3 + l];
-                s.Ga[l] = (p == prev ? hk[k] : b.dH[pb + l]) - b.Dt[cb + l];
+                in.h[k] = a.H[cb + l];
+#pragma unroll
+                for (int q = 0; q < 3; q++) in.x[q][k] = rx[q * H::L3 + l];
+                in.dt[k] = b.Dt[cb + l];
+                in.A[k] = b.A[cb + l];
+                in.dhp[k] = in.p != chained ? b.dH[pb + l] : (R)0;
             }
         }
-        edge_tables<R, LEN>(a, c, p, (R)1, s);
+        in.e = edge_in(a, c, in.p, (R)1);
+    };
+    constexpr bool PF = sweep_prefetch<R, LEN>();
+    In cur, nxt;
+    if (PF) fetch(cur, nd[i0], -1);
+    int prev = -1;
+    R hk[H::KL];
+    for (int i = i0; i < i1; i++) {
+        if (!PF) fetch(cur, nd[i], prev);
+        const int c = cur.c;
+        const bool chained = cur.p == prev;
+        const long long cb = (long long)c * H::L3;
+#pragma unroll
+        for (int k = 0; k < H::KL; k++) {
+            const int l = threadIdx.x + k * H::BLOCK;
+            if (l < H::L3) {
+                s.X0[l] = cur.h[k];
+                s.X1[l] = cur.x[0][k];
+                s.X2[l] = cur.x[1][k];
+                s.X3[l] = cur.x[2][k];
+                s.Ga[l] = (chained ? hk[k] : cur.dhp[k]) - cur.dt[k];
+            }
+        }
+        if (PF && i + 1 < i1) fetch(nxt, nd[i + 1], c);
+        edge_tables<R, LEN>(a, cur.e, s);
         transform_adj<R, LEN>(s, a.T);
 #pragma unroll
         for (int k = 0; k < H::KL; k++) {
             const int l = threadIdx.x + k * H::BLOCK;
             if (l < H::L3) {
-                hk[k] = b.A[cb + l] + s.Gb[l];
+                hk[k] = cur.A[k] + s.Gb[l];
                 b.dH[cb + l] = hk[k];
             }
         }
         prev = c;
+        if (PF) cur = nxt;
     }
 }
 
