@@ -6,9 +6,10 @@
 //           clamped box sums of radius qs along i, j, k (:2-76, as window
 //           sums rather than prefix-sum differences); phi = the twelve SSC
 //           values minus their minimum over max(mean, 1e-6) (:184-201).
-// Backward: the adjoint of each step in reverse (the box passes are
-//           symmetric, so they are their own adjoint); the distances D are
-//           recomputed.  All kernels are elementwise or short stencils (HBM bound).
+// Backward: the adjoint of each step in reverse (the 3-D box is symmetric,
+//           so it is its own adjoint); the distances D are recomputed.  All
+//           kernels are elementwise or short stencils (HBM bound); the three
+//           box passes run as one LDS-tiled kernel.
 #include "soft_common.h"
 
 namespace deeds {
@@ -64,6 +65,95 @@ __global__ void k_sf_box(R* __restrict__ out, const R* __restrict__ in, Dims d, 
     R acc = 0;
     for (int t = lo; t <= hi; t++) acc += base[t * stride];
     out[idx] = acc;
+}
+
+// The three box passes (i, then j, then k) of the six channels in one kernel:
+// a workgroup stages a BI x BJ x BK output tile with its RAD halo in LDS (zero
+// beyond the volume, which is the clamped window), sums along i into a
+// second buffer, along j back into the first, and along k into the output.
+// Every window sum adds its terms in ascending order as k_sf_box does, and
+// the zeros it adds fall before or after the in-volume terms, so the result
+// is k_sf_box's, three passes through HBM fewer.
+constexpr int kBI = 32, kBJ = 8, kBK = 8;
+template <int RAD>
+struct Box3 {
+    static constexpr int XI = kBI + 2 * RAD, XJ = kBJ + 2 * RAD, XK = kBK + 2 * RAD;
+    static constexpr int NA = XI * XJ * XK, NB = kBI * XJ * XK;
+};
+template <typename R, int RAD>
+__global__ __launch_bounds__(256) void k_sf_box3(R* __restrict__ out, const R* __restrict__ in, Dims d, int ti,
+                                                  int tj) {
+    using X = Box3<RAD>;
+    extern __shared__ __align__(16) unsigned char sf_smem[];
+    R* A = reinterpret_cast<R*>(sf_smem);  // [XK][XJ][XI], then [XK][BJ][BI]
+    R* B = A + X::NA;                      // [XK][XJ][BI]
+    const int tile = blockIdx.x, ch = blockIdx.y;
+    const int bi = tile % ti, bj = (tile / ti) % tj, bk = tile / (ti * tj);
+    const int i0 = bi * kBI - RAD, j0 = bj * kBJ - RAD, k0 = bk * kBK - RAD;
+    const R* src = in + ch * d.sz;
+    for (int e = threadIdx.x; e < X::NA; e += blockDim.x) {
+        const int ii = e % X::XI, jj = (e / X::XI) % X::XJ, kk = e / (X::XI * X::XJ);
+        const int i = i0 + ii, j = j0 + jj, k = k0 + kk;
+        A[e] = d.in(i, j, k) ? src[d.at(i, j, k)] : (R)0;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < X::NB; e += blockDim.x) {
+        const int ii = e % kBI, r = e / kBI;  // r = jj + XJ * kk
+        const R* a = A + r * X::XI + ii;
+        R acc = 0;
+#pragma unroll
+        for (int t = 0; t <= 2 * RAD; t++) acc += a[t];
+        B[e] = acc;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kBI * kBJ * X::XK; e += blockDim.x) {
+        const int ii = e % kBI, jo = (e / kBI) % kBJ, kk = e / (kBI * kBJ);
+        const R* b = B + (kk * X::XJ + jo) * kBI + ii;
+        R acc = 0;
+#pragma unroll
+        for (int t = 0; t <= 2 * RAD; t++) acc += b[t * kBI];
+        A[e] = acc;
+    }
+    __syncthreads();
+    R* dst = out + ch * d.sz;
+    for (int e = threadIdx.x; e < kBI * kBJ * kBK; e += blockDim.x) {
+        const int ii = e % kBI, jo = (e / kBI) % kBJ, ko = e / (kBI * kBJ);
+        const int i = bi * kBI + ii, j = bj * kBJ + jo, k = bk * kBK + ko;
+        if (!d.in(i, j, k)) continue;
+        const R* c = A + ko * kBJ * kBI + jo * kBI + ii;
+        R acc = 0;
+#pragma unroll
+        for (int t = 0; t <= 2 * RAD; t++) acc += c[t * kBJ * kBI];
+        dst[d.at(i, j, k)] = acc;
+    }
+}
+
+// The 3-D box of radius r over the six channels: fused for radius 1..3, else
+// three k_sf_box launches through tmp, in the given axis order (0, 1, 2 or,
+// for the adjoint, its reverse).  The 1-D boxes are symmetric and commute, so
+// the 3-D box is its own adjoint in either order.
+template <typename R>
+void box3(R* out, const R* in, R* tmp, const Dims& d, int r, bool forward, hipStream_t s) {
+    const unsigned g6 = ceil_div(6 * d.sz, 256);
+    if (r >= 1 && r <= 3) {
+        const int ti = (int)ceil_div(d.m, kBI), tj = (int)ceil_div(d.n, kBJ), tk = (int)ceil_div(d.o, kBK);
+        const dim3 grid((unsigned)(ti * tj * tk), 6);
+        auto go = [&](auto kern, int NA, int NB) {
+            const size_t smem = (size_t)(NA + NB) * sizeof(R);
+            set_smem((const void*)kern, smem);
+            kern<<<grid, 256, smem, s>>>(out, in, d, ti, tj);
+        };
+        if (r == 1) go(k_sf_box3<R, 1>, Box3<1>::NA, Box3<1>::NB);
+        else if (r == 2) go(k_sf_box3<R, 2>, Box3<2>::NA, Box3<2>::NB);
+        else go(k_sf_box3<R, 3>, Box3<3>::NA, Box3<3>::NB);
+        DEEDS_LAUNCH_CHECK();
+        return;
+    }
+    const int a0 = forward ? 0 : 2, a2 = forward ? 2 : 0;
+    k_sf_box<R><<<g6, 256, 0, s>>>(out, in, d, r, a0);
+    k_sf_box<R><<<g6, 256, 0, s>>>(tmp, out, d, r, 1);
+    k_sf_box<R><<<g6, 256, 0, s>>>(out, tmp, d, r, a2);
+    DEEDS_LAUNCH_CHECK();
 }
 
 // the twelve SSC distances of voxel (i,j,k) with the out-of-volume fallback of MINDSSCbox.h:186-190
@@ -185,10 +275,7 @@ void soft_features(R* phi, const R* im, int m, int n, int o, int qs, R* w0, R* w
     const unsigned g6 = ceil_div(6 * d.sz, 256), g1 = ceil_div(d.sz, 256);
     k_sf_ssd<R><<<g6, 256, 0, s>>>(w0, im, d, qs);
     DEEDS_LAUNCH_CHECK();
-    k_sf_box<R><<<g6, 256, 0, s>>>(w1, w0, d, qs, 0);
-    k_sf_box<R><<<g6, 256, 0, s>>>(w0, w1, d, qs, 1);
-    k_sf_box<R><<<g6, 256, 0, s>>>(w1, w0, d, qs, 2);
-    DEEDS_LAUNCH_CHECK();
+    box3<R>(w1, w0, w0, d, qs, true, s);
     k_sf_phi<R><<<g1, 256, 0, s>>>(phi, w1, d, qs);
     DEEDS_LAUNCH_CHECK();
 }
@@ -200,16 +287,13 @@ void soft_features_backward(R* gim, const R* gphi, const R* im, int m, int n, in
     const unsigned g6 = ceil_div(6 * d.sz, 256), g1 = ceil_div(d.sz, 256);
     // recompute the box-filtered distances D into w1
     k_sf_ssd<R><<<g6, 256, 0, s>>>(w0, im, d, qs);
-    k_sf_box<R><<<g6, 256, 0, s>>>(w1, w0, d, qs, 0);
-    k_sf_box<R><<<g6, 256, 0, s>>>(w0, w1, d, qs, 1);
-    k_sf_box<R><<<g6, 256, 0, s>>>(w1, w0, d, qs, 2);
     DEEDS_LAUNCH_CHECK();
-    // w2 (12 per voxel) = d/dv, w0 = d/dD, then back through the box passes (w1, w0) and the SSD
+    box3<R>(w1, w0, w0, d, qs, true, s);
+    // w2 (12 per voxel) = d/dv, w0 = d/dD, then back through the box (w1) and the SSD
     k_sf_phi_bwd<R><<<g1, 256, 0, s>>>(w2, gphi, w1, d, qs);
     k_sf_gather<R><<<g6, 256, 0, s>>>(w0, w2, d, qs);
-    k_sf_box<R><<<g6, 256, 0, s>>>(w1, w0, d, qs, 2);
-    k_sf_box<R><<<g6, 256, 0, s>>>(w0, w1, d, qs, 1);
-    k_sf_box<R><<<g6, 256, 0, s>>>(w1, w0, d, qs, 0);
+    DEEDS_LAUNCH_CHECK();
+    box3<R>(w1, w0, w0, d, qs, false, s);
     k_sf_ssd_bwd<R><<<g1, 256, 0, s>>>(gim, w1, im, d, qs);
     DEEDS_LAUNCH_CHECK();
 }
