@@ -469,13 +469,40 @@ class PairRun {
         return a >= b ? 0 : 1;
     }
 
+    // The MIND codes of warped0 and im1b depend on the patch radius only, so
+    // the next level that changes the radius can take them from a second pair
+    // of buffers filled beforehand.  Queued on s0 after level l's pre-BP
+    // work, they run while the GPU would otherwise wait for level l's trees
+    // (the first level's trees are built while the GPU has little else to
+    // do).  The temporaries are the forward direction's, used on s0 only.
+    void mind_ahead(int l) {
+        if (graph_ || ahead_level_ >= 0) return;
+        int l2 = l + 1;
+        while (l2 < L_ && !lv_[l2].new_mind) l2++;
+        if (l2 >= L_) return;
+        const size_t sz = (size_t)in_.m * in_.n * in_.o;
+        if (!mind_mov2_) {
+            mind_mov2_ = get<uint64_t>("mind.mov2", sz);
+            mind_fix2_ = get<uint64_t>("mind.fix2", sz);
+        }
+        const int qs = lv_[l2].mind_qs;
+        launch_descriptor(mind_mov2_, warped0_, in_.m, in_.n, in_.o, qs, sh_.mw[0][0], sh_.mw[0][1], s0_);
+        launch_descriptor(mind_fix2_, im1b_, in_.m, in_.n, in_.o, qs, sh_.mw[0][0], sh_.mw[0][1], s0_);
+        ahead_level_ = l2;
+    }
+
     // Pre-BP work of both directions: MIND of the images when the patch radius
     // changes, then per direction upsample, warp, MIND, padding, data cost
     // (the `first` direction's data cost before the other's; -1: concurrent).
     void pre_bp(int l, int first = -1) {
         const LevelCfg& c = lv_[l];
         if (!graph_) DEEDS_HIP(hipEventRecord(EV(l, 0), s0_));
-        if (c.new_mind) {
+        if (c.new_mind && ahead_level_ == l) {
+            // computed ahead (mind_ahead): the codes of this patch radius take the place of the previous ones
+            std::swap(mind_mov_, mind_mov2_);
+            std::swap(mind_fix_, mind_fix2_);
+            ahead_level_ = -1;
+        } else if (c.new_mind) {
             // both directions' MIND of warped0 / im1b (deedsBCV0.cpp:230-236)
             launch_descriptor(mind_mov_, warped0_, in_.m, in_.n, in_.o, c.mind_qs, sh_.mw[0][0], sh_.mw[0][1], s0_);
             launch_descriptor(mind_fix_, im1b_, in_.m, in_.n, in_.o, c.mind_qs, sh_.mw[0][0], sh_.mw[0][1], s0_);
@@ -649,6 +676,8 @@ class PairRun {
     Affine A_{};
     float *im1_, *im1b_, *warped0_;
     uint64_t *mind_mov_, *mind_fix_;
+    uint64_t *mind_mov2_ = nullptr, *mind_fix2_ = nullptr;  // codes computed ahead (mind_ahead)
+    int ahead_level_ = -1;
     float* cost_[2];
     int16_t* inds_[2];
     int16_t* comp_[2];
@@ -876,6 +905,7 @@ void run_level_sync(deeds_ctx* ctx, const std::vector<PairIn>& pairs, const deed
         // one pair: the direction with the longer expected BP computes its data cost first
         if (P == 1) {
             runs[0]->pre_bp(l, runs[0]->first_direction(l));
+            if (l == 0) runs[0]->mind_ahead(0);
         } else if (replay) {
             DEEDS_HIP(hipGraphLaunch(G->pre[l], s0));
             s1_behind_s0();
